@@ -1,0 +1,219 @@
+/*
+ * tracer_hip.h -- C-ABI of the MI355X time-step hot path of tracer_code.
+ *
+ * The reference (seantrim/tracer_code, Fortran 90 + OpenMP) has no FFI; its
+ * hot path is a set of subroutines on module-global state.  Every entry point
+ * below replaces the body of one of those subroutines (or one of the inline
+ * OpenMP tracer loops of energy_time) and is meant to be bound from Fortran
+ * with ISO_C_BINDING (the interface module is shown in INTEGRATION.md).
+ *
+ * Conventions
+ *  - All functions return int: 0 = ok, >0 = the reference's own `stop`
+ *    conditions, <0 = library / HIP errors.  tc_last_error() gives text.
+ *  - Host arrays use the reference's Fortran layout: a ghosted grid field
+ *    f(-2:nx+2,-2:nz+2) is passed as the address of f(-2,-2), i fastest,
+ *    leading dimension nx+5.  Composition fields Cnew/Cvis/Cbuoy are
+ *    type-fastest, C(1:ntype,-2:nx+2,-2:nz+2), and count_type is
+ *    count_type(0:ntype,-2:nx+2,-2:nz+2) (basics.f90:42-98).
+ *  - One host thread and one HIP stream per context; not re-entrant.
+ *    Host arrays are valid when a call returns (synchronous at the boundary).
+ *  - Device state (multigrid hierarchy, fields, tracers) is owned by the
+ *    library; host arrays stay owned by the caller.
+ *  - Only order=2 finite differences, iterate_order=1 (the Euler/Richardson
+ *    smoother), uniform grids (gridX=gridZ=0), Newtonian rheology
+ *    (non_Newtonian=0, ivis in {0,1,2}) and global time steps
+ *    (local_time=0) are accepted; anything else returns TC_ERR_UNSUPPORTED.
+ */
+#ifndef TRACER_HIP_H
+#define TRACER_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TC_MAX_TYPES 4
+#define TC_MAX_LEVELS 6
+#define TC_MAX_MULTI 52 /* multi(0:51), multigrid_cycle.f90:6 */
+
+/* Status codes. */
+enum {
+    TC_OK = 0,
+    TC_ERR_MAX_CYCLES = 1,    /* iterative_solver.f90:164-167 */
+    TC_ERR_RESID_BLOWUP = 2,  /* iterative_solver.f90:275-278 */
+    TC_ERR_TRACER_DOMAIN = 3, /* composition.f90:709-712 */
+    TC_ERR_SINGULAR = 4,      /* DGBTRF INFO>0 on the coarsest grid */
+    TC_ERR_ARG = -1,
+    TC_ERR_HIP = -2,
+    TC_ERR_UNSUPPORTED = -3,
+    TC_ERR_NOMEM = -4
+};
+
+/* Solver precision modes (tc_set_mode). */
+enum {
+    TC_MODE_FP64 = 0,  /* plain path: the reference's Jacobi-Richardson W-cycle in FP64 */
+    TC_MODE_MIXED = 1  /* FP32 sweeps/transfers inside an FP64 defect-correction loop */
+};
+
+/* Field identifiers for tc_put_field / tc_get_field. */
+enum {
+    TC_F_T = 0,      /* T(-2:nx+2,-2:nz+2) */
+    TC_F_SF = 1,     /* SF (streamfunction) */
+    TC_F_TVIS = 2,   /* Tvis (smoothed T for viscosity) */
+    TC_F_TBUOY = 3,  /* Tbuoy (T for buoyancy) */
+    TC_F_CNEW = 4,   /* Cnew(1:ntype,...) */
+    TC_F_CVIS = 5,   /* Cvis(1:ntype,...) */
+    TC_F_CBUOY = 6,  /* Cbuoy(1:ntype,...) */
+    TC_F_U = 7,      /* u: the ghosted (-2:nx+2,-2:nz+2) frame; the reference's u(-1:nx+1,-1:nz+1) sits inside it */
+    TC_F_W = 8,      /* w: same frame as u */
+    TC_F_STRAIN = 9, /* strain */
+    TC_F_VIS = 10,   /* vis_grid(1,...) = vis after a solve */
+    TC_F_CONDUCT = 11,
+    TC_F_NEMPTY = 12,     /* int32 nempty(-2:nx+2,-2:nz+2) */
+    TC_F_COUNT_TYPE = 13, /* int32 count_type(0:ntype,...) */
+    TC_F_T0 = 14,    /* energy_time stage fields T0/T1/T2 (device-resident driver) */
+    TC_F_T1 = 15,
+    TC_F_T2 = 16,
+    TC_F_COUNT
+};
+
+/* Tracer position sets (energy.f90 arrays rtr/str, rtr0/str0, rtr1/str1, rtr2/str2). */
+enum { TC_TR_X = 0, TC_TR_X0 = 1, TC_TR_X1 = 2, TC_TR_X2 = 3 };
+
+/*
+ * Run parameters: the compile-time inputs of main.f90:8-116 plus what the
+ * Fortran host derives at start-up (FD_coefficients main.f90:350-469,
+ * multigrid_setup multigrid_cycle.f90:1-99, derivative_eigenvalues
+ * energy.f90:501-588, compute_smoothing_parameters smoother.f90:31-73).
+ * Index j of a 5-point table D(-2:2) is stored at [j+2]; Dint[m-1][j+1]
+ * holds D(m,j) of the interpolation table D(1:2,-1:1).
+ */
+typedef struct tc_params {
+    int32_t nx, nz;       /* cells after optimize_grid */
+    int32_t grids;        /* multigrid levels after optimize_grid */
+    int32_t order;        /* must be 2 */
+    int32_t iterate_order;/* must be 1 */
+    int32_t Vbc, Tbot;    /* velocity / bottom thermal BC switches */
+    int32_t Nmulti, vis_iter, max_cycles;
+    int32_t nsteps;       /* W/V-cycle length */
+    int32_t multi[TC_MAX_MULTI]; /* multi(0:nsteps+1) */
+    int32_t comp, ntype, ivis, torder, RK_update;
+    int32_t pad0;
+    double aspect;
+    double tolerance, eig_ratio, eig_ratio_min;
+    double courant, courantT, courantC, courant_stream;
+    double RaT, visT, visP, H;
+    double RaC[TC_MAX_TYPES], visC[TC_MAX_TYPES];
+    double conduct_factor[TC_MAX_TYPES], Htype[TC_MAX_TYPES];
+    double delta_T;                  /* T smoothing threshold */
+    double delta_C[TC_MAX_TYPES];    /* C smoothing thresholds */
+    double D1[5], D2[5], D3[5], D4[5];
+    double Dint[2][3];
+    /* Fourier symbol maxima (derivative_eigenvalues) */
+    double Tr_base, Trr_base, Trrr_base, Trrrr_base, Trs_base, Trrs_base, Trrss_base;
+} tc_params;
+
+typedef struct tc_ctx tc_ctx;
+
+/* Timing / counters for the last call(s); filled by tc_get_stats. */
+typedef struct tc_stats {
+    int32_t q_last;          /* multigrid cycles of the last solve */
+    int32_t n_eta;           /* smoother_vis passes at level 1 of the last solve */
+    int32_t n_temp;          /* smoother_time_T iterations of the last call */
+    int32_t n_comp[TC_MAX_TYPES];
+    int32_t guess_used;      /* learned guess accepted in the last solve (1) or rejected (0) */
+    int32_t pad;
+    double residual_last;    /* residual_mag(1) of the last solve */
+    double res_ref_last;     /* res_ref(1) of the last solve */
+    double eig_fine;
+    double dt_last;          /* dt of the last time step */
+    double time;             /* model time */
+    int64_t sweeps_exec[TC_MAX_LEVELS]; /* smoother sweeps executed per level (cumulative) */
+    int64_t residuals_exec[TC_MAX_LEVELS];
+    int64_t coarse_solves;
+    int64_t vis_iters;       /* smoother_vis iterations (cumulative) */
+    int64_t c_iters;         /* smoother_time iterations (cumulative) */
+    int64_t t_iters;         /* smoother_time_T iterations (cumulative) */
+    int64_t mg_cycles;       /* multigrid cycles (cumulative) */
+} tc_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces storage (basics.f90:37-110) + the device half of compute_basics
+ * (basics.f90:112-196): allocates the level hierarchy and ntr tracers. */
+int tc_init(const tc_params* p, int64_t ntr, tc_ctx** out);
+int tc_finalize(tc_ctx* ctx);
+const char* tc_last_error(const tc_ctx* ctx);
+int tc_set_mode(tc_ctx* ctx, int mode);
+int tc_get_stats(const tc_ctx* ctx, tc_stats* out);
+int tc_sync(tc_ctx* ctx);
+
+/* ---- data movement (the Fortran host keeps its own copies) ------------- */
+int tc_put_field(tc_ctx* ctx, int field, const void* host);
+int tc_get_field(tc_ctx* ctx, int field, void* host);
+/* rtr/str/Ttype of ntr tracers (arrays.f90:3,8-9, ID order). */
+int tc_put_tracers(tc_ctx* ctx, const double* rtr, const double* str, const int32_t* ttype);
+/* which = TC_TR_X / _X0 / _X1 / _X2; returned in ID order. */
+int tc_get_tracers(tc_ctx* ctx, int which, double* r, double* s);
+/* Model time and dt (energy.f90 stability_RK4 writes dt). */
+int tc_set_time(tc_ctx* ctx, double time, double dt);
+
+/* ---- drop-in entry points ---------------------------------------------- */
+/* multigrid (iterative_solver.f90:79-173): solves for SF from the device
+ * Tbuoy/Cbuoy/Tvis/Cvis using SF as the warm start; q_out = cycles used,
+ * resid_out = residual_mag(1), resref_out = res_ref(1). */
+int tc_multigrid(tc_ctx* ctx, int32_t* q_out, double* resid_out, double* resref_out);
+/* compute_velocities (momentum.f90:532-551) + velocityBCs. */
+int tc_compute_velocities(tc_ctx* ctx);
+/* compute_strain_rate_invariant (momentum.f90:32-62). */
+int tc_compute_strain_rate_invariant(tc_ctx* ctx);
+/* smoother_time_T(Tin) (smoother.f90:342-376); src = TC_F_T/_T0/_T1/_T2. */
+int tc_smoother_time_T(tc_ctx* ctx, int src_field, int32_t* n_temp);
+/* tracers_to_corners (composition.f90:669-790) on the position set `which`,
+ * followed (smooth=1) by smoother_time and load_conductivity_array. */
+int tc_tracers_to_corners(tc_ctx* ctx, int which, int smooth, int32_t* n_comp);
+/* smoother_time (smoother.f90:286-340) alone. */
+int tc_smoother_time(tc_ctx* ctx, int32_t* n_comp);
+/* load_conductivity_array (energy.f90:31-58). */
+int tc_load_conductivity(tc_ctx* ctx);
+/* One RK stage of the tracer loops of energy_time (energy.f90:182-225,
+ * 270-312, 355-378, 420-440), including compute_derivatives of u and w
+ * (energy.f90:170-171 etc.); dt is the full time step. */
+int tc_tracer_stage(tc_ctx* ctx, int stage, double dt);
+/* stability_RK4 (energy.f90:600-646): returns dt from the device u,w. */
+int tc_stability_RK4(tc_ctx* ctx, double* dt_out);
+/* One T-field RK stage of energy_time (energy.f90:157-167, 248-260,
+ * 339-351, 405-416) on the device fields. */
+int tc_energy_stage(tc_ctx* ctx, int stage, double dt);
+/* nsteps iterations of the body of the time loop (main.f90:174-205) with
+ * tracer=0, comp=1, RK4 and RK_update=1: multigrid, compute_velocities,
+ * strain, energy_time; everything stays on the device. */
+int tc_time_step(tc_ctx* ctx, int nsteps);
+
+/* ---- host start-up helper ----------------------------------------------- */
+/* derivative_eigenvalues (energy.f90:501-598) for callers without the
+ * Fortran host: Fourier-symbol maxima of the FD tables, single-precision
+ * T_test products as in the reference.  base = Tr, Trr, Trrr, Trrrr, Trs,
+ * Trrs, Trrss (the *_base scalars of tc_params).  No GPU needed. */
+int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
+                              double base[7]);
+
+/* ---- measurement hooks (no counterpart in the reference) --------------- */
+/* Kernel ids for tc_kernel_timing. */
+enum { TC_K_SWEEP_L1 = 0, TC_K_COUNT };
+/* enable=1: bracket every launch of the timed kernels with HIP events on the
+ * library stream (the stream the kernels run on); enable=0 stops. */
+int tc_set_profiling(tc_ctx* ctx, int enable);
+/* Average device duration (microseconds) and launch count of kernel `which`
+ * since profiling was enabled. */
+int tc_kernel_timing(tc_ctx* ctx, int which, double* avg_us, int64_t* count);
+
+/* ---- learned initial guess (no counterpart in the reference) ----------- */
+/* Enables/disables the convolutional warm start used as the multigrid
+ * initial guess (the hook is error(1,:,:)=SF, iterative_solver.f90:87). */
+int tc_guess_enable(tc_ctx* ctx, int enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRACER_HIP_H */

@@ -1,0 +1,122 @@
+"""GPU parity of the HIP hot path against the CPU oracle (same seeded inputs).
+
+Bars: the FP64 path reproduces the oracle's streamfunction and velocities to
+1e-10 relative after a multigrid solve (same cycle count), tracer positions
+to 1e-12 after RK stages, and integer counts / tracer-ratio composition bit
+for bit given identical positions.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import load_oracle_state, rel_err
+from tracer_code_amd import Solver, config, ic
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (preset, overrides)
+    "cfg0_32": ("cfg0", dict(nx=32, nz=32)),
+    "cfg0_64": ("cfg0", dict(nx=64, nz=64)),
+    "cfg1_96x32": ("cfg1", dict(nx=96, nz=32, tpc=8)),
+}
+
+
+def make_pair(case, seed=1):
+    preset, ov = CASES[case]
+    params, cfg = config.preset_params(preset, **ov)
+    T, r, s, ty = ic.workload(cfg, seed=seed)
+    print(f"[case {case}] grid {params.nx}x{params.nz} grids={params.grids} ntr={len(r)}", flush=True)
+    o = load_oracle_state(params, T, r, s, ty)
+    g = Solver(params, len(r))
+    g.initialize(T, r, s, ty)
+    return params, o, g
+
+
+def interior(a):
+    return a[2:-2, 2:-2]
+
+
+@pytest.mark.parametrize("case", ["cfg0_32", "cfg1_96x32"])
+def test_startup_fields_match(case):
+    params, o, g = make_pair(case)
+    with g:
+        assert np.array_equal(g.get("Tvis"), o.plane(oracle.OR_TVIS))
+        assert np.array_equal(g.get("Tbuoy"), o.plane(oracle.OR_TBUOY))
+        assert np.array_equal(g.get("nempty"), o.plane(oracle.OR_NEMPTY, 0, np.int32))
+        ct = g.get("count_type")
+        for t in range(params.ntype + 1):
+            assert np.array_equal(ct[..., t], o.plane(oracle.OR_COUNT, t, np.int32))
+        cn, cv, cb = g.get("Cnew"), g.get("Cvis"), g.get("Cbuoy")
+        for t in range(params.ntype):
+            assert np.array_equal(cn[..., t], o.plane(oracle.OR_CNEW, t + 1))
+            assert np.array_equal(cb[..., t], o.plane(oracle.OR_CBUOY, t + 1))
+            assert rel_err(cv[..., t], o.plane(oracle.OR_CVIS, t + 1)) == 0.0
+        assert rel_err(g.get("conduct"), o.plane(oracle.OR_CONDUCT)) <= 1e-15
+
+
+@pytest.mark.parametrize("case", ["cfg0_32", "cfg0_64", "cfg1_96x32"])
+def test_multigrid_solve_matches(case):
+    params, o, g = make_pair(case)
+    with g:
+        qg, resid, resref = g.multigrid()
+        assert o.multigrid() == 0
+        qo = int(o.scalar(oracle.ORS_Q))
+        print(f"q gpu={qg} oracle={qo} resid={resid:.3e} ref={resref:.3e}", flush=True)
+        assert qg == qo
+        assert abs(resref - o.scalar(oracle.ORS_RESREF1)) <= 1e-12 * abs(resref)
+        e_sf = rel_err(g.get("SF"), o.plane(oracle.OR_SF))
+        e_vis = rel_err(interior(g.get("vis")), interior(o.plane(oracle.OR_VIS, 1)))
+        g.compute_velocities()
+        o.compute_velocities()
+        e_u = rel_err(g.get("u"), o.plane(oracle.OR_U))
+        e_w = rel_err(g.get("w"), o.plane(oracle.OR_W))
+        print(f"rel err SF={e_sf:.2e} vis={e_vis:.2e} u={e_u:.2e} w={e_w:.2e}", flush=True)
+        assert e_vis <= 1e-13
+        assert e_sf <= 1e-10 and e_u <= 1e-10 and e_w <= 1e-10
+
+
+@pytest.mark.parametrize("case", ["cfg0_32", "cfg1_96x32"])
+def test_tracer_stages_given_velocities(case):
+    """RK4 stages on identical u,w: positions agree to 1e-12 (bitwise expected)."""
+    params, o, g = make_pair(case)
+    with g:
+        assert o.multigrid() == 0
+        o.compute_velocities()
+        g.put("u", o.plane(oracle.OR_U).copy())
+        g.put("w", o.plane(oracle.OR_W).copy())
+        dt = o.stability_RK4()
+        for stage in (1, 2, 3, 4):
+            g.tracer_stage(stage, dt)
+            o.tracer_stage(stage, dt)
+        for which, (ro, so) in {"x": (oracle.OR_RTR, oracle.OR_STR), "x1": (oracle.OR_RTR1, oracle.OR_STR1),
+                                "x2": (oracle.OR_RTR2, oracle.OR_STR2)}.items():
+            rg, sg = g.get_tracers(which)
+            assert np.max(np.abs(rg - o.tracers(ro))) <= 1e-12
+            assert np.max(np.abs(sg - o.tracers(so))) <= 1e-12
+        # binning of identical positions is bit-exact
+        g.tracers_to_corners("x", smooth=False)
+        o.set_scalar(oracle.ORS_SMOOTH_VIS, 0)
+        assert o.tracers_to_corners(0) == 0
+        assert np.array_equal(g.get("nempty"), o.plane(oracle.OR_NEMPTY, 0, np.int32))
+        cn = g.get("Cnew")
+        for t in range(params.ntype):
+            assert np.array_equal(cn[..., t], o.plane(oracle.OR_CNEW, t + 1))
+
+
+@pytest.mark.parametrize("case", ["cfg0_32"])
+def test_time_steps_match(case):
+    params, o, g = make_pair(case)
+    with g:
+        nsteps = 2
+        g.time_step(nsteps)
+        for _ in range(nsteps):
+            assert o.time_step() == 0
+        e_sf = rel_err(g.get("SF"), o.plane(oracle.OR_SF))
+        e_T = rel_err(g.get("T"), o.plane(oracle.OR_T))
+        rg, sg = g.get_tracers("x")
+        e_r = float(np.max(np.abs(rg - o.tracers(oracle.OR_RTR))))
+        e_s = float(np.max(np.abs(sg - o.tracers(oracle.OR_STR))))
+        print(f"after {nsteps} steps: SF {e_sf:.2e} T {e_T:.2e} r {e_r:.2e} s {e_s:.2e}", flush=True)
+        assert e_sf <= 1e-9 and e_T <= 1e-9
+        assert e_r <= 1e-10 and e_s <= 1e-10

@@ -1,0 +1,264 @@
+"""Run parameters for the HIP hot path.
+
+Builds the C-ABI parameter block (tc_params) from the reference's input
+variables (main.f90:8-116) and performs the start-up derivations its Fortran
+host does before the time loop:
+
+* optimize_grid          (iterative_solver.f90:1-61)
+* multigrid_setup        (multigrid_cycle.f90:1-99)
+* FD_coefficients        (main.f90:350-469)
+* compute_smoothing_parameters (smoother.f90:31-73)
+* derivative_eigenvalues (energy.f90:501-598; computed by the library's host
+  helper tc_derivative_eigenvalues, single-precision products as in the
+  reference)
+
+PRESETS holds the three benchmark workloads: 128x128 (aspect 1, Ra=1e5),
+768x256 (aspect 3, Ra=1e6) and 2048x1024 (aspect 2, Ra=1e7, 42M tracers).
+"""
+import ctypes
+import math
+
+from . import _abi
+
+# W-cycle visit sequences of multigrid_cycle.f90 (the active ones).
+_CYCLES = {
+    2: (1, 2),
+    3: (1, 2, 3, 2, 3, 2),
+    4: (1, 2, 3, 4, 3, 4, 3, 2, 3, 4, 3, 4, 3, 2),
+    5: (1, 2, 3, 4, 5, 4, 5, 4, 3, 4, 5, 4, 3, 2, 3, 4, 5, 4, 3, 4, 5, 4, 5, 4, 3, 2),
+    6: (1, 2, 3, 4, 5, 6, 5, 6, 5, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 3, 2, 3, 4, 5, 6, 5, 4, 3, 4, 5, 6, 5, 4, 5, 6,
+        5, 6, 5, 4, 3, 2),
+}
+
+
+def optimize_grid(nx, nz, grids, n_points_min=4):
+    """optimize_grid (iterative_solver.f90:1-61): number of levels and the
+    rounded resolution.  Returns (nx, nz, grids)."""
+    n_points_max = min(nx, nz)
+    n_coarse_min, n_coarse_max = 1, grids - 1
+    n_max = n_points_max * 2 ** n_coarse_max
+    if nz <= nx:
+        nxbig, n_in = 1, nz
+    else:
+        nxbig, n_in = 0, nx
+    n_temp, n_coarse_temp, n_points_temp = n_max, 0, 0
+    for n_points in range(n_points_min, n_points_max + 1):
+        for n_coarse in range(n_coarse_min, n_coarse_max + 1):
+            n = n_points * 2 ** n_coarse
+            if n >= n_in and n < n_temp:
+                n_temp, n_coarse_temp, n_points_temp = n, n_coarse, n_points
+    grids = n_coarse_temp + 1
+    if nxbig == 0:
+        nx = n_temp
+    else:
+        nz = n_temp
+    n_in = nx if nz <= nx else nz
+    q = n_in / 2 ** n_coarse_temp
+    n_points_temp = int(math.floor(q + 0.5)) if q >= 0 else -int(math.floor(-q + 0.5))  # Fortran NINT
+    n_temp = n_points_temp * 2 ** n_coarse_temp
+    if nxbig == 0:
+        nz = n_temp
+    else:
+        nx = n_temp
+    return nx, nz, grids
+
+
+def multigrid_setup(grids):
+    """multigrid_setup (multigrid_cycle.f90:1-99): (nsteps, multi(0:51))."""
+    seq = _CYCLES[grids]
+    multi = [0] * _abi.TC_MAX_MULTI
+    for p, lv in enumerate(seq, start=1):
+        multi[p] = lv
+    nsteps = len(seq)
+    multi[0] = multi[nsteps]
+    multi[nsteps + 1] = multi[1]
+    return nsteps, multi
+
+
+def fd_coefficients(order=2):
+    """FD_coefficients (main.f90:350-469): Fornberg weights on the node
+    sequence 0, 1, -1, 2, -2, ... and the enforced (anti)symmetries.
+    Returns D1, D2, D3, D4 as 5-tables (index j at [j+2]) and D(1:2,-1:1)."""
+    Dnumber = order
+    bigM = max(4, Dnumber)
+    bigN = order + bigM - 2
+    span = order // 2 + 1
+    span_interp = (order + Dnumber - 2) // 2
+    a = [0.0] * (bigN + 2)
+    k = 1
+    for i in range(1, bigN + 1, 2):
+        a[i] = float(k)
+        a[i + 1] = -float(k)
+        k += 1
+    # delta[m+1][n][i], m = -1..bigM
+    delta = [[[0.0] * (bigN + 1) for _ in range(bigN + 1)] for _ in range(bigM + 2)]
+
+    def dl(m, n, i):
+        return delta[m + 1][n][i]
+
+    def set_dl(m, n, i, v):
+        delta[m + 1][n][i] = v
+
+    x0 = 0.0
+    set_dl(0, 0, 0, 1.0)
+    c1 = 1.0
+    for n in range(1, bigN + 1):
+        c2 = 1.0
+        for i in range(0, n):
+            c3 = a[n] - a[i]
+            c2 = c2 * c3
+            if n <= bigM:
+                set_dl(n, n - 1, i, 0.0)
+            for m in range(0, min(n, bigM) + 1):
+                set_dl(m, n, i, ((a[n] - x0) * dl(m, n - 1, i) - float(m) * dl(m - 1, n - 1, i)) / c3)
+        for m in range(0, min(n, bigM) + 1):
+            set_dl(m, n, n, (c1 / c2) * (float(m) * dl(m - 1, n - 1, n - 1) - (a[n - 1] - x0) * dl(m, n - 1, n - 1)))
+        c1 = c2
+    D1 = [0.0] * 5
+    D2 = [0.0] * 5
+    D3 = [0.0] * 5
+    D4 = [0.0] * 5
+    D1[2] = dl(1, order, 0)
+    D2[2] = dl(2, order, 0)
+    k = 1
+    for i in range(1, order + 1, 2):
+        D1[2 + k] = dl(1, order, i)
+        D1[2 - k] = dl(1, order, i + 1)
+        D2[2 + k] = dl(2, order, i)
+        D2[2 - k] = dl(2, order, i + 1)
+        k += 1
+    Dint = [[0.0] * 3 for _ in range(2)]
+    for m in range(2, Dnumber + 1, 2):
+        Dint[m - 2][1] = dl(m - 1, order + m - 2, 0)
+        Dint[m - 1][1] = dl(m, order + m - 2, 0)
+        k = 1
+        for i in range(1, order + m - 1, 2):
+            Dint[m - 2][1 + k] = dl(m - 1, order + m - 2, i)
+            Dint[m - 2][1 - k] = dl(m - 1, order + m - 2, i + 1)
+            Dint[m - 1][1 + k] = dl(m, order + m - 2, i)
+            Dint[m - 1][1 - k] = dl(m, order + m - 2, i + 1)
+            k += 1
+    for m in range(1, Dnumber + 1, 2):
+        Dint[m - 1][1] = 0.0
+        for i in range(1, span_interp + 1):
+            Dint[m - 1][1 + i] = -Dint[m - 1][1 - i]
+            Dint[m][1 + i] = Dint[m][1 - i]
+    D3[2] = dl(3, order + 2, 0)
+    D4[2] = dl(4, order + 2, 0)
+    k = 1
+    for i in range(1, order + 3, 2):
+        D3[2 + k] = dl(3, order + 2, i)
+        D3[2 - k] = dl(3, order + 2, i + 1)
+        D4[2 + k] = dl(4, order + 2, i)
+        D4[2 - k] = dl(4, order + 2, i + 1)
+        k += 1
+    D1[2] = 0.0
+    D3[2] = 0.0
+    for i in range(1, span + 1):
+        D1[2 + i] = -D1[2 - i]
+        D2[2 + i] = D2[2 - i]
+        D3[2 + i] = -D3[2 - i]
+        D4[2 + i] = D4[2 - i]
+    return D1, D2, D3, D4, Dint
+
+
+def smoothing_parameters(ivis, RaT, visT, delta_visT, comp, ntype, conduct_factor, delta_visC, visC):
+    """compute_smoothing_parameters (smoother.f90:31-73): (delta_T, delta_C)."""
+    if ivis == 0 or RaT == 0.0:
+        delta_T = 2.0
+    elif visT != 1.0:
+        delta_T = math.log(delta_visT) / math.log(visT)
+    else:
+        delta_T = 2.0
+    delta_C = [2.0] * max(ntype, 1)
+    if comp == 1:
+        for i in range(ntype):
+            if conduct_factor[i] != 1.0:
+                delta_C[i] = abs(math.log(delta_visC[i]) / math.log(1.0 / conduct_factor[i]))
+        if ivis == 2:
+            for i in range(ntype):
+                if visC[i] != 1.0:
+                    delta_C[i] = abs(math.log(delta_visC[i]) / math.log(visC[i]))
+    return delta_T, delta_C
+
+
+_BASE_CACHE = {}
+
+
+def derivative_eigenvalues(D1, D2, D3, D4, lib=None):
+    """Fourier-symbol maxima (energy.f90:501-598) via the library's host
+    helper; cached per FD table."""
+    key = (tuple(D1), tuple(D2), tuple(D3), tuple(D4))
+    if key in _BASE_CACHE:
+        return _BASE_CACHE[key]
+    lib = lib or _abi.load_library()
+    arr5 = ctypes.c_double * 5
+    base = (ctypes.c_double * 7)()
+    st = lib.tc_derivative_eigenvalues(arr5(*D1), arr5(*D2), arr5(*D3), arr5(*D4), base)
+    if st != 0:
+        raise RuntimeError(f"tc_derivative_eigenvalues failed: {st}")
+    out = tuple(base)
+    _BASE_CACHE[key] = out
+    return out
+
+
+# Benchmark workloads (reference parameter names, main.f90:8-116).
+PRESETS = {
+    "cfg0": dict(aspect=1.0, nx=128, nz=128, RaT=1e5, RaC=(0.5e5,), ivis=1, visT=100.0, visC=(1.0,), tpc=20,
+                 dlayer=0.2, initial_comp=0, t_variant="cos", steps=10),
+    "cfg1": dict(aspect=3.0, nx=768, nz=256, RaT=1e6, RaC=(0.6e6,), ivis=2, visT=1e4, visC=(1.0 / 30.0,), tpc=40,
+                 dlayer=0.1, initial_comp=1, t_variant="cos", steps=100),
+    "cfg2": dict(aspect=2.0, nx=2048, nz=1024, RaT=1e7, RaC=(0.5e7,), ivis=2, visT=1e5, visC=(0.01,), tpc=20,
+                 dlayer=0.15, initial_comp=0, t_variant="fourier4", steps=50),
+}
+
+
+def make_params(nx, nz, aspect=1.0, grids=6, mingrid=4, RaT=1e5, comp=1, ntype=1, RaC=(0.5e5,), ivis=1,
+                visT=100.0, visP=1.0, visC=(1.0,), delta_visT=100.0, delta_visC=(2.0,), conduct_factor=(1.0,),
+                H=0.0, Htype=(0.0,), tolerance=1e-8, Nmulti=70, vis_iter=12, max_cycles=1000, eig_ratio=4.0,
+                eig_ratio_min=4.0, courant=0.99, courantT=0.99, courantC=0.99, courant_stream=0.99, torder=4,
+                RK_update=1, Vbc=0, Tbot=0, order=2, lib=None, **_ignored):
+    """tc_params for one run (order=2, iterate_order=1, uniform grid)."""
+    nx, nz, grids = optimize_grid(nx, nz, grids, mingrid)
+    nsteps, multi = multigrid_setup(grids)
+    D1, D2, D3, D4, Dint = fd_coefficients(order)
+    base = derivative_eigenvalues(D1, D2, D3, D4, lib=lib)
+    pad = lambda seq, fill: (list(seq) + [fill] * _abi.TC_MAX_TYPES)[:_abi.TC_MAX_TYPES]
+    delta_T, delta_C = smoothing_parameters(ivis, RaT, visT, delta_visT, comp, ntype, pad(conduct_factor, 1.0),
+                                            pad(delta_visC, 2.0), pad(visC, 1.0))
+    p = _abi.TcParams()
+    p.nx, p.nz, p.grids, p.order, p.iterate_order = nx, nz, grids, order, 1
+    p.Vbc, p.Tbot = Vbc, Tbot
+    p.Nmulti, p.vis_iter, p.max_cycles, p.nsteps = Nmulti, vis_iter, max_cycles, nsteps
+    for i, m in enumerate(multi):
+        p.multi[i] = m
+    p.comp, p.ntype, p.ivis, p.torder, p.RK_update = comp, ntype, ivis, torder, RK_update
+    p.aspect, p.tolerance, p.eig_ratio, p.eig_ratio_min = aspect, tolerance, eig_ratio, eig_ratio_min
+    p.courant, p.courantT, p.courantC, p.courant_stream = courant, courantT, courantC, courant_stream
+    p.RaT, p.visT, p.visP, p.H = RaT, visT, visP, H
+    for i, v in enumerate(pad(RaC, 0.0)):
+        p.RaC[i] = v
+    for i, v in enumerate(pad(visC, 1.0)):
+        p.visC[i] = v
+    for i, v in enumerate(pad(conduct_factor, 1.0)):
+        p.conduct_factor[i] = v
+    for i, v in enumerate(pad(Htype, 0.0)):
+        p.Htype[i] = v
+    p.delta_T = delta_T
+    for i, v in enumerate(pad(delta_C, 2.0)):
+        p.delta_C[i] = v
+    for j in range(5):
+        p.D1[j], p.D2[j], p.D3[j], p.D4[j] = D1[j], D2[j], D3[j], D4[j]
+    for m in range(2):
+        for j in range(3):
+            p.Dint[m][j] = Dint[m][j]
+    (p.Tr_base, p.Trr_base, p.Trrr_base, p.Trrrr_base, p.Trs_base, p.Trrs_base, p.Trrss_base) = base
+    return p
+
+
+def preset_params(name, **overrides):
+    """tc_params of a named workload; keyword overrides replace preset values
+    (e.g. nx/nz for a reduced-size parity case)."""
+    cfg = dict(PRESETS[name])
+    cfg.update(overrides)
+    return make_params(**cfg), cfg

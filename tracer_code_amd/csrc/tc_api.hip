@@ -1,0 +1,1278 @@
+// C-ABI of the MI355X hot path (include/tracer_hip.h) and the host-side
+// orchestration of the kernels: the multigrid driver with its data-dependent
+// control flow, the T/C filter loops, tracers_to_corners and the RK stages
+// of energy_time.  One HIP stream carries the hot path; a second stream
+// (default priority) factors and inverts the coarsest-grid matrix while the
+// first W-cycle descent runs.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "tc_kernels.h"
+
+using namespace tc;
+
+namespace {
+
+constexpr double PII = 3.1415926535897932;
+
+// slots of the device scalar array
+enum { S_RESID1 = 0, S_RESREF1, S_RMIN, S_SMOOTH, S_EIGA, S_EIGD, S_COUNT };
+
+bool is_pow2(double x)
+{
+    if (!(x > 0.0) || !std::isfinite(x)) return false;
+    int e;
+    double m = std::frexp(x, &e);
+    return m == 0.5;
+}
+
+Spacing make_spacing(const Level& L)
+{
+    Spacing h;
+    h.dr = L.dr; h.ds = L.ds; h.dr2 = L.dr2; h.ds2 = L.ds2; h.dr3 = L.dr3; h.ds3 = L.ds3; h.dr4 = L.dr4; h.ds4 = L.ds4;
+    h.drds = L.drds; h.drds2 = L.drds2; h.dr2ds = L.dr2ds; h.dr2ds2 = L.dr2ds2;
+    h.idr = 1.0 / L.dr; h.ids = 1.0 / L.ds; h.idr2 = 1.0 / L.dr2; h.ids2 = 1.0 / L.ds2; h.idr3 = 1.0 / L.dr3;
+    h.ids3 = 1.0 / L.ds3; h.idr4 = 1.0 / L.dr4; h.ids4 = 1.0 / L.ds4; h.idrds = 1.0 / L.drds;
+    h.idrds2 = 1.0 / L.drds2; h.idr2ds = 1.0 / L.dr2ds; h.idr2ds2 = 1.0 / L.dr2ds2;
+    return h;
+}
+
+bool level_pow2(const Level& L)
+{
+    return is_pow2(L.dr) && is_pow2(L.ds) && is_pow2(L.dr2) && is_pow2(L.ds2) && is_pow2(L.dr3) && is_pow2(L.ds3) &&
+           is_pow2(L.dr4) && is_pow2(L.ds4) && is_pow2(L.drds) && is_pow2(L.drds2) && is_pow2(L.dr2ds) &&
+           is_pow2(L.dr2ds2);
+}
+
+}  // namespace
+
+// Device-side objects that need only host bookkeeping.
+struct tc_dev {
+    FD F;
+    Spacing h[TC_MAX_LEVELS + 1];
+    bool pow2[TC_MAX_LEVELS + 1];
+    double* d_scal = nullptr;
+    double* h_scal = nullptr;
+    double* partialA = nullptr;
+    double* partialB = nullptr;
+    int max_partial = 0;
+    bool coarse_ready = false;
+    std::vector<double> hbuf;  // host staging plane
+};
+
+static tc_dev& DEV(tc_ctx* ctx) { return *static_cast<tc_dev*>(ctx->impl); }
+
+namespace {
+
+// Allocations are zeroed on the context stream, which orders them before
+// every later use (the streams are non-blocking, so the null stream would not).
+int dalloc(tc_ctx* ctx, double** p, size_t n)
+{
+    TC_HIP(hipMalloc((void**)p, n * sizeof(double)));
+    TC_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), ctx->stream));
+    return 0;
+}
+int dalloc_i(tc_ctx* ctx, int32_t** p, size_t n)
+{
+    TC_HIP(hipMalloc((void**)p, n * sizeof(int32_t)));
+    TC_HIP(hipMemsetAsync(*p, 0, n * sizeof(int32_t), ctx->stream));
+    return 0;
+}
+
+EigScalars eig_scalars(const tc_params& p, const Level& L)
+{
+    // (iterative_solver.f90:674-685)
+    EigScalars e;
+    e.Ts_max = p.Tr_base / L.ds;
+    e.Tss_max = -p.Trr_base / L.ds2;
+    e.Tsss_max = p.Trrr_base / L.ds3;
+    e.Tssss_max = p.Trrrr_base / L.ds4;
+    e.Tr_max = p.Tr_base / L.dr;
+    e.Trr_max = -p.Trr_base / L.dr2;
+    e.Trrr_max = p.Trrr_base / L.dr3;
+    e.Trrrr_max = p.Trrrr_base / L.dr4;
+    e.Trs_max = -p.Trs_base / L.drds;
+    e.Trrss_max = p.Trrss_base / L.dr2ds2;
+    e.Trss_max = p.Trrs_base / L.drds2;
+    e.Trrs_max = p.Trrs_base / L.dr2ds;
+    return e;
+}
+
+size_t plane_bytes(const Grid& g) { return g.elems * sizeof(double); }
+
+// host <-> device copies of a ghosted Fortran-layout plane (ld = nx+5)
+int put_plane(tc_ctx* ctx, double* dev, const Grid& g, const double* host)
+{
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
+    TC_HIP(hipMemcpy2DAsync(dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double), host, w * sizeof(double),
+                            w * sizeof(double), g.nz + 2 * SPAN + 1, hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+int get_plane(tc_ctx* ctx, const double* dev, const Grid& g, double* host)
+{
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
+    TC_HIP(hipMemcpy2DAsync(host, w * sizeof(double), dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double),
+                            w * sizeof(double), g.nz + 2 * SPAN + 1, hipMemcpyDeviceToHost, ctx->stream));
+    return 0;
+}
+int put_iplane(tc_ctx* ctx, int32_t* dev, const Grid& g, const int32_t* host)
+{
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
+    TC_HIP(hipMemcpy2DAsync(dev + g.at(-SPAN, -SPAN), g.ld * sizeof(int32_t), host, w * sizeof(int32_t),
+                            w * sizeof(int32_t), g.nz + 2 * SPAN + 1, hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+int get_iplane(tc_ctx* ctx, const int32_t* dev, const Grid& g, int32_t* host)
+{
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
+    TC_HIP(hipMemcpy2DAsync(host, w * sizeof(int32_t), dev + g.at(-SPAN, -SPAN), g.ld * sizeof(int32_t),
+                            w * sizeof(int32_t), g.nz + 2 * SPAN + 1, hipMemcpyDeviceToHost, ctx->stream));
+    return 0;
+}
+
+int copy_plane(tc_ctx* ctx, double* dst, const double* src, const Grid& g)
+{
+    TC_HIP(hipMemcpyAsync(dst, src, plane_bytes(g), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
+// kernel timing: event pairs around timed launches, folded into totals
+constexpr int PROF_PAIRS = 4096;
+int prof_flush(tc_ctx* ctx)
+{
+    if (ctx->prof_used == 0) return 0;
+    TC_HIP(hipEventSynchronize(ctx->prof_ev[2 * ctx->prof_used - 1]));
+    for (int q = 0; q < ctx->prof_used; ++q) {
+        float ms = 0.f;
+        TC_HIP(hipEventElapsedTime(&ms, ctx->prof_ev[2 * q], ctx->prof_ev[2 * q + 1]));
+        ctx->prof_ms[TC_K_SWEEP_L1] += ms;
+        ctx->prof_n[TC_K_SWEEP_L1]++;
+    }
+    ctx->prof_used = 0;
+    return 0;
+}
+int prof_begin(tc_ctx* ctx)
+{
+    if (ctx->prof_used == PROF_PAIRS) TC_TRY(prof_flush(ctx));
+    TC_HIP(hipEventRecord(ctx->prof_ev[2 * ctx->prof_used], ctx->stream));
+    return 0;
+}
+int prof_end(tc_ctx* ctx)
+{
+    TC_HIP(hipEventRecord(ctx->prof_ev[2 * ctx->prof_used + 1], ctx->stream));
+    ctx->prof_used++;
+    return 0;
+}
+
+// read device scalars [0, n) into the pinned mirror (synchronises the stream)
+int fetch_scalars(tc_ctx* ctx, tc_dev& d, int n)
+{
+    TC_HIP(hipMemcpyAsync(d.h_scal, d.d_scal, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// viscosity hierarchy
+// ---------------------------------------------------------------------------
+int grad_eig_min(tc_ctx* ctx, tc_dev& d, int level, double* rmin)
+{
+    Level& L = ctx->lv[level];
+    EigScalars e = eig_scalars(ctx->p, L);
+    int nb = launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], ctx->p.ivis, e, 1, d.partialA);
+    ctx->ratio_level = (ctx->p.ivis == 0) ? -1 : level;
+    launch_reduce_final(ctx->stream, d.partialA, nb, 1, d.d_scal, S_RMIN);
+    TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+    *rmin = (ctx->p.ivis == 0) ? 1e99 : d.h_scal[S_RMIN];
+    ctx->stats.vis_iters++;
+    return 0;
+}
+
+// smoother_vis (smoother.f90:165-263)
+int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
+{
+    Level& L = ctx->lv[level];
+    const tc_params& p = ctx->p;
+    int option, Niter;
+    double lb, la;
+    if (level == 1 && interpolate == 0) { option = 1; Niter = 6; lb = -5.0; la = -2.0; }
+    else { option = 0; Niter = p.vis_iter; lb = -2.0; la = -1.0; }
+    if (interpolate == 0 || level == 1) { ctx->Nvis_store = 0; ctx->nn_store = 1; }
+    int kmin, kmax;
+    if (p.RaT == 0.0) { kmin = 0; kmax = L.g.nz; } else { kmin = 1; kmax = L.g.nz - 1; }
+    int n = 0, trigger = 0;
+    const Grid* gf = level > 1 ? &ctx->lv[level - 1].g : &L.g;
+    const double* Vf = level > 1 ? ctx->lv[level - 1].vis : nullptr;
+    for (;;) {
+        for (int nn = ctx->nn_store; nn <= Niter; ++nn) {
+            double rmin;
+            TC_TRY(grad_eig_min(ctx, d, level, &rmin));
+            if (option == 1) {
+                if ((ctx->Nvis_store >= 1 && rmin > p.eig_ratio_min) || p.ivis == 0 || n > 20) {
+                    ctx->stats.eig_fine = rmin;
+                    ctx->stats.n_eta = n;
+                    trigger = 1;
+                    break;
+                }
+            } else if (interpolate == 0) {
+                if (rmin > p.eig_ratio || p.ivis == 0) { trigger = 1; ctx->nn_store = nn; break; }
+            } else {
+                if (ctx->Nvis_store >= 1 || p.ivis == 0) { trigger = 1; break; }
+            }
+            const int ntemp = Niter - nn + 1;
+            const double c = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+            TC_TRY(copy_plane(ctx, L.vtmp, L.vis, L.g));
+            launch_vis_sweep(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax, c);
+            launch_visc_bcs(ctx->stream, L.vis, L.g, Vf, *gf, level, 0, p.RaT, p.visT * p.visT);
+        }
+        if (trigger) break;
+        ctx->Nvis_store++;
+        ctx->nn_store = 1;
+        if (option == 1) { ctx->nn_store = 1; Niter = 6; lb = -2.0; la = -1.0; }
+        n++;
+    }
+    return 0;
+}
+
+int save_restore_level(tc_ctx* ctx, Level& L, bool save)
+{
+    double* a[6] = {L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz};
+    double* b[6] = {L.visf, L.visxf, L.viszf, L.visxxf, L.viszzf, L.visxzf};
+    for (int t = 0; t < 6; ++t) TC_TRY(save ? copy_plane(ctx, b[t], a[t], L.g) : copy_plane(ctx, a[t], b[t], L.g));
+    return 0;
+}
+
+// compute_viscosity (iterative_solver.f90:331-377)
+int compute_viscosity(tc_ctx* ctx, tc_dev& d, int level)
+{
+    const tc_params& p = ctx->p;
+    if (level == 1) {
+        RaCArr lc{};
+        PtrArr cv{};
+        for (int t = 0; t < p.ntype; ++t) { lc.v[t] = (p.comp == 1 && p.ivis == 2) ? std::log(p.visC[t]) : 0.0; cv.v[t] = ctx->Cvis[t]; }
+        launch_visf(ctx->stream, ctx->lv[1].vis, ctx->Tvis, cv, ctx->zg, ctx->lv[1].g, p.ivis, p.comp, p.ntype, p.RaT,
+                    std::log(p.visT), std::log(p.visP), lc);
+        return smoother_vis(ctx, d, 1, 0);
+    }
+    Level& Lf = ctx->lv[level - 1];
+    Level& L = ctx->lv[level];
+    TC_TRY(save_restore_level(ctx, Lf, true));
+    TC_TRY(smoother_vis(ctx, d, level - 1, 1));
+    launch_restrict(ctx->stream, Lf.vis, Lf.g, L.vis, L.g);
+    launch_visc_bcs(ctx->stream, L.vis, L.g, Lf.vis, Lf.g, level, 1, p.RaT, p.visT * p.visT);
+    if (level < ctx->grids) {
+        TC_TRY(smoother_vis(ctx, d, level, 0));
+    } else {
+        EigScalars e = eig_scalars(p, L);
+        launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 0, nullptr);
+    }
+    TC_TRY(save_restore_level(ctx, Lf, false));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid (iterative_solver.f90:79-173), Newtonian branch
+// ---------------------------------------------------------------------------
+int iterate_stream(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, int p, bool& first_sweep)
+{
+    Level& L = ctx->lv[level];
+    const tc_params& P = ctx->p;
+    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
+    const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
+    for (int n = 1; n <= Niter; ++n) {
+        const double ntemp = (double)(Niter - n + 1);
+        const double temp = std::cos((2.0 * ntemp - 1.0) * PII / (double)(2 * Niter));
+        double cconst = 1.0;
+        if (dir != 1) {
+            const double lb = -2.0, la = -1.0;
+            cconst = 2.0 / (-lb - la + (lb - la) * temp);
+        }
+        const bool timed = ctx->prof && level == 1;
+        if (timed) TC_TRY(prof_begin(ctx));
+        launch_sweep(ctx->stream, d.pow2[level], use_ratio, L, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level], P.ivis,
+                     temp, cconst, zsg);
+        if (timed) TC_TRY(prof_end(ctx));
+        L.cur = 1 - L.cur;
+        if (first_sweep) {
+            // error(1) started as a copy of SF; from the second sweep on the
+            // reference reads walls zeroed by enforceBCs_stream
+            launch_bc_stream(ctx->stream, L.err[1 - L.cur], L.g, P.Vbc);
+            first_sweep = false;
+        }
+    }
+    ctx->stats.sweeps_exec[level - 1] += Niter;
+    // compute_residual
+    int nb = launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis,
+                             level == 1 ? d.partialA : nullptr);
+    ctx->stats.residuals_exec[level - 1]++;
+    if (level == 1) launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
+    (void)p;
+    return 0;
+}
+
+int coarse_setup(tc_ctx* ctx, tc_dev& d)
+{
+    Level& L = ctx->lv[ctx->grids];
+    const tc_params& P = ctx->p;
+    TC_HIP(hipEventRecord(ctx->ev_visc_ready, ctx->stream));
+    TC_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_visc_ready, 0));
+    TC_HIP(hipMemsetAsync(ctx->Mc, 0, sizeof(double) * (size_t)ctx->ldab * ctx->ngc, ctx->side));
+    launch_coarse_assemble(ctx->side, ctx->Mc, ctx->RowB, L, d.F, d.h[ctx->grids], P.ivis, P.Vbc, ctx->kl, ctx->ku,
+                           ctx->ngc);
+    launch_dgbtf2(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->d_info);
+    launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->Ainv);
+    TC_HIP(hipEventRecord(ctx->ev_inv_ready, ctx->side));
+    d.coarse_ready = true;
+    return 0;
+}
+
+int run_multigrid(tc_ctx* ctx, int32_t* q_out)
+{
+    tc_dev& d = DEV(ctx);
+    const tc_params& P = ctx->p;
+    const int G = ctx->grids;
+    Level& L1 = ctx->lv[1];
+    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
+    // error(1,:,:) = SF
+    L1.cur = 0;
+    TC_TRY(copy_plane(ctx, L1.err[0], ctx->SF, L1.g));
+    // compute_RHS(1), compute_reference_residual(1)
+    {
+        RaCArr rc{};
+        PtrArr cb{};
+        for (int t = 0; t < P.ntype; ++t) { rc.v[t] = P.RaC[t]; cb.v[t] = ctx->Cbuoy[t]; }
+        int nb = launch_rhs1(ctx->stream, ctx->Tbuoy, cb, L1.rhs, L1.g, d.F, L1.dr, P.RaT, P.comp, P.ntype, rc, d.partialA);
+        launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESREF1);
+    }
+    TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+    ctx->res_ref[1] = d.h_scal[S_RESREF1] / (double)(L1.g.nx - 2) / (double)(L1.g.nz - 2);
+    // viscosity hierarchy, coarse matrix + LU (+ inverse), local time steps
+    for (int L = 1; L <= G; ++L) TC_TRY(compute_viscosity(ctx, d, L));
+    if (ctx->time == 0.0 || P.ivis > 0 || !d.coarse_ready) TC_TRY(coarse_setup(ctx, d));
+    for (int L = 1; L <= G; ++L) {
+        EigScalars e = eig_scalars(P, ctx->lv[L]);
+        launch_local_dt(ctx->stream, ctx->lv[L], e, P.courant_stream);
+    }
+    TC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_inv_ready, 0));
+    bool first_sweep = true;
+    const int* multi = P.multi;
+    int q = 1;
+    int status = 0;
+    for (;;) {
+        bool done = false;
+        for (int p = 1; p <= P.nsteps; ++p) {
+            const int level = multi[p];
+            const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
+            if (multi[p - 1] < multi[p]) {
+                // compute_RHS(level) = restrict(residual of level-1)
+                Level& Lf = ctx->lv[level - 1];
+                launch_restrict(ctx->stream, Lf.res, Lf.g, ctx->lv[level].rhs, ctx->lv[level].g);
+            }
+            if (level < G) {
+                TC_TRY(iterate_stream(ctx, d, level, P.Nmulti, dir, p, first_sweep));
+            } else {
+                Level& Lc = ctx->lv[G];
+                launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, Lc, Lc.err[Lc.cur], ctx->ngc, zsg);
+                ctx->stats.coarse_solves++;
+            }
+            if (level == 1) {
+                TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+                ctx->residual_mag[1] = d.h_scal[S_RESID1] / (double)(L1.g.nx - 1) / (double)(L1.g.nz - 1);
+                if (ctx->residual_mag[1] > 1e10 * ctx->res_ref[1]) {
+                    ctx->err = "Multigrid Convergence Problem: Abnormally High Residual Detected -- stopping";
+                    status = TC_ERR_RESID_BLOWUP;
+                    break;
+                }
+                if (ctx->residual_mag[1] / ctx->res_ref[1] <= P.tolerance) { done = true; break; }
+            }
+            if (multi[p + 1] < multi[p]) {
+                Level& Lc = ctx->lv[level];
+                Level& Lf = ctx->lv[level - 1];
+                launch_prolong(ctx->stream, Lf.err[Lf.cur], Lf.g, Lc.err[Lc.cur], Lc.g, zsg);
+            }
+        }
+        ctx->stats.mg_cycles++;
+        if (status) break;
+        if (done) break;
+        q++;
+        if (q > P.max_cycles) {
+            ctx->err = "Multigrid Convergence Problem: Too Many Multigrid Cycles Needed";
+            status = TC_ERR_MAX_CYCLES;
+            break;
+        }
+    }
+    // LU singularity check (DGBTRF INFO)
+    int32_t info = 0;
+    TC_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    info = *ctx->h_flag;
+    if (info != 0 && status == 0) {
+        ctx->err = "DGBTRF: singular coarse-grid matrix";
+        status = TC_ERR_SINGULAR;
+    }
+    if (status == 0) TC_TRY(copy_plane(ctx, ctx->SF, L1.err[L1.cur], L1.g));
+    ctx->stats.q_last = q;
+    ctx->stats.residual_last = ctx->residual_mag[1];
+    ctx->stats.res_ref_last = ctx->res_ref[1];
+    if (q_out) *q_out = q;
+    return status;
+}
+
+// ---------------------------------------------------------------------------
+// filters, binning, stages
+// ---------------------------------------------------------------------------
+int smoothness(tc_ctx* ctx, tc_dev& d, const double* f, double* diff)
+{
+    int nb = launch_smoothness(ctx->stream, f, ctx->g, d.partialA);
+    launch_reduce_final(ctx->stream, d.partialA, nb, 2, d.d_scal, S_SMOOTH);
+    TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+    *diff = d.h_scal[S_SMOOTH];
+    return 0;
+}
+
+double* field_ptr(tc_ctx* ctx, int which)
+{
+    switch (which) {
+    case TC_F_T: return ctx->T; case TC_F_T0: return ctx->T0; case TC_F_T1: return ctx->T1; case TC_F_T2: return ctx->T2;
+    default: return nullptr;
+    }
+}
+
+// smoother_time_T (smoother.f90:342-376)
+int smoother_time_T(tc_ctx* ctx, tc_dev& d, const double* Tin, int32_t* n_out)
+{
+    const tc_params& P = ctx->p;
+    if (Tin != ctx->Tvis) TC_TRY(copy_plane(ctx, ctx->Tvis, Tin, ctx->g));
+    TC_TRY(copy_plane(ctx, ctx->Tbuoy, ctx->Tvis, ctx->g));
+    int n = 0;
+    if (P.RaT != 0.0) {
+        for (;;) {
+            double diff;
+            TC_TRY(smoothness(ctx, d, ctx->Tvis, &diff));
+            if (diff < P.delta_T) break;
+            TC_TRY(copy_plane(ctx, ctx->tmp2, ctx->Tvis, ctx->g));
+            launch_smooth_T(ctx->stream, ctx->tmp2, ctx->Tvis, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2,
+                            P.courantT);
+            launch_bcs_T(ctx->stream, ctx->Tvis, ctx->g, P.Tbot);
+            n++;
+            ctx->stats.t_iters++;
+        }
+        ctx->stats.n_temp = n;
+    }
+    if (n_out) *n_out = n;
+    return 0;
+}
+
+// smoother_time (smoother.f90:286-340)
+int smoother_time(tc_ctx* ctx, tc_dev& d, int32_t* n_comp)
+{
+    const tc_params& P = ctx->p;
+    int Niter = 10;
+    const double lb = -5.0, la = -1.0;
+    for (int t = 0; t < P.ntype; ++t) {
+        TC_TRY(copy_plane(ctx, ctx->Cvis[t], ctx->Cnew[t], ctx->g));
+        TC_TRY(copy_plane(ctx, ctx->Cbuoy[t], ctx->Cnew[t], ctx->g));
+    }
+    for (int t = 0; t < P.ntype; ++t) {
+        int nn = 0;
+        bool trigger = false;
+        double* C = ctx->Cvis[t];
+        for (;;) {
+            for (int n = 1; n <= Niter; ++n) {
+                double diff;
+                TC_TRY(smoothness(ctx, d, C, &diff));
+                if (diff < P.delta_C[t]) { trigger = true; break; }
+                double c;
+                if (nn == 0) {
+                    const int ntemp = Niter - n + 1;
+                    c = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+                } else {
+                    c = 1.0;
+                }
+                TC_TRY(copy_plane(ctx, ctx->tmp2, C, ctx->g));
+                launch_smooth_C(ctx->stream, ctx->tmp2, C, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2, c,
+                                P.courantC);
+                launch_bcs_comp(ctx->stream, C, ctx->g);
+                ctx->stats.c_iters++;
+            }
+            if (trigger) break;
+            Niter = 1;
+            nn++;
+        }
+        ctx->stats.n_comp[t] = nn;
+        if (n_comp) n_comp[t] = nn;
+    }
+    return 0;
+}
+
+int load_conductivity(tc_ctx* ctx, tc_dev& d)
+{
+    const tc_params& P = ctx->p;
+    RaCArr lcf{};
+    PtrArr cv{};
+    for (int t = 0; t < P.ntype; ++t) {
+        lcf.v[t] = (P.comp != 0) ? std::log(1.0 / P.conduct_factor[t]) : 0.0;
+        cv.v[t] = ctx->Cvis[t];
+    }
+    launch_conduct(ctx->stream, cv, ctx->conduct, ctx->conduct_r, ctx->conduct_s, ctx->g, d.F, ctx->lv[1].dr,
+                   ctx->lv[1].ds, P.comp, P.ntype, lcf);
+    return 0;
+}
+
+void tracer_set(tc_ctx* ctx, int which, double** r, double** s)
+{
+    switch (which) {
+    case TC_TR_X0: *r = ctx->rtr0; *s = ctx->str0; break;
+    case TC_TR_X1: *r = ctx->rtr1; *s = ctx->str1; break;
+    case TC_TR_X2: *r = ctx->rtr2; *s = ctx->str2; break;
+    default: *r = ctx->rtr; *s = ctx->str; break;
+    }
+}
+
+// tracers_to_corners (composition.f90:669-790)
+int tracers_to_corners(tc_ctx* ctx, tc_dev& d, int which, int smooth, int32_t* n_comp)
+{
+    const tc_params& P = ctx->p;
+    double *r, *s;
+    tracer_set(ctx, which, &r, &s);
+    CntArr cnt{};
+    for (int t = 0; t <= P.ntype; ++t) {
+        cnt.v[t] = ctx->count[t];
+        launch_zero_i32(ctx->stream, ctx->count[t], ctx->g.elems);
+    }
+    launch_zero_i32(ctx->stream, ctx->d_flag, 1);
+    launch_bin(ctx->stream, r, s, ctx->ttype, ctx->ntr, cnt, ctx->g, ctx->lv[1].dr, ctx->lv[1].ds, P.aspect, ctx->d_flag);
+    PtrArr cn{};
+    for (int t = 0; t < P.ntype; ++t) cn.v[t] = ctx->Cnew[t];
+    launch_ratio(ctx->stream, cnt, ctx->nempty, cn, ctx->g, P.ntype, P.comp);
+    TC_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    if (*ctx->h_flag) {
+        ctx->err = "Tracer advected outside of domain - stopping";
+        return TC_ERR_TRACER_DOMAIN;
+    }
+    if (P.comp == 1) {
+        for (int t = 0; t < P.ntype; ++t) launch_bcs_comp(ctx->stream, ctx->Cnew[t], ctx->g);
+        if (smooth) TC_TRY(smoother_time(ctx, d, n_comp));
+    }
+    return load_conductivity(ctx, d);
+}
+
+int tracer_stage(tc_ctx* ctx, tc_dev& d, int stage, double dt)
+{
+    D3Arr Dint{{ctx->p.Dint[0][0], ctx->p.Dint[0][1], ctx->p.Dint[0][2]}};
+    launch_derivs(ctx->stream, ctx->u, ctx->w, ctx->DERr_u, ctx->DERr_w, ctx->g, Dint, ctx->lv[1].dr);
+    launch_tracer_stage(ctx->stream, stage, ctx->p.torder, dt, ctx->ntr, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0,
+                        ctx->rtr1, ctx->str1, ctx->rtr2, ctx->str2, ctx->u, ctx->w, ctx->DERr_u, ctx->DERr_w, ctx->g,
+                        Dint, ctx->lv[1].dr, ctx->lv[1].ds);
+    (void)d;
+    return 0;
+}
+
+int energy_stage(tc_ctx* ctx, tc_dev& d, int stage, double dt)
+{
+    const tc_params& P = ctx->p;
+    if (P.RaT == 0.0) return 0;
+    RaCArr ht{};
+    PtrArr cv{};
+    for (int t = 0; t < P.ntype; ++t) { ht.v[t] = P.Htype[t]; cv.v[t] = ctx->Cvis[t]; }
+    launch_energy_stage(ctx->stream, stage, dt, ctx->T, ctx->T0, ctx->T1, ctx->T2, ctx->u, ctx->w, ctx->conduct,
+                        ctx->conduct_r, ctx->conduct_s, cv, ctx->g, d.F, ctx->lv[1].dr, ctx->lv[1].ds, ctx->lv[1].dr2,
+                        ctx->lv[1].ds2, P.comp, P.ntype, P.H, ht);
+    double* dst = (stage == 1 || stage == 3) ? ctx->T1 : (stage == 2 ? ctx->T2 : ctx->T);
+    launch_bcs_T(ctx->stream, dst, ctx->g, P.Tbot);
+    return 0;
+}
+
+int stability_RK4(tc_ctx* ctx, tc_dev& d, double* dt_out)
+{
+    const tc_params& P = ctx->p;
+    const Level& L = ctx->lv[1];
+    double dt;
+    if (P.RaT != 0.0) {
+        const double Ts_max = P.Tr_base / L.ds, Tss_max = -P.Trr_base / L.ds2;
+        const double Tr_max = P.Tr_base / L.dr, Trr_max = -P.Trr_base / L.dr2;
+        int nb = launch_stability(ctx->stream, ctx->u, ctx->w, ctx->conduct, ctx->conduct_r, ctx->conduct_s, ctx->g,
+                                  Tr_max, Ts_max, Trr_max, Tss_max, d.partialA, d.partialB);
+        launch_reduce_final(ctx->stream, d.partialA, nb, 2, d.d_scal, S_EIGA);
+        launch_reduce_final(ctx->stream, d.partialB, nb, 2, d.d_scal, S_EIGD);
+        TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+        const double eA = d.h_scal[S_EIGA], eD = d.h_scal[S_EIGD];
+        double dtA, dtD;
+        if (P.torder == 2) { dtA = 1.0 / eA; dtD = 2.0 / eD; }
+        else { dtA = 2.8 / eA; dtD = 2.8 / eD; }
+        if (eA == 0.0) dt = dtD;
+        else dt = 1.0 / (1.0 / dtA + 1.0 / dtD);
+    } else {
+        dt = 0.5;
+    }
+    *dt_out = P.courant * dt;
+    return 0;
+}
+
+int velocities(tc_ctx* ctx, tc_dev& d)
+{
+    launch_velocities(ctx->stream, ctx->SF, ctx->u, ctx->w, ctx->g, d.F, ctx->lv[1].dr, ctx->lv[1].ds, ctx->p.Vbc);
+    return 0;
+}
+
+int copy_C(tc_ctx* ctx, double* const* dst, double* const* src)
+{
+    for (int t = 0; t < ctx->p.ntype; ++t) TC_TRY(copy_plane(ctx, dst[t], src[t], ctx->g));
+    return 0;
+}
+
+// energy_time (energy.f90:117-453) for tracer=0, comp=1, local_time=0
+int energy_time(tc_ctx* ctx, tc_dev& d)
+{
+    const tc_params& P = ctx->p;
+    double dt;
+    TC_TRY(stability_RK4(ctx, d, &dt));
+    ctx->dt = dt;
+    TC_TRY(copy_C(ctx, ctx->C0, ctx->Cnew));
+    TC_TRY(copy_plane(ctx, ctx->T0, ctx->T, ctx->g));
+    TC_TRY(energy_stage(ctx, d, 1, dt));
+    TC_TRY(tracer_stage(ctx, d, 1, dt));
+    if (P.torder >= 2) {
+        if (P.RK_update == 1) {
+            TC_TRY(tracers_to_corners(ctx, d, TC_TR_X1, 1, nullptr));
+            TC_TRY(smoother_time_T(ctx, d, ctx->T1, nullptr));
+            TC_TRY(run_multigrid(ctx, nullptr));
+            TC_TRY(velocities(ctx, d));
+        }
+        TC_TRY(energy_stage(ctx, d, 2, dt));
+        TC_TRY(tracer_stage(ctx, d, 2, dt));
+    }
+    if (P.torder >= 3) {
+        if (P.RK_update == 1) {
+            TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
+            TC_TRY(tracers_to_corners(ctx, d, TC_TR_X2, 1, nullptr));
+            TC_TRY(smoother_time_T(ctx, d, ctx->T2, nullptr));
+            TC_TRY(run_multigrid(ctx, nullptr));
+            TC_TRY(velocities(ctx, d));
+        }
+        TC_TRY(energy_stage(ctx, d, 3, dt));
+        TC_TRY(tracer_stage(ctx, d, 3, dt));
+    }
+    if (P.torder >= 4) {
+        if (P.RK_update == 1) {
+            TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
+            TC_TRY(tracers_to_corners(ctx, d, TC_TR_X1, 1, nullptr));
+            TC_TRY(smoother_time_T(ctx, d, ctx->T1, nullptr));
+            TC_TRY(run_multigrid(ctx, nullptr));
+            TC_TRY(velocities(ctx, d));
+        }
+        TC_TRY(energy_stage(ctx, d, 4, dt));
+        TC_TRY(tracer_stage(ctx, d, 4, dt));
+    }
+    TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
+    TC_TRY(tracers_to_corners(ctx, d, TC_TR_X, 1, nullptr));
+    TC_TRY(smoother_time_T(ctx, d, ctx->T, nullptr));
+    return 0;
+}
+
+// host copies of the grid-dependent time-step tables
+void host_dt_vis(const tc_params& p, const Level& L, std::vector<double>& plane)
+{
+    // stability_vis_smoother (smoother.f90:465-486), uniform grid
+    const Grid& g = L.g;
+    const int w = g.nx + 2 * SPAN + 1;
+    plane.assign((size_t)w * (g.nz + 2 * SPAN + 1), 0.0);
+    const double Ts_max = p.Tr_base / L.ds, Tss_max = -p.Trr_base / L.ds2;
+    const double Tr_max = p.Tr_base / L.dr, Trr_max = -p.Trr_base / L.dr2;
+    for (int i = 0; i <= g.nx; ++i)
+        for (int k = 1; k <= g.nz - 1; ++k) {
+            double eR = std::fabs(Trr_max / (1.0 * 1.0)) + std::fabs(Tss_max / (1.0 * 1.0));
+            double eI = std::fabs(0.0 / std::pow(1.0, 3.0)) * Tr_max + std::fabs(0.0 / std::pow(1.0, 3.0)) * Ts_max;
+            plane[(size_t)(k + SPAN) * w + (i + SPAN)] = 2.0 * eR / (eR * eR + eI * eI);
+        }
+}
+
+void host_dts(const tc_params& p, const Level& L, std::vector<double>& plane)
+{
+    // stability_RK4_smoother (smoother.f90:423-444), uniform grid
+    const Grid& g = L.g;
+    const int w = g.nx + 2 * SPAN + 1;
+    plane.assign((size_t)w * (g.nz + 2 * SPAN + 1), 0.0);
+    const double Ts_max = p.Tr_base / L.ds, Tss_max = -p.Trr_base / L.ds2;
+    const double Tr_max = p.Tr_base / L.dr, Trr_max = -p.Trr_base / L.dr2;
+    for (int i = 0; i <= g.nx; ++i)
+        for (int k = 0; k <= g.nz; ++k) {
+            double eR = std::fabs(Trr_max / (1.0 * 1.0)) + std::fabs(Tss_max / (1.0 * 1.0));
+            double eI = std::fabs(0.0 / std::pow(1.0, 3.0)) * Tr_max + std::fabs(0.0 / std::pow(1.0, 3.0)) * Ts_max;
+            plane[(size_t)(k + SPAN) * w + (i + SPAN)] = 2.0 * eR / (eR * eR + eI * eI);
+        }
+}
+
+int validate(const tc_params* p, std::string& why)
+{
+    if (p->order != 2) { why = "order must be 2"; return TC_ERR_UNSUPPORTED; }
+    if (p->iterate_order != 1) { why = "iterate_order must be 1"; return TC_ERR_UNSUPPORTED; }
+    if (p->grids < 2 || p->grids > TC_MAX_LEVELS) { why = "grids must be 2..6"; return TC_ERR_ARG; }
+    if (p->ivis < 0 || p->ivis > 2) { why = "ivis must be 0, 1 or 2 (Newtonian)"; return TC_ERR_UNSUPPORTED; }
+    if (p->ntype < 0 || p->ntype > TC_MAX_TYPES || (p->comp == 1 && p->ntype < 1)) { why = "bad ntype"; return TC_ERR_ARG; }
+    if (p->torder != 2 && p->torder != 4) { why = "torder must be 2 or 4"; return TC_ERR_ARG; }
+    if (p->nsteps < 2 || p->nsteps > TC_MAX_MULTI - 2) { why = "bad nsteps"; return TC_ERR_ARG; }
+    if (p->nx < 4 || p->nz < 4) { why = "grid too small"; return TC_ERR_ARG; }
+    const int f = 1 << (p->grids - 1);
+    if (p->nx % f || p->nz % f || p->nx / f < 2 || p->nz / f < 2) { why = "nx, nz must be divisible by 2^(grids-1)"; return TC_ERR_ARG; }
+    for (int q = 1; q <= p->nsteps; ++q)
+        if (p->multi[q] < 1 || p->multi[q] > p->grids) { why = "multi() out of range"; return TC_ERR_ARG; }
+    if (p->multi[0] != p->multi[p->nsteps] || p->multi[p->nsteps + 1] != p->multi[1]) { why = "multi() must wrap"; return TC_ERR_ARG; }
+    if (p->Nmulti < 1 || p->max_cycles < 1) { why = "bad Nmulti/max_cycles"; return TC_ERR_ARG; }
+    return 0;
+}
+
+void free_ctx(tc_ctx* ctx)
+{
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    for (int L = 1; L <= TC_MAX_LEVELS; ++L) {
+        Level& l = ctx->lv[L];
+        double* a[] = {l.err[0], l.err[1], l.rhs, l.vis, l.visx, l.visz, l.visxx, l.viszz, l.visxz, l.dtstream, l.ratio,
+                       l.dtvis, l.visf, l.visxf, l.viszf, l.visxxf, l.viszzf, l.visxzf, l.vtmp, l.res};
+        for (double* p : a) F(p);
+    }
+    double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
+                   ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
+                   ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
+                   ctx->str1, ctx->rtr2, ctx->str2};
+    for (double* p : b) F(p);
+    for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
+    for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
+    F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag);
+    if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    tc_dev* d = static_cast<tc_dev*>(ctx->impl);
+    if (d) {
+        F(d->d_scal); F(d->partialA); F(d->partialB);
+        if (d->h_scal) (void)hipHostFree(d->h_scal);
+        delete d;
+    }
+    for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+    if (ctx->ev_inv_ready) (void)hipEventDestroy(ctx->ev_inv_ready);
+    if (ctx->ev_visc_ready) (void)hipEventDestroy(ctx->ev_visc_ready);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+}
+
+int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
+{
+    ctx->p = *p;
+    ctx->ntr = ntr;
+    ctx->grids = p->grids;
+    tc_dev* d = new (std::nothrow) tc_dev();
+    if (!d) return TC_ERR_NOMEM;
+    ctx->impl = d;
+    TC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    TC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    TC_HIP(hipEventCreateWithFlags(&ctx->ev_inv_ready, hipEventDisableTiming));
+    TC_HIP(hipEventCreateWithFlags(&ctx->ev_visc_ready, hipEventDisableTiming));
+    // FD tables (order 2): Drs(i,k)=D1(k)*D1(i) ... (main.f90:448-459)
+    for (int j = 0; j < 5; ++j) { d->F.D1[j] = p->D1[j]; d->F.D2[j] = p->D2[j]; d->F.D3[j] = p->D3[j]; d->F.D4[j] = p->D4[j]; }
+    for (int dk = -1; dk <= 1; ++dk)
+        for (int di = -1; di <= 1; ++di) {
+            const int t = (dk + 1) * 3 + (di + 1);
+            d->F.Drs[t] = p->D1[dk + 2] * p->D1[di + 2];
+            d->F.Drss[t] = p->D2[dk + 2] * p->D1[di + 2];
+            d->F.Drrs[t] = p->D1[dk + 2] * p->D2[di + 2];
+            d->F.Drrss[t] = p->D2[dk + 2] * p->D2[di + 2];
+        }
+    // level hierarchy (basics.f90:167-182)
+    for (int j = 1; j <= ctx->grids; ++j) {
+        Level& L = ctx->lv[j];
+        const int nxl = p->nx / (1 << (j - 1)), nzl = p->nz / (1 << (j - 1));
+        L.g = make_grid(nxl, nzl);
+        L.dr = p->aspect / (double)nxl;
+        L.ds = 1.0 / (double)nzl;
+        L.dr2 = L.dr * L.dr; L.ds2 = L.ds * L.ds;
+        L.dr3 = std::pow(L.dr, 3.0); L.ds3 = std::pow(L.ds, 3.0);
+        L.dr4 = std::pow(L.dr, 4.0); L.ds4 = std::pow(L.ds, 4.0);
+        L.drds = L.dr * L.ds; L.drds2 = L.dr * L.ds2; L.dr2ds = L.dr2 * L.ds; L.dr2ds2 = L.dr2 * L.ds2;
+        d->h[j] = make_spacing(L);
+        d->pow2[j] = level_pow2(L);
+        const size_t n = L.g.elems;
+        double** a[] = {&L.err[0], &L.err[1], &L.rhs, &L.vis, &L.visx, &L.visz, &L.visxx, &L.viszz, &L.visxz,
+                        &L.dtstream, &L.ratio, &L.dtvis, &L.visf, &L.visxf, &L.viszf, &L.visxxf, &L.viszzf, &L.visxzf,
+                        &L.vtmp, &L.res};
+        for (double** q : a) TC_TRY(dalloc(ctx, q, n));
+    }
+    ctx->g = ctx->lv[1].g;
+    const size_t n1 = ctx->g.elems;
+    double** b[] = {&ctx->T, &ctx->SF, &ctx->u, &ctx->w, &ctx->strain, &ctx->Tvis, &ctx->Tbuoy, &ctx->conduct,
+                    &ctx->conduct_r, &ctx->conduct_s, &ctx->dts, &ctx->DERr_u, &ctx->DERr_w, &ctx->T0, &ctx->T1,
+                    &ctx->T2, &ctx->tmp, &ctx->tmp2};
+    for (double** q : b) TC_TRY(dalloc(ctx, q, n1));
+    for (int t = 0; t < p->ntype; ++t) {
+        TC_TRY(dalloc(ctx, &ctx->Cnew[t], n1)); TC_TRY(dalloc(ctx, &ctx->Cvis[t], n1));
+        TC_TRY(dalloc(ctx, &ctx->Cbuoy[t], n1)); TC_TRY(dalloc(ctx, &ctx->C0[t], n1));
+    }
+    for (int t = 0; t <= p->ntype; ++t) TC_TRY(dalloc_i(ctx, &ctx->count[t], n1));
+    TC_TRY(dalloc_i(ctx, &ctx->nempty, n1));
+    // zg(-3:nz+3), uniform grid with the mirrored ghosts of build_grid (grid.f90:97-108)
+    {
+        std::vector<double> zg(p->nz + 7);
+        const double ds = 1.0 / (double)p->nz;
+        for (int k = 0; k <= p->nz; ++k) zg[k + 3] = ds * (double)k;
+        for (int k = 1; k <= 3; ++k) { zg[-k + 3] = -zg[k + 3]; zg[p->nz + k + 3] = 1.0 - zg[p->nz - k + 3] + 1.0; }
+        zg[3] = 0.0;
+        zg[p->nz + 3] = 1.0;
+        TC_TRY(dalloc(ctx, &ctx->zg, zg.size()));
+        TC_HIP(hipMemcpyAsync(ctx->zg, zg.data(), zg.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    // coarse matrix bandwidth (Compute_Matrix_coarse analysis=1)
+    {
+        const int nxc = ctx->lv[ctx->grids].g.nx, nzc = ctx->lv[ctx->grids].g.nz, N = nzc + 1;
+        ctx->ngc = (nxc + 1) * (nzc + 1);
+        const int limit[5] = {0, 1, 2, 1, 0};
+        int ku = 0, kl = 0;
+        for (int pp = 1; pp <= ctx->ngc; ++pp) {
+            int kp = pp - N * ((pp - 1) / N) - 1, ip = (pp - 1) / N;
+            for (int j = -2; j <= 2; ++j) {
+                int iq = j + ip;
+                if (iq > 0 && iq < nxc)
+                    for (int kq = -limit[j + 2] + kp; kq <= kp + limit[j + 2]; ++kq)
+                        if (kq > 0 && kq < nzc) {
+                            int t = (N * iq + kq + 1) - pp;
+                            ku = std::max(ku, t);
+                            kl = std::max(kl, -t);
+                        }
+            }
+        }
+        ctx->kl = kl; ctx->ku = ku; ctx->ldab = 2 * kl + ku + 1;
+        TC_TRY(dalloc(ctx, &ctx->Mc, (size_t)ctx->ldab * ctx->ngc));
+        TC_TRY(dalloc(ctx, &ctx->RowB, ctx->ngc));
+        TC_TRY(dalloc(ctx, &ctx->Ainv, (size_t)ctx->ngc * ctx->ngc));
+        TC_TRY(dalloc(ctx, &ctx->Bc, ctx->ngc));
+        TC_TRY(dalloc_i(ctx, &ctx->ipiv, ctx->ngc));
+        TC_TRY(dalloc_i(ctx, &ctx->d_info, 1));
+    }
+    // tracers
+    if (ntr > 0) {
+        double** c[] = {&ctx->rtr, &ctx->str, &ctx->rtr0, &ctx->str0, &ctx->rtr1, &ctx->str1, &ctx->rtr2, &ctx->str2};
+        for (double** q : c) TC_TRY(dalloc(ctx, q, (size_t)ntr));
+        TC_TRY(dalloc_i(ctx, &ctx->ttype, (size_t)ntr));
+    }
+    TC_TRY(dalloc_i(ctx, &ctx->d_flag, 1));
+    TC_HIP(hipHostMalloc((void**)&ctx->h_flag, sizeof(int32_t), hipHostMallocDefault));
+    // reduction buffers
+    {
+        const Grid& g = ctx->g;
+        int nb = ((g.nx + 2 * SPAN + 1 + 63) / 64) * ((g.nz + 2 * SPAN + 1 + 3) / 4) + 16;
+        d->max_partial = nb;
+        TC_TRY(dalloc(ctx, &d->partialA, nb));
+        TC_TRY(dalloc(ctx, &d->partialB, nb));
+        TC_TRY(dalloc(ctx, &d->d_scal, 16));
+        TC_HIP(hipHostMalloc((void**)&d->h_scal, 16 * sizeof(double), hipHostMallocDefault));
+    }
+    // grid-dependent tables: dt_vis per level, dts_array
+    {
+        std::vector<double> plane;
+        for (int j = 1; j <= ctx->grids; ++j) {
+            host_dt_vis(*p, ctx->lv[j], plane);
+            TC_TRY(put_plane(ctx, ctx->lv[j].dtvis, ctx->lv[j].g, plane.data()));
+            TC_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        host_dts(*p, ctx->lv[1], plane);
+        TC_TRY(put_plane(ctx, ctx->dts, ctx->g, plane.data()));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    TC_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int tc_init(const tc_params* p, int64_t ntr, tc_ctx** out)
+{
+    if (!p || !out || ntr < 0) return TC_ERR_ARG;
+    *out = nullptr;
+    std::string why;
+    int v = validate(p, why);
+    if (v) {
+        fprintf(stderr, "tc_init: %s\n", why.c_str());
+        return v;
+    }
+    tc_ctx* ctx = new (std::nothrow) tc_ctx();
+    if (!ctx) return TC_ERR_NOMEM;
+    int st = init_ctx(ctx, p, ntr);
+    if (st) {
+        fprintf(stderr, "tc_init: %s\n", ctx->err.c_str());
+        free_ctx(ctx);
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return 0;
+}
+
+int tc_finalize(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
+    free_ctx(ctx);
+    delete ctx;
+    return 0;
+}
+
+const char* tc_last_error(const tc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tc_set_mode(tc_ctx* ctx, int mode)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED) return TC_ERR_ARG;
+    ctx->mode = mode;
+    return 0;
+}
+
+int tc_get_stats(const tc_ctx* ctx, tc_stats* out)
+{
+    if (!ctx || !out) return TC_ERR_ARG;
+    *out = ctx->stats;
+    out->dt_last = ctx->dt;
+    out->time = ctx->time;
+    return 0;
+}
+
+int tc_sync(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_put_field(tc_ctx* ctx, int field, const void* host)
+{
+    if (!ctx || !host) return TC_ERR_ARG;
+    const int nt = ctx->p.ntype;
+    const Grid& g = ctx->g;
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1), h = (size_t)(g.nz + 2 * SPAN + 1);
+    double* dptr = nullptr;
+    switch (field) {
+    case TC_F_T: dptr = ctx->T; break; case TC_F_SF: dptr = ctx->SF; break; case TC_F_TVIS: dptr = ctx->Tvis; break;
+    case TC_F_TBUOY: dptr = ctx->Tbuoy; break; case TC_F_U: dptr = ctx->u; break; case TC_F_W: dptr = ctx->w; break;
+    case TC_F_STRAIN: dptr = ctx->strain; break; case TC_F_VIS: dptr = ctx->lv[1].vis; break;
+    case TC_F_CONDUCT: dptr = ctx->conduct; break; case TC_F_T0: dptr = ctx->T0; break; case TC_F_T1: dptr = ctx->T1; break;
+    case TC_F_T2: dptr = ctx->T2; break;
+    default: break;
+    }
+    if (dptr) {
+        TC_TRY(put_plane(ctx, dptr, g, (const double*)host));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (field == TC_F_CNEW || field == TC_F_CVIS || field == TC_F_CBUOY) {
+        if (nt < 1) return TC_ERR_ARG;
+        std::vector<double> tmp(w * h);
+        const double* src = (const double*)host;
+        for (int t = 0; t < nt; ++t) {
+            for (size_t e = 0; e < w * h; ++e) tmp[e] = src[e * nt + t];
+            double* dst = field == TC_F_CNEW ? ctx->Cnew[t] : (field == TC_F_CVIS ? ctx->Cvis[t] : ctx->Cbuoy[t]);
+            TC_TRY(put_plane(ctx, dst, g, tmp.data()));
+            TC_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        return 0;
+    }
+    if (field == TC_F_NEMPTY) {
+        TC_TRY(put_iplane(ctx, ctx->nempty, g, (const int32_t*)host));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (field == TC_F_COUNT_TYPE) {
+        std::vector<int32_t> tmp(w * h);
+        const int32_t* src = (const int32_t*)host;
+        for (int t = 0; t <= nt; ++t) {
+            for (size_t e = 0; e < w * h; ++e) tmp[e] = src[e * (nt + 1) + t];
+            TC_TRY(put_iplane(ctx, ctx->count[t], g, tmp.data()));
+            TC_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        return 0;
+    }
+    return TC_ERR_ARG;
+}
+
+int tc_get_field(tc_ctx* ctx, int field, void* host)
+{
+    if (!ctx || !host) return TC_ERR_ARG;
+    const int nt = ctx->p.ntype;
+    const Grid& g = ctx->g;
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1), h = (size_t)(g.nz + 2 * SPAN + 1);
+    const double* dptr = nullptr;
+    switch (field) {
+    case TC_F_T: dptr = ctx->T; break; case TC_F_SF: dptr = ctx->SF; break; case TC_F_TVIS: dptr = ctx->Tvis; break;
+    case TC_F_TBUOY: dptr = ctx->Tbuoy; break; case TC_F_U: dptr = ctx->u; break; case TC_F_W: dptr = ctx->w; break;
+    case TC_F_STRAIN: dptr = ctx->strain; break; case TC_F_VIS: dptr = ctx->lv[1].vis; break;
+    case TC_F_CONDUCT: dptr = ctx->conduct; break; case TC_F_T0: dptr = ctx->T0; break; case TC_F_T1: dptr = ctx->T1; break;
+    case TC_F_T2: dptr = ctx->T2; break;
+    default: break;
+    }
+    if (dptr) {
+        TC_TRY(get_plane(ctx, dptr, g, (double*)host));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (field == TC_F_CNEW || field == TC_F_CVIS || field == TC_F_CBUOY) {
+        if (nt < 1) return TC_ERR_ARG;
+        std::vector<double> tmp(w * h);
+        double* dst = (double*)host;
+        for (int t = 0; t < nt; ++t) {
+            const double* src = field == TC_F_CNEW ? ctx->Cnew[t] : (field == TC_F_CVIS ? ctx->Cvis[t] : ctx->Cbuoy[t]);
+            TC_TRY(get_plane(ctx, src, g, tmp.data()));
+            TC_HIP(hipStreamSynchronize(ctx->stream));
+            for (size_t e = 0; e < w * h; ++e) dst[e * nt + t] = tmp[e];
+        }
+        return 0;
+    }
+    if (field == TC_F_NEMPTY) {
+        TC_TRY(get_iplane(ctx, ctx->nempty, g, (int32_t*)host));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    if (field == TC_F_COUNT_TYPE) {
+        std::vector<int32_t> tmp(w * h);
+        int32_t* dst = (int32_t*)host;
+        for (int t = 0; t <= nt; ++t) {
+            TC_TRY(get_iplane(ctx, ctx->count[t], g, tmp.data()));
+            TC_HIP(hipStreamSynchronize(ctx->stream));
+            for (size_t e = 0; e < w * h; ++e) dst[e * (nt + 1) + t] = tmp[e];
+        }
+        return 0;
+    }
+    return TC_ERR_ARG;
+}
+
+int tc_put_tracers(tc_ctx* ctx, const double* rtr, const double* str, const int32_t* ttype)
+{
+    if (!ctx || !rtr || !str || !ttype) return TC_ERR_ARG;
+    if (ctx->ntr == 0) return 0;
+    for (int64_t i = 0; i < ctx->ntr; ++i)
+        if (ttype[i] < 0 || ttype[i] > ctx->p.ntype) {
+            ctx->err = "tracer type outside 0..ntype";
+            return TC_ERR_ARG;
+        }
+    const size_t nb = (size_t)ctx->ntr * sizeof(double);
+    TC_HIP(hipMemcpyAsync(ctx->rtr, rtr, nb, hipMemcpyHostToDevice, ctx->stream));
+    TC_HIP(hipMemcpyAsync(ctx->str, str, nb, hipMemcpyHostToDevice, ctx->stream));
+    TC_HIP(hipMemcpyAsync(ctx->ttype, ttype, (size_t)ctx->ntr * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_get_tracers(tc_ctx* ctx, int which, double* r, double* s)
+{
+    if (!ctx || !r || !s) return TC_ERR_ARG;
+    if (ctx->ntr == 0) return 0;
+    double *dr, *ds;
+    tracer_set(ctx, which, &dr, &ds);
+    const size_t nb = (size_t)ctx->ntr * sizeof(double);
+    TC_HIP(hipMemcpyAsync(r, dr, nb, hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipMemcpyAsync(s, ds, nb, hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_set_time(tc_ctx* ctx, double time, double dt)
+{
+    if (!ctx) return TC_ERR_ARG;
+    ctx->time = time;
+    ctx->dt = dt;
+    return 0;
+}
+
+int tc_multigrid(tc_ctx* ctx, int32_t* q_out, double* resid_out, double* resref_out)
+{
+    if (!ctx) return TC_ERR_ARG;
+    int st = run_multigrid(ctx, q_out);
+    if (resid_out) *resid_out = ctx->residual_mag[1];
+    if (resref_out) *resref_out = ctx->res_ref[1];
+    if (st == 0) TC_HIP(hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+int tc_compute_velocities(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    TC_TRY(velocities(ctx, DEV(ctx)));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_compute_strain_rate_invariant(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    tc_dev& d = DEV(ctx);
+    launch_strain(ctx->stream, ctx->u, ctx->w, ctx->strain, ctx->g, d.F, ctx->lv[1].dr, ctx->lv[1].ds);
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_smoother_time_T(tc_ctx* ctx, int src_field, int32_t* n_temp)
+{
+    if (!ctx) return TC_ERR_ARG;
+    double* src = field_ptr(ctx, src_field);
+    if (!src) return TC_ERR_ARG;
+    TC_TRY(smoother_time_T(ctx, DEV(ctx), src, n_temp));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_tracers_to_corners(tc_ctx* ctx, int which, int smooth, int32_t* n_comp)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (which < TC_TR_X || which > TC_TR_X2) return TC_ERR_ARG;
+    TC_TRY(tracers_to_corners(ctx, DEV(ctx), which, smooth, n_comp));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_smoother_time(tc_ctx* ctx, int32_t* n_comp)
+{
+    if (!ctx) return TC_ERR_ARG;
+    TC_TRY(smoother_time(ctx, DEV(ctx), n_comp));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_load_conductivity(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    TC_TRY(load_conductivity(ctx, DEV(ctx)));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_tracer_stage(tc_ctx* ctx, int stage, double dt)
+{
+    if (!ctx || stage < 1 || stage > 4) return TC_ERR_ARG;
+    TC_TRY(tracer_stage(ctx, DEV(ctx), stage, dt));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_stability_RK4(tc_ctx* ctx, double* dt_out)
+{
+    if (!ctx || !dt_out) return TC_ERR_ARG;
+    return stability_RK4(ctx, DEV(ctx), dt_out);
+}
+
+int tc_energy_stage(tc_ctx* ctx, int stage, double dt)
+{
+    if (!ctx || stage < 1 || stage > 4) return TC_ERR_ARG;
+    TC_TRY(energy_stage(ctx, DEV(ctx), stage, dt));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_time_step(tc_ctx* ctx, int nsteps)
+{
+    if (!ctx || nsteps < 0) return TC_ERR_ARG;
+    tc_dev& d = DEV(ctx);
+    for (int s = 0; s < nsteps; ++s) {
+        TC_TRY(run_multigrid(ctx, nullptr));
+        TC_TRY(velocities(ctx, d));
+        launch_strain(ctx->stream, ctx->u, ctx->w, ctx->strain, ctx->g, d.F, ctx->lv[1].dr, ctx->lv[1].ds);
+        TC_TRY(energy_time(ctx, d));
+        ctx->time += ctx->dt;
+        ctx->tstep++;
+    }
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_set_profiling(tc_ctx* ctx, int enable)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (enable) {
+        if (ctx->prof_ev.empty()) {
+            ctx->prof_ev.resize(2 * PROF_PAIRS);
+            for (auto& e : ctx->prof_ev) TC_HIP(hipEventCreate(&e));
+        }
+        for (int k = 0; k < TC_K_COUNT; ++k) { ctx->prof_ms[k] = 0.0; ctx->prof_n[k] = 0; }
+        ctx->prof_used = 0;
+        ctx->prof = true;
+    } else {
+        TC_TRY(prof_flush(ctx));
+        ctx->prof = false;
+    }
+    return 0;
+}
+
+int tc_kernel_timing(tc_ctx* ctx, int which, double* avg_us, int64_t* count)
+{
+    if (!ctx || which < 0 || which >= TC_K_COUNT || !avg_us || !count) return TC_ERR_ARG;
+    TC_TRY(prof_flush(ctx));
+    *count = ctx->prof_n[which];
+    *avg_us = ctx->prof_n[which] ? 1000.0 * ctx->prof_ms[which] / (double)ctx->prof_n[which] : 0.0;
+    return 0;
+}
+
+int tc_guess_enable(tc_ctx* ctx, int enable)
+{
+    if (!ctx) return TC_ERR_ARG;
+    ctx->guess = enable ? 1 : 0;
+    return 0;
+}
+
+/* derivative_eigenvalues (energy.f90:501-598): Fourier-symbol maxima of the
+ * FD operators, with T_test's cmplx() products rounded to single precision
+ * as the reference does.  base[] = Tr, Trr, Trrr, Trrrr, Trs, Trrs, Trrss. */
+int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
+                              double base[7])
+{
+    if (!D1 || !D2 || !D3 || !D4 || !base) return TC_ERR_ARG;
+    const int n_angle = (1 << 11) + 1;
+    const double d_angle = 2.0 * PII / (double)(n_angle - 1);
+    auto ttest = [](int i, int k, double kr, double ks, double& re, double& im) {
+        volatile float a = (float)std::cos(kr * (double)i), b = (float)std::sin(kr * (double)i);
+        volatile float c = (float)std::cos(ks * (double)k), dd = (float)std::sin(ks * (double)k);
+        volatile float ac = a * c, bd = b * dd, ad = a * dd, bc = b * c;
+        volatile float x = ac - bd, y = ad + bc;
+        re = (double)x;
+        im = (double)y;
+    };
+    double m[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int angle = 1; angle <= n_angle; ++angle) {
+        const double kr = (double)(angle - 1) * d_angle;
+        double re, im, tr_im = 0, trr_re = 0, t3_im = 0, t4_re = 0;
+        for (int j = -1; j <= 1; ++j) {
+            ttest(j, 0, kr, 0.0, re, im);
+            tr_im += D1[2 + j] * im;
+            trr_re += D2[2 + j] * re;
+        }
+        for (int j = -2; j <= 2; ++j) {
+            ttest(j, 0, kr, 0.0, re, im);
+            t3_im += D3[2 + j] * im;
+            t4_re += D4[2 + j] * re;
+        }
+        m[0] = std::max(m[0], std::fabs(tr_im));
+        m[1] = std::max(m[1], std::fabs(trr_re));
+        m[2] = std::max(m[2], std::fabs(t3_im));
+        m[3] = std::max(m[3], std::fabs(t4_re));
+        for (int angle2 = 1; angle2 <= n_angle; ++angle2) {
+            const double ks = (double)(angle2 - 1) * d_angle;
+            double trs = 0, trrs = 0, trrss = 0;
+            for (int j = -1; j <= 1; ++j)
+                for (int jj = -1; jj <= 1; ++jj) {
+                    ttest(j, jj, kr, ks, re, im);
+                    trs += (D1[2 + jj] * D1[2 + j]) * re;
+                    trrs += (D1[2 + jj] * D2[2 + j]) * im;
+                    trrss += (D2[2 + jj] * D2[2 + j]) * re;
+                }
+            m[4] = std::max(m[4], std::fabs(trs));
+            m[5] = std::max(m[5], std::fabs(trrs));
+            m[6] = std::max(m[6], std::fabs(trrss));
+        }
+    }
+    for (int t = 0; t < 7; ++t) base[t] = m[t];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,302 @@
+// Coarsest-grid direct solve of the multigrid W-cycle
+// (Compute_Matrix_coarse iterative_solver.f90:1026-1179, DGBEQU/DGBTRF at
+// :1165/:97, Compute_coarse_vector/DGBTRS/Stream_grid_coarse :139-142,
+// :1181-1204).
+//
+// The reference factors the row-equilibrated banded matrix with MKL and
+// back-substitutes 8 times per 6-level W-cycle.  Sequential banded
+// substitution is latency-bound on a GPU, so here the matrix is factored
+// once per solve (LAPACK DGBTF2 semantics, one workgroup), its explicit
+// inverse is formed column-parallel from those factors (DGBTRS applied to
+// each unit vector), and every coarse visit becomes one dense GEMV that
+// streams the inverse from HBM / Infinity Cache.
+#include "tc_kernels.h"
+
+namespace tc {
+
+// Band storage: A(p,q) (1-based) at ab[(kl+ku+p-q) + (q-1)*ldab] (0-based row).
+#define ABx(r, q) ab[(size_t)(r) + (size_t)(q) * (size_t)ldab]
+
+// stream_space with build=1 (iterative_solver.f90:487-560), uniform grid:
+// coefficient of S(iq,kq)*Sval in the row of interior point (i,k).
+__device__ double stream_coeff(const FD& F, const Spacing& h, int ivis, double vis, double vx, double vz, double vxx,
+                               double vzz, double vxz, int i, int k, int iq, int kq, double Sval)
+{
+    const int di = iq - i, dk = kq - k;
+    double Sr = 0, Srr = 0, Srrr = 0, Srrrr = 0, Ss = 0, Sss = 0, Ssss = 0, Sssss = 0;
+    if (k == kq) {
+        Sr = F.D1[di + 2] * Sval / h.dr;
+        Srr = F.D2[di + 2] * Sval / h.dr2;
+        Srrr = F.D3[di + 2] * Sval / h.dr3;
+        Srrrr = F.D4[di + 2] * Sval / h.dr4;
+    }
+    if (i == iq) {
+        Ss = F.D1[dk + 2] * Sval / h.ds;
+        Sss = F.D2[dk + 2] * Sval / h.ds2;
+        Ssss = F.D3[dk + 2] * Sval / h.ds3;
+        Sssss = F.D4[dk + 2] * Sval / h.ds4;
+    }
+    double m9 = 0.0, m9s = 0.0, m9r = 0.0, m9rs = 0.0;
+    if (di >= -1 && di <= 1 && dk >= -1 && dk <= 1) {
+        const int t = (dk + 1) * 3 + (di + 1);
+        m9 = F.Drs[t]; m9s = F.Drss[t]; m9r = F.Drrs[t]; m9rs = F.Drrss[t];
+    }
+    double Srs = m9 * Sval / h.drds;
+    double Srss = m9s * Sval / h.drds2;
+    double Srrs = m9r * Sval / h.dr2ds;
+    double Srrss = m9rs * Sval / h.dr2ds2;
+    (void)Sr; (void)Ss;
+    double t1, t2, t3, t4, t5, t6;
+    if (ivis == 0) {
+        t1 = 0.0; t2 = vis * (Srrrr + Sssss); t3 = 2.0 * vis * Srrss; t4 = 0.0; t5 = 0.0; t6 = 0.0;
+    } else {
+        t1 = (vxx - vzz) * (Srr - Sss);
+        t2 = vis * (Srrrr + Sssss);
+        t3 = 2.0 * vis * Srrss;
+        t4 = 4.0 * vxz * Srs;
+        t5 = 2.0 * vx * (Srrr + Srss);
+        t6 = 2.0 * vz * (Ssss + Srrs);
+    }
+    return t1 + t2 + t3 + t4 + t5 + t6;
+}
+
+// One thread per matrix row p (Database=2 ordering p = (nz_c+1)*ip + kp + 1):
+// set the in-domain stencil entries, add the folded ghost entries, identity
+// rows on the walls, then DGBEQU's row factor R(p) = 1/max|A(p,:)| and the
+// row scaling.  The band must be zero on entry.
+__global__ __launch_bounds__(256) void k_coarse_assemble(double* __restrict__ ab, double* __restrict__ RowB,
+                                                         const double* __restrict__ V, const double* __restrict__ vx,
+                                                         const double* __restrict__ vz, const double* __restrict__ vxx,
+                                                         const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                                         Grid g, FD F, Spacing h, int ivis, int Vbc, int kl, int ku, int n)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (p > n) return;
+    const int ldab = 2 * kl + ku + 1, kl_ku_1 = kl + ku + 1;
+    const int nxc = g.nx, nzc = g.nz, N = nzc + 1;
+    const int kp = p - N * ((p - 1) / N) - 1, ip = (p - 1) / N;
+    const int limit[5] = {0, 1, 2, 1, 0};
+    if (ip > 0 && ip < nxc && kp > 0 && kp < nzc) {
+        const size_t c = g.at(ip, kp);
+        const double vis = V[c], gx = vx[c], gz = vz[c], gxx = vxx[c], gzz = vzz[c], gxz = vxz[c];
+        for (int j = -2; j <= 2; ++j) {
+            int iq = j + ip;
+            if (iq >= 0 && iq <= nxc)
+                for (int kq = -limit[j + 2] + kp; kq <= kp + limit[j + 2]; ++kq)
+                    if (kq >= 0 && kq <= nzc) {
+                        int q = N * iq + kq + 1;
+                        ABx(kl_ku_1 + p - q - 1, q - 1) = stream_coeff(F, h, ivis, vis, gx, gz, gxx, gzz, gxz, ip, kp, iq, kq, 1.0);
+                    }
+        }
+        for (int j = -2; j <= 2; ++j) {
+            int iq = j + ip;
+            for (int kq = -limit[j + 2] + kp; kq <= kp + limit[j + 2]; ++kq) {
+                if (iq < 0 || iq > nxc || kq < 0 || kq > nzc) {
+                    int iiq, kkq;
+                    double xval, zval;
+                    if (iq < 0) { iiq = -iq; xval = -1.0; }
+                    else if (iq > nxc) { iiq = 2 * nxc - iq; xval = -1.0; }
+                    else { iiq = iq; xval = 1.0; }
+                    if (kq < 0) { kkq = -kq; zval = (Vbc == 0) ? -1.0 : 1.0; }
+                    else if (kq > nzc) { kkq = 2 * nzc - kq; zval = (Vbc == 0) ? -1.0 : 1.0; }
+                    else { kkq = kq; zval = 1.0; }
+                    double Sval = xval * zval;
+                    int q = N * iiq + kkq + 1;
+                    double& a = ABx(kl_ku_1 + p - q - 1, q - 1);
+                    a = a + stream_coeff(F, h, ivis, vis, gx, gz, gxx, gzz, gxz, ip, kp, iq, kq, Sval);
+                }
+            }
+        }
+    } else {
+        ABx(kl_ku_1 - 1, p - 1) = 1.0;
+    }
+    // DGBEQU row factor over the band of row p, then the row scaling.
+    const double smlnum = 2.2250738585072014e-308, bignum = 1.0 / smlnum;
+    double rmax = 0.0;
+    int q0 = p - kl > 1 ? p - kl : 1, q1 = p + ku < n ? p + ku : n;
+    for (int q = q0; q <= q1; ++q) {
+        double v = fabs(ABx(kl_ku_1 + p - q - 1, q - 1));
+        if (v > rmax) rmax = v;
+    }
+    double rv = rmax > smlnum ? rmax : smlnum;
+    rv = rv < bignum ? rv : bignum;
+    double R = 1.0 / rv;
+    RowB[p - 1] = R;
+    // rows p with p-q in [-kl, ku] are scaled (iterative_solver.f90:1168-1172)
+    int s0 = p - ku > 1 ? p - ku : 1, s1 = p + kl < n ? p + kl : n;
+    for (int q = s0; q <= s1; ++q) {
+        double& a = ABx(kl_ku_1 + p - q - 1, q - 1);
+        a = R * a;
+    }
+}
+
+// DGBTF2 (LAPACK, unblocked banded LU with partial pivoting) by one
+// workgroup; same operations per element as the reference algorithm.
+__global__ __launch_bounds__(256) void k_dgbtf2(double* __restrict__ ab, int ldab, int n, int kl, int ku,
+                                                int32_t* __restrict__ ipiv, int32_t* __restrict__ info_out)
+{
+    const int kv = ku + kl;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ int sh_jp;
+    __shared__ int sh_ju;
+    __shared__ double sh_piv, sh_old;
+    __shared__ int sh_info;
+    for (int j = ku + 1; j < (kv < n ? kv : n); ++j)
+        for (int i = kv - j + tid; i < kl; i += nt) ABx(i, j) = 0.0;
+    if (tid == 0) { sh_ju = 0; sh_info = 0; }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (j + kv < n)
+            for (int i = tid; i < kl; i += nt) ABx(i, j + kv) = 0.0;
+        const int km = kl < n - 1 - j ? kl : n - 1 - j;
+        // IDAMAX over rows kv..kv+km of column j (first maximum wins), wave 0
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = 1 << 30;
+            for (int t = tid; t <= km; t += 64) {
+                double v = fabs(ABx(kv + t, j));
+                if (v > best) { best = v; bi = t; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                double ob = __shfl_down(best, off, 64);
+                int oi = __shfl_down(bi, off, 64);
+                if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+            }
+            if (tid == 0) {
+                sh_jp = bi;
+                sh_piv = ABx(kv + bi, j);
+                sh_old = ABx(kv, j);
+            }
+        }
+        __syncthreads();
+        const int jp = sh_jp;
+        const double piv = sh_piv;
+        if (tid == 0) ipiv[j] = jp + j + 1;
+        if (piv != 0.0) {
+            int jt = j + ku + jp;
+            if (jt > n - 1) jt = n - 1;
+            const int ju = sh_ju > jt ? sh_ju : jt;
+            // column j: swap + DSCAL by the reciprocal pivot; columns j+1..ju: swap
+            const double rp = 1.0 / piv;
+            if (tid == 0) ABx(kv, j) = piv;
+            for (int t = 1 + tid; t <= km; t += nt) {
+                double v = (t == jp) ? sh_old : ABx(kv + t, j);
+                ABx(kv + t, j) = v * rp;
+            }
+            if (jp != 0)
+                for (int t = 1 + tid; t <= ju - j; t += nt) {
+                    double a = ABx(kv + jp - t, j + t);
+                    ABx(kv + jp - t, j + t) = ABx(kv - t, j + t);
+                    ABx(kv - t, j + t) = a;
+                }
+            __syncthreads();
+            if (tid == 0) sh_ju = ju;
+            // DGER rank-1 update of the trailing band
+            if (km > 0 && ju > j) {
+                const int ncol = ju - j;
+                const int tot = ncol * km;
+                for (int e = tid; e < tot; e += nt) {
+                    int jj = e / km + 1, t = e % km + 1;
+                    double y = ABx(kv - jj, j + jj);
+                    if (y != 0.0) {
+                        double temp = -1.0 * y;
+                        ABx(kv + t - jj, j + jj) += ABx(kv + t, j) * temp;
+                    }
+                }
+            }
+        } else if (tid == 0 && sh_info == 0) {
+            sh_info = j + 1;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *info_out = sh_info;
+}
+
+// Explicit inverse from the DGBTF2 factors: thread c applies DGBTRS('N')
+// to the unit vector e_c, writing column c of A^-1 into the row-major Ainv
+// (lanes of a wavefront touch consecutive columns: coalesced).
+__global__ __launch_bounds__(64) void k_band_inverse(const double* __restrict__ ab, int ldab, int n, int kl, int ku,
+                                                     const int32_t* __restrict__ ipiv, double* __restrict__ X)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const size_t nn = (size_t)n;
+    for (int i = 0; i < n; ++i) X[(size_t)i * nn + c] = (i == c) ? 1.0 : 0.0;
+    const int kd = ku + kl;
+    for (int j = 0; j < n - 1; ++j) {
+        const int lm = kl < n - 1 - j ? kl : n - 1 - j;
+        const int l = ipiv[j] - 1;
+        if (l != j) {
+            double a = X[(size_t)l * nn + c];
+            X[(size_t)l * nn + c] = X[(size_t)j * nn + c];
+            X[(size_t)j * nn + c] = a;
+        }
+        double bj = X[(size_t)j * nn + c];
+        if (bj != 0.0) {
+            double temp = -1.0 * bj;
+            for (int t = 1; t <= lm; ++t) X[(size_t)(j + t) * nn + c] += ABx(kd + t, j) * temp;
+        }
+    }
+    const int k = kl + ku;
+    for (int j = n - 1; j >= 0; --j) {
+        double bj = X[(size_t)j * nn + c];
+        if (bj != 0.0) {
+            bj = bj / ABx(k, j);
+            X[(size_t)j * nn + c] = bj;
+            const int i0 = j - k > 0 ? j - k : 0;
+            for (int i = j - 1; i >= i0; --i) X[(size_t)i * nn + c] -= bj * ABx(k + i - j, j);
+        }
+    }
+}
+
+// Coarse solve: B(p) = RHS(grids,i,k)*R(p) on interior rows (0 on walls),
+// x = A^-1 B, error(grids) <- x on the interior with the ghost mirrors of
+// enforceBCs_stream (walls stay 0).  One workgroup per interior row.
+__global__ __launch_bounds__(256) void k_coarse_apply(const double* __restrict__ Ainv, const double* __restrict__ RowB,
+                                                      const double* __restrict__ rhs, double* __restrict__ E, Grid g,
+                                                      int n, double zsg)
+{
+    const int nxc = g.nx, nzc = g.nz, N = nzc + 1;
+    const int ip = blockIdx.x + 1, kp = blockIdx.y + 1;  // interior unknown
+    const int prow = N * ip + kp;                         // 0-based row of A^-1
+    const double* row = Ainv + (size_t)prow * (size_t)n;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int kq = q % N, iq = q / N;
+        if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) acc += row[q] * (rhs[g.at(iq, kq)] * RowB[q]);
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) {
+        E[g.at(ip, kp)] = acc;
+        stream_ghosts(E, g, ip, kp, acc, zsg);
+    }
+}
+
+#undef ABx
+
+void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Level& L, const FD& F, const Spacing& h,
+                            int ivis, int Vbc, int kl, int ku, int ngc)
+{
+    hipLaunchKernelGGL(k_coarse_assemble, dim3((ngc + 255) / 256), dim3(256), 0, st, Mc, RowB, L.vis, L.visx, L.visz,
+                       L.visxx, L.viszz, L.visxz, L.g, F, h, ivis, Vbc, kl, ku, ngc);
+}
+
+void launch_dgbtf2(hipStream_t st, double* ab, int ldab, int n, int kl, int ku, int32_t* ipiv, int32_t* info)
+{
+    hipLaunchKernelGGL(k_dgbtf2, dim3(1), dim3(256), 0, st, ab, ldab, n, kl, ku, ipiv, info);
+}
+
+void launch_band_inverse(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
+                         double* Ainv)
+{
+    hipLaunchKernelGGL(k_band_inverse, dim3((n + 63) / 64), dim3(64), 0, st, ab, ldab, n, kl, ku, ipiv, Ainv);
+}
+
+void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,
+                         double zsg)
+{
+    hipLaunchKernelGGL(k_coarse_apply, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, RowB, L.rhs, E, L.g, ngc,
+                       zsg);
+}
+
+}  // namespace tc

@@ -1,0 +1,556 @@
+// Field and tracer kernels: compute_velocities/velocityBCs, the strain-rate
+// invariant, the T and C filters (smoother_time_T, smoother_time), the
+// tracer-ratio binning (tracers_to_corners), load_conductivity_array,
+// compute_derivatives, the RK tracer stages and the T-field RK stages of
+// energy_time.  Uniform grids (xr=zs=1, xrr=zss=0).
+#include "tc_kernels.h"
+
+namespace tc {
+
+static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+// compute_velocities (momentum.f90:538-549): u = dSF/ds, w = -dSF/dr on 0..nx x 0..nz.
+__global__ __launch_bounds__(256) void k_velocities(const double* __restrict__ SF, double* __restrict__ u,
+                                                    double* __restrict__ w, Grid g, FD F, double dr, double ds)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.nx || k > g.nz) return;
+    const size_t c = g.at(i, k);
+    double SFr = dot3(F.D1, SF, c, 1) / dr;
+    double SFs = dot3(F.D1, SF, c, g.ld) / ds;
+    u[c] = SFs;
+    w[c] = -SFr;
+}
+
+// velocityBCs (composition.f90:806-836), reference order, one workgroup.
+__global__ void k_velocity_bcs(double* __restrict__ u, double* __restrict__ w, Grid g, int Vbc)
+{
+    const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
+    for (int k = tid; k <= nz; k += nt) { w[g.at(-1, k)] = w[g.at(1, k)]; w[g.at(nx + 1, k)] = w[g.at(nx - 1, k)]; }
+    for (int i = tid; i <= nx; i += nt) {
+        if (Vbc == 0) { u[g.at(i, -1)] = u[g.at(i, 1)]; u[g.at(i, nz + 1)] = u[g.at(i, nz - 1)]; }
+        else { u[g.at(i, -1)] = -u[g.at(i, 1)]; u[g.at(i, nz + 1)] = -u[g.at(i, nz - 1)]; }
+    }
+    __syncthreads();
+    for (int i = -1 + tid; i <= nx + 1; i += nt) { w[g.at(i, -1)] = -w[g.at(i, 1)]; w[g.at(i, nz + 1)] = -w[g.at(i, nz - 1)]; }
+    for (int k = -1 + tid; k <= nz + 1; k += nt) { u[g.at(-1, k)] = -u[g.at(1, k)]; u[g.at(nx + 1, k)] = -u[g.at(nx - 1, k)]; }
+    __syncthreads();
+    for (int k = -1 + tid; k <= nz + 1; k += nt) { u[g.at(0, k)] = 0.0; u[g.at(nx, k)] = 0.0; }
+    for (int i = -1 + tid; i <= nx + 1; i += nt) { w[g.at(i, 0)] = 0.0; w[g.at(i, nz)] = 0.0; }
+    if (Vbc == 1) {
+        __syncthreads();
+        for (int i = -1 + tid; i <= nx + 1; i += nt) { u[g.at(i, 0)] = 0.0; u[g.at(i, nz)] = 0.0; }
+    }
+}
+
+// compute_strain_rate_invariant (momentum.f90:39-52).
+__global__ __launch_bounds__(256) void k_strain(const double* __restrict__ u, const double* __restrict__ w,
+                                                double* __restrict__ st, Grid g, FD F, double dr, double ds)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.nx || k > g.nz) return;
+    const size_t c = g.at(i, k);
+    double ur = dot3(F.D1, u, c, 1) / dr;
+    double us = dot3(F.D1, u, c, g.ld) / ds;
+    double wr = dot3(F.D1, w, c, 1) / dr;
+    double ws = dot3(F.D1, w, c, g.ld) / ds;
+    double a = us + wr;
+    st[c] = sqrt(0.5 * (2.0 * (ur * ur + ws * ws) + a * a));
+}
+
+// Symmetric ghosts of a node field: sides for k=0..nz, then full rows
+// (strain momentum.f90:54-61, conductivity energy.f90:42-49).
+__global__ void k_sym_bcs(double* __restrict__ f, Grid g)
+{
+    const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
+    for (int k = tid; k <= nz; k += nt)
+        for (int i = 1; i <= SPAN; ++i) { f[g.at(-i, k)] = f[g.at(i, k)]; f[g.at(nx + i, k)] = f[g.at(nx - i, k)]; }
+    __syncthreads();
+    for (int i = -SPAN + tid; i <= nx + SPAN; i += nt)
+        for (int k = 1; k <= SPAN; ++k) { f[g.at(i, -k)] = f[g.at(i, k)]; f[g.at(i, nz + k)] = f[g.at(i, nz - k)]; }
+}
+
+// smoothness (smoother.f90:75-95): per-block maxima of the largest jump to
+// the five forward neighbours, i=0..nx-1, k=1..nz-1.
+__global__ __launch_bounds__(256) void k_smoothness(const double* __restrict__ T, Grid g, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        const double t0 = T[c];
+        double a1 = fabs(T[g.at(i, k - 1)] - t0);
+        double a2 = fabs(T[g.at(i + 1, k - 1)] - t0);
+        double a3 = fabs(T[g.at(i + 1, k)] - t0);
+        double a4 = fabs(T[g.at(i, k + 1)] - t0);
+        double a5 = fabs(T[g.at(i + 1, k + 1)] - t0);
+        a = a1;
+        if (a2 > a) a = a2;
+        if (a3 > a) a = a3;
+        if (a4 > a) a = a4;
+        if (a5 > a) a = a5;
+    }
+    a = block_max(a);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+}
+
+// One smoother_time_T sweep (smoother.f90:363-371) on 0..nx x 0..nz.
+__global__ __launch_bounds__(256) void k_smooth_T(const double* __restrict__ T0, double* __restrict__ T,
+                                                  const double* __restrict__ dts, Grid g, FD F, double dr2, double ds2,
+                                                  double courantT)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.nx || k > g.nz) return;
+    const size_t c = g.at(i, k);
+    double Trr = dot3(F.D2, T0, c, 1) / dr2;
+    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
+    double temp = Trr + Tss;
+    double d = dts[c] * courantT;
+    T[c] = T0[c] + d * temp;
+}
+
+// One smoother_time sweep (smoother.f90:319-329) with the clamp to [0,1];
+// cfac = (Chebyshev factor c, courantC) packed as c*dts*temp.
+__global__ __launch_bounds__(256) void k_smooth_C(const double* __restrict__ T0, double* __restrict__ C,
+                                                  const double* __restrict__ dts, Grid g, FD F, double dr2, double ds2,
+                                                  double cfac, double courantC)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.nx || k > g.nz) return;
+    const size_t c = g.at(i, k);
+    double Trr = dot3(F.D2, T0, c, 1) / dr2;
+    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
+    double temp = Trr + Tss;
+    double d = dts[c] * courantC;
+    double v = T0[c] + cfac * d * temp;
+    if (v < 0.0) v = 0.0;
+    if (v > 1.0) v = 1.0;
+    C[c] = v;
+}
+
+// enforceBCs (energy.f90:455-475), one workgroup.
+__global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot)
+{
+    const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
+    for (int k = tid; k <= nz; k += nt)
+        for (int i = 1; i <= SPAN; ++i) { T[g.at(-i, k)] = T[g.at(i, k)]; T[g.at(nx + i, k)] = T[g.at(nx - i, k)]; }
+    __syncthreads();
+    for (int i = -SPAN + tid; i <= nx + SPAN; i += nt)
+        for (int k = 1; k <= SPAN; ++k) {
+            if (Tbot == 0) T[g.at(i, -k)] = -(T[g.at(i, k)] - 1.0) + 1.0;
+            else T[g.at(i, -k)] = T[g.at(i, k)];
+            T[g.at(i, nz + k)] = -T[g.at(i, nz - k)];
+        }
+    __syncthreads();
+    for (int i = -SPAN + tid; i <= nx + SPAN; i += nt) {
+        if (Tbot == 0) T[g.at(i, 0)] = 1.0;
+        T[g.at(i, nz)] = 0.0;
+    }
+}
+
+// enforceBCs_comp (composition.f90:792-803) in the reference's interleaved
+// order, one workgroup.
+__global__ void k_bcs_comp(double* __restrict__ C, Grid g)
+{
+    const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
+    for (int i = 1; i <= SPAN; ++i) {
+        for (int k = tid; k <= nz; k += nt) { C[g.at(-i, k)] = C[g.at(i, k)]; C[g.at(nx + i, k)] = C[g.at(nx - i, k)]; }
+        __syncthreads();
+        for (int ii = -SPAN + tid; ii <= nx + SPAN; ii += nt) { C[g.at(ii, -i)] = C[g.at(ii, i)]; C[g.at(ii, nz + i)] = C[g.at(ii, nz - i)]; }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ p, size_t n)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0;
+}
+
+// tracers_to_corners counting loop (composition.f90:704-715): nearest node
+// by Fortran NINT (half away from zero) of r/dr and s/ds, domain check.
+__global__ __launch_bounds__(256) void k_bin(const double* __restrict__ r, const double* __restrict__ s,
+                                             const int32_t* __restrict__ ty, int64_t n, CntArr cnt, Grid g, double dr,
+                                             double ds, double aspect, int32_t* __restrict__ flag)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const double rv = r[id], sv = s[id];
+    // the reference's test (r<=0 or r>=aspect or s<=0 or s>=1), written so
+    // that a NaN position is also rejected before it can form an index
+    if (!(rv > 0.0 && rv < aspect && sv > 0.0 && sv < 1.0)) {
+        atomicOr(flag, 1);
+        return;
+    }
+    const int i = (int)round(rv / dr);
+    const int k = (int)round(sv / ds);
+    atomicAdd(&cnt.v[ty[id]][g.at(i, k)], 1);
+}
+
+// Mirrored ghost counts (composition.f90:745-754), nempty (:756) and the
+// tracer ratio (:757-776) in one pass over the ghosted grid.
+__global__ __launch_bounds__(256) void k_ratio(CntArr cnt, int32_t* __restrict__ nempty, PtrArr Cnew, Grid g, int ntype,
+                                               int comp)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
+    const int nx = g.nx, nz = g.nz;
+    if (i > nx + SPAN || k > nz + SPAN) return;
+    const bool gi = i < 0 || i > nx, gk = k < 0 || k > nz;
+    const size_t c = g.at(i, k);
+    int tot = 0;
+    if (gi && gk) {
+        for (int t = 0; t <= ntype; ++t) cnt.v[t][c] = 0;
+    } else if (gi || gk) {
+        const int mi = i < 0 ? -i : (i > nx ? 2 * nx - i : i);
+        const int mk = k < 0 ? -k : (k > nz ? 2 * nz - k : k);
+        const size_t m = g.at(mi, mk);
+        for (int t = 0; t <= ntype; ++t) {
+            int v = cnt.v[t][m];
+            cnt.v[t][c] = v;
+            tot += v;
+        }
+    } else {
+        int ct[TC_MAX_TYPES + 1];
+        for (int t = 0; t <= ntype; ++t) { ct[t] = cnt.v[t][c]; tot += ct[t]; }
+        if (tot > 0 && comp == 1)
+            for (int t = 1; t <= ntype; ++t) {
+                double v;
+                if (ct[t] == tot) v = 1.0;
+                else if (ct[t] == 0) v = 0.0;
+                else v = (double)ct[t] / (double)tot;
+                Cnew.v[t - 1][c] = v;
+            }
+    }
+    nempty[c] = tot;
+}
+
+// conductivity + load_conductivity_array (energy.f90:16-58): the ghost
+// values of conduct are the mirrored interior ones, so every point of the
+// ghosted grid is computed from Cvis at its mirror point.
+__global__ __launch_bounds__(256) void k_conduct(PtrArr Cvis, double* __restrict__ K, Grid g, int comp, int ntype,
+                                                 RaCArr lcf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
+    const int nx = g.nx, nz = g.nz;
+    if (i > nx + SPAN || k > nz + SPAN) return;
+    const int mi = i < 0 ? -i : (i > nx ? 2 * nx - i : i);
+    const int mk = k < 0 ? -k : (k > nz ? 2 * nz - k : k);
+    const size_t m = g.at(mi, mk);
+    double v = 1.0;
+    if (comp != 0)
+        for (int j = 0; j < ntype; ++j) v = exp(-Cvis.v[j][m] * lcf.v[j]);
+    K[g.at(i, k)] = v;
+}
+
+__global__ __launch_bounds__(256) void k_conduct_grad(const double* __restrict__ K, double* __restrict__ Kr,
+                                                      double* __restrict__ Ks, Grid g, FD F, double dr, double ds)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    if (i > g.nx + 1 || k > g.nz + 1) return;
+    const size_t c = g.at(i, k);
+    Kr[c] = dot3(F.D1, K, c, 1) / dr;
+    Ks[c] = dot3(F.D1, K, c, g.ld) / ds;
+}
+
+// compute_derivatives (composition.f90:900-919) of u and w, k=-1..nz+1.
+__global__ __launch_bounds__(256) void k_derivs(const double* __restrict__ u, const double* __restrict__ w,
+                                                double* __restrict__ Du, double* __restrict__ Dw, Grid g, D3Arr Dint,
+                                                double dr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - 1;
+    if (i > g.nx || k > g.nz + 1) return;
+    const size_t c = g.at(i, k);
+    double a = Dint.v[0] * u[c - 1];
+    a += Dint.v[1] * u[c];
+    a += Dint.v[2] * u[c + 1];
+    Du[c] = a / dr;
+    double b = Dint.v[0] * w[c - 1];
+    b += Dint.v[1] * w[c];
+    b += Dint.v[2] * w[c + 1];
+    Dw[c] = b / dr;
+}
+
+// interpolate (composition.f90:838-898), order=2.
+__device__ __forceinline__ void interp_uw(double r, double s, const double* __restrict__ u, const double* __restrict__ w,
+                                          const double* __restrict__ Du, const double* __restrict__ Dw, const Grid& g,
+                                          const D3Arr& D, double dr, double ds, double& ou, double& ow)
+{
+    int i = (int)(r / dr);
+    int k = (int)(s / ds);
+    // in-domain positions give 0<=i<=nx-1, 0<=k<=nz-1; the clamp only keeps
+    // the gathers inside the ghost ring for invalid input
+    i = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
+    k = k < 0 ? 0 : (k > g.nz - 1 ? g.nz - 1 : k);
+    double hr = r - (double)i * dr;
+    double fu[4], fw[4];
+    int ib;
+    double a1;
+    if (hr <= 0.5 * dr) { ib = i; a1 = hr; }
+    else { hr = dr - hr; ib = i + 1; a1 = -hr; }
+#pragma unroll
+    for (int jj = -1; jj <= 2; ++jj) {
+        const size_t c = g.at(ib, k + jj);
+        fu[jj + 1] = u[c] + Du[c] * a1;
+        fw[jj + 1] = w[c] + Dw[c] * a1;
+    }
+    double hs = s - (double)k * ds;
+    if (hs <= 0.5 * ds) {
+        double au = D.v[0] * fu[0]; au += D.v[1] * fu[1]; au += D.v[2] * fu[2];
+        double aw = D.v[0] * fw[0]; aw += D.v[1] * fw[1]; aw += D.v[2] * fw[2];
+        ou = fu[1] + (au / ds) * hs;
+        ow = fw[1] + (aw / ds) * hs;
+    } else {
+        hs = ds - hs;
+        double au = D.v[0] * fu[1]; au += D.v[1] * fu[2]; au += D.v[2] * fu[3];
+        double aw = D.v[0] * fw[1]; aw += D.v[1] * fw[2]; aw += D.v[2] * fw[3];
+        ou = fu[2] + (au / ds) * (-hs);
+        ow = fw[2] + (aw / ds) * (-hs);
+    }
+}
+
+// RK stage `stage` of the tracer loops (energy.f90:182-225, 270-312,
+// 355-378, 420-440), tracer=0, uniform grids, global dt.
+__global__ __launch_bounds__(256) void k_tracer_stage(int stage, int torder, double dt, int64_t n, double* __restrict__ r,
+                                                      double* __restrict__ s, double* __restrict__ r0,
+                                                      double* __restrict__ s0, double* __restrict__ r1,
+                                                      double* __restrict__ s1, double* __restrict__ r2,
+                                                      double* __restrict__ s2, const double* __restrict__ u,
+                                                      const double* __restrict__ w, const double* __restrict__ Du,
+                                                      const double* __restrict__ Dw, Grid g, D3Arr D, double dr, double ds)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    double tu, tw;
+    if (stage == 1) {
+        const double x0 = r[id], z0 = s[id];  // rtr0 = rtr (energy.f90:147-148)
+        r0[id] = x0; s0[id] = z0;
+        interp_uw(x0, z0, u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        const double dtstage = (torder == 2) ? dt : 0.5 * dt;
+        r[id] = tu; s[id] = tw;
+        r1[id] = x0 + dtstage * tu;
+        s1[id] = z0 + dtstage * tw;
+    } else if (stage == 2) {
+        interp_uw(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        const double dtstage = 0.5 * dt;
+        if (torder == 2) {
+            double a = r[id] + tu, b = s[id] + tw;
+            r[id] = r0[id] + dtstage * a;
+            s[id] = s0[id] + dtstage * b;
+        } else {
+            r[id] = r[id] + 2.0 * tu;
+            s[id] = s[id] + 2.0 * tw;
+            r2[id] = r0[id] + dtstage * tu;
+            s2[id] = s0[id] + dtstage * tw;
+        }
+    } else if (stage == 3) {
+        interp_uw(r2[id], s2[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        r[id] = r[id] + 2.0 * tu;
+        r1[id] = r0[id] + dt * tu;
+        s[id] = s[id] + 2.0 * tw;
+        s1[id] = s0[id] + dt * tw;
+    } else {
+        interp_uw(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        const double dto6 = dt / 6.0;
+        r[id] = r0[id] + dto6 * (r[id] + tu);
+        s[id] = s0[id] + dto6 * (s[id] + tw);
+    }
+}
+
+// energy_space (energy.f90:60-80) with heating (energy.f90:1-14).
+__device__ __forceinline__ double energy_space(const double* __restrict__ T0, size_t c, ptrdiff_t ld, const FD& F,
+                                               double dr, double ds, double dr2, double ds2, double K, double Kr,
+                                               double Ks, double uu, double ww, double heat)
+{
+    double Tr = dot3(F.D1, T0, c, 1) / dr;
+    double Ts = dot3(F.D1, T0, c, ld) / ds;
+    double Trr = dot3(F.D2, T0, c, 1) / dr2;
+    double Tss = dot3(F.D2, T0, c, ld) / ds2;
+    return K * (Trr + Tss) + (Kr * Tr + Ks * Ts) - uu * Tr - ww * Ts + heat;
+}
+
+__global__ __launch_bounds__(256) void k_energy_stage(int stage, double dt, double* __restrict__ T,
+                                                      const double* __restrict__ T0, double* __restrict__ T1,
+                                                      double* __restrict__ T2, const double* __restrict__ u,
+                                                      const double* __restrict__ w, const double* __restrict__ K,
+                                                      const double* __restrict__ Kr, const double* __restrict__ Ks,
+                                                      PtrArr Cvis, Grid g, FD F, double dr, double ds, double dr2,
+                                                      double ds2, int comp, int ntype, double H, RaCArr Htype)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > g.nx || k > g.nz) return;
+    const size_t c = g.at(i, k);
+    double heat;
+    if (comp == 0) heat = H;
+    else {
+        double sum = 0.0;
+        for (int j = 0; j < ntype; ++j) sum += Cvis.v[j][c];
+        heat = H * (1.0 - sum);
+        for (int j = 0; j < ntype; ++j) heat = Htype.v[j] * Cvis.v[j][c] + heat;
+    }
+    const double* src = (stage == 1) ? T0 : (stage == 3 ? T2 : T1);
+    double temp = energy_space(src, c, g.ld, F, dr, ds, dr2, ds2, K[c], Kr[c], Ks[c], u[c], w[c], heat);
+    if (stage == 1) { T[c] = temp; T1[c] = T0[c] + 0.5 * dt * temp; }
+    else if (stage == 2) { T[c] = T[c] + 2.0 * temp; T2[c] = T0[c] + 0.5 * dt * temp; }
+    else if (stage == 3) { T[c] = T[c] + 2.0 * temp; T1[c] = T0[c] + dt * temp; }
+    else { T[c] = T0[c] + (dt / 6.0) * (T[c] + temp); }
+}
+
+// stability_RK4 maxima of eigA / eigD (energy.f90:614-623, 686-706), tracer=0.
+__global__ __launch_bounds__(256) void k_stability(const double* __restrict__ u, const double* __restrict__ w,
+                                                   const double* __restrict__ K, const double* __restrict__ Kr,
+                                                   const double* __restrict__ Ks, Grid g, double Tr_max, double Ts_max,
+                                                   double Trr_max, double Tss_max, double* __restrict__ pA,
+                                                   double* __restrict__ pD)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    double eA = 0.0, eD = 0.0;
+    if (i <= g.nx && k <= g.nz) {
+        const size_t c = g.at(i, k);
+        const double Kc = K[c];
+        eA = fabs(-(Kc * 0.0 / 1.0) - u[c] / 1.0 + Kr[c] / 1.0) * Tr_max +
+             fabs(-(Kc * 0.0 / 1.0) - w[c] / 1.0 + Ks[c] / 1.0) * Ts_max;
+        eD = Kc * (fabs(Trr_max / 1.0) + fabs(Tss_max / 1.0));
+    }
+    eA = block_max(eA);
+    eD = block_max(eD);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        pA[blockIdx.y * gridDim.x + blockIdx.x] = eA;
+        pD[blockIdx.y * gridDim.x + blockIdx.x] = eD;
+    }
+}
+
+// ---------------------------------------------------------------------------
+void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,
+                       double ds, int Vbc)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_velocities, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, SF, u, w, g, F, dr, ds);
+    hipLaunchKernelGGL(k_velocity_bcs, dim3(1), dim3(256), 0, st, u, w, g, Vbc);
+}
+
+void launch_strain(hipStream_t st, const double* u, const double* w, double* strain, const Grid& g, const FD& F,
+                   double dr, double ds)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_strain, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, u, w, strain, g, F, dr, ds);
+    hipLaunchKernelGGL(k_sym_bcs, dim3(1), dim3(256), 0, st, strain, g);
+}
+
+int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx, g.nz - 1, b);
+    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_smooth_T(hipStream_t st, const double* T0, double* T, const double* dts, const Grid& g, const FD& F,
+                     double dr2, double ds2, double courantT)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_smooth_T, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, T0, T, dts, g, F, dr2, ds2, courantT);
+}
+
+void launch_smooth_C(hipStream_t st, const double* T0, double* C, const double* dts, const Grid& g, const FD& F,
+                     double dr2, double ds2, double cfac, double courantC)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_smooth_C, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, T0, C, dts, g, F, dr2, ds2, cfac, courantC);
+}
+
+void launch_sym_bcs(hipStream_t st, double* f, const Grid& g)
+{
+    hipLaunchKernelGGL(k_sym_bcs, dim3(1), dim3(256), 0, st, f, g);
+}
+
+void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot)
+{
+    hipLaunchKernelGGL(k_bcs_T, dim3(1), dim3(256), 0, st, T, g, Tbot);
+}
+
+void launch_bcs_comp(hipStream_t st, double* C, const Grid& g)
+{
+    hipLaunchKernelGGL(k_bcs_comp, dim3(1), dim3(256), 0, st, C, g);
+}
+
+void launch_zero_i32(hipStream_t st, int32_t* p, size_t n)
+{
+    unsigned nb = (unsigned)((n + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_zero_i32, dim3(nb), dim3(256), 0, st, p, n);
+}
+
+void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
+                const Grid& g, double dr, double ds, double aspect, int32_t* flag)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, s, ty, n, cnt, g, dr, ds, aspect,
+                       flag);
+}
+
+void launch_ratio(hipStream_t st, const CntArr& cnt, int32_t* nempty, const PtrArr& Cnew, const Grid& g, int ntype,
+                  int comp)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_ratio, grid2(g.nx + 2 * SPAN + 1, g.nz + 2 * SPAN + 1, b), b, 0, st, cnt, nempty, Cnew, g, ntype,
+                       comp);
+}
+
+void launch_conduct(hipStream_t st, const PtrArr& Cvis, double* K, double* Kr, double* Ks, const Grid& g, const FD& F,
+                    double dr, double ds, int comp, int ntype, const RaCArr& lcf)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_conduct, grid2(g.nx + 2 * SPAN + 1, g.nz + 2 * SPAN + 1, b), b, 0, st, Cvis, K, g, comp, ntype,
+                       lcf);
+    hipLaunchKernelGGL(k_conduct_grad, grid2(g.nx + 3, g.nz + 3, b), b, 0, st, K, Kr, Ks, g, F, dr, ds);
+}
+
+void launch_derivs(hipStream_t st, const double* u, const double* w, double* Du, double* Dw, const Grid& g,
+                   const D3Arr& Dint, double dr)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_derivs, grid2(g.nx + 1, g.nz + 3, b), b, 0, st, u, w, Du, Dw, g, Dint, dr);
+}
+
+void launch_tracer_stage(hipStream_t st, int stage, int torder, double dt, int64_t n, double* r, double* s, double* r0,
+                         double* s0, double* r1, double* s1, double* r2, double* s2, const double* u, const double* w,
+                         const double* Du, const double* Dw, const Grid& g, const D3Arr& Dint, double dr, double ds)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tracer_stage, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, torder, dt, n, r, s,
+                       r0, s0, r1, s1, r2, s2, u, w, Du, Dw, g, Dint, dr, ds);
+}
+
+void launch_energy_stage(hipStream_t st, int stage, double dt, double* T, const double* T0, double* T1, double* T2,
+                         const double* u, const double* w, const double* K, const double* Kr, const double* Ks,
+                         const PtrArr& Cvis, const Grid& g, const FD& F, double dr, double ds, double dr2, double ds2,
+                         int comp, int ntype, double H, const RaCArr& Htype)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_energy_stage, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, stage, dt, T, T0, T1, T2, u, w, K, Kr, Ks,
+                       Cvis, g, F, dr, ds, dr2, ds2, comp, ntype, H, Htype);
+}
+
+int launch_stability(hipStream_t st, const double* u, const double* w, const double* K, const double* Kr,
+                     const double* Ks, const Grid& g, double TrA, double TsA, double TrrD, double TssD, double* partialA,
+                     double* partialD)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1, g.nz + 1, b);
+    hipLaunchKernelGGL(k_stability, gr, b, 0, st, u, w, K, Kr, Ks, g, TrA, TsA, TrrD, TssD, partialA, partialD);
+    return (int)(gr.x * gr.y);
+}
+
+}  // namespace tc

@@ -1,0 +1,130 @@
+// Internal declarations of the MI355X (gfx950) implementation of the
+// tracer_code time-step hot path.  Host orchestration lives in tc_api.hip,
+// kernels in tc_mg.hip (multigrid + viscosity hierarchy), tc_coarse.hip
+// (coarsest-grid banded LU / inverse) and tc_fields.hip (velocities, T/C
+// filters, energy equation, tracers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/tracer_hip.h"
+
+namespace tc {
+
+constexpr int SPAN = 2;       // ghost width of every grid field (order/2+1)
+constexpr int ORG_PAD = 8;    // elements before column i=0 so that i=0 is 64-byte aligned
+
+// A ghosted 2-D plane f(-2:nx+2,-2:nz+2) in device memory.  Element (i,k)
+// sits at data[org + k*ld + i]; ld is a multiple of 32 doubles.
+struct Grid {
+    int nx = 0, nz = 0;   // cells of this level
+    int ld = 0;           // row pitch in elements
+    int org = 0;          // offset of (0,0)
+    size_t elems = 0;     // allocation size in elements
+    __host__ __device__ inline size_t at(int i, int k) const { return (size_t)org + (size_t)((ptrdiff_t)k * ld + i); }
+};
+
+inline Grid make_grid(int nx, int nz)
+{
+    Grid g;
+    g.nx = nx;
+    g.nz = nz;
+    int need = nx + 2 * SPAN + 1 + ORG_PAD;
+    g.ld = (need + 31) / 32 * 32;
+    g.org = ORG_PAD + SPAN * g.ld;
+    g.elems = (size_t)g.ld * (size_t)(nz + 2 * SPAN + 1) + 2 * ORG_PAD;
+    return g;
+}
+
+// Per-level device state of the multigrid hierarchy (the reference keeps
+// these as level-fastest fine-size arrays, basics.f90:87-98; here each level
+// owns compact planes).
+struct Level {
+    Grid g;
+    double *err[2] = {nullptr, nullptr};  // error(L): ping-pong for Jacobi sweeps
+    int cur = 0;                          // which err buffer holds error(L)
+    double *rhs = nullptr, *vis = nullptr, *visx = nullptr, *visz = nullptr, *visxx = nullptr;
+    double *viszz = nullptr, *visxz = nullptr, *dtstream = nullptr, *ratio = nullptr, *dtvis = nullptr;
+    double *visf = nullptr, *visxf = nullptr, *viszf = nullptr, *visxxf = nullptr, *viszzf = nullptr, *visxzf = nullptr;
+    double *vtmp = nullptr;               // viscosity smoother copy (T0 of smoother_vis)
+    double *res = nullptr;                // compute_residual output of this level
+    // spacings (basics.f90:167-182)
+    double dr, ds, dr2, ds2, dr3, ds3, dr4, ds4, drds, drds2, dr2ds, dr2ds2;
+    // FP32 copies for the mixed-precision path
+    float *err32[2] = {nullptr, nullptr};
+    float *rhs32 = nullptr, *vis32 = nullptr, *visx32 = nullptr, *visz32 = nullptr, *a132 = nullptr, *visxz32 = nullptr;
+    float *dts32 = nullptr, *lb32 = nullptr;
+    int cur32 = 0;
+};
+
+}  // namespace tc
+
+struct tc_ctx {
+    tc_params p;
+    int64_t ntr = 0;
+    int mode = TC_MODE_FP64;
+    int guess = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;   // coarse LU / inverse overlap stream
+    hipEvent_t ev_inv_ready = nullptr, ev_visc_ready = nullptr;
+    std::string err;
+    // hierarchy
+    int grids = 0;
+    tc::Level lv[TC_MAX_LEVELS + 1];
+    int ratio_level = -1;           // level whose ratio array is valid (others read as 1e99)
+    int Nvis_store = 0, nn_store = 1;
+    double res_ref[TC_MAX_LEVELS + 1] = {0};
+    double residual_mag[TC_MAX_MULTI] = {0};
+    // fine-grid fields (level-1 geometry)
+    tc::Grid g;
+    double *T = nullptr, *SF = nullptr, *u = nullptr, *w = nullptr, *strain = nullptr, *Tvis = nullptr, *Tbuoy = nullptr;
+    double *conduct = nullptr, *conduct_r = nullptr, *conduct_s = nullptr, *dts = nullptr;
+    double *DERr_u = nullptr, *DERr_w = nullptr, *T0 = nullptr, *T1 = nullptr, *T2 = nullptr;
+    double *residual = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+    double *Cnew[TC_MAX_TYPES] = {}, *Cvis[TC_MAX_TYPES] = {}, *Cbuoy[TC_MAX_TYPES] = {}, *C0[TC_MAX_TYPES] = {};
+    int32_t *count[TC_MAX_TYPES + 1] = {}, *nempty = nullptr;
+    double* zg = nullptr;  // zg(-3:nz+3)
+    // coarsest grid direct solve
+    int ngc = 0, kl = 0, ku = 0, ldab = 0;
+    double *Mc = nullptr, *RowB = nullptr, *Ainv = nullptr, *Bc = nullptr;
+    int32_t* ipiv = nullptr;
+    int32_t* d_info = nullptr;
+    // tracers (SoA)
+    double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
+    double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;
+    int32_t* ttype = nullptr;
+    int32_t* d_flag = nullptr;   // tracer-out-of-domain flag
+    int32_t* h_flag = nullptr;   // pinned
+    // model time
+    double time = 0.0, dt = 0.0;
+    int tstep = 0;
+    tc_stats stats{};
+    // learned guess
+    void* guess_net = nullptr;
+    // implementation-private host bookkeeping (tc_api.hip)
+    void* impl = nullptr;
+    // kernel timing (tc_set_profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    int prof_used = 0;
+    double prof_ms[TC_K_COUNT] = {0};
+    int64_t prof_n[TC_K_COUNT] = {0};
+};
+
+#define TC_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(_e);              \
+            return TC_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define TC_TRY(expr)                  \
+    do {                              \
+        int _s = (expr);              \
+        if (_s != 0) return _s;       \
+    } while (0)

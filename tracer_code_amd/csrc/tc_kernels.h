@@ -1,0 +1,118 @@
+// Kernel-side helpers and the host launchers of every kernel file.
+#pragma once
+#include "tc_internal.h"
+#include "tc_stencil.h"
+
+namespace tc {
+
+struct RaCArr { double v[TC_MAX_TYPES]; };
+struct PtrArr { double* v[TC_MAX_TYPES]; };
+struct CntArr { int32_t* v[TC_MAX_TYPES + 1]; };
+struct D3Arr { double v[3]; };
+
+// Scaled Fourier-symbol maxima of test_eigenvalues / iterate_local_time_steps
+// (iterative_solver.f90:674-685, 709-720).
+struct EigScalars {
+    double Tr_max, Trr_max, Trrr_max, Trrrr_max, Ts_max, Tss_max, Tsss_max, Tssss_max;
+    double Trs_max, Trss_max, Trrs_max, Trrss_max;
+};
+
+// Block-wide reductions (any block shape up to 1024 threads, wave64).
+// The combination order is fixed, so results are run-to-run deterministic.
+__device__ __forceinline__ int flat_tid() { return threadIdx.z * blockDim.x * blockDim.y + threadIdx.y * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int flat_nthreads() { return blockDim.x * blockDim.y * blockDim.z; }
+
+template <int OP>
+__device__ __forceinline__ double red_op(double a, double b)
+{
+    if constexpr (OP == 0) return a + b;
+    else if constexpr (OP == 1) return b < a ? b : a;
+    else return b > a ? b : a;
+}
+
+template <int OP>
+__device__ __forceinline__ double block_reduce(double a)
+{
+    __shared__ double sh[16];
+    const int tid = flat_tid(), nt = flat_nthreads();
+    for (int off = 32; off > 0; off >>= 1) a = red_op<OP>(a, __shfl_down(a, off, 64));
+    const int wid = tid >> 6, nw = (nt + 63) >> 6;
+    __syncthreads();
+    if ((tid & 63) == 0) sh[wid] = a;
+    __syncthreads();
+    if (tid == 0) {
+        double r = sh[0];
+        for (int w = 1; w < nw; ++w) r = red_op<OP>(r, sh[w]);
+        a = r;
+    }
+    return a;
+}
+__device__ __forceinline__ double block_sum(double a) { return block_reduce<0>(a); }
+__device__ __forceinline__ double block_min(double a) { return block_reduce<1>(a); }
+__device__ __forceinline__ double block_max(double a) { return block_reduce<2>(a); }
+
+// --- tc_mg.hip ---------------------------------------------------------------
+void launch_sweep(hipStream_t st, bool pow2, bool use_ratio, const Level& L, const double* S0, double* E, const FD& F,
+                  const Spacing& h, int ivis, double temp, double cconst, double zsg);
+void launch_bc_stream(hipStream_t st, double* S, const Grid& g, int Vbc);
+int launch_residual(hipStream_t st, bool pow2, const Level& L, const double* S, double* R, const FD& F,
+                    const Spacing& h, int ivis, double* partial);
+void launch_restrict(hipStream_t st, const double* in, const Grid& gi, double* out, const Grid& go);
+void launch_prolong(hipStream_t st, double* Ef, const Grid& gf, const double* Ec, const Grid& gc, double zsg);
+int launch_rhs1(hipStream_t st, const double* Tb, const PtrArr& Cb, double* rhs, const Grid& g, const FD& F,
+                double dr, double RaT, int comp, int ntype, const RaCArr& RaC, double* partial);
+int launch_abs_partial(hipStream_t st, const double* f, const Grid& g, double* partial);
+void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, double* out, int slot);
+void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
+                 const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC);
+int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
+                        const EigScalars& e, int want_ratio, double* partial);
+void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream);
+void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
+                      const FD& F, const Spacing& h, int kmin, int kmax, double c);
+void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
+                     double RaT, double visT2);
+
+// --- tc_coarse.hip -------------------------------------------------------------
+void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Level& L, const FD& F, const Spacing& h,
+                            int ivis, int Vbc, int kl, int ku, int ngc);
+void launch_dgbtf2(hipStream_t st, double* ab, int ldab, int n, int kl, int ku, int32_t* ipiv, int32_t* info);
+void launch_band_inverse(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
+                         double* Ainv);
+void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,
+                         double zsg);
+
+// --- tc_fields.hip -------------------------------------------------------------
+void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,
+                       double ds, int Vbc);
+void launch_strain(hipStream_t st, const double* u, const double* w, double* strain, const Grid& g, const FD& F,
+                   double dr, double ds);
+int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial);
+void launch_smooth_T(hipStream_t st, const double* T0, double* T, const double* dts, const Grid& g, const FD& F,
+                     double dr2, double ds2, double courantT);
+void launch_smooth_C(hipStream_t st, const double* T0, double* C, const double* dts, const Grid& g, const FD& F,
+                     double dr2, double ds2, double cfac, double courantC);
+void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot);
+void launch_bcs_comp(hipStream_t st, double* C, const Grid& g);
+void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
+void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
+                const Grid& g, double dr, double ds, double aspect, int32_t* flag);
+void launch_ratio(hipStream_t st, const CntArr& cnt, int32_t* nempty, const PtrArr& Cnew, const Grid& g, int ntype,
+                  int comp);
+void launch_conduct(hipStream_t st, const PtrArr& Cvis, double* K, double* Kr, double* Ks, const Grid& g, const FD& F,
+                    double dr, double ds, int comp, int ntype, const RaCArr& lcf);
+void launch_derivs(hipStream_t st, const double* u, const double* w, double* Du, double* Dw, const Grid& g,
+                   const D3Arr& Dint, double dr);
+void launch_tracer_stage(hipStream_t st, int stage, int torder, double dt, int64_t n, double* r, double* s, double* r0,
+                         double* s0, double* r1, double* s1, double* r2, double* s2, const double* u, const double* w,
+                         const double* Du, const double* Dw, const Grid& g, const D3Arr& Dint, double dr, double ds);
+void launch_energy_stage(hipStream_t st, int stage, double dt, double* T, const double* T0, double* T1, double* T2,
+                         const double* u, const double* w, const double* K, const double* Kr, const double* Ks,
+                         const PtrArr& Cvis, const Grid& g, const FD& F, double dr, double ds, double dr2, double ds2,
+                         int comp, int ntype, double H, const RaCArr& Htype);
+void launch_sym_bcs(hipStream_t st, double* f, const Grid& g);
+int launch_stability(hipStream_t st, const double* u, const double* w, const double* K, const double* Kr,
+                     const double* Ks, const Grid& g, double TrA, double TsA, double TrrD, double TssD, double* partialA,
+                     double* partialD);
+
+}  // namespace tc

@@ -1,0 +1,479 @@
+// Multigrid kernels of the plain FP64 path: the Jacobi-Richardson sweep of
+// iterate_stream, compute_residual, restrict, prolong, compute_RHS and the
+// viscosity hierarchy (compute_viscosity, smoother_vis, viscosity_gradients,
+// test_eigenvalues, iterate_local_time_steps).  Reference:
+// iterative_solver.f90 and smoother.f90 of seantrim/tracer_code.
+#include "tc_kernels.h"
+
+namespace tc {
+
+// ---------------------------------------------------------------------------
+// iterate_stream sweep (iterative_solver.f90:871-912) fused with the ghost
+// part of enforceBCs_stream.  RATIO selects the per-point Chebyshev factor
+// computed from ratio(level) (iterative_solver.f90:880-885); otherwise the
+// factor is the scalar cconst (1 on the upward leg, 2/(3-temp) when the
+// level's ratio is the reset value 1e99).
+// ---------------------------------------------------------------------------
+template <bool POW2, bool RATIO>
+__global__ __launch_bounds__(256) void k_sweep(const double* __restrict__ S0, double* __restrict__ E,
+                                               const double* __restrict__ vis, const double* __restrict__ vx,
+                                               const double* __restrict__ vz, const double* __restrict__ vxx,
+                                               const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                               const double* __restrict__ rhs, const double* __restrict__ dts,
+                                               const double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
+                                               double temp, double cconst, double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    double t = stream_space<POW2>(F, h, ivis, S0, c, g.ld, vis[c], vx[c], vz[c], vxx[c], vzz[c], vxz[c], rhs[c]);
+    double cc;
+    if constexpr (RATIO) {
+        const double la = -1.0;
+        double r = ratio[c], lb;
+        if (r > 1e99) lb = -2.0;
+        else lb = -2.0 * (r * r) / (1.0 + r * r);
+        cc = 2.0 / (-lb - la + (lb - la) * temp);
+    } else {
+        cc = cconst;
+    }
+    double v = S0[c] + cc * dts[c] * t;
+    E[c] = v;
+    stream_ghosts(E, g, i, k, v, zsg);
+}
+
+// Full enforceBCs_stream on one level, one workgroup (boundary work only).
+__global__ void k_bc_stream(double* __restrict__ S, Grid g, int Vbc)
+{
+    const int nx = g.nx, nz = g.nz;
+    for (int k = -SPAN + (int)threadIdx.x; k <= nz + SPAN; k += blockDim.x) { S[g.at(0, k)] = 0.0; S[g.at(nx, k)] = 0.0; }
+    for (int i = -SPAN + (int)threadIdx.x; i <= nx + SPAN; i += blockDim.x) { S[g.at(i, 0)] = 0.0; S[g.at(i, nz)] = 0.0; }
+    __syncthreads();
+    for (int k = threadIdx.x; k <= nz; k += blockDim.x)
+        for (int i = 1; i <= SPAN; ++i) {
+            S[g.at(-i, k)] = -S[g.at(i, k)];
+            S[g.at(nx + i, k)] = -S[g.at(nx - i, k)];
+        }
+    __syncthreads();
+    for (int ii = -SPAN + (int)threadIdx.x; ii <= nx + SPAN; ii += blockDim.x)
+        for (int i = 1; i <= SPAN; ++i) {
+            if (Vbc == 1) {
+                S[g.at(ii, -i)] = S[g.at(ii, i)];
+                S[g.at(ii, nz + i)] = S[g.at(ii, nz - i)];
+            } else {
+                S[g.at(ii, -i)] = -S[g.at(ii, i)];
+                S[g.at(ii, nz + i)] = -S[g.at(ii, nz - i)];
+            }
+        }
+}
+
+// compute_residual (iterative_solver.f90:258-279): residual = RHS - L(S);
+// optionally per-block partial sums of |residual| for residual_mag.
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ S, double* __restrict__ R,
+                                                  const double* __restrict__ vis, const double* __restrict__ vx,
+                                                  const double* __restrict__ vz, const double* __restrict__ vxx,
+                                                  const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                                  const double* __restrict__ rhs, Grid g, FD F, Spacing h, int ivis,
+                                                  double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        double r = stream_space<POW2>(F, h, ivis, S, c, g.ld, vis[c], vx[c], vz[c], vxx[c], vzz[c], vxz[c], rhs[c]);
+        R[c] = r;
+        a = fabs(r);
+    }
+    if (partial) {
+        a = block_sum(a);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// restrict / restrict_coefficients (iterative_solver.f90:596-664), with the
+// reference's corner weights (ii-1,kk+1 twice, ii+1,kk-1 absent).
+__global__ __launch_bounds__(256) void k_restrict(const double* __restrict__ in, Grid gi, double* __restrict__ out, Grid go)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > go.nx - 1 || k > go.nz - 1) return;
+    const int ii = 2 * i, kk = 2 * k;
+    double corners = (in[gi.at(ii - 1, kk - 1)] + in[gi.at(ii - 1, kk + 1)] + in[gi.at(ii + 1, kk + 1)] +
+                      in[gi.at(ii - 1, kk + 1)]) / 16.0;
+    double sides = (in[gi.at(ii - 1, kk)] + in[gi.at(ii, kk - 1)] + in[gi.at(ii + 1, kk)] + in[gi.at(ii, kk + 1)]) / 8.0;
+    double centre = in[gi.at(ii, kk)] / 4.0;
+    out[go.at(i, k)] = corners + sides + centre;
+}
+
+// prolong (iterative_solver.f90:175-233): one thread per fine interior
+// point; the four parity classes of the reference's loops are disjoint.
+// Fused with the ghost part of enforceBCs_stream(lf).
+__global__ __launch_bounds__(256) void k_prolong(double* __restrict__ Ef, Grid gf, const double* __restrict__ Ec, Grid gc,
+                                                 double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > gf.nx - 1 || k > gf.nz - 1) return;
+    const size_t c = gf.at(i, k);
+    double add;
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) add = Ec[gc.at(i / 2, k / 2)];
+    else if (ie) add = (Ec[gc.at(i / 2, (k - 1) / 2)] + Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
+    else if (ke) add = (Ec[gc.at((i - 1) / 2, k / 2)] + Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
+    else
+        add = (Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
+               Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+    double v = Ef[c] + add;
+    Ef[c] = v;
+    stream_ghosts(Ef, gf, i, k, v, zsg);
+}
+
+// compute_RHS(1) (iterative_solver.f90:571-590) + per-block |RHS| sums
+// for compute_reference_residual.
+__global__ __launch_bounds__(256) void k_rhs1(const double* __restrict__ Tb, PtrArr Cb,
+                                              double* __restrict__ rhs, Grid g, FD F, double dr, double RaT, int comp,
+                                              int ntype, RaCArr RaC, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        double r = 0.0;
+        if (RaT != 0.0) {
+            double Tr = dot3(F.D1, Tb, c, 1) / dr;
+            r = RaT * Tr;
+        }
+        if (comp == 1)
+            for (int j = 0; j < ntype; ++j) {
+                double Cr = dot3(F.D1, Cb.v[j], c, 1) / dr;
+                r = r - RaC.v[j] * Cr;
+            }
+        rhs[c] = r;
+        a = fabs(r);
+    }
+    a = block_sum(a);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+}
+
+// Sum of |f| over the interior of a level (compute_reference_residual).
+__global__ __launch_bounds__(256) void k_abs_partial(const double* __restrict__ f, Grid g, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) a = fabs(f[g.at(i, k)]);
+    a = block_sum(a);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+}
+
+// Deterministic final reduction of n partials by one workgroup:
+// op 0 = sum, 1 = min, 2 = max.  Writes out[slot].
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ partial, int n, int op,
+                                                      double* __restrict__ out, int slot)
+{
+    double a = (op == 0) ? 0.0 : (op == 1 ? INFINITY : -INFINITY);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        double v = partial[t];
+        if (op == 0) a += v;
+        else if (op == 1) a = v < a ? v : a;
+        else a = v > a ? v : a;
+    }
+    if (op == 0) a = block_sum(a);
+    else if (op == 1) a = block_min(a);
+    else a = block_max(a);
+    if (threadIdx.x == 0) out[slot] = a;
+}
+
+// ---------------------------------------------------------------------------
+// Viscosity hierarchy
+// ---------------------------------------------------------------------------
+
+// visf (momentum.f90:1-30) on the ghosted level-1 grid
+// (compute_viscosity, iterative_solver.f90:340-346).
+__global__ __launch_bounds__(256) void k_visf(double* __restrict__ vis, const double* __restrict__ Tvis,
+                                              PtrArr Cvis, const double* __restrict__ zg,
+                                              Grid g, int ivis, int comp, int ntype, double RaT, double lvisT,
+                                              double lvisP, RaCArr lvisC)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
+    if (i > g.nx + SPAN || k > g.nz + SPAN) return;
+    const size_t c = g.at(i, k);
+    const double zgk = zg[k + 3];
+    double v = 1.0;
+    if (ivis == 1) {
+        v = exp(-Tvis[c] * lvisT + (1.0 - zgk) * lvisP);
+    } else if (ivis == 2 && comp == 1) {
+        double dp = 0.0;
+        for (int t = 0; t < ntype; ++t) dp += Cvis.v[t][c] * lvisC.v[t];
+        if (RaT != 0.0) v = exp(-Tvis[c] * lvisT - dp + (1.0 - zgk) * lvisP);
+        else v = exp(-dp + (1.0 - zgk) * lvisP);
+    }
+    vis[c] = v;
+}
+
+// viscosity_gradients + test_eigenvalues (iterative_solver.f90:290-329,
+// 666-698, eigI 750-803, eigR 806-857) on uniform grids, with per-block
+// minima of the ratio for smoother_vis (smoother.f90:209).
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__ V, double* __restrict__ vx,
+                                                      double* __restrict__ vz, double* __restrict__ vxx,
+                                                      double* __restrict__ vzz, double* __restrict__ vxz,
+                                                      double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
+                                                      EigScalars e, int want_ratio, double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double m = INFINITY;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        double gx = 0.0, gz = 0.0, gxx = 0.0, gzz = 0.0, gxz = 0.0;
+        if (ivis != 0) {
+            gx = sdiv<POW2>(dot3(F.D1, V, c, 1), h.dr, h.idr);
+            gxx = sdiv<POW2>(dot3(F.D2, V, c, 1), h.dr2, h.idr2);
+            gz = sdiv<POW2>(dot3(F.D1, V, c, g.ld), h.ds, h.ids);
+            gzz = sdiv<POW2>(dot3(F.D2, V, c, g.ld), h.ds2, h.ids2);
+            gxz = sdiv<POW2>(sum9(F.Drs, V, c, g.ld), h.drds, h.idrds);
+        }
+        vx[c] = gx; vz[c] = gz; vxx[c] = gxx; vzz[c] = gzz; vxz[c] = gxz;
+        if (want_ratio && ivis != 0) {
+            const double vis = V[c];
+            // eigI with xr=zs=1, xrr=..=0: only the odd-derivative terms survive
+            double t1 = (gxx + gzz) * (0.0 + 0.0);
+            double t2 = vis * (0.0 + 0.0);
+            double t3 = 2.0 * vis * 0.0;
+            double t4 = 4.0 * gxz * 0.0;
+            double t5 = 2.0 * gx * (e.Trrr_max + e.Trss_max);
+            double t6 = 2.0 * gz * (e.Tsss_max + e.Trrs_max);
+            double eI = fabs(t1) + fabs(t2) + fabs(t3) + fabs(t4) + fabs(t5) + fabs(t6);
+            double r1 = (gxx - gzz) * (e.Trr_max - e.Tss_max);
+            double r2 = vis * (e.Trrrr_max + e.Tssss_max);
+            double r3 = 2.0 * vis * e.Trrss_max;
+            double r4 = 4.0 * gxz * e.Trs_max;
+            double r5 = 2.0 * gx * (0.0 + 0.0);
+            double r6 = 2.0 * gz * (0.0 + 0.0);
+            double eR = -(r1 + r2 + r3 + r4 + r5 + r6);
+            double r = fabs(eR / eI);
+            ratio[c] = r;
+            m = r;
+        }
+    }
+    if (partial) {
+        m = block_min(m);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
+}
+
+// iterate_local_time_steps (iterative_solver.f90:700-748), iterate_order=1.
+__global__ __launch_bounds__(256) void k_local_dt(const double* __restrict__ V, const double* __restrict__ vx,
+                                                  const double* __restrict__ vz, const double* __restrict__ vxx,
+                                                  const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                                  double* __restrict__ dts, Grid g, EigScalars e, double courant_stream)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    const double vis = V[c], gx = vx[c], gz = vz[c], gxx = vxx[c], gzz = vzz[c], gxz = vxz[c];
+    double t1 = (gxx + gzz) * (0.0 + 0.0);
+    double t2 = vis * (0.0 + 0.0);
+    double t3 = 2.0 * vis * 0.0;
+    double t4 = 4.0 * gxz * 0.0;
+    double t5 = 2.0 * gx * (e.Trrr_max + e.Trss_max);
+    double t6 = 2.0 * gz * (e.Tsss_max + e.Trrs_max);
+    double eI = fabs(t1) + fabs(t2) + fabs(t3) + fabs(t4) + fabs(t5) + fabs(t6);
+    double r1 = (gxx - gzz) * (e.Trr_max - e.Tss_max);
+    double r2 = vis * (e.Trrrr_max + e.Tssss_max);
+    double r3 = 2.0 * vis * e.Trrss_max;
+    double r4 = 4.0 * gxz * e.Trs_max;
+    double r5 = 2.0 * gx * (0.0 + 0.0);
+    double r6 = 2.0 * gz * (0.0 + 0.0);
+    double eR = -(r1 + r2 + r3 + r4 + r5 + r6);
+    dts[c] = -courant_stream * 2.0 * eR / (eR * eR + eI * eI);
+}
+
+// One smoother_vis sweep (smoother.f90:239-248) from the copy T0 into V on
+// i=0..nx, k=kmin..kmax.
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_vis_sweep(const double* __restrict__ T0, double* __restrict__ V,
+                                                   const double* __restrict__ dtv, Grid g, FD F, Spacing h, int kmin,
+                                                   int kmax, double c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + kmin;
+    if (i > g.nx || k > kmax) return;
+    const size_t cc = g.at(i, k);
+    double Trr = sdiv<POW2>(dot3(F.D2, T0, cc, 1), h.dr2, h.idr2);
+    double Tss = sdiv<POW2>(dot3(F.D2, T0, cc, g.ld), h.ds2, h.ids2);
+    double temp = Trr + Tss;
+    double dts = dtv[cc] * 0.99;
+    V[cc] = T0[cc] + c * dts * temp;
+}
+
+// viscosityBCs / viscosityBCs_initialize (iterative_solver.f90:379-453),
+// one workgroup, phases separated by barriers.
+__global__ void k_visc_bcs(double* __restrict__ V, Grid g, const double* __restrict__ Vf, Grid gf, int level,
+                           int initialize, double RaT, double visT2)
+{
+    const int nx = g.nx, nz = g.nz;
+    if (level > 1) {
+        if (initialize)
+            for (int k = 1 + (int)threadIdx.x; k <= nz - 1; k += blockDim.x) {
+                V[g.at(0, k)] = Vf[gf.at(0, 2 * k)];
+                V[g.at(nx, k)] = Vf[gf.at(gf.nx, 2 * k)];
+            }
+        for (int i = threadIdx.x; i <= nx; i += blockDim.x) {
+            V[g.at(i, 0)] = Vf[gf.at(2 * i, 0)];
+            V[g.at(i, nz)] = Vf[gf.at(2 * i, gf.nz)];
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k <= nz; k += blockDim.x)
+        for (int i = 1; i <= SPAN; ++i) {
+            V[g.at(-i, k)] = V[g.at(i, k)];
+            V[g.at(nx + i, k)] = V[g.at(nx - i, k)];
+        }
+    __syncthreads();
+    for (int i = -SPAN + (int)threadIdx.x; i <= nx + SPAN; i += blockDim.x)
+        for (int k = 1; k <= SPAN; ++k) {
+            if (RaT != 0.0) {
+                V[g.at(i, -k)] = 1.0 / V[g.at(i, k)] / visT2;
+                V[g.at(i, nz + k)] = 1.0 / V[g.at(i, nz - k)];
+            } else {
+                V[g.at(i, -k)] = V[g.at(i, k)];
+                V[g.at(i, nz + k)] = V[g.at(i, nz - k)];
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+void launch_sweep(hipStream_t st, bool pow2, bool use_ratio, const Level& L, const double* S0, double* E, const FD& F,
+                  const Spacing& h, int ivis, double temp, double cconst, double zsg)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    if (pow2) {
+        if (use_ratio)
+            hipLaunchKernelGGL((k_sweep<true, true>), gr, b, 0, st, S0, E, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
+                               L.rhs, L.dtstream, L.ratio, L.g, F, h, ivis, temp, cconst, zsg);
+        else
+            hipLaunchKernelGGL((k_sweep<true, false>), gr, b, 0, st, S0, E, L.vis, L.visx, L.visz, L.visxx, L.viszz,
+                               L.visxz, L.rhs, L.dtstream, L.ratio, L.g, F, h, ivis, temp, cconst, zsg);
+    } else {
+        if (use_ratio)
+            hipLaunchKernelGGL((k_sweep<false, true>), gr, b, 0, st, S0, E, L.vis, L.visx, L.visz, L.visxx, L.viszz,
+                               L.visxz, L.rhs, L.dtstream, L.ratio, L.g, F, h, ivis, temp, cconst, zsg);
+        else
+            hipLaunchKernelGGL((k_sweep<false, false>), gr, b, 0, st, S0, E, L.vis, L.visx, L.visz, L.visxx, L.viszz,
+                               L.visxz, L.rhs, L.dtstream, L.ratio, L.g, F, h, ivis, temp, cconst, zsg);
+    }
+}
+
+void launch_bc_stream(hipStream_t st, double* S, const Grid& g, int Vbc)
+{
+    hipLaunchKernelGGL(k_bc_stream, dim3(1), dim3(256), 0, st, S, g, Vbc);
+}
+
+int launch_residual(hipStream_t st, bool pow2, const Level& L, const double* S, double* R, const FD& F,
+                    const Spacing& h, int ivis, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    if (pow2)
+        hipLaunchKernelGGL((k_residual<true>), gr, b, 0, st, S, R, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
+                           L.rhs, L.g, F, h, ivis, partial);
+    else
+        hipLaunchKernelGGL((k_residual<false>), gr, b, 0, st, S, R, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
+                           L.rhs, L.g, F, h, ivis, partial);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_restrict(hipStream_t st, const double* in, const Grid& gi, double* out, const Grid& go)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_restrict, grid2(go.nx - 1, go.nz - 1, b), b, 0, st, in, gi, out, go);
+}
+
+void launch_prolong(hipStream_t st, double* Ef, const Grid& gf, const double* Ec, const Grid& gc, double zsg)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_prolong, grid2(gf.nx - 1, gf.nz - 1, b), b, 0, st, Ef, gf, Ec, gc, zsg);
+}
+
+int launch_rhs1(hipStream_t st, const double* Tb, const PtrArr& Cb, double* rhs, const Grid& g, const FD& F,
+                double dr, double RaT, int comp, int ntype, const RaCArr& RaC, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx - 1, g.nz - 1, b);
+    hipLaunchKernelGGL(k_rhs1, gr, b, 0, st, Tb, Cb, rhs, g, F, dr, RaT, comp, ntype, RaC, partial);
+    return (int)(gr.x * gr.y);
+}
+
+int launch_abs_partial(hipStream_t st, const double* f, const Grid& g, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx - 1, g.nz - 1, b);
+    hipLaunchKernelGGL(k_abs_partial, gr, b, 0, st, f, g, partial);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, double* out, int slot)
+{
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, partial, n, op, out, slot);
+}
+
+void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
+                 const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_visf, grid2(g.nx + 2 * SPAN + 1, g.nz + 2 * SPAN + 1, b), b, 0, st, vis, Tvis, Cvis, zg, g, ivis,
+                       comp, ntype, RaT, lvisT, lvisP, lvisC);
+}
+
+int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
+                        const EigScalars& e, int want_ratio, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    if (pow2)
+        hipLaunchKernelGGL((k_vis_grad_eig<true>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial);
+    else
+        hipLaunchKernelGGL((k_vis_grad_eig<false>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_local_dt, grid2(L.g.nx - 1, L.g.nz - 1, b), b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz,
+                       L.visxz, L.dtstream, L.g, e, courant_stream);
+}
+
+void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
+                      const FD& F, const Spacing& h, int kmin, int kmax, double c)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1, kmax - kmin + 1, b);
+    if (pow2)
+        hipLaunchKernelGGL((k_vis_sweep<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c);
+    else
+        hipLaunchKernelGGL((k_vis_sweep<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c);
+}
+
+void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
+                     double RaT, double visT2)
+{
+    hipLaunchKernelGGL(k_visc_bcs, dim3(1), dim3(256), 0, st, V, g, Vf, gf, level, init, RaT, visT2);
+}
+
+}  // namespace tc

@@ -1,0 +1,212 @@
+"""Host-side mirror of the reference's hot-path routines over the C-ABI.
+
+`Solver` owns one library context (device state) and exposes the reference's
+routine names: multigrid, compute_velocities, compute_strain_rate_invariant,
+smoother_time_T, tracers_to_corners, smoother_time, load_conductivity_array,
+the RK stages of energy_time and whole time steps.  Arrays cross the boundary
+in the reference's Fortran layout: a ghosted field f(-2:nx+2,-2:nz+2) is a
+numpy array of shape (nz+5, nx+5) whose element [k+2, i+2] is f(i,k);
+composition fields are (nz+5, nx+5, ntype) (type fastest) and count_type is
+(nz+5, nx+5, ntype+1).
+
+Errors are raised as TcError carrying the library status (the reference's
+`stop` conditions are positive codes).  There is no CPU fallback: without the
+built library the constructor raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+
+
+class TcError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"tracer_hip status {status}: {msg}")
+        self.status = status
+
+
+_FIELD = {
+    "T": _abi.TC_F_T, "SF": _abi.TC_F_SF, "Tvis": _abi.TC_F_TVIS, "Tbuoy": _abi.TC_F_TBUOY,
+    "Cnew": _abi.TC_F_CNEW, "Cvis": _abi.TC_F_CVIS, "Cbuoy": _abi.TC_F_CBUOY, "u": _abi.TC_F_U,
+    "w": _abi.TC_F_W, "strain": _abi.TC_F_STRAIN, "vis": _abi.TC_F_VIS, "conduct": _abi.TC_F_CONDUCT,
+    "nempty": _abi.TC_F_NEMPTY, "count_type": _abi.TC_F_COUNT_TYPE, "T0": _abi.TC_F_T0, "T1": _abi.TC_F_T1,
+    "T2": _abi.TC_F_T2,
+}
+_TRACERS = {"x": _abi.TC_TR_X, "x0": _abi.TC_TR_X0, "x1": _abi.TC_TR_X1, "x2": _abi.TC_TR_X2}
+
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is None:
+        _lib = _abi.load_library()
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Solver:
+    def __init__(self, params, ntr=0, lib=None):
+        self.lib = lib or library()
+        self.params = params
+        self.nx, self.nz, self.ntype = params.nx, params.nz, params.ntype
+        self.ntr = int(ntr)
+        h = ctypes.c_void_p()
+        st = self.lib.tc_init(ctypes.byref(params), self.ntr, ctypes.byref(h))
+        if st != 0:
+            raise TcError(st, "tc_init failed")
+        self.h = h
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tc_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != 0:
+            msg = self.lib.tc_last_error(self.h)
+            raise TcError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- data movement -------------------------------------------------------
+    def _shape(self, name):
+        base = (self.nz + 5, self.nx + 5)
+        if name in ("Cnew", "Cvis", "Cbuoy"):
+            return base + (self.ntype,), np.float64
+        if name == "count_type":
+            return base + (self.ntype + 1,), np.int32
+        if name == "nempty":
+            return base, np.int32
+        return base, np.float64
+
+    def put(self, name, arr):
+        shape, dt = self._shape(name)
+        a = np.ascontiguousarray(arr, dtype=dt)
+        if a.shape != shape:
+            raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
+        self._check(self.lib.tc_put_field(self.h, _FIELD[name], _ptr(a)), f"put {name}")
+
+    def get(self, name):
+        shape, dt = self._shape(name)
+        a = np.empty(shape, dtype=dt)
+        self._check(self.lib.tc_get_field(self.h, _FIELD[name], _ptr(a)), f"get {name}")
+        return a
+
+    def put_tracers(self, rtr, str_, ttype):
+        r = np.ascontiguousarray(rtr, dtype=np.float64)
+        s = np.ascontiguousarray(str_, dtype=np.float64)
+        t = np.ascontiguousarray(ttype, dtype=np.int32)
+        if not (r.shape == s.shape == t.shape == (self.ntr,)):
+            raise ValueError("tracer arrays must have length ntr")
+        self._check(self.lib.tc_put_tracers(self.h, _ptr(r), _ptr(s), _ptr(t)), "put tracers")
+
+    def get_tracers(self, which="x"):
+        r = np.empty(self.ntr)
+        s = np.empty(self.ntr)
+        self._check(self.lib.tc_get_tracers(self.h, _TRACERS[which], _ptr(r), _ptr(s)), "get tracers")
+        return r, s
+
+    def set_time(self, time, dt=0.0):
+        self._check(self.lib.tc_set_time(self.h, float(time), float(dt)), "set_time")
+
+    def set_mode(self, mode):
+        self._check(self.lib.tc_set_mode(self.h, int(mode)), "set_mode")
+
+    def stats(self):
+        s = _abi.TcStats()
+        self._check(self.lib.tc_get_stats(self.h, ctypes.byref(s)), "stats")
+        return s
+
+    def sync(self):
+        self._check(self.lib.tc_sync(self.h), "sync")
+
+    # -- the reference's routines ------------------------------------------
+    def multigrid(self):
+        """multigrid (iterative_solver.f90:79-173) -> (q, residual_mag(1), res_ref(1))."""
+        q = ctypes.c_int32()
+        r = ctypes.c_double()
+        rr = ctypes.c_double()
+        self._check(self.lib.tc_multigrid(self.h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(rr)), "multigrid")
+        return q.value, r.value, rr.value
+
+    def compute_velocities(self):
+        self._check(self.lib.tc_compute_velocities(self.h), "compute_velocities")
+
+    def compute_strain_rate_invariant(self):
+        self._check(self.lib.tc_compute_strain_rate_invariant(self.h), "compute_strain_rate_invariant")
+
+    def smoother_time_T(self, src="T"):
+        n = ctypes.c_int32()
+        self._check(self.lib.tc_smoother_time_T(self.h, _FIELD[src], ctypes.byref(n)), "smoother_time_T")
+        return n.value
+
+    def tracers_to_corners(self, which="x", smooth=True):
+        n = (ctypes.c_int32 * _abi.TC_MAX_TYPES)()
+        self._check(self.lib.tc_tracers_to_corners(self.h, _TRACERS[which], int(bool(smooth)), n),
+                    "tracers_to_corners")
+        return list(n)[: self.ntype]
+
+    def smoother_time(self):
+        n = (ctypes.c_int32 * _abi.TC_MAX_TYPES)()
+        self._check(self.lib.tc_smoother_time(self.h, n), "smoother_time")
+        return list(n)[: self.ntype]
+
+    def load_conductivity_array(self):
+        self._check(self.lib.tc_load_conductivity(self.h), "load_conductivity")
+
+    def tracer_stage(self, stage, dt):
+        self._check(self.lib.tc_tracer_stage(self.h, int(stage), float(dt)), "tracer_stage")
+
+    def energy_stage(self, stage, dt):
+        self._check(self.lib.tc_energy_stage(self.h, int(stage), float(dt)), "energy_stage")
+
+    def stability_RK4(self):
+        dt = ctypes.c_double()
+        self._check(self.lib.tc_stability_RK4(self.h, ctypes.byref(dt)), "stability_RK4")
+        return dt.value
+
+    def set_profiling(self, enable):
+        self._check(self.lib.tc_set_profiling(self.h, int(bool(enable))), "set_profiling")
+
+    def kernel_timing(self, which=_abi.TC_K_SWEEP_L1):
+        avg = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.lib.tc_kernel_timing(self.h, int(which), ctypes.byref(avg), ctypes.byref(n)), "kernel_timing")
+        return avg.value, n.value
+
+    def time_step(self, nsteps=1):
+        self._check(self.lib.tc_time_step(self.h, int(nsteps)), "time_step")
+
+    # -- start-up ------------------------------------------------------------
+    def initialize(self, T, rtr, str_, ttype, SF=None):
+        """The restart start-up of the reference (main.f90:121-163 with
+        restart=1, grid.f90:56-95): load T (boundary conditions applied),
+        SF and the tracers, bin the tracers without smoothing, then
+        smoother_time_T(T), smoother_time and load_conductivity_array."""
+        self.put("T", T)
+        sf = np.zeros((self.nz + 5, self.nx + 5)) if SF is None else SF
+        self.put("SF", sf)
+        if self.ntr:
+            self.put_tracers(rtr, str_, ttype)
+            self.tracers_to_corners("x", smooth=False)
+        self.smoother_time_T("T")
+        if self.params.comp:
+            self.smoother_time()
+        self.load_conductivity_array()
+        self.set_time(0.0, 0.0)
