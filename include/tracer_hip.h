@@ -78,6 +78,9 @@ enum {
     TC_F_COUNT
 };
 
+/* Bits of tc_params.fixes. */
+enum { TC_FIX_ZERO_COARSE = 1 };
+
 /* Tracer position sets (energy.f90 arrays rtr/str, rtr0/str0, rtr1/str1, rtr2/str2). */
 enum { TC_TR_X = 0, TC_TR_X0 = 1, TC_TR_X1 = 2, TC_TR_X2 = 3 };
 
@@ -99,7 +102,13 @@ typedef struct tc_params {
     int32_t nsteps;       /* W/V-cycle length */
     int32_t multi[TC_MAX_MULTI]; /* multi(0:nsteps+1) */
     int32_t comp, ntype, ivis, torder, RK_update;
-    int32_t pad0;
+    /* Bit mask of departures from the reference multigrid (0 = the
+     * reference exactly).  TC_FIX_ZERO_COARSE: error(L), 1<L<grids, is
+     * zeroed each time the cycle descends into level L (the standard
+     * coarse-grid correction).  The reference zeroes it once (main.f90:162)
+     * and otherwise starts each visit from the stale correction, which
+     * diverges at large viscosity contrasts (1e5 from T x 100 from C). */
+    int32_t fixes;
     double aspect;
     double tolerance, eig_ratio, eig_ratio_min;
     double courant, courantT, courantC, courant_stream;

@@ -1028,8 +1028,7 @@ static void visc_bcs(or_state* s, int level, int initialize)
     const double visT2 = s->p.visT * s->p.visT;
     for (int k = 1; k <= SPAN; ++k)
         for (int i = -SPAN; i <= nxl + SPAN; ++i) {
-            if (s->p.RaT != 0.0) {
-                V[IX(s, i, -k)] = 1.0 / V[IX(s, i, k)] / visT2;
+            if (s->p.RaT != 0.0) {                V[IX(s, i, -k)] = 1.0 / V[IX(s, i, k)] / visT2;
                 V[IX(s, i, nzl + k)] = 1.0 / V[IX(s, i, nzl - k)];
             } else {
                 V[IX(s, i, -k)] = V[IX(s, i, k)];
@@ -1453,6 +1452,8 @@ int or_multigrid(or_state* s)
             if (multi[p - 1] < multi[p]) {
                 or_compute_RHS(s, level);
                 if (p <= G && q == 1) or_compute_reference_residual(s, level);
+                /* optional standard coarse-grid correction (not the reference) */
+                if (s->p.fixes & TC_FIX_ZERO_COARSE) memset(s->error[level], 0, s->plane * sizeof(double));
             }
             if (level < G) {
                 or_iterate_stream(s, level, s->p.Nmulti, dir, p);

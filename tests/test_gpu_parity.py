@@ -19,6 +19,8 @@ CASES = {
     "cfg0_32": ("cfg0", dict(nx=32, nz=32)),
     "cfg0_64": ("cfg0", dict(nx=64, nz=64)),
     "cfg1_96x32": ("cfg1", dict(nx=96, nz=32, tpc=8)),
+    # 2048x1024 physics (visT=1e5, visC=0.01) at reduced size, zeroed coarse corrections
+    "cfg2_128x64": ("cfg2", dict(nx=128, nz=64, tpc=4)),
 }
 
 
@@ -55,7 +57,7 @@ def test_startup_fields_match(case):
         assert rel_err(g.get("conduct"), o.plane(oracle.OR_CONDUCT)) <= 1e-15
 
 
-@pytest.mark.parametrize("case", ["cfg0_32", "cfg0_64", "cfg1_96x32"])
+@pytest.mark.parametrize("case", ["cfg0_32", "cfg0_64", "cfg1_96x32", "cfg2_128x64"])
 def test_multigrid_solve_matches(case):
     params, o, g = make_pair(case)
     with g:

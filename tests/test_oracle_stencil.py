@@ -265,3 +265,20 @@ def test_multigrid_stop_conditions():
     params, o = small_state(nx=16, nz=16, grids=3, ivis=1)
     o.plane(oracle.OR_SF)[:] = 1e30
     assert o.multigrid() == 2  # abnormally high residual
+
+
+def test_reference_cycle_diverges_where_zeroed_correction_converges():
+    """At the 2048x1024 workload's physics (visT=1e5, visC=0.01) the
+    reference's W-cycle, whose intermediate corrections are never re-zeroed,
+    grows the residual; zeroing error(L) on each descent converges."""
+    def run(zero, cycles):
+        params, cfg = config.preset_params("cfg2", nx=128, nz=64, tpc=1, fixes=zero,
+                                           max_cycles=cycles, tolerance=1e-8)
+        T, r, s, ty = ic.workload(cfg)
+        o = load_oracle_state(params, T, r, s, ty)
+        st = o.multigrid()
+        return st, o.scalar(oracle.ORS_RESID1) / o.scalar(oracle.ORS_RESREF1)
+    st_ref, rel_ref = run(0, 40)
+    assert st_ref != 0 and rel_ref > 1e-2
+    st_z, rel_z = run(1, 60)
+    assert st_z == 0 and rel_z <= 1e-8

@@ -31,6 +31,7 @@ TC_F_T0, TC_F_T1, TC_F_T2 = 14, 15, 16
 
 TC_TR_X, TC_TR_X0, TC_TR_X1, TC_TR_X2 = 0, 1, 2, 3
 TC_K_SWEEP_L1 = 0
+TC_FIX_ZERO_COARSE = 1
 
 c_double5 = ctypes.c_double * 5
 
@@ -43,7 +44,7 @@ class TcParams(ctypes.Structure):
         ("Nmulti", ctypes.c_int32), ("vis_iter", ctypes.c_int32), ("max_cycles", ctypes.c_int32),
         ("nsteps", ctypes.c_int32), ("multi", ctypes.c_int32 * TC_MAX_MULTI),
         ("comp", ctypes.c_int32), ("ntype", ctypes.c_int32), ("ivis", ctypes.c_int32),
-        ("torder", ctypes.c_int32), ("RK_update", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("torder", ctypes.c_int32), ("RK_update", ctypes.c_int32), ("fixes", ctypes.c_int32),
         ("aspect", ctypes.c_double),
         ("tolerance", ctypes.c_double), ("eig_ratio", ctypes.c_double), ("eig_ratio_min", ctypes.c_double),
         ("courant", ctypes.c_double), ("courantT", ctypes.c_double), ("courantC", ctypes.c_double),

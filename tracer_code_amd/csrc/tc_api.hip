@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <new>
 
 #include "tc_kernels.h"
@@ -60,6 +61,8 @@ struct tc_dev {
     double* partialB = nullptr;
     int max_partial = 0;
     bool coarse_ready = false;
+    bool trace = false;
+    bool coarse_lu = false;
     std::vector<double> hbuf;  // host staging plane
 };
 
@@ -370,17 +373,27 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
                 // compute_RHS(level) = restrict(residual of level-1)
                 Level& Lf = ctx->lv[level - 1];
                 launch_restrict(ctx->stream, Lf.res, Lf.g, ctx->lv[level].rhs, ctx->lv[level].g);
+                if (P.fixes & TC_FIX_ZERO_COARSE) {
+                    Level& Lc = ctx->lv[level];
+                    TC_HIP(hipMemsetAsync(Lc.err[Lc.cur], 0, plane_bytes(Lc.g), ctx->stream));
+                }
             }
             if (level < G) {
                 TC_TRY(iterate_stream(ctx, d, level, P.Nmulti, dir, p, first_sweep));
             } else {
                 Level& Lc = ctx->lv[G];
-                launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, Lc, Lc.err[Lc.cur], ctx->ngc, zsg);
+                if (d.coarse_lu)
+                    launch_coarse_lusolve(ctx->stream, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv,
+                                          ctx->RowB, Lc, Lc.err[Lc.cur], zsg);
+                else
+                    launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, Lc, Lc.err[Lc.cur], ctx->ngc, zsg);
                 ctx->stats.coarse_solves++;
             }
             if (level == 1) {
                 TC_TRY(fetch_scalars(ctx, d, S_COUNT));
                 ctx->residual_mag[1] = d.h_scal[S_RESID1] / (double)(L1.g.nx - 1) / (double)(L1.g.nz - 1);
+                // the reference's per-cycle line (iterative_solver.f90:135), TC_TRACE=1
+                if (d.trace) fprintf(stderr, "%5d %20.8g %20.8g\n", q, ctx->residual_mag[1], ctx->res_ref[1]);
                 if (ctx->residual_mag[1] > 1e10 * ctx->res_ref[1]) {
                     ctx->err = "Multigrid Convergence Problem: Abnormally High Residual Detected -- stopping";
                     status = TC_ERR_RESID_BLOWUP;
@@ -765,6 +778,12 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
     tc_dev* d = new (std::nothrow) tc_dev();
     if (!d) return TC_ERR_NOMEM;
     ctx->impl = d;
+    {
+        const char* e = getenv("TC_TRACE");
+        d->trace = e && e[0] == '1';
+        const char* c = getenv("TC_COARSE_LU");
+        d->coarse_lu = c && c[0] == '1';
+    }
     TC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     TC_HIP(hipEventCreateWithFlags(&ctx->ev_inv_ready, hipEventDisableTiming));
@@ -842,6 +861,10 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
             }
         }
         ctx->kl = kl; ctx->ku = ku; ctx->ldab = 2 * kl + ku + 1;
+        if (8 * (size_t)(kl + ku + 1) * sizeof(double) > 65536 || (size_t)ctx->ngc * ctx->ngc > ((size_t)1 << 31)) {
+            ctx->err = "coarsest grid too large for the explicit coarse inverse (use more levels)";
+            return TC_ERR_UNSUPPORTED;
+        }
         TC_TRY(dalloc(ctx, &ctx->Mc, (size_t)ctx->ldab * ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->RowB, ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Ainv, (size_t)ctx->ngc * ctx->ngc));

@@ -212,40 +212,80 @@ __global__ __launch_bounds__(256) void k_dgbtf2(double* __restrict__ ab, int lda
     if (tid == 0) *info_out = sh_info;
 }
 
-// Explicit inverse from the DGBTF2 factors: thread c applies DGBTRS('N')
-// to the unit vector e_c, writing column c of A^-1 into the row-major Ainv
-// (lanes of a wavefront touch consecutive columns: coalesced).
+// Explicit inverse from the DGBTF2 factors: lane c of a wavefront applies
+// DGBTRS('N') to the unit vector e_c (the same operations, in the same
+// order, as the reference algorithm), writing column c of A^-1 into the
+// row-major Ainv (lanes touch consecutive columns: coalesced).  The rows a
+// substitution step touches form a window of kl+1 (forward) or kl+ku+1
+// (backward) rows that slides by one row per step; it lives in LDS as a
+// circular buffer, so each step is LDS traffic plus one coalesced global
+// store of the row that leaves the window.
+template <int INV_COLS>
 __global__ __launch_bounds__(64) void k_band_inverse(const double* __restrict__ ab, int ldab, int n, int kl, int ku,
                                                      const int32_t* __restrict__ ipiv, double* __restrict__ X)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    extern __shared__ double win[];  // [W][INV_COLS]
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x * INV_COLS + lane;
+    const bool active = c < n;
     const size_t nn = (size_t)n;
-    for (int i = 0; i < n; ++i) X[(size_t)i * nn + c] = (i == c) ? 1.0 : 0.0;
     const int kd = ku + kl;
-    for (int j = 0; j < n - 1; ++j) {
-        const int lm = kl < n - 1 - j ? kl : n - 1 - j;
-        const int l = ipiv[j] - 1;
-        if (l != j) {
-            double a = X[(size_t)l * nn + c];
-            X[(size_t)l * nn + c] = X[(size_t)j * nn + c];
-            X[(size_t)j * nn + c] = a;
-        }
-        double bj = X[(size_t)j * nn + c];
-        if (bj != 0.0) {
-            double temp = -1.0 * bj;
-            for (int t = 1; t <= lm; ++t) X[(size_t)(j + t) * nn + c] += ABx(kd + t, j) * temp;
-        }
+    // ---- forward: L^-1 P e_c, window rows [j, j+kl] -----------------------
+    const int Wf = kl + 1;
+    const int cmin = blockIdx.x * INV_COLS;
+    const int jstart = cmin - kl > 0 ? cmin - kl : 0;  // rows before this are zero for every lane
+    for (int t = 0; t < Wf; ++t) {
+        const int row = jstart + t;
+        win[((row % Wf) * INV_COLS) + lane] = (row == c) ? 1.0 : 0.0;
     }
-    const int k = kl + ku;
-    for (int j = n - 1; j >= 0; --j) {
-        double bj = X[(size_t)j * nn + c];
-        if (bj != 0.0) {
-            bj = bj / ABx(k, j);
-            X[(size_t)j * nn + c] = bj;
-            const int i0 = j - k > 0 ? j - k : 0;
-            for (int i = j - 1; i >= i0; --i) X[(size_t)i * nn + c] -= bj * ABx(k + i - j, j);
+    for (int i = 0; i < jstart && active; ++i) X[(size_t)i * nn + c] = 0.0;
+    for (int j = jstart; j < n; ++j) {
+        if (j < n - 1) {
+            const int lm = kl < n - 1 - j ? kl : n - 1 - j;
+            const int l = ipiv[j] - 1;
+            if (l != j) {
+                double& a = win[(l % Wf) * INV_COLS + lane];
+                double& b = win[(j % Wf) * INV_COLS + lane];
+                const double tmp = a;
+                a = b;
+                b = tmp;
+            }
+            const double bj = win[(j % Wf) * INV_COLS + lane];
+            if (bj != 0.0) {
+                const double temp = -1.0 * bj;
+                for (int t = 1; t <= lm; ++t) {
+                    double& x = win[((j + t) % Wf) * INV_COLS + lane];
+                    x += ab[(size_t)(kd + t) + (size_t)j * ldab] * temp;
+                }
+            }
         }
+        // row j is final for the forward pass; row j+kl+1 enters the window
+        const double yj = win[(j % Wf) * INV_COLS + lane];
+        if (active) X[(size_t)j * nn + c] = yj;
+        const int enter = j + Wf;
+        win[(j % Wf) * INV_COLS + lane] = (enter == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // ---- backward: U^-1 y, window rows [j-k, j] ---------------------------
+    const int k = kl + ku;
+    const int Wb = k + 1;
+    for (int t = 0; t < Wb && t < n; ++t) {
+        const int row = n - 1 - t;
+        win[(row % Wb) * INV_COLS + lane] = active ? X[(size_t)row * nn + c] : 0.0;
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        double bj = win[(j % Wb) * INV_COLS + lane];
+        if (bj != 0.0) {
+            bj = bj / ab[(size_t)k + (size_t)j * ldab];
+            const int i0 = j - k > 0 ? j - k : 0;
+            for (int i = j - 1; i >= i0; --i) {
+                double& x = win[(i % Wb) * INV_COLS + lane];
+                x -= bj * ab[(size_t)(k + i - j) + (size_t)j * ldab];
+            }
+        }
+        if (active) X[(size_t)j * nn + c] = bj;
+        const int enter = j - Wb;
+        if (enter >= 0) win[(j % Wb) * INV_COLS + lane] = active ? X[(size_t)enter * nn + c] : 0.0;
     }
 }
 
@@ -272,7 +312,71 @@ __global__ __launch_bounds__(256) void k_coarse_apply(const double* __restrict__
     }
 }
 
+// Coarse solve by banded substitution with the DGBTF2 factors (DGBTRS 'N',
+// iterative_solver.f90:139-142): one workgroup, the solution vector in LDS,
+// one barrier per substitution step.
+__global__ __launch_bounds__(256) void k_coarse_lusolve(const double* __restrict__ ab, int ldab, int n, int kl, int ku,
+                                                        const int32_t* __restrict__ ipiv,
+                                                        const double* __restrict__ RowB, const double* __restrict__ rhs,
+                                                        double* __restrict__ E, Grid g, double zsg)
+{
+    extern __shared__ double b[];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nxc = g.nx, nzc = g.nz, N = nzc + 1;
+    for (int q = tid; q < n; q += nt) {
+        const int kq = q % N, iq = q / N;
+        b[q] = (iq > 0 && iq < nxc && kq > 0 && kq < nzc) ? rhs[g.at(iq, kq)] * RowB[q] : 0.0;
+    }
+    __syncthreads();
+    const int kd = ku + kl;
+    for (int j = 0; j < n - 1; ++j) {
+        const int lm = kl < n - 1 - j ? kl : n - 1 - j;
+        const int l = ipiv[j] - 1;
+        double bj;
+        if (l != j) {
+            const double bl = b[l], bjj = b[j];
+            __syncthreads();
+            if (tid == 0) { b[l] = bjj; b[j] = bl; }
+            bj = bl;
+        } else {
+            bj = b[j];
+        }
+        __syncthreads();
+        if (bj != 0.0) {
+            const double temp = -1.0 * bj;
+            for (int t = 1 + tid; t <= lm; t += nt) b[j + t] += ABx(kd + t, j) * temp;
+        }
+        __syncthreads();
+    }
+    const int k = kl + ku;
+    for (int j = n - 1; j >= 0; --j) {
+        double bj = b[j];
+        if (bj != 0.0) {
+            bj = bj / ABx(k, j);
+            const int i0 = j - k > 0 ? j - k : 0;
+            for (int i = j - 1 - tid; i >= i0; i -= nt) b[i] -= bj * ABx(k + i - j, j);
+        }
+        __syncthreads();
+        if (tid == 0) b[j] = bj;
+        __syncthreads();
+    }
+    for (int q = tid; q < n; q += nt) {
+        const int kq = q % N, iq = q / N;
+        if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) {
+            E[g.at(iq, kq)] = b[q];
+            stream_ghosts(E, g, iq, kq, b[q], zsg);
+        }
+    }
+}
+
 #undef ABx
+
+void launch_coarse_lusolve(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
+                           const double* RowB, const Level& L, double* E, double zsg)
+{
+    hipLaunchKernelGGL(k_coarse_lusolve, dim3(1), dim3(256), sizeof(double) * (size_t)n, st, ab, ldab, n, kl, ku, ipiv,
+                       RowB, L.rhs, E, L.g, zsg);
+}
 
 void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Level& L, const FD& F, const Spacing& h,
                             int ivis, int Vbc, int kl, int ku, int ngc)
@@ -289,7 +393,20 @@ void launch_dgbtf2(hipStream_t st, double* ab, int ldab, int n, int kl, int ku, 
 void launch_band_inverse(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
                          double* Ainv)
 {
-    hipLaunchKernelGGL(k_band_inverse, dim3((n + 63) / 64), dim3(64), 0, st, ab, ldab, n, kl, ku, ipiv, Ainv);
+    // window of kl+ku+1 rows per column; keep the workgroup's LDS within 64 KiB
+    const size_t row_bytes = sizeof(double) * (size_t)(kl + ku + 1);
+    if (64 * row_bytes <= 65536)
+        hipLaunchKernelGGL((k_band_inverse<64>), dim3((n + 63) / 64), dim3(64), 64 * row_bytes, st, ab, ldab, n, kl, ku,
+                           ipiv, Ainv);
+    else if (32 * row_bytes <= 65536)
+        hipLaunchKernelGGL((k_band_inverse<32>), dim3((n + 31) / 32), dim3(32), 32 * row_bytes, st, ab, ldab, n, kl, ku,
+                           ipiv, Ainv);
+    else if (16 * row_bytes <= 65536)
+        hipLaunchKernelGGL((k_band_inverse<16>), dim3((n + 15) / 16), dim3(16), 16 * row_bytes, st, ab, ldab, n, kl, ku,
+                           ipiv, Ainv);
+    else
+        hipLaunchKernelGGL((k_band_inverse<8>), dim3((n + 7) / 8), dim3(8), 8 * row_bytes, st, ab, ldab, n, kl, ku, ipiv,
+                           Ainv);
 }
 
 void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,

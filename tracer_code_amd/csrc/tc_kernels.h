@@ -79,6 +79,8 @@ void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Leve
 void launch_dgbtf2(hipStream_t st, double* ab, int ldab, int n, int kl, int ku, int32_t* ipiv, int32_t* info);
 void launch_band_inverse(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
                          double* Ainv);
+void launch_coarse_lusolve(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
+                           const double* RowB, const Level& L, double* E, double zsg);
 void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,
                          double zsg);
 
