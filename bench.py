@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X hot path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--mode fp64|mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg2] [--mode mixed|dc64|fp64]
 
 A step is one pass of the reference's time loop body (main.f90:174-205):
 multigrid, compute_velocities, the strain rate, and energy_time with RK4 and
@@ -9,10 +9,16 @@ tracer stages over all tracers, five tracer-to-node binnings with the C
 filter, the T filter and the T-field RK stages.  Inputs are synthetic
 (seeded) and resident in HBM when the timed region starts.
 
+Modes: `mixed` (default) and `dc64` solve by defect correction with a
+double-double streamfunction and FP32 / FP64 W-cycle corrections; `fp64` is
+the reference's own cycle, which FP64 rounding keeps above a 1e-8 residual at
+2048x1024 (it stops with "Too Many Multigrid Cycles").
+
 Prints one JSON line.  `roofline` reports the level-1 Jacobi-Richardson sweep
-(the dominant kernel): algorithmic bytes per launch = 88 B per interior point
-(11 FP64 arrays of the reference: S, eta and its 5 derivatives, RHS,
-dt_stream, ratio read, error written) over the HIP-event-timed average launch
+(the dominant kernel): algorithmic bytes per launch per interior point are
+80 B in FP64 (read S, eta and its 5 derivatives, RHS, dt_stream; write the
+new iterate) and 36 B in FP32 (read S, eta, eta_x, eta_z, eta_xx-eta_zz,
+eta_xz, RHS, dt_stream; write), over the HIP-event-timed average launch
 duration on the library's stream.  `cpu_baseline` times the CPU oracle's
 level-1 sweep on this host and converts it to steps/s with the sweep counts
 the GPU executed per step.
@@ -27,7 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_POINT = 88.0
+SWEEP_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}
+SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
 
 
 def cpu_baseline(params, sweeps_per_step, budget_s=12.0):
@@ -80,7 +87,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "dc64", "fp64"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -98,7 +105,7 @@ def main():
     print(f"[bench] {args.config}: {params.nx}x{params.nz}, {params.grids} levels, {ntr} tracers, "
           f"inputs generated in {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
     solver = Solver(params, ntr)
-    solver.set_mode(_abi.TC_MODE_MIXED if args.mode == "mixed" else _abi.TC_MODE_FP64)
+    solver.set_mode(args.mode)
     solver.initialize(T, r, s, ty)
     del T, r, s, ty
     for w in range(args.warmup):
@@ -125,10 +132,11 @@ def main():
     cycles_per_step = (after.mg_cycles - cycles0) / args.steps
     steps_per_s = args.steps / elapsed
     pts1 = (params.nx - 1) * (params.nz - 1)
-    bytes_per_launch = SWEEP_BYTES_PER_POINT * pts1
+    bytes_per_launch = SWEEP_BYTES_PER_POINT[args.mode] * pts1
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else None
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"smoother_l1_{args.config}_hbm_bytes.txt")
+    kname, kfile = SWEEP_KERNEL[args.mode]
+    tfile = os.path.join(ROOT, "profiles", f"{kfile}_l1_{params.nx}x{params.nz}_hbm_bytes.txt")
     if os.path.exists(tfile):
         for line in open(tfile):
             if line.startswith("bytes_per_launch"):
@@ -144,7 +152,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "fp64" else "f32+f64",
+        "dtype": {"fp64": "f64", "dc64": "f64 (double-double residual)",
+                  "mixed": "f32 cycle, f64/double-double residual"}[args.mode],
         "data": "synthetic (seeded T0 with Fourier modes, jittered tracers)",
         "config": {
             "workload": f"{params.nx}x{params.nz} aspect {params.aspect:g}, Ra={params.RaT:g}, "
@@ -155,7 +164,7 @@ def main():
             "sweeps_per_step": sweeps_per_step,
         },
         "roofline": {
-            "kernel": "k_sweep level 1 (Jacobi-Richardson smoother)",
+            "kernel": f"{kname} level 1 (Jacobi-Richardson smoother)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -53,7 +53,8 @@ enum {
 /* Solver precision modes (tc_set_mode). */
 enum {
     TC_MODE_FP64 = 0,  /* plain path: the reference's Jacobi-Richardson W-cycle in FP64 */
-    TC_MODE_MIXED = 1  /* FP32 sweeps/transfers inside an FP64 defect-correction loop */
+    TC_MODE_MIXED = 1, /* defect correction: double-double psi and fine residual, FP32 W-cycle corrections */
+    TC_MODE_DC64 = 2   /* defect correction as MIXED with FP64 W-cycle corrections */
 };
 
 /* Field identifiers for tc_put_field / tc_get_field. */
@@ -75,6 +76,9 @@ enum {
     TC_F_T0 = 14,    /* energy_time stage fields T0/T1/T2 (device-resident driver) */
     TC_F_T1 = 15,
     TC_F_T2 = 16,
+    TC_F_SF_LO = 17, /* get only: low part of the double-double streamfunction of the last
+                        defect-correction solve (SF holds the high part) */
+    TC_F_RHS = 18,   /* get only: RHS(1) of the last solve (compute_RHS, iterative_solver.f90:571-590) */
     TC_F_COUNT
 };
 

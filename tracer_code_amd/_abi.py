@@ -23,11 +23,13 @@ TC_ERR_NOMEM = -4
 
 TC_MODE_FP64 = 0
 TC_MODE_MIXED = 1
+TC_MODE_DC64 = 2
 
 # field ids
 TC_F_T, TC_F_SF, TC_F_TVIS, TC_F_TBUOY, TC_F_CNEW, TC_F_CVIS, TC_F_CBUOY = range(7)
 TC_F_U, TC_F_W, TC_F_STRAIN, TC_F_VIS, TC_F_CONDUCT, TC_F_NEMPTY, TC_F_COUNT_TYPE = range(7, 14)
 TC_F_T0, TC_F_T1, TC_F_T2 = 14, 15, 16
+TC_F_SF_LO, TC_F_RHS = 17, 18
 
 TC_TR_X, TC_TR_X0, TC_TR_X1, TC_TR_X2 = 0, 1, 2, 3
 TC_K_SWEEP_L1 = 0

@@ -332,17 +332,14 @@ int coarse_setup(tc_ctx* ctx, tc_dev& d)
     return 0;
 }
 
-int run_multigrid(tc_ctx* ctx, int32_t* q_out)
+// compute_RHS(1), compute_reference_residual(1), the viscosity hierarchy,
+// the coarse matrix factorisation/inverse (side stream) and the local time
+// steps: the set-up part of multigrid (iterative_solver.f90:79-100).
+int solve_setup(tc_ctx* ctx, tc_dev& d)
 {
-    tc_dev& d = DEV(ctx);
     const tc_params& P = ctx->p;
     const int G = ctx->grids;
     Level& L1 = ctx->lv[1];
-    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
-    // error(1,:,:) = SF
-    L1.cur = 0;
-    TC_TRY(copy_plane(ctx, L1.err[0], ctx->SF, L1.g));
-    // compute_RHS(1), compute_reference_residual(1)
     {
         RaCArr rc{};
         PtrArr cb{};
@@ -352,7 +349,6 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
     }
     TC_TRY(fetch_scalars(ctx, d, S_COUNT));
     ctx->res_ref[1] = d.h_scal[S_RESREF1] / (double)(L1.g.nx - 2) / (double)(L1.g.nz - 2);
-    // viscosity hierarchy, coarse matrix + LU (+ inverse), local time steps
     for (int L = 1; L <= G; ++L) TC_TRY(compute_viscosity(ctx, d, L));
     if (ctx->time == 0.0 || P.ivis > 0 || !d.coarse_ready) TC_TRY(coarse_setup(ctx, d));
     for (int L = 1; L <= G; ++L) {
@@ -360,6 +356,58 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
         launch_local_dt(ctx->stream, ctx->lv[L], e, P.courant_stream);
     }
     TC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_inv_ready, 0));
+    return 0;
+}
+
+// Stop test of the reference after a level-1 visit (iterative_solver.f90:
+// 130-145): reads the device sum of |residual(1)|.
+int stop_test(tc_ctx* ctx, tc_dev& d, int q, bool& done)
+{
+    const Level& L1 = ctx->lv[1];
+    TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+    ctx->residual_mag[1] = d.h_scal[S_RESID1] / (double)(L1.g.nx - 1) / (double)(L1.g.nz - 1);
+    // the reference's per-cycle line (iterative_solver.f90:135), TC_TRACE=1
+    if (d.trace) fprintf(stderr, "%5d %20.8g %20.8g\n", q, ctx->residual_mag[1], ctx->res_ref[1]);
+    if (ctx->residual_mag[1] > 1e10 * ctx->res_ref[1]) {
+        ctx->err = "Multigrid Convergence Problem: Abnormally High Residual Detected -- stopping";
+        return TC_ERR_RESID_BLOWUP;
+    }
+    done = ctx->residual_mag[1] / ctx->res_ref[1] <= ctx->p.tolerance;
+    return 0;
+}
+
+int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_out)
+{
+    // LU singularity check (DGBTRF INFO)
+    TC_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    const int32_t info = *ctx->h_flag;
+    if (info != 0 && status == 0) {
+        ctx->err = "DGBTRF: singular coarse-grid matrix";
+        status = TC_ERR_SINGULAR;
+    }
+    if (status == 0) TC_TRY(copy_plane(ctx, ctx->SF, psi, ctx->lv[1].g));
+    ctx->stats.q_last = q;
+    ctx->stats.residual_last = ctx->residual_mag[1];
+    ctx->stats.res_ref_last = ctx->res_ref[1];
+    if (q_out) *q_out = q;
+    return status;
+}
+
+int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out);
+
+int run_multigrid(tc_ctx* ctx, int32_t* q_out)
+{
+    if (ctx->mode != TC_MODE_FP64) return run_multigrid_dc(ctx, q_out);
+    tc_dev& d = DEV(ctx);
+    const tc_params& P = ctx->p;
+    const int G = ctx->grids;
+    Level& L1 = ctx->lv[1];
+    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
+    // error(1,:,:) = SF
+    L1.cur = 0;
+    TC_TRY(copy_plane(ctx, L1.err[0], ctx->SF, L1.g));
+    TC_TRY(solve_setup(ctx, d));
     bool first_sweep = true;
     const int* multi = P.multi;
     int q = 1;
@@ -390,16 +438,8 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
                 ctx->stats.coarse_solves++;
             }
             if (level == 1) {
-                TC_TRY(fetch_scalars(ctx, d, S_COUNT));
-                ctx->residual_mag[1] = d.h_scal[S_RESID1] / (double)(L1.g.nx - 1) / (double)(L1.g.nz - 1);
-                // the reference's per-cycle line (iterative_solver.f90:135), TC_TRACE=1
-                if (d.trace) fprintf(stderr, "%5d %20.8g %20.8g\n", q, ctx->residual_mag[1], ctx->res_ref[1]);
-                if (ctx->residual_mag[1] > 1e10 * ctx->res_ref[1]) {
-                    ctx->err = "Multigrid Convergence Problem: Abnormally High Residual Detected -- stopping";
-                    status = TC_ERR_RESID_BLOWUP;
-                    break;
-                }
-                if (ctx->residual_mag[1] / ctx->res_ref[1] <= P.tolerance) { done = true; break; }
+                status = stop_test(ctx, d, q, done);
+                if (status || done) break;
             }
             if (multi[p + 1] < multi[p]) {
                 Level& Lc = ctx->lv[level];
@@ -417,21 +457,181 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
             break;
         }
     }
-    // LU singularity check (DGBTRF INFO)
-    int32_t info = 0;
-    TC_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    TC_HIP(hipStreamSynchronize(ctx->stream));
-    info = *ctx->h_flag;
-    if (info != 0 && status == 0) {
-        ctx->err = "DGBTRF: singular coarse-grid matrix";
-        status = TC_ERR_SINGULAR;
+    return finish_solve(ctx, status, q, L1.err[L1.cur], q_out);
+}
+
+// ---------------------------------------------------------------------------
+// defect-correction multigrid (TC_MODE_MIXED, TC_MODE_DC64)
+// ---------------------------------------------------------------------------
+int falloc(tc_ctx* ctx, float** p, size_t n)
+{
+    TC_HIP(hipMalloc((void**)p, n * sizeof(float)));
+    TC_HIP(hipMemsetAsync(*p, 0, n * sizeof(float), ctx->stream));
+    return 0;
+}
+
+int dc_alloc(tc_ctx* ctx)
+{
+    if (ctx->psi_hi) return 0;
+    for (int j = 1; j <= ctx->grids; ++j) {
+        Level& L = ctx->lv[j];
+        float** a[] = {&L.err32[0], &L.err32[1], &L.rhs32, &L.res32, &L.vis32, &L.visx32, &L.visz32, &L.a132,
+                       &L.visxz32, &L.dts32, &L.lb32};
+        for (float** q : a) TC_TRY(falloc(ctx, q, L.g.elems));
     }
-    if (status == 0) TC_TRY(copy_plane(ctx, ctx->SF, L1.err[L1.cur], L1.g));
-    ctx->stats.q_last = q;
-    ctx->stats.residual_last = ctx->residual_mag[1];
-    ctx->stats.res_ref_last = ctx->res_ref[1];
-    if (q_out) *q_out = q;
-    return status;
+    TC_TRY(dalloc(ctx, &ctx->psi_hi, ctx->g.elems));
+    TC_TRY(dalloc(ctx, &ctx->psi_lo, ctx->g.elems));
+    return 0;
+}
+
+// Jacobi-Richardson sweeps of one visit on a correction level (FP32 or
+// FP64), followed (level > 1) by compute_residual of the level.
+int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
+{
+    Level& L = ctx->lv[level];
+    const tc_params& P = ctx->p;
+    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
+    const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
+    const int Niter = P.Nmulti;
+    // level 1 smooths the correction against the defect of psi
+    Level view = L;
+    if (level == 1) view.rhs = L.res;
+    const float* rhs32 = (level == 1) ? L.res32 : L.rhs32;
+    for (int n = 1; n <= Niter; ++n) {
+        const double ntemp = (double)(Niter - n + 1);
+        const double temp = std::cos((2.0 * ntemp - 1.0) * PII / (double)(2 * Niter));
+        double cconst = 1.0;
+        if (dir != 1) {
+            const double lb = -2.0, la = -1.0;
+            cconst = 2.0 / (-lb - la + (lb - la) * temp);
+        }
+        const bool timed = ctx->prof && level == 1;
+        if (timed) TC_TRY(prof_begin(ctx));
+        if (f32) {
+            launch_sweep32(ctx->stream, use_ratio, L, L.err32[L.cur32], L.err32[1 - L.cur32], rhs32, d.F, d.h[level],
+                           temp, cconst, zsg);
+            L.cur32 = 1 - L.cur32;
+        } else {
+            launch_sweep(ctx->stream, d.pow2[level], use_ratio, view, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level],
+                         P.ivis, temp, cconst, zsg);
+            L.cur = 1 - L.cur;
+        }
+        if (timed) TC_TRY(prof_end(ctx));
+    }
+    ctx->stats.sweeps_exec[level - 1] += Niter;
+    if (level > 1) {
+        if (f32) launch_residual32(ctx->stream, L, L.err32[L.cur32], L.res32, L.rhs32, d.F, d.h[level]);
+        else launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis, nullptr);
+        ctx->stats.residuals_exec[level - 1]++;
+    }
+    return 0;
+}
+
+// psi += e(1); e(1) = 0; residual(1) = RHS - L(psi) in double-double with
+// the |residual| block sums of the stop test.
+int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
+{
+    Level& L1 = ctx->lv[1];
+    const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
+    launch_dd_absorb(ctx->stream, ctx->psi_hi, ctx->psi_lo, f32 ? L1.err32[L1.cur32] : nullptr,
+                     f32 ? nullptr : L1.err[L1.cur], L1.g, zsg);
+    int nb = launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr,
+                                d.F, d.h[1], ctx->p.ivis, d.partialA);
+    launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
+    ctx->stats.residuals_exec[0]++;
+    return 0;
+}
+
+// The reference's cycle schedule applied to corrections: psi is a
+// double-double; level 1 smooths a zero-started correction e(1) against the
+// exact defect of psi and folds it into psi after each visit; the level-2
+// correction is prolonged straight into psi (and the defect recomputed)
+// before the level-1 post-smoothing; coarser levels start every descent
+// from a zero correction; the coarsest grid is solved in FP64.  In exact
+// arithmetic one cycle is the reference's cycle with zeroed coarse
+// corrections; the stop test reads the double-double residual of psi.
+int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
+{
+    tc_dev& d = DEV(ctx);
+    const tc_params& P = ctx->p;
+    const int G = ctx->grids;
+    const bool f32 = ctx->mode == TC_MODE_MIXED;
+    Level& L1 = ctx->lv[1];
+    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
+    TC_TRY(dc_alloc(ctx));
+    TC_TRY(copy_plane(ctx, ctx->psi_hi, ctx->SF, L1.g));
+    launch_bc_stream(ctx->stream, ctx->psi_hi, L1.g, P.Vbc);
+    TC_HIP(hipMemsetAsync(ctx->psi_lo, 0, plane_bytes(L1.g), ctx->stream));
+    TC_TRY(solve_setup(ctx, d));
+    if (f32)
+        for (int L = 1; L <= G; ++L) launch_coeffs_to_f32(ctx->stream, ctx->lv[L], L == ctx->ratio_level, P.ivis);
+    L1.cur = 0;
+    L1.cur32 = 0;
+    if (f32) TC_HIP(hipMemsetAsync(L1.err32[0], 0, L1.g.elems * sizeof(float), ctx->stream));
+    else TC_HIP(hipMemsetAsync(L1.err[0], 0, plane_bytes(L1.g), ctx->stream));
+    {
+        int nb = launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res,
+                                    f32 ? L1.res32 : nullptr, d.F, d.h[1], P.ivis, d.partialA);
+        (void)nb;
+    }
+    const int* multi = P.multi;
+    int q = 1;
+    int status = 0;
+    for (;;) {
+        bool done = false;
+        for (int p = 1; p <= P.nsteps; ++p) {
+            const int level = multi[p];
+            const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
+            Level& L = ctx->lv[level];
+            if (multi[p - 1] < multi[p]) {
+                Level& Lf = ctx->lv[level - 1];
+                if (f32) {
+                    launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
+                    TC_HIP(hipMemsetAsync(L.err32[L.cur32], 0, L.g.elems * sizeof(float), ctx->stream));
+                } else {
+                    launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
+                    TC_HIP(hipMemsetAsync(L.err[L.cur], 0, plane_bytes(L.g), ctx->stream));
+                }
+            }
+            if (level < G) {
+                TC_TRY(dc_iterate(ctx, d, level, dir, f32));
+            } else {
+                if (f32) launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg);
+                else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
+                ctx->stats.coarse_solves++;
+            }
+            if (level == 1) {
+                TC_TRY(dc_absorb_residual(ctx, d, f32));
+                status = stop_test(ctx, d, q, done);
+                if (status || done) break;
+            }
+            if (multi[p + 1] < multi[p]) {
+                Level& Lf = ctx->lv[level - 1];
+                if (level == 2) {
+                    // fold the coarse-grid correction into psi; level 1 then
+                    // post-smooths a zero correction against the new defect
+                    launch_prolong_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf.g, f32 ? L.err32[L.cur32] : nullptr,
+                                      f32 ? nullptr : L.err[L.cur], L.g, zsg);
+                    launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res,
+                                       f32 ? Lf.res32 : nullptr, d.F, d.h[1], P.ivis, nullptr);
+                    ctx->stats.residuals_exec[0]++;
+                } else if (f32) {
+                    launch_prolong32(ctx->stream, Lf.err32[Lf.cur32], Lf.g, L.err32[L.cur32], L.g, zsg);
+                } else {
+                    launch_prolong(ctx->stream, Lf.err[Lf.cur], Lf.g, L.err[L.cur], L.g, zsg);
+                }
+            }
+        }
+        ctx->stats.mg_cycles++;
+        if (status || done) break;
+        q++;
+        if (q > P.max_cycles) {
+            ctx->err = "Multigrid Convergence Problem: Too Many Multigrid Cycles Needed";
+            status = TC_ERR_MAX_CYCLES;
+            break;
+        }
+    }
+    return finish_solve(ctx, status, q, ctx->psi_hi, q_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,11 +947,14 @@ void free_ctx(tc_ctx* ctx)
         double* a[] = {l.err[0], l.err[1], l.rhs, l.vis, l.visx, l.visz, l.visxx, l.viszz, l.visxz, l.dtstream, l.ratio,
                        l.dtvis, l.visf, l.visxf, l.viszf, l.visxxf, l.viszzf, l.visxzf, l.vtmp, l.res};
         for (double* p : a) F(p);
+        float* a32[] = {l.err32[0], l.err32[1], l.rhs32, l.res32, l.vis32, l.visx32, l.visz32, l.a132, l.visxz32,
+                        l.dts32, l.lb32};
+        for (float* p : a32) F(p);
     }
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
                    ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
-                   ctx->str1, ctx->rtr2, ctx->str2};
+                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo};
     for (double* p : b) F(p);
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
@@ -951,7 +1154,7 @@ const char* tc_last_error(const tc_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 int tc_set_mode(tc_ctx* ctx, int mode)
 {
     if (!ctx) return TC_ERR_ARG;
-    if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED) return TC_ERR_ARG;
+    if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED && mode != TC_MODE_DC64) return TC_ERR_ARG;
     ctx->mode = mode;
     return 0;
 }
@@ -1035,6 +1238,11 @@ int tc_get_field(tc_ctx* ctx, int field, void* host)
     case TC_F_STRAIN: dptr = ctx->strain; break; case TC_F_VIS: dptr = ctx->lv[1].vis; break;
     case TC_F_CONDUCT: dptr = ctx->conduct; break; case TC_F_T0: dptr = ctx->T0; break; case TC_F_T1: dptr = ctx->T1; break;
     case TC_F_T2: dptr = ctx->T2; break;
+    case TC_F_RHS: dptr = ctx->lv[1].rhs; break;
+    case TC_F_SF_LO:
+        if (!ctx->psi_lo) { ctx->err = "no defect-correction solve has run"; return TC_ERR_ARG; }
+        dptr = ctx->psi_lo;
+        break;
     default: break;
     }
     if (dptr) {

@@ -1,0 +1,447 @@
+// Defect-correction multigrid (TC_MODE_MIXED and TC_MODE_DC64).
+//
+// At 2048x1024 the FP64 rounding of the fourth differences of an FP64
+// streamfunction leaves a relative residual of ~8e-8, above the 1e-8
+// tolerance.  The defect-correction form keeps psi as a double-double
+// (hi + lo), evaluates the fine-grid residual RHS - L(psi) with error-free
+// transformations, and computes each correction with one W-cycle of the
+// reference's schedule in FP32 (MIXED) or FP64 (DC64) started from zero.  In
+// exact arithmetic a cycle is the same affine map as the reference's cycle
+// with zeroed coarse corrections; only the rounding differs.  The coarsest
+// grid is solved in FP64 in both forms.
+#include "tc_kernels.h"
+
+namespace tc {
+
+// ---- double-double helpers (Knuth TwoSum, FMA TwoProd) ---------------------
+struct dd {
+    double hi, lo;
+};
+__device__ __forceinline__ dd two_sum(double a, double b)
+{
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+__device__ __forceinline__ dd quick_two_sum(double a, double b)
+{
+    double s = a + b;
+    return {s, b - (s - a)};
+}
+__device__ __forceinline__ dd dd_add(dd a, dd b)
+{
+    dd s = two_sum(a.hi, b.hi);
+    dd t = two_sum(a.lo, b.lo);
+    s.lo += t.hi;
+    s = quick_two_sum(s.hi, s.lo);
+    s.lo += t.lo;
+    return quick_two_sum(s.hi, s.lo);
+}
+__device__ __forceinline__ dd dd_mul_d(dd a, double b)
+{
+    double p = a.hi * b;
+    double e = fma(a.hi, b, -p);
+    e = fma(a.lo, b, e);
+    return quick_two_sum(p, e);
+}
+__device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+// a / b for a double b (one Newton correction)
+__device__ __forceinline__ dd dd_div_d(dd a, double b)
+{
+    double q1 = a.hi / b;
+    dd p = {q1 * b, fma(q1, b, -(q1 * b))};
+    dd r = dd_add(a, dd_neg(p));
+    double q2 = r.hi / b;
+    return quick_two_sum(q1, q2);
+}
+
+// sum_j D(j) S(c + j*st) in double-double, S = hi + lo
+__device__ __forceinline__ dd dd_dot(const double* D, int j0, int j1, const double* __restrict__ hi,
+                                     const double* __restrict__ lo, size_t c, ptrdiff_t st)
+{
+    dd acc = {0.0, 0.0};
+    for (int j = j0; j <= j1; ++j) {
+        const double w = D[j + 2];
+        if (w == 0.0) continue;
+        const size_t q = c + j * st;
+        acc = dd_add(acc, dd_mul_d({hi[q], lo[q]}, w));
+    }
+    return acc;
+}
+__device__ __forceinline__ dd dd_sum9(const double* M, const double* __restrict__ hi, const double* __restrict__ lo,
+                                      size_t c, ptrdiff_t ld)
+{
+    dd acc = {0.0, 0.0};
+    for (int dk = -1; dk <= 1; ++dk)
+        for (int di = -1; di <= 1; ++di) {
+            const double w = M[(dk + 1) * 3 + (di + 1)];
+            if (w == 0.0) continue;
+            const size_t q = c + dk * ld + di;
+            acc = dd_add(acc, dd_mul_d({hi[q], lo[q]}, w));
+        }
+    return acc;
+}
+
+// Fine-grid residual r = RHS - L(hi+lo) in double-double; writes r (FP64)
+// and its FP32 rounding, and per-block sums of |r|.
+__global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ hi, const double* __restrict__ lo,
+                                                     const double* __restrict__ vis, const double* __restrict__ vx,
+                                                     const double* __restrict__ vz, const double* __restrict__ vxx,
+                                                     const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                                     const double* __restrict__ rhs, double* __restrict__ r64,
+                                                     float* __restrict__ r32, Grid g, FD F, Spacing h, int ivis,
+                                                     double* __restrict__ partial)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        const ptrdiff_t ld = g.ld;
+        dd Srr = dd_div_d(dd_dot(F.D2, -1, 1, hi, lo, c, 1), h.dr2);
+        dd Sss = dd_div_d(dd_dot(F.D2, -1, 1, hi, lo, c, ld), h.ds2);
+        dd Srrr = dd_div_d(dd_dot(F.D3, -2, 2, hi, lo, c, 1), h.dr3);
+        dd Ssss = dd_div_d(dd_dot(F.D3, -2, 2, hi, lo, c, ld), h.ds3);
+        dd Srrrr = dd_div_d(dd_dot(F.D4, -2, 2, hi, lo, c, 1), h.dr4);
+        dd Sssss = dd_div_d(dd_dot(F.D4, -2, 2, hi, lo, c, ld), h.ds4);
+        dd Srs = dd_div_d(dd_sum9(F.Drs, hi, lo, c, ld), h.drds);
+        dd Srss = dd_div_d(dd_sum9(F.Drss, hi, lo, c, ld), h.drds2);
+        dd Srrs = dd_div_d(dd_sum9(F.Drrs, hi, lo, c, ld), h.dr2ds);
+        dd Srrss = dd_div_d(dd_sum9(F.Drrss, hi, lo, c, ld), h.dr2ds2);
+        const double v = vis[c];
+        dd sum = dd_mul_d(dd_add(Srrrr, Sssss), v);
+        sum = dd_add(sum, dd_mul_d(Srrss, 2.0 * v));
+        if (ivis != 0) {
+            sum = dd_add(sum, dd_mul_d(dd_add(Srr, dd_neg(Sss)), vxx[c] - vzz[c]));
+            sum = dd_add(sum, dd_mul_d(Srs, 4.0 * vxz[c]));
+            sum = dd_add(sum, dd_mul_d(dd_add(Srrr, Srss), 2.0 * vx[c]));
+            sum = dd_add(sum, dd_mul_d(dd_add(Ssss, Srrs), 2.0 * vz[c]));
+        }
+        dd r = dd_add({rhs[c], 0.0}, dd_neg(sum));
+        const double rv = r.hi + r.lo;
+        r64[c] = rv;
+        if (r32) r32[c] = (float)rv;
+        a = fabs(rv);
+    }
+    if (partial) {
+        a = block_sum(a);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// psi(hi,lo) += e on the interior with the enforceBCs_stream ghost mirrors;
+// the correction e is cleared at the same points (its walls stay zero), so
+// the next visit of level 1 starts from a zero correction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dd_absorb(double* __restrict__ hi, double* __restrict__ lo, T* __restrict__ e,
+                                                   Grid g, double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    dd s = dd_add({hi[c], lo[c]}, {(double)e[c], 0.0});
+    hi[c] = s.hi;
+    lo[c] = s.lo;
+    stream_ghosts(hi, g, i, k, s.hi, zsg);
+    stream_ghosts(lo, g, i, k, s.lo, zsg);
+    e[c] = T(0);
+    stream_ghosts(e, g, i, k, T(0), T(zsg));
+}
+
+// psi(hi,lo) += bilinear prolongation of the level-2 correction
+// (prolong, iterative_solver.f90:175-233) with the ghost mirrors.  Folding
+// the coarse-grid correction into psi before the level-1 post-smoothing keeps
+// the smooth part out of the FP32 level-1 correction, whose rounding would
+// otherwise leave a high-frequency defect proportional to its magnitude.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong_dd(double* __restrict__ hi, double* __restrict__ lo, Grid gf,
+                                                    const T* __restrict__ Ec, Grid gc, double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > gf.nx - 1 || k > gf.nz - 1) return;
+    const size_t c = gf.at(i, k);
+    double add;
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) add = (double)Ec[gc.at(i / 2, k / 2)];
+    else if (ie) add = ((double)Ec[gc.at(i / 2, (k - 1) / 2)] + (double)Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
+    else if (ke) add = ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
+    else
+        add = ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
+               (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+    dd s = dd_add({hi[c], lo[c]}, {add, 0.0});
+    hi[c] = s.hi;
+    lo[c] = s.lo;
+    stream_ghosts(hi, gf, i, k, s.hi, zsg);
+    stream_ghosts(lo, gf, i, k, s.lo, zsg);
+}
+
+// ---- FP32 inner cycle --------------------------------------------------------
+struct FD32 {
+    float D1[5], D2[5], D3[5], D4[5];
+    float Drs[9], Drss[9], Drrs[9], Drrss[9];
+};
+struct H32 {
+    float idr, ids, idr2, ids2, idr3, ids3, idr4, ids4, idrds, idrds2, idr2ds, idr2ds2;
+};
+
+// FP32 fields and weights; A is the arithmetic type.  With A = double every
+// product of two FP32 values is exact and only the sums round, which keeps
+// the residual passed to the next level accurate relative to the (small)
+// defect even where rhs and L(S) nearly cancel.
+#pragma clang fp contract(fast)
+template <typename A>
+__device__ __forceinline__ A dot3f(const float* D, const float* __restrict__ f, size_t c, ptrdiff_t st)
+{
+    return A(D[1]) * A(f[c - st]) + A(D[2]) * A(f[c]) + A(D[3]) * A(f[c + st]);
+}
+template <typename A>
+__device__ __forceinline__ A dot5f(const float* D, const float* __restrict__ f, size_t c, ptrdiff_t st)
+{
+    return A(D[0]) * A(f[c - 2 * st]) + A(D[1]) * A(f[c - st]) + A(D[2]) * A(f[c]) + A(D[3]) * A(f[c + st]) +
+           A(D[4]) * A(f[c + 2 * st]);
+}
+template <typename A>
+__device__ __forceinline__ A sum9f(const float* M, const float* __restrict__ f, size_t c, ptrdiff_t ld)
+{
+    return A(M[0]) * A(f[c - ld - 1]) + A(M[1]) * A(f[c - ld]) + A(M[2]) * A(f[c - ld + 1]) + A(M[3]) * A(f[c - 1]) +
+           A(M[4]) * A(f[c]) + A(M[5]) * A(f[c + 1]) + A(M[6]) * A(f[c + ld - 1]) + A(M[7]) * A(f[c + ld]) +
+           A(M[8]) * A(f[c + ld + 1]);
+}
+// rhs - L(S); a1 = vis_xx - vis_zz
+template <typename A>
+__device__ __forceinline__ A stream_space32(const FD32& F, const H32& h, const float* __restrict__ S, size_t c,
+                                            ptrdiff_t ld, float vis, float vx, float vz, float a1, float vxz,
+                                            float rhs)
+{
+    A Srr = dot3f<A>(F.D2, S, c, 1) * A(h.idr2);
+    A Sss = dot3f<A>(F.D2, S, c, ld) * A(h.ids2);
+    A Srrr = dot5f<A>(F.D3, S, c, 1) * A(h.idr3);
+    A Ssss = dot5f<A>(F.D3, S, c, ld) * A(h.ids3);
+    A Srrrr = dot5f<A>(F.D4, S, c, 1) * A(h.idr4);
+    A Sssss = dot5f<A>(F.D4, S, c, ld) * A(h.ids4);
+    A Srs = sum9f<A>(F.Drs, S, c, ld) * A(h.idrds);
+    A Srss = sum9f<A>(F.Drss, S, c, ld) * A(h.idrds2);
+    A Srrs = sum9f<A>(F.Drrs, S, c, ld) * A(h.idr2ds);
+    A Srrss = sum9f<A>(F.Drrss, S, c, ld) * A(h.idr2ds2);
+    A L = A(a1) * (Srr - Sss) + A(vis) * (Srrrr + Sssss) + A(2) * A(vis) * Srrss + A(4) * A(vxz) * Srs +
+          A(2) * A(vx) * (Srrr + Srss) + A(2) * A(vz) * (Ssss + Srrs);
+    return A(rhs) - L;
+}
+
+template <bool RATIO>
+__global__ __launch_bounds__(256) void k_sweep32(const float* __restrict__ S0, float* __restrict__ E,
+                                                 const float* __restrict__ vis, const float* __restrict__ vx,
+                                                 const float* __restrict__ vz, const float* __restrict__ a1,
+                                                 const float* __restrict__ vxz, const float* __restrict__ rhs,
+                                                 const float* __restrict__ dts, const float* __restrict__ lb32, Grid g,
+                                                 FD32 F, H32 h, float temp, float cconst, float zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    float t = stream_space32<float>(F, h, S0, c, g.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);
+    float cc = cconst;
+    if constexpr (RATIO) {
+        const float lb = lb32[c];
+        cc = 2.0f / (-lb + 1.0f + (lb + 1.0f) * temp);
+    }
+    float v = S0[c] + cc * dts[c] * t;
+    E[c] = v;
+    stream_ghosts(E, g, i, k, v, zsg);
+}
+
+__global__ __launch_bounds__(256) void k_residual32(const float* __restrict__ S, float* __restrict__ R,
+                                                    const float* __restrict__ vis, const float* __restrict__ vx,
+                                                    const float* __restrict__ vz, const float* __restrict__ a1,
+                                                    const float* __restrict__ vxz, const float* __restrict__ rhs,
+                                                    Grid g, FD32 F, H32 h)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    R[c] = (float)stream_space32<double>(F, h, S, c, g.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);
+}
+
+__global__ __launch_bounds__(256) void k_restrict32(const float* __restrict__ in, Grid gi, float* __restrict__ out,
+                                                    Grid go)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > go.nx - 1 || k > go.nz - 1) return;
+    const int ii = 2 * i, kk = 2 * k;
+    float corners = (in[gi.at(ii - 1, kk - 1)] + in[gi.at(ii - 1, kk + 1)] + in[gi.at(ii + 1, kk + 1)] +
+                     in[gi.at(ii - 1, kk + 1)]) * (1.0f / 16.0f);
+    float sides = (in[gi.at(ii - 1, kk)] + in[gi.at(ii, kk - 1)] + in[gi.at(ii + 1, kk)] + in[gi.at(ii, kk + 1)]) *
+                  (1.0f / 8.0f);
+    out[go.at(i, k)] = corners + sides + in[gi.at(ii, kk)] * 0.25f;
+}
+
+__global__ __launch_bounds__(256) void k_prolong32(float* __restrict__ Ef, Grid gf, const float* __restrict__ Ec,
+                                                   Grid gc, float zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > gf.nx - 1 || k > gf.nz - 1) return;
+    const size_t c = gf.at(i, k);
+    float add;
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) add = Ec[gc.at(i / 2, k / 2)];
+    else if (ie) add = (Ec[gc.at(i / 2, (k - 1) / 2)] + Ec[gc.at(i / 2, (k + 1) / 2)]) * 0.5f;
+    else if (ke) add = (Ec[gc.at((i - 1) / 2, k / 2)] + Ec[gc.at((i + 1) / 2, k / 2)]) * 0.5f;
+    else
+        add = (Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
+               Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) * 0.25f;
+    float v = Ef[c] + add;
+    Ef[c] = v;
+    stream_ghosts(Ef, gf, i, k, v, zsg);
+}
+#pragma clang fp contract(off)
+
+// Coarse solve from an FP32 right-hand side with the FP64 inverse, result
+// rounded to FP32.
+__global__ __launch_bounds__(256) void k_coarse_apply32(const double* __restrict__ Ainv, const double* __restrict__ RowB,
+                                                        const float* __restrict__ rhs, float* __restrict__ E, Grid g,
+                                                        int n, float zsg)
+{
+    const int nxc = g.nx, nzc = g.nz, N = nzc + 1;
+    const int ip = blockIdx.x + 1, kp = blockIdx.y + 1;
+    const double* row = Ainv + (size_t)(N * ip + kp) * (size_t)n;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int kq = q % N, iq = q / N;
+        if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) acc += row[q] * ((double)rhs[g.at(iq, kq)] * RowB[q]);
+    }
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) {
+        const float v = (float)acc;
+        E[g.at(ip, kp)] = v;
+        stream_ghosts(E, g, ip, kp, v, zsg);
+    }
+}
+
+// FP64 -> FP32 coefficient copies (a1 = vis_xx - vis_zz; lb from the ratio
+// as in iterate_stream, iterative_solver.f90:880-884).
+__global__ __launch_bounds__(256) void k_coeffs_to_f32(const double* __restrict__ vis, const double* __restrict__ vx,
+                                                       const double* __restrict__ vz, const double* __restrict__ vxx,
+                                                       const double* __restrict__ vzz, const double* __restrict__ vxz,
+                                                       const double* __restrict__ dts, const double* __restrict__ ratio,
+                                                       float* __restrict__ vis32, float* __restrict__ vx32,
+                                                       float* __restrict__ vz32, float* __restrict__ a132,
+                                                       float* __restrict__ vxz32, float* __restrict__ dts32,
+                                                       float* __restrict__ lb32, Grid g, int ivis)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    vis32[c] = (float)vis[c];
+    vx32[c] = ivis ? (float)vx[c] : 0.0f;
+    vz32[c] = ivis ? (float)vz[c] : 0.0f;
+    a132[c] = ivis ? (float)(vxx[c] - vzz[c]) : 0.0f;
+    vxz32[c] = ivis ? (float)vxz[c] : 0.0f;
+    dts32[c] = (float)dts[c];
+    if (lb32) {
+        const double r = ratio[c];
+        lb32[c] = (float)((r > 1e99) ? -2.0 : -2.0 * (r * r) / (1.0 + r * r));
+    }
+}
+
+// ---- launchers -------------------------------------------------------------
+static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+FD32 fd32_from(const FD& F)
+{
+    FD32 o;
+    for (int j = 0; j < 5; ++j) { o.D1[j] = (float)F.D1[j]; o.D2[j] = (float)F.D2[j]; o.D3[j] = (float)F.D3[j]; o.D4[j] = (float)F.D4[j]; }
+    for (int j = 0; j < 9; ++j) { o.Drs[j] = (float)F.Drs[j]; o.Drss[j] = (float)F.Drss[j]; o.Drrs[j] = (float)F.Drrs[j]; o.Drrss[j] = (float)F.Drrss[j]; }
+    return o;
+}
+H32 h32_from(const Spacing& h)
+{
+    return H32{(float)h.idr, (float)h.ids, (float)h.idr2, (float)h.ids2, (float)h.idr3, (float)h.ids3,
+               (float)h.idr4, (float)h.ids4, (float)h.idrds, (float)h.idrds2, (float)h.idr2ds, (float)h.idr2ds2};
+}
+
+int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
+                       double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    hipLaunchKernelGGL(k_residual_dd, gr, b, 0, st, hi, lo, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz, rhs, r64,
+                       r32, L.g, F, h, ivis, partial);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_dd_absorb(hipStream_t st, double* hi, double* lo, float* e32, double* e64, const Grid& g, double zsg)
+{
+    dim3 b(64, 4);
+    if (e32) hipLaunchKernelGGL((k_dd_absorb<float>), grid2(g.nx - 1, g.nz - 1, b), b, 0, st, hi, lo, e32, g, zsg);
+    else hipLaunchKernelGGL((k_dd_absorb<double>), grid2(g.nx - 1, g.nz - 1, b), b, 0, st, hi, lo, e64, g, zsg);
+}
+
+void launch_prolong_dd(hipStream_t st, double* hi, double* lo, const Grid& gf, const float* e32, const double* e64,
+                       const Grid& gc, double zsg)
+{
+    dim3 b(64, 4);
+    if (e32)
+        hipLaunchKernelGGL((k_prolong_dd<float>), grid2(gf.nx - 1, gf.nz - 1, b), b, 0, st, hi, lo, gf, e32, gc, zsg);
+    else
+        hipLaunchKernelGGL((k_prolong_dd<double>), grid2(gf.nx - 1, gf.nz - 1, b), b, 0, st, hi, lo, gf, e64, gc, zsg);
+}
+
+void launch_sweep32(hipStream_t st, bool use_ratio, const Level& L, const float* S0, float* E, const float* rhs,
+                    const FD& F, const Spacing& h, double temp, double cconst, double zsg)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    const FD32 f = fd32_from(F);
+    const H32 hh = h32_from(h);
+    if (use_ratio)
+        hipLaunchKernelGGL((k_sweep32<true>), gr, b, 0, st, S0, E, L.vis32, L.visx32, L.visz32, L.a132, L.visxz32, rhs,
+                           L.dts32, L.lb32, L.g, f, hh, (float)temp, (float)cconst, (float)zsg);
+    else
+        hipLaunchKernelGGL((k_sweep32<false>), gr, b, 0, st, S0, E, L.vis32, L.visx32, L.visz32, L.a132, L.visxz32, rhs,
+                           L.dts32, L.lb32, L.g, f, hh, (float)temp, (float)cconst, (float)zsg);
+}
+
+void launch_residual32(hipStream_t st, const Level& L, const float* S, float* R, const float* rhs, const FD& F,
+                       const Spacing& h)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_residual32, grid2(L.g.nx - 1, L.g.nz - 1, b), b, 0, st, S, R, L.vis32, L.visx32, L.visz32,
+                       L.a132, L.visxz32, rhs, L.g, fd32_from(F), h32_from(h));
+}
+
+void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* out, const Grid& go)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_restrict32, grid2(go.nx - 1, go.nz - 1, b), b, 0, st, in, gi, out, go);
+}
+
+void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_prolong32, grid2(gf.nx - 1, gf.nz - 1, b), b, 0, st, Ef, gf, Ec, gc, (float)zsg);
+}
+
+void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
+                           float* E, int ngc, double zsg)
+{
+    hipLaunchKernelGGL(k_coarse_apply32, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, RowB, rhs, E, L.g, ngc,
+                       (float)zsg);
+}
+
+void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_coeffs_to_f32, grid2(L.g.nx - 1, L.g.nz - 1, b), b, 0, st, L.vis, L.visx, L.visz, L.visxx,
+                       L.viszz, L.visxz, L.dtstream, L.ratio, L.vis32, L.visx32, L.visz32, L.a132, L.visxz32, L.dts32,
+                       with_lb ? L.lb32 : nullptr, L.g, ivis);
+}
+
+}  // namespace tc

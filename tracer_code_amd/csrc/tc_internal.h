@@ -56,7 +56,7 @@ struct Level {
     // FP32 copies for the mixed-precision path
     float *err32[2] = {nullptr, nullptr};
     float *rhs32 = nullptr, *vis32 = nullptr, *visx32 = nullptr, *visz32 = nullptr, *a132 = nullptr, *visxz32 = nullptr;
-    float *dts32 = nullptr, *lb32 = nullptr;
+    float *dts32 = nullptr, *lb32 = nullptr, *res32 = nullptr;
     int cur32 = 0;
 };
 
@@ -84,6 +84,7 @@ struct tc_ctx {
     double *conduct = nullptr, *conduct_r = nullptr, *conduct_s = nullptr, *dts = nullptr;
     double *DERr_u = nullptr, *DERr_w = nullptr, *T0 = nullptr, *T1 = nullptr, *T2 = nullptr;
     double *residual = nullptr, *tmp = nullptr, *tmp2 = nullptr;
+    double *psi_hi = nullptr, *psi_lo = nullptr;  // double-double streamfunction of the defect-correction modes
     double *Cnew[TC_MAX_TYPES] = {}, *Cvis[TC_MAX_TYPES] = {}, *Cbuoy[TC_MAX_TYPES] = {}, *C0[TC_MAX_TYPES] = {};
     int32_t *count[TC_MAX_TYPES + 1] = {}, *nempty = nullptr;
     double* zg = nullptr;  // zg(-3:nz+3)

@@ -84,6 +84,22 @@ void launch_coarse_lusolve(hipStream_t st, const double* ab, int ldab, int n, in
 void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,
                          double zsg);
 
+// --- tc_dc.hip ------------------------------------------------------------------
+int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
+                       double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial);
+void launch_dd_absorb(hipStream_t st, double* hi, double* lo, float* e32, double* e64, const Grid& g, double zsg);
+void launch_prolong_dd(hipStream_t st, double* hi, double* lo, const Grid& gf, const float* e32, const double* e64,
+                       const Grid& gc, double zsg);
+void launch_sweep32(hipStream_t st, bool use_ratio, const Level& L, const float* S0, float* E, const float* rhs,
+                    const FD& F, const Spacing& h, double temp, double cconst, double zsg);
+void launch_residual32(hipStream_t st, const Level& L, const float* S, float* R, const float* rhs, const FD& F,
+                       const Spacing& h);
+void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* out, const Grid& go);
+void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg);
+void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
+                           float* E, int ngc, double zsg);
+void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
+
 // --- tc_fields.hip -------------------------------------------------------------
 void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,
                        double ds, int Vbc);

@@ -31,7 +31,7 @@ _FIELD = {
     "Cnew": _abi.TC_F_CNEW, "Cvis": _abi.TC_F_CVIS, "Cbuoy": _abi.TC_F_CBUOY, "u": _abi.TC_F_U,
     "w": _abi.TC_F_W, "strain": _abi.TC_F_STRAIN, "vis": _abi.TC_F_VIS, "conduct": _abi.TC_F_CONDUCT,
     "nempty": _abi.TC_F_NEMPTY, "count_type": _abi.TC_F_COUNT_TYPE, "T0": _abi.TC_F_T0, "T1": _abi.TC_F_T1,
-    "T2": _abi.TC_F_T2,
+    "T2": _abi.TC_F_T2, "SF_lo": _abi.TC_F_SF_LO, "RHS": _abi.TC_F_RHS,
 }
 _TRACERS = {"x": _abi.TC_TR_X, "x0": _abi.TC_TR_X0, "x1": _abi.TC_TR_X1, "x2": _abi.TC_TR_X2}
 
@@ -125,7 +125,13 @@ class Solver:
     def set_time(self, time, dt=0.0):
         self._check(self.lib.tc_set_time(self.h, float(time), float(dt)), "set_time")
 
+    MODES = {"fp64": _abi.TC_MODE_FP64, "mixed": _abi.TC_MODE_MIXED, "dc64": _abi.TC_MODE_DC64}
+
     def set_mode(self, mode):
+        """'fp64' (the reference's cycle), 'mixed' or 'dc64' (defect correction
+        with FP32 or FP64 corrections), or the TC_MODE_* integer."""
+        if isinstance(mode, str):
+            mode = self.MODES[mode]
         self._check(self.lib.tc_set_mode(self.h, int(mode)), "set_mode")
 
     def stats(self):
