@@ -16,6 +16,9 @@ OUT = os.path.join(HERE, "libtracer_hip.so")
 SOURCES = ["tc_api.hip", "tc_mg.hip", "tc_coarse.hip", "tc_fields.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "_build")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",
@@ -40,5 +43,30 @@ def build(verbose=False):
     return OUT
 
 
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module and its two programs with flang
+    (tc_layout: struct layout check, no GPU; tc_solve_demo: a Fortran caller
+    of the solver).  They find libtracer_hip.so through an $ORIGIN rpath.
+    Returns None when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    mod_o = os.path.join(FBUILD, "tracer_hip.o")
+    link = ["-L", HERE, "-ltracer_hip", "-Wl,-rpath,$ORIGIN/../.."]
+    cmds = [[FLANG, "-O2", "-c", "-J", FBUILD, os.path.join(FDIR, "tracer_hip.f90"), "-o", mod_o]]
+    for prog in ("tc_layout", "tc_solve_demo"):
+        cmds.append([FLANG, "-O2", "-I", FBUILD, os.path.join(FDIR, prog + ".f90"), mod_o] + link +
+                    ["-o", os.path.join(FBUILD, prog)])
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed building the Fortran interface")
+    return FBUILD
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))
