@@ -63,6 +63,10 @@ struct tc_dev {
     bool coarse_ready = false;
     bool trace = false;
     bool coarse_lu = false;
+    bool tb = true;            // temporally blocked sweeps on levels >= 2
+    bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
+    int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
+    int32_t* h_ctl = nullptr;  // pinned mirror
     std::vector<double> hbuf;  // host staging plane
 };
 
@@ -279,40 +283,87 @@ int compute_viscosity(tc_ctx* ctx, tc_dev& d, int level)
 // ---------------------------------------------------------------------------
 // multigrid (iterative_solver.f90:79-173), Newtonian branch
 // ---------------------------------------------------------------------------
-int iterate_stream(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, int p, bool& first_sweep)
+// Chebyshev factors of sweep n of a visit (iterative_solver.f90:873-899):
+// temp for the per-point ratio form, cconst for the scalar form (c=1 on
+// the upward leg, lb=-2 / la=-1 otherwise).
+void cheb_factors(int n, int Niter, int dir, double& temp, double& cconst)
+{
+    const double ntemp = (double)(Niter - n + 1);
+    temp = std::cos((2.0 * ntemp - 1.0) * PII / (double)(2 * Niter));
+    cconst = 1.0;
+    if (dir != 1) {
+        const double lb = -2.0, la = -1.0;
+        cconst = 2.0 / (-lb - la + (lb - la) * temp);
+    }
+}
+
+// The Niter sweeps of one visit of `level` on its error buffers (FP64
+// err[] or FP32 err32[]).  view.rhs is the FP64 right-hand side, rhs32 the
+// FP32 one.  In the defect-correction modes the levels >= 2 run several
+// sweeps per launch (k_sweep_tb, the 13-tap form of the operator) unless
+// TC_TB=0; the reference-form path and level 1 run one sweep per launch.
+int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, const Level& view, const float* rhs32,
+               bool* first_sweep)
 {
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
     const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
     const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
-    for (int n = 1; n <= Niter; ++n) {
-        const double ntemp = (double)(Niter - n + 1);
-        const double temp = std::cos((2.0 * ntemp - 1.0) * PII / (double)(2 * Niter));
-        double cconst = 1.0;
-        if (dir != 1) {
-            const double lb = -2.0, la = -1.0;
-            cconst = 2.0 / (-lb - la + (lb - la) * temp);
+    const bool tb = d.tb && level >= 2 && ctx->mode != TC_MODE_FP64;
+    int n = 1;
+    while (n <= Niter) {
+        if (tb) {
+            const int nk = std::min(tb_max_sweeps(), Niter - n + 1);
+            double temp[8], cc[8];
+            for (int s = 0; s < nk; ++s) cheb_factors(n + s, Niter, dir, temp[s], cc[s]);
+            if (f32) {
+                launch_sweep_tb32(ctx->stream, use_ratio, L, rhs32, L.err32[L.cur32], L.err32[1 - L.cur32], d.F,
+                                  d.h[level], temp, cc, nk, zsg);
+                L.cur32 = 1 - L.cur32;
+            } else {
+                launch_sweep_tb64(ctx->stream, use_ratio, L, view.rhs, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level],
+                                  temp, cc, nk, zsg);
+                L.cur = 1 - L.cur;
+            }
+            n += nk;
+            continue;
         }
+        double temp, cconst;
+        cheb_factors(n, Niter, dir, temp, cconst);
         const bool timed = ctx->prof && level == 1;
         if (timed) TC_TRY(prof_begin(ctx));
-        launch_sweep(ctx->stream, d.pow2[level], use_ratio, L, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level], P.ivis,
-                     temp, cconst, zsg);
+        if (f32) {
+            launch_sweep32(ctx->stream, use_ratio, L, L.err32[L.cur32], L.err32[1 - L.cur32], rhs32, d.F, d.h[level],
+                           temp, cconst, zsg);
+            L.cur32 = 1 - L.cur32;
+        } else {
+            launch_sweep(ctx->stream, d.pow2[level], use_ratio, view, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level],
+                         P.ivis, temp, cconst, zsg);
+            L.cur = 1 - L.cur;
+        }
         if (timed) TC_TRY(prof_end(ctx));
-        L.cur = 1 - L.cur;
-        if (first_sweep) {
+        if (first_sweep && *first_sweep) {
             // error(1) started as a copy of SF; from the second sweep on the
             // reference reads walls zeroed by enforceBCs_stream
             launch_bc_stream(ctx->stream, L.err[1 - L.cur], L.g, P.Vbc);
-            first_sweep = false;
+            *first_sweep = false;
         }
+        ++n;
     }
     ctx->stats.sweeps_exec[level - 1] += Niter;
+    return 0;
+}
+
+int iterate_stream(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool& first_sweep)
+{
+    Level& L = ctx->lv[level];
+    const tc_params& P = ctx->p;
+    TC_TRY(run_sweeps(ctx, d, level, Niter, dir, false, L, nullptr, &first_sweep));
     // compute_residual
     int nb = launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis,
                              level == 1 ? d.partialA : nullptr);
     ctx->stats.residuals_exec[level - 1]++;
     if (level == 1) launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
-    (void)p;
     return 0;
 }
 
@@ -326,7 +377,7 @@ int coarse_setup(tc_ctx* ctx, tc_dev& d)
     launch_coarse_assemble(ctx->side, ctx->Mc, ctx->RowB, L, d.F, d.h[ctx->grids], P.ivis, P.Vbc, ctx->kl, ctx->ku,
                            ctx->ngc);
     launch_dgbtf2(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->d_info);
-    launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->Ainv);
+    launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->Ainv, ctx->Ur);
     TC_HIP(hipEventRecord(ctx->ev_inv_ready, ctx->side));
     d.coarse_ready = true;
     return 0;
@@ -355,7 +406,18 @@ int solve_setup(tc_ctx* ctx, tc_dev& d)
         EigScalars e = eig_scalars(P, ctx->lv[L]);
         launch_local_dt(ctx->stream, ctx->lv[L], e, P.courant_stream);
     }
-    TC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_inv_ready, 0));
+    d.inv_pending = true;
+    return 0;
+}
+
+// The coarse factorisation runs on the side stream while the first descent
+// of the cycle smooths the finer levels; the first coarse solve waits for it.
+int wait_coarse(tc_ctx* ctx, tc_dev& d)
+{
+    if (d.inv_pending) {
+        TC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_inv_ready, 0));
+        d.inv_pending = false;
+    }
     return 0;
 }
 
@@ -378,6 +440,7 @@ int stop_test(tc_ctx* ctx, tc_dev& d, int q, bool& done)
 
 int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_out)
 {
+    TC_TRY(wait_coarse(ctx, DEV(ctx)));
     // LU singularity check (DGBTRF INFO)
     TC_HIP(hipMemcpyAsync(ctx->h_flag, ctx->d_info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     TC_HIP(hipStreamSynchronize(ctx->stream));
@@ -427,9 +490,10 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
                 }
             }
             if (level < G) {
-                TC_TRY(iterate_stream(ctx, d, level, P.Nmulti, dir, p, first_sweep));
+                TC_TRY(iterate_stream(ctx, d, level, P.Nmulti, dir, first_sweep));
             } else {
                 Level& Lc = ctx->lv[G];
+                TC_TRY(wait_coarse(ctx, d));
                 if (d.coarse_lu)
                     launch_coarse_lusolve(ctx->stream, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv,
                                           ctx->RowB, Lc, Lc.err[Lc.cur], zsg);
@@ -490,35 +554,11 @@ int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
 {
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
-    const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
-    const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
-    const int Niter = P.Nmulti;
     // level 1 smooths the correction against the defect of psi
     Level view = L;
     if (level == 1) view.rhs = L.res;
     const float* rhs32 = (level == 1) ? L.res32 : L.rhs32;
-    for (int n = 1; n <= Niter; ++n) {
-        const double ntemp = (double)(Niter - n + 1);
-        const double temp = std::cos((2.0 * ntemp - 1.0) * PII / (double)(2 * Niter));
-        double cconst = 1.0;
-        if (dir != 1) {
-            const double lb = -2.0, la = -1.0;
-            cconst = 2.0 / (-lb - la + (lb - la) * temp);
-        }
-        const bool timed = ctx->prof && level == 1;
-        if (timed) TC_TRY(prof_begin(ctx));
-        if (f32) {
-            launch_sweep32(ctx->stream, use_ratio, L, L.err32[L.cur32], L.err32[1 - L.cur32], rhs32, d.F, d.h[level],
-                           temp, cconst, zsg);
-            L.cur32 = 1 - L.cur32;
-        } else {
-            launch_sweep(ctx->stream, d.pow2[level], use_ratio, view, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level],
-                         P.ivis, temp, cconst, zsg);
-            L.cur = 1 - L.cur;
-        }
-        if (timed) TC_TRY(prof_end(ctx));
-    }
-    ctx->stats.sweeps_exec[level - 1] += Niter;
+    TC_TRY(run_sweeps(ctx, d, level, P.Nmulti, dir, f32, view, rhs32, nullptr));
     if (level > 1) {
         if (f32) launch_residual32(ctx->stream, L, L.err32[L.cur32], L.res32, L.rhs32, d.F, d.h[level]);
         else launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis, nullptr);
@@ -596,6 +636,7 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
             if (level < G) {
                 TC_TRY(dc_iterate(ctx, d, level, dir, f32));
             } else {
+                TC_TRY(wait_coarse(ctx, d));
                 if (f32) launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg);
                 else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
                 ctx->stats.coarse_solves++;
@@ -637,21 +678,48 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
 // ---------------------------------------------------------------------------
 // filters, binning, stages
 // ---------------------------------------------------------------------------
-int smoothness(tc_ctx* ctx, tc_dev& d, const double* f, double* diff)
-{
-    int nb = launch_smoothness(ctx->stream, f, ctx->g, d.partialA);
-    launch_reduce_final(ctx->stream, d.partialA, nb, 2, d.d_scal, S_SMOOTH);
-    TC_TRY(fetch_scalars(ctx, d, S_COUNT));
-    *diff = d.h_scal[S_SMOOTH];
-    return 0;
-}
-
 double* field_ptr(tc_ctx* ctx, int which)
 {
     switch (which) {
     case TC_F_T: return ctx->T; case TC_F_T0: return ctx->T0; case TC_F_T1: return ctx->T1; case TC_F_T2: return ctx->T2;
     default: return nullptr;
     }
+}
+
+// Data-dependent filter loop of smoother_time_T / smoother_time on the
+// device: iteration m checks the smoothness of the current iterate (stop
+// when below delta), else smooths it into the other buffer with factor
+// cfac(m) and applies the boundary routine.  Iterations are enqueued in
+// batches; the stop flag turns the rest of a batch into no-ops, and the
+// host reads the flag (and the stopping index) once per batch.  Returns the
+// number of smoothing iterations; the result ends in A.
+template <typename CFac>
+int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double delta, double courant, CFac cfac,
+                int* iters)
+{
+    const tc_params& P = ctx->p;
+    launch_zero_i32(ctx->stream, d.ctl, 4);
+    int m = 0, batch = 8;
+    for (;;) {
+        for (int b = 0; b < batch; ++b, ++m) {
+            double* cur = (m % 2 == 0) ? A : B;
+            double* nxt = (m % 2 == 0) ? B : A;
+            const int nb = launch_smoothness_stop(ctx->stream, cur, ctx->g, d.partialA, d.ctl);
+            launch_stop_test(ctx->stream, d.partialA, nb, delta, m, d.ctl);
+            launch_filter_step(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2,
+                               cfac(m), courant, d.ctl);
+            if (comp) launch_bcs_comp(ctx->stream, nxt, ctx->g, d.ctl);
+            else launch_bcs_T(ctx->stream, nxt, ctx->g, P.Tbot, d.ctl);
+        }
+        TC_HIP(hipMemcpyAsync(d.h_ctl, d.ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        if (d.h_ctl[0]) break;
+        if (batch < 64) batch *= 2;
+    }
+    const int mt = d.h_ctl[1];
+    if (mt % 2 == 1) TC_TRY(copy_plane(ctx, A, B, ctx->g));
+    *iters = mt;
+    return 0;
 }
 
 // smoother_time_T (smoother.f90:342-376)
@@ -662,59 +730,36 @@ int smoother_time_T(tc_ctx* ctx, tc_dev& d, const double* Tin, int32_t* n_out)
     TC_TRY(copy_plane(ctx, ctx->Tbuoy, ctx->Tvis, ctx->g));
     int n = 0;
     if (P.RaT != 0.0) {
-        for (;;) {
-            double diff;
-            TC_TRY(smoothness(ctx, d, ctx->Tvis, &diff));
-            if (diff < P.delta_T) break;
-            TC_TRY(copy_plane(ctx, ctx->tmp2, ctx->Tvis, ctx->g));
-            launch_smooth_T(ctx->stream, ctx->tmp2, ctx->Tvis, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2,
-                            P.courantT);
-            launch_bcs_T(ctx->stream, ctx->Tvis, ctx->g, P.Tbot);
-            n++;
-            ctx->stats.t_iters++;
-        }
+        TC_TRY(filter_loop(ctx, d, ctx->Tvis, ctx->tmp2, false, P.delta_T, P.courantT, [](int) { return 1.0; }, &n));
+        ctx->stats.t_iters += n;
         ctx->stats.n_temp = n;
     }
     if (n_out) *n_out = n;
     return 0;
 }
 
-// smoother_time (smoother.f90:286-340)
+// smoother_time (smoother.f90:286-340): a Chebyshev pass of 10 iterations
+// (lb=-5, la=-1), then single iterations with factor 1 until smooth; nn
+// counts the passes after the first.
 int smoother_time(tc_ctx* ctx, tc_dev& d, int32_t* n_comp)
 {
     const tc_params& P = ctx->p;
-    int Niter = 10;
+    const int Niter = 10;
     const double lb = -5.0, la = -1.0;
     for (int t = 0; t < P.ntype; ++t) {
         TC_TRY(copy_plane(ctx, ctx->Cvis[t], ctx->Cnew[t], ctx->g));
         TC_TRY(copy_plane(ctx, ctx->Cbuoy[t], ctx->Cnew[t], ctx->g));
     }
     for (int t = 0; t < P.ntype; ++t) {
-        int nn = 0;
-        bool trigger = false;
-        double* C = ctx->Cvis[t];
-        for (;;) {
-            for (int n = 1; n <= Niter; ++n) {
-                double diff;
-                TC_TRY(smoothness(ctx, d, C, &diff));
-                if (diff < P.delta_C[t]) { trigger = true; break; }
-                double c;
-                if (nn == 0) {
-                    const int ntemp = Niter - n + 1;
-                    c = 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
-                } else {
-                    c = 1.0;
-                }
-                TC_TRY(copy_plane(ctx, ctx->tmp2, C, ctx->g));
-                launch_smooth_C(ctx->stream, ctx->tmp2, C, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2, c,
-                                P.courantC);
-                launch_bcs_comp(ctx->stream, C, ctx->g);
-                ctx->stats.c_iters++;
-            }
-            if (trigger) break;
-            Niter = 1;
-            nn++;
-        }
+        auto cfac = [&](int m) {
+            if (m >= Niter) return 1.0;
+            const int ntemp = Niter - (m + 1) + 1;
+            return 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+        };
+        int iters = 0;
+        TC_TRY(filter_loop(ctx, d, ctx->Cvis[t], ctx->tmp2, true, P.delta_C[t], P.courantC, cfac, &iters));
+        const int nn = iters < Niter ? 0 : iters - Niter + 1;
+        ctx->stats.c_iters += iters;
         ctx->stats.n_comp[t] = nn;
         if (n_comp) n_comp[t] = nn;
     }
@@ -953,7 +998,7 @@ void free_ctx(tc_ctx* ctx)
     }
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
-                   ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
+                   ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
                    ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo};
     for (double* p : b) F(p);
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
@@ -962,8 +1007,9 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
-        F(d->d_scal); F(d->partialA); F(d->partialB);
+        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
+        if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         delete d;
     }
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
@@ -986,6 +1032,8 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         d->trace = e && e[0] == '1';
         const char* c = getenv("TC_COARSE_LU");
         d->coarse_lu = c && c[0] == '1';
+        const char* t = getenv("TC_TB");
+        d->tb = !(t && t[0] == '0');
     }
     TC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
@@ -1072,6 +1120,7 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_TRY(dalloc(ctx, &ctx->RowB, ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Ainv, (size_t)ctx->ngc * ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Bc, ctx->ngc));
+        TC_TRY(dalloc(ctx, &ctx->Ur, (size_t)ctx->ngc * (size_t)(ctx->kl + ctx->ku + 1)));
         TC_TRY(dalloc_i(ctx, &ctx->ipiv, ctx->ngc));
         TC_TRY(dalloc_i(ctx, &ctx->d_info, 1));
     }
@@ -1091,6 +1140,8 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_TRY(dalloc(ctx, &d->partialA, nb));
         TC_TRY(dalloc(ctx, &d->partialB, nb));
         TC_TRY(dalloc(ctx, &d->d_scal, 16));
+        TC_TRY(dalloc_i(ctx, &d->ctl, 4));
+        TC_HIP(hipHostMalloc((void**)&d->h_ctl, 4 * sizeof(int32_t), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_scal, 16 * sizeof(double), hipHostMallocDefault));
     }
     // grid-dependent tables: dt_vis per level, dts_array

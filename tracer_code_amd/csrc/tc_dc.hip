@@ -10,6 +10,7 @@
 // with zeroed coarse corrections; only the rounding differs.  The coarsest
 // grid is solved in FP64 in both forms.
 #include "tc_kernels.h"
+#include "tc_stencil32.h"
 
 namespace tc {
 
@@ -178,59 +179,8 @@ __global__ __launch_bounds__(256) void k_prolong_dd(double* __restrict__ hi, dou
     stream_ghosts(lo, gf, i, k, s.lo, zsg);
 }
 
-// ---- FP32 inner cycle --------------------------------------------------------
-struct FD32 {
-    float D1[5], D2[5], D3[5], D4[5];
-    float Drs[9], Drss[9], Drrs[9], Drrss[9];
-};
-struct H32 {
-    float idr, ids, idr2, ids2, idr3, ids3, idr4, ids4, idrds, idrds2, idr2ds, idr2ds2;
-};
-
-// FP32 fields and weights; A is the arithmetic type.  With A = double every
-// product of two FP32 values is exact and only the sums round, which keeps
-// the residual passed to the next level accurate relative to the (small)
-// defect even where rhs and L(S) nearly cancel.
+// ---- FP32 inner cycle (stencil in tc_stencil32.h) ---------------------------
 #pragma clang fp contract(fast)
-template <typename A>
-__device__ __forceinline__ A dot3f(const float* D, const float* __restrict__ f, size_t c, ptrdiff_t st)
-{
-    return A(D[1]) * A(f[c - st]) + A(D[2]) * A(f[c]) + A(D[3]) * A(f[c + st]);
-}
-template <typename A>
-__device__ __forceinline__ A dot5f(const float* D, const float* __restrict__ f, size_t c, ptrdiff_t st)
-{
-    return A(D[0]) * A(f[c - 2 * st]) + A(D[1]) * A(f[c - st]) + A(D[2]) * A(f[c]) + A(D[3]) * A(f[c + st]) +
-           A(D[4]) * A(f[c + 2 * st]);
-}
-template <typename A>
-__device__ __forceinline__ A sum9f(const float* M, const float* __restrict__ f, size_t c, ptrdiff_t ld)
-{
-    return A(M[0]) * A(f[c - ld - 1]) + A(M[1]) * A(f[c - ld]) + A(M[2]) * A(f[c - ld + 1]) + A(M[3]) * A(f[c - 1]) +
-           A(M[4]) * A(f[c]) + A(M[5]) * A(f[c + 1]) + A(M[6]) * A(f[c + ld - 1]) + A(M[7]) * A(f[c + ld]) +
-           A(M[8]) * A(f[c + ld + 1]);
-}
-// rhs - L(S); a1 = vis_xx - vis_zz
-template <typename A>
-__device__ __forceinline__ A stream_space32(const FD32& F, const H32& h, const float* __restrict__ S, size_t c,
-                                            ptrdiff_t ld, float vis, float vx, float vz, float a1, float vxz,
-                                            float rhs)
-{
-    A Srr = dot3f<A>(F.D2, S, c, 1) * A(h.idr2);
-    A Sss = dot3f<A>(F.D2, S, c, ld) * A(h.ids2);
-    A Srrr = dot5f<A>(F.D3, S, c, 1) * A(h.idr3);
-    A Ssss = dot5f<A>(F.D3, S, c, ld) * A(h.ids3);
-    A Srrrr = dot5f<A>(F.D4, S, c, 1) * A(h.idr4);
-    A Sssss = dot5f<A>(F.D4, S, c, ld) * A(h.ids4);
-    A Srs = sum9f<A>(F.Drs, S, c, ld) * A(h.idrds);
-    A Srss = sum9f<A>(F.Drss, S, c, ld) * A(h.idrds2);
-    A Srrs = sum9f<A>(F.Drrs, S, c, ld) * A(h.idr2ds);
-    A Srrss = sum9f<A>(F.Drrss, S, c, ld) * A(h.idr2ds2);
-    A L = A(a1) * (Srr - Sss) + A(vis) * (Srrrr + Sssss) + A(2) * A(vis) * Srrss + A(4) * A(vxz) * Srs +
-          A(2) * A(vx) * (Srrr + Srss) + A(2) * A(vz) * (Ssss + Srrs);
-    return A(rhs) - L;
-}
-
 template <bool RATIO>
 __global__ __launch_bounds__(256) void k_sweep32(const float* __restrict__ S0, float* __restrict__ E,
                                                  const float* __restrict__ vis, const float* __restrict__ vx,
@@ -353,19 +303,6 @@ __global__ __launch_bounds__(256) void k_coeffs_to_f32(const double* __restrict_
 
 // ---- launchers -------------------------------------------------------------
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
-
-FD32 fd32_from(const FD& F)
-{
-    FD32 o;
-    for (int j = 0; j < 5; ++j) { o.D1[j] = (float)F.D1[j]; o.D2[j] = (float)F.D2[j]; o.D3[j] = (float)F.D3[j]; o.D4[j] = (float)F.D4[j]; }
-    for (int j = 0; j < 9; ++j) { o.Drs[j] = (float)F.Drs[j]; o.Drss[j] = (float)F.Drss[j]; o.Drrs[j] = (float)F.Drrs[j]; o.Drrss[j] = (float)F.Drrss[j]; }
-    return o;
-}
-H32 h32_from(const Spacing& h)
-{
-    return H32{(float)h.idr, (float)h.ids, (float)h.idr2, (float)h.ids2, (float)h.idr3, (float)h.ids3,
-               (float)h.idr4, (float)h.ids4, (float)h.idrds, (float)h.idrds2, (float)h.idr2ds, (float)h.idr2ds2};
-}
 
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial)

@@ -74,8 +74,10 @@ __global__ void k_sym_bcs(double* __restrict__ f, Grid g)
 
 // smoothness (smoother.f90:75-95): per-block maxima of the largest jump to
 // the five forward neighbours, i=0..nx-1, k=1..nz-1.
-__global__ __launch_bounds__(256) void k_smoothness(const double* __restrict__ T, Grid g, double* __restrict__ partial)
+__global__ __launch_bounds__(256) void k_smoothness(const double* __restrict__ T, Grid g, double* __restrict__ partial,
+                                                    const int32_t* __restrict__ stop)
 {
+    if (stop && stop[0]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     double a = 0.0;
@@ -134,8 +136,60 @@ __global__ __launch_bounds__(256) void k_smooth_C(const double* __restrict__ T0,
 }
 
 // enforceBCs (energy.f90:455-475), one workgroup.
-__global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot)
+// One filter iteration from T0 into T for the device-controlled loops of
+// smoother_time_T / smoother_time: the smoothing update of k_smooth_T /
+// k_smooth_C on i=0..nx, k=0..nz, and the ghost cells copied from T0 (the
+// reference smooths from a full copy of the field, so the ghosts the
+// boundary routine then reads are the previous iterate's).  Skipped once
+// the loop's stop flag is set.
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_filter_step(const double* __restrict__ T0, double* __restrict__ T,
+                                                     const double* __restrict__ dts, Grid g, FD F, double dr2,
+                                                     double ds2, double cfac, double courant,
+                                                     const int32_t* __restrict__ stop)
 {
+    if (stop[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
+    if (i > g.nx + SPAN || k > g.nz + SPAN) return;
+    const size_t c = g.at(i, k);
+    if (i < 0 || i > g.nx || k < 0 || k > g.nz) {
+        T[c] = T0[c];
+        return;
+    }
+    double Trr = dot3(F.D2, T0, c, 1) / dr2;
+    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
+    double temp = Trr + Tss;
+    if constexpr (COMP) {
+        double d = dts[c] * courant;
+        double v = T0[c] + cfac * d * temp;
+        if (v < 0.0) v = 0.0;
+        if (v > 1.0) v = 1.0;
+        T[c] = v;
+    } else {
+        double d = dts[c] * courant;
+        T[c] = T0[c] + d * temp;
+    }
+}
+
+// Stop test of the filter loops: max of the block maxima of k_smoothness;
+// when it is below delta the flag is set and the iteration index recorded.
+__global__ __launch_bounds__(256) void k_stop_test(const double* __restrict__ partial, int n, double delta, int m,
+                                                   int32_t* __restrict__ ctl)
+{
+    if (ctl[0]) return;
+    double a = -INFINITY;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) a = fmax(a, partial[t]);
+    a = block_max(a);
+    if (threadIdx.x == 0 && a < delta) {
+        ctl[0] = 1;
+        ctl[1] = m;
+    }
+}
+
+__global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot, const int32_t* __restrict__ stop)
+{
+    if (stop && stop[0]) return;
     const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
     for (int k = tid; k <= nz; k += nt)
         for (int i = 1; i <= SPAN; ++i) { T[g.at(-i, k)] = T[g.at(i, k)]; T[g.at(nx + i, k)] = T[g.at(nx - i, k)]; }
@@ -155,8 +209,9 @@ __global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot)
 
 // enforceBCs_comp (composition.f90:792-803) in the reference's interleaved
 // order, one workgroup.
-__global__ void k_bcs_comp(double* __restrict__ C, Grid g)
+__global__ void k_bcs_comp(double* __restrict__ C, Grid g, const int32_t* __restrict__ stop)
 {
+    if (stop && stop[0]) return;
     const int nx = g.nx, nz = g.nz, tid = threadIdx.x, nt = blockDim.x;
     for (int i = 1; i <= SPAN; ++i) {
         for (int k = tid; k <= nz; k += nt) { C[g.at(-i, k)] = C[g.at(i, k)]; C[g.at(nx + i, k)] = C[g.at(nx - i, k)]; }
@@ -451,7 +506,7 @@ int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* p
 {
     dim3 b(64, 4);
     dim3 gr = grid2(g.nx, g.nz - 1, b);
-    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial);
+    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial, nullptr);
     return (int)(gr.x * gr.y);
 }
 
@@ -474,14 +529,38 @@ void launch_sym_bcs(hipStream_t st, double* f, const Grid& g)
     hipLaunchKernelGGL(k_sym_bcs, dim3(1), dim3(256), 0, st, f, g);
 }
 
-void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot)
+void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot, const int32_t* stop)
 {
-    hipLaunchKernelGGL(k_bcs_T, dim3(1), dim3(256), 0, st, T, g, Tbot);
+    hipLaunchKernelGGL(k_bcs_T, dim3(1), dim3(1024), 0, st, T, g, Tbot, stop);
 }
 
-void launch_bcs_comp(hipStream_t st, double* C, const Grid& g)
+void launch_bcs_comp(hipStream_t st, double* C, const Grid& g, const int32_t* stop)
 {
-    hipLaunchKernelGGL(k_bcs_comp, dim3(1), dim3(256), 0, st, C, g);
+    hipLaunchKernelGGL(k_bcs_comp, dim3(1), dim3(1024), 0, st, C, g, stop);
+}
+
+int launch_smoothness_stop(hipStream_t st, const double* T0, const Grid& g, double* partial, const int32_t* stop)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx, g.nz - 1, b);
+    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial, stop);
+    return (int)(gr.x * gr.y);
+}
+
+void launch_stop_test(hipStream_t st, const double* partial, int n, double delta, int m, int32_t* ctl)
+{
+    hipLaunchKernelGGL(k_stop_test, dim3(1), dim3(256), 0, st, partial, n, delta, m, ctl);
+}
+
+void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                        const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1 + 2 * SPAN, g.nz + 1 + 2 * SPAN, b);
+    if (comp)
+        hipLaunchKernelGGL((k_filter_step<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
+    else
+        hipLaunchKernelGGL((k_filter_step<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
 }
 
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n)

@@ -91,6 +91,7 @@ struct tc_ctx {
     // coarsest grid direct solve
     int ngc = 0, kl = 0, ku = 0, ldab = 0;
     double *Mc = nullptr, *RowB = nullptr, *Ainv = nullptr, *Bc = nullptr;
+    double* Ur = nullptr;  // row-major copy of the U band (explicit inverse)
     int32_t* ipiv = nullptr;
     int32_t* d_info = nullptr;
     // tracers (SoA)

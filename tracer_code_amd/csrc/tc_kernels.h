@@ -78,7 +78,7 @@ void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Leve
                             int ivis, int Vbc, int kl, int ku, int ngc);
 void launch_dgbtf2(hipStream_t st, double* ab, int ldab, int n, int kl, int ku, int32_t* ipiv, int32_t* info);
 void launch_band_inverse(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
-                         double* Ainv);
+                         double* Ainv, double* Ur);
 void launch_coarse_lusolve(hipStream_t st, const double* ab, int ldab, int n, int kl, int ku, const int32_t* ipiv,
                            const double* RowB, const Level& L, double* E, double zsg);
 void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, double* E, int ngc,
@@ -100,6 +100,13 @@ void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* Row
                            float* E, int ngc, double zsg);
 void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
 
+// --- tc_tb.hip ------------------------------------------------------------------
+int tb_max_sweeps();
+void launch_sweep_tb64(hipStream_t st, bool use_ratio, const Level& L, const double* rhs, const double* S0, double* E,
+                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg);
+void launch_sweep_tb32(hipStream_t st, bool use_ratio, const Level& L, const float* rhs, const float* S0, float* E,
+                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg);
+
 // --- tc_fields.hip -------------------------------------------------------------
 void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,
                        double ds, int Vbc);
@@ -110,8 +117,12 @@ void launch_smooth_T(hipStream_t st, const double* T0, double* T, const double* 
                      double dr2, double ds2, double courantT);
 void launch_smooth_C(hipStream_t st, const double* T0, double* C, const double* dts, const Grid& g, const FD& F,
                      double dr2, double ds2, double cfac, double courantC);
-void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot);
-void launch_bcs_comp(hipStream_t st, double* C, const Grid& g);
+void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot, const int32_t* stop = nullptr);
+void launch_bcs_comp(hipStream_t st, double* C, const Grid& g, const int32_t* stop = nullptr);
+int launch_smoothness_stop(hipStream_t st, const double* T0, const Grid& g, double* partial, const int32_t* stop);
+void launch_stop_test(hipStream_t st, const double* partial, int n, double delta, int m, int32_t* ctl);
+void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                        const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop);
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
 void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
                 const Grid& g, double dr, double ds, double aspect, int32_t* flag);

@@ -24,7 +24,52 @@ struct Spacing {
     double idr, ids, idr2, ids2, idr3, ids3, idr4, ids4, idrds, idrds2, idr2ds, idr2ds2;
 };
 
-// dot_product(D(-1:1), f(c-st:c+st)) with D given as the 5-table D[-2..2].
+// The stencil sums read the field through an accessor S(di, dk) that
+// returns S(i+di, k+dk); the pointer forms below wrap a strided plane.
+// dot_product(D(-1:1), S along one axis) with D given as the 5-table D[-2..2].
+template <int AX, typename G>
+__device__ __forceinline__ double dot3g(const double* D, const G& S)
+{
+    double acc = D[1] * S(AX == 0 ? -1 : 0, AX == 0 ? 0 : -1);
+    acc += D[2] * S(0, 0);
+    acc += D[3] * S(AX == 0 ? 1 : 0, AX == 0 ? 0 : 1);
+    return acc;
+}
+template <int AX, typename G>
+__device__ __forceinline__ double dot5g(const double* D, const G& S)
+{
+    double acc = D[0] * S(AX == 0 ? -2 : 0, AX == 0 ? 0 : -2);
+    acc += D[1] * S(AX == 0 ? -1 : 0, AX == 0 ? 0 : -1);
+    acc += D[2] * S(0, 0);
+    acc += D[3] * S(AX == 0 ? 1 : 0, AX == 0 ? 0 : 1);
+    acc += D[4] * S(AX == 0 ? 2 : 0, AX == 0 ? 0 : 2);
+    return acc;
+}
+// sum(M(-1:1,-1:1)*S(...)) in column-major order; M[(dk+1)*3+(di+1)].
+template <typename G>
+__device__ __forceinline__ double sum9g(const double* M, const G& S)
+{
+    double acc = M[0] * S(-1, -1);
+    acc += M[1] * S(0, -1);
+    acc += M[2] * S(1, -1);
+    acc += M[3] * S(-1, 0);
+    acc += M[4] * S(0, 0);
+    acc += M[5] * S(1, 0);
+    acc += M[6] * S(-1, 1);
+    acc += M[7] * S(0, 1);
+    acc += M[8] * S(1, 1);
+    return acc;
+}
+
+// accessor of a strided plane around element c
+template <typename T>
+struct PlaneAt {
+    const T* __restrict__ f;
+    size_t c;
+    ptrdiff_t ld;
+    __device__ __forceinline__ T operator()(int di, int dk) const { return f[c + (ptrdiff_t)dk * ld + di]; }
+};
+
 __device__ __forceinline__ double dot3(const double* D, const double* __restrict__ f, size_t c, ptrdiff_t st)
 {
     double acc = D[1] * f[c - st];
@@ -41,19 +86,9 @@ __device__ __forceinline__ double dot5(const double* D, const double* __restrict
     acc += D[4] * f[c + 2 * st];
     return acc;
 }
-// sum(M(-1:1,-1:1)*f(...)) in column-major order; M[(dk+1)*3+(di+1)].
 __device__ __forceinline__ double sum9(const double* M, const double* __restrict__ f, size_t c, ptrdiff_t ld)
 {
-    double acc = M[0] * f[c - ld - 1];
-    acc += M[1] * f[c - ld];
-    acc += M[2] * f[c - ld + 1];
-    acc += M[3] * f[c - 1];
-    acc += M[4] * f[c];
-    acc += M[5] * f[c + 1];
-    acc += M[6] * f[c + ld - 1];
-    acc += M[7] * f[c + ld];
-    acc += M[8] * f[c + ld + 1];
-    return acc;
+    return sum9g(M, PlaneAt<double>{f, c, ld});
 }
 
 struct FD {
@@ -63,23 +98,20 @@ struct FD {
 
 // L(S) - RHS evaluated as in stream_space (build=0, scaling=0), negated:
 // returns RHS - L(S).  The six terms are summed left to right.
-template <bool POW2>
-__device__ __forceinline__ double stream_space(const FD& F, const Spacing& h, int ivis, const double* __restrict__ S,
-                                               size_t c, ptrdiff_t ld, double vis, double vx, double vz, double vxx,
-                                               double vzz, double vxz, double rhs)
+template <bool POW2, typename G>
+__device__ __forceinline__ double stream_space_g(const FD& F, const Spacing& h, int ivis, const G& S, double vis,
+                                                 double vx, double vz, double vxx, double vzz, double vxz, double rhs)
 {
-    double Sr = sdiv<POW2>(dot3(F.D1, S, c, 1), h.dr, h.idr);
-    double Ss = sdiv<POW2>(dot3(F.D1, S, c, ld), h.ds, h.ids);
-    double Srr = sdiv<POW2>(dot3(F.D2, S, c, 1), h.dr2, h.idr2);
-    double Sss = sdiv<POW2>(dot3(F.D2, S, c, ld), h.ds2, h.ids2);
-    double Srrr = sdiv<POW2>(dot5(F.D3, S, c, 1), h.dr3, h.idr3);
-    double Ssss = sdiv<POW2>(dot5(F.D3, S, c, ld), h.ds3, h.ids3);
-    double Srrrr = sdiv<POW2>(dot5(F.D4, S, c, 1), h.dr4, h.idr4);
-    double Sssss = sdiv<POW2>(dot5(F.D4, S, c, ld), h.ds4, h.ids4);
-    double Srs = sdiv<POW2>(sum9(F.Drs, S, c, ld), h.drds, h.idrds);
-    double Srss = sdiv<POW2>(sum9(F.Drss, S, c, ld), h.drds2, h.idrds2);
-    double Srrs = sdiv<POW2>(sum9(F.Drrs, S, c, ld), h.dr2ds, h.idr2ds);
-    double Srrss = sdiv<POW2>(sum9(F.Drrss, S, c, ld), h.dr2ds2, h.idr2ds2);
+    double Srr = sdiv<POW2>(dot3g<0>(F.D2, S), h.dr2, h.idr2);
+    double Sss = sdiv<POW2>(dot3g<1>(F.D2, S), h.ds2, h.ids2);
+    double Srrr = sdiv<POW2>(dot5g<0>(F.D3, S), h.dr3, h.idr3);
+    double Ssss = sdiv<POW2>(dot5g<1>(F.D3, S), h.ds3, h.ids3);
+    double Srrrr = sdiv<POW2>(dot5g<0>(F.D4, S), h.dr4, h.idr4);
+    double Sssss = sdiv<POW2>(dot5g<1>(F.D4, S), h.ds4, h.ids4);
+    double Srs = sdiv<POW2>(sum9g(F.Drs, S), h.drds, h.idrds);
+    double Srss = sdiv<POW2>(sum9g(F.Drss, S), h.drds2, h.idrds2);
+    double Srrs = sdiv<POW2>(sum9g(F.Drrs, S), h.dr2ds, h.idr2ds);
+    double Srrss = sdiv<POW2>(sum9g(F.Drrss, S), h.dr2ds2, h.idr2ds2);
     double t1, t2, t3, t4, t5, t6;
     if (ivis == 0) {
         t1 = 0.0;
@@ -96,6 +128,13 @@ __device__ __forceinline__ double stream_space(const FD& F, const Spacing& h, in
     }
     double ss = t1 + t2 + t3 + t4 + t5 + t6 - rhs;
     return -ss;
+}
+template <bool POW2>
+__device__ __forceinline__ double stream_space(const FD& F, const Spacing& h, int ivis, const double* __restrict__ S,
+                                               size_t c, ptrdiff_t ld, double vis, double vx, double vz, double vxx,
+                                               double vzz, double vxz, double rhs)
+{
+    return stream_space_g<POW2>(F, h, ivis, PlaneAt<double>{S, c, ld}, vis, vx, vz, vxx, vzz, vxz, rhs);
 }
 
 // Ghost mirrors of enforceBCs_stream (iterative_solver.f90:965-989) for an

@@ -1,0 +1,202 @@
+// Temporally blocked Jacobi-Richardson sweeps for the correction levels of
+// the defect-correction cycle.  One launch performs up to TB_K consecutive
+// sweeps of a visit (iterate_stream, iterative_solver.f90:871-912, with the
+// ghost part of enforceBCs_stream :965-989).
+//
+// Each workgroup stages its TB_T x TB_T tile plus a halo of 2*TB_K cells in
+// LDS, one cell per thread, and sweeps the staged region TB_K times.  The
+// band of still-valid cells shrinks by the stencil reach of 2 per sweep, so
+// the tile itself stays exact.  It then writes only its tile and the tile's
+// ghost mirrors.
+//
+// The operator is applied in its 13-tap form: per cell the weights of
+// S(i+di,k+dk) are combined once per launch from eta, eta_x, eta_z,
+// eta_xx-eta_zz and eta_xz (pre-scaled by dt_stream), so a sweep costs 13
+// LDS reads and 13 FMAs.  This is the same operator with a different
+// rounding order, so these kernels serve the defect-correction modes; the
+// reference-form FP64 path keeps its one-sweep kernels.
+#include "tc_kernels.h"
+
+namespace tc {
+
+constexpr int TB_T = 16, TB_K = 4;
+constexpr int TB_H = 2 * TB_K;            // halo width
+constexpr int TB_R = TB_T + 2 * TB_H;     // staged region 32 x 32
+constexpr int TB_CELLS = TB_R * TB_R;
+constexpr int TB_THREADS = TB_CELLS;      // one thread per staged cell (1024)
+
+template <typename T>
+struct SweepFactors {
+    T temp[TB_K];
+    T cconst[TB_K];
+};
+
+// FD tables and reciprocal spacings of one level in the arithmetic type.
+template <typename T>
+struct Tables {
+    T D1[5], D2[5], D3[5], D4[5];
+    T idr2, ids2, idr3, ids3, idr4, ids4, idrds, idrds2, idr2ds, idr2ds2;
+};
+
+// Per-level coefficient arrays; a1 = eta_xx - eta_zz for FP32, or eta_xx
+// and eta_zz separately for FP64.
+template <typename T>
+struct Coefs {
+    const T *vis, *vx, *vz, *vxx, *vzz, *a1, *vxz, *rhs, *dts, *ratio, *lb;
+};
+
+// dt_stream-scaled weights of the 13 taps and the scaled RHS of one cell.
+template <typename T>
+struct W13 {
+    T c, xm1, xp1, xm2, xp2, zm1, zp1, zm2, zp2, mm, pm, mp, pp, rd, lb;
+};
+
+template <typename T, bool RATIO>
+__device__ __forceinline__ void tb_weights(const Coefs<T>& C, const Tables<T>& Tb, size_t c, W13<T>& w)
+{
+    const T vis = C.vis[c], vx = C.vx[c], vz = C.vz[c], vxz = C.vxz[c], dts = C.dts[c];
+    T a1;
+    if constexpr (sizeof(T) == 4) a1 = C.a1[c];
+    else a1 = C.vxx[c] - C.vzz[c];
+    const T two = T(2);
+    // axis parts: a1 (S_rr - S_ss) + 2 vx S_rrr + 2 vz S_sss + vis (S_rrrr + S_ssss)
+    auto wx = [&](int d) { return a1 * Tb.D2[d + 2] * Tb.idr2 + two * vx * Tb.D3[d + 2] * Tb.idr3 + vis * Tb.D4[d + 2] * Tb.idr4; };
+    auto wz = [&](int d) { return -a1 * Tb.D2[d + 2] * Tb.ids2 + two * vz * Tb.D3[d + 2] * Tb.ids3 + vis * Tb.D4[d + 2] * Tb.ids4; };
+    // 3x3 part: 4 vxz S_rs + 2 vx S_rss + 2 vz S_rrs + 2 vis S_rrss
+    auto wb = [&](int di, int dk) {
+        return T(4) * vxz * Tb.D1[dk + 2] * Tb.D1[di + 2] * Tb.idrds + two * vx * Tb.D2[dk + 2] * Tb.D1[di + 2] * Tb.idrds2 +
+               two * vz * Tb.D1[dk + 2] * Tb.D2[di + 2] * Tb.idr2ds + two * vis * Tb.D2[dk + 2] * Tb.D2[di + 2] * Tb.idr2ds2;
+    };
+    w.c = dts * (wx(0) + wz(0) + wb(0, 0));
+    w.xm1 = dts * (wx(-1) + wb(-1, 0));
+    w.xp1 = dts * (wx(1) + wb(1, 0));
+    w.xm2 = dts * wx(-2);
+    w.xp2 = dts * wx(2);
+    w.zm1 = dts * (wz(-1) + wb(0, -1));
+    w.zp1 = dts * (wz(1) + wb(0, 1));
+    w.zm2 = dts * wz(-2);
+    w.zp2 = dts * wz(2);
+    w.mm = dts * wb(-1, -1);
+    w.pm = dts * wb(1, -1);
+    w.mp = dts * wb(-1, 1);
+    w.pp = dts * wb(1, 1);
+    w.rd = dts * C.rhs[c];
+    w.lb = T(-2);
+    if constexpr (RATIO) {
+        if constexpr (sizeof(T) == 4) {
+            w.lb = C.lb[c];
+        } else {
+            const T r = C.ratio[c];
+            w.lb = (r > T(1e99)) ? T(-2) : T(-2) * (r * r) / (T(1) + r * r);
+        }
+    }
+}
+
+// Mirror writes of enforceBCs_stream for interior (i,k) into the staged
+// region with global origin (gi0, gk0); only cells inside it.
+template <typename T>
+__device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int i, int k, int gi0, int gk0, T v, T zsg)
+{
+    const int nx = g.nx, nz = g.nz;
+    auto put = [&](int ii, int kk, T x) {
+        const int li = ii - gi0, lk = kk - gk0;
+        if (li >= 0 && li < TB_R && lk >= 0 && lk < TB_R) B[lk * TB_R + li] = x;
+    };
+    const bool lo_i = i <= SPAN, hi_i = i >= nx - SPAN, lo_k = k <= SPAN, hi_k = k >= nz - SPAN;
+    if (lo_i) put(-i, k, -v);
+    if (hi_i) put(2 * nx - i, k, -v);
+    if (lo_k) put(i, -k, zsg * v);
+    if (hi_k) put(i, 2 * nz - k, zsg * v);
+    if (lo_i && lo_k) put(-i, -k, zsg * (-v));
+    if (lo_i && hi_k) put(-i, 2 * nz - k, zsg * (-v));
+    if (hi_i && lo_k) put(2 * nx - i, -k, zsg * (-v));
+    if (hi_i && hi_k) put(2 * nx - i, 2 * nz - k, zsg * (-v));
+}
+
+#pragma clang fp contract(fast)
+template <typename T, bool RATIO>
+__global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
+                                                        Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg)
+{
+    __shared__ T buf[2][TB_CELLS];
+    const int l = threadIdx.x;
+    const int li = l % TB_R, lk = l / TB_R;
+    const int gi0 = 1 + blockIdx.x * TB_T - TB_H, gk0 = 1 + blockIdx.y * TB_T - TB_H;  // staged region origin
+    const int i = gi0 + li, k = gk0 + lk;
+    T v = T(0);
+    if (i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
+    buf[0][l] = v;
+    buf[1][l] = v;
+    // sweep s updates the cells at least 2(s+1) from the region edge
+    const int m = min(min(li, TB_R - 1 - li), min(lk, TB_R - 1 - lk));
+    const bool interior = i >= 1 && i <= g.nx - 1 && k >= 1 && k <= g.nz - 1;
+    const int band = (interior && m >= 2) ? m / 2 : 0;
+    W13<T> w;
+    if (band > 0) tb_weights<T, RATIO>(C, Tb, g.at(i, k), w);
+    __syncthreads();
+    int cur = 0;
+    for (int s = 0; s < nk; ++s) {
+        if (band > s) {
+            const T* __restrict__ A = buf[cur];
+            T* __restrict__ B = buf[1 - cur];
+            const T s0 = A[l];
+            T acc = w.c * s0;
+            acc += w.xm1 * A[l - 1] + w.xp1 * A[l + 1] + w.xm2 * A[l - 2] + w.xp2 * A[l + 2];
+            acc += w.zm1 * A[l - TB_R] + w.zp1 * A[l + TB_R] + w.zm2 * A[l - 2 * TB_R] + w.zp2 * A[l + 2 * TB_R];
+            acc += w.mm * A[l - TB_R - 1] + w.pm * A[l - TB_R + 1] + w.mp * A[l + TB_R - 1] + w.pp * A[l + TB_R + 1];
+            T cc = sf.cconst[s];
+            if constexpr (RATIO) cc = T(2) / (-w.lb + T(1) + (w.lb + T(1)) * sf.temp[s]);
+            const T nv = s0 + cc * (w.rd - acc);
+            B[l] = nv;
+            tb_ghosts(B, g, i, k, gi0, gk0, nv, zsg);
+        }
+        __syncthreads();
+        cur = 1 - cur;
+    }
+    // the tile (interior cells) and its ghost mirrors
+    if (li >= TB_H && li < TB_H + TB_T && lk >= TB_H && lk < TB_H + TB_T && interior) {
+        const T r = buf[cur][l];
+        Eout[g.at(i, k)] = r;
+        stream_ghosts(Eout, g, i, k, r, zsg);
+    }
+}
+#pragma clang fp contract(off)
+
+// ---- launchers -------------------------------------------------------------
+template <typename T>
+static Tables<T> make_tables(const FD& F, const Spacing& h)
+{
+    Tables<T> t;
+    for (int j = 0; j < 5; ++j) { t.D1[j] = (T)F.D1[j]; t.D2[j] = (T)F.D2[j]; t.D3[j] = (T)F.D3[j]; t.D4[j] = (T)F.D4[j]; }
+    t.idr2 = (T)h.idr2; t.ids2 = (T)h.ids2; t.idr3 = (T)h.idr3; t.ids3 = (T)h.ids3; t.idr4 = (T)h.idr4; t.ids4 = (T)h.ids4;
+    t.idrds = (T)h.idrds; t.idrds2 = (T)h.idrds2; t.idr2ds = (T)h.idr2ds; t.idr2ds2 = (T)h.idr2ds2;
+    return t;
+}
+
+static inline dim3 tb_grid(const Grid& g) { return dim3((g.nx - 1 + TB_T - 1) / TB_T, (g.nz - 1 + TB_T - 1) / TB_T); }
+
+int tb_max_sweeps() { return TB_K; }
+
+void launch_sweep_tb64(hipStream_t st, bool use_ratio, const Level& L, const double* rhs, const double* S0, double* E,
+                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg)
+{
+    Coefs<double> C{L.vis, L.visx, L.visz, L.visxx, L.viszz, nullptr, L.visxz, rhs, L.dtstream, L.ratio, nullptr};
+    SweepFactors<double> sf{};
+    for (int s = 0; s < nk; ++s) { sf.temp[s] = temp[s]; sf.cconst[s] = cconst[s]; }
+    const Tables<double> tb = make_tables<double>(F, h);
+    if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<double, true>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, zsg);
+    else hipLaunchKernelGGL((k_sweep_tb<double, false>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, zsg);
+}
+
+void launch_sweep_tb32(hipStream_t st, bool use_ratio, const Level& L, const float* rhs, const float* S0, float* E,
+                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg)
+{
+    Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, rhs, L.dts32, nullptr, L.lb32};
+    SweepFactors<float> sf{};
+    for (int s = 0; s < nk; ++s) { sf.temp[s] = (float)temp[s]; sf.cconst[s] = (float)cconst[s]; }
+    const Tables<float> tb = make_tables<float>(F, h);
+    if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<float, true>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, (float)zsg);
+    else hipLaunchKernelGGL((k_sweep_tb<float, false>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, (float)zsg);
+}
+
+}  // namespace tc
