@@ -1,0 +1,66 @@
+"""Errors of the accelerated solver forms against the FP64 reference-form
+path after several time steps from the same state: streamfunction,
+velocity, Nusselt number, rms velocity and mean composition (total mass),
+plus the tracer positions and the tracer counts.
+
+All modes solve to the same 1e-8 relative residual, so they differ by what
+that tolerance allows.  The size (256x128 at the 2048x1024 physics) is one
+where the reference form still converges in FP64.  The diagnostics are
+evaluated by the oracle's `stats` (basics.f90:236-324) on each path's fields.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+from tracer_code_amd import Solver, config, ic
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 3
+
+
+def run(mode, params, T, r, s, ty):
+    g = Solver(params, len(r))
+    g.initialize(T, r, s, ty)
+    g.set_mode(mode)
+    g.time_step(STEPS)
+    g.compute_velocities()
+    out = {k: g.get(k) for k in ("SF", "u", "w", "T", "conduct", "Cnew", "nempty", "count_type")}
+    out["r"], out["s"] = g.get_tracers("x")
+    g.close()
+    return out
+
+
+def diagnostics(params, f):
+    o = oracle.Oracle(params, 0)
+    nx = params.nx
+    for oid, key in ((oracle.OR_T, "T"), (oracle.OR_U, "u"), (oracle.OR_W, "w"), (oracle.OR_CONDUCT, "conduct")):
+        o.plane(oid)[:, :nx + 5] = f[key]
+    o.plane(oracle.OR_CNEW, 1)[:, :nx + 5] = f["Cnew"][..., 0]
+    o.plane(oracle.OR_NEMPTY, 0, np.int32)[:, :nx + 5] = f["nempty"]
+    st = np.zeros(9)
+    o.stats(st.ctypes.data)
+    o.close()
+    return {"vrms": st[0], "Nu": st[1], "Cavg": st[3]}
+
+
+def test_accelerated_forms_against_fp64():
+    params, cfg = config.preset_params("cfg2", nx=256, nz=128, tpc=4)
+    T, r, s, ty = ic.workload(cfg, seed=7)
+    print(f"[accuracy 256x128 ntr={len(r)} steps={STEPS}]", flush=True)
+    ref = run("fp64", params, T, r, s, ty)
+    dref = diagnostics(params, ref)
+    for mode in ("dc64", "mixed"):
+        f = run(mode, params, T, r, s, ty)
+        d = diagnostics(params, f)
+        err = {k: rel_err(f[k], ref[k]) for k in ("SF", "u", "w", "T")}
+        derr = {k: abs(d[k] - dref[k]) / abs(dref[k]) for k in d}
+        pos = max(float(np.max(np.abs(f["r"] - ref["r"]))), float(np.max(np.abs(f["s"] - ref["s"]))))
+        counts_equal = bool(np.array_equal(f["count_type"], ref["count_type"]))
+        print(f"{mode}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()) + " "
+              + " ".join(f"{k}={v:.2e}" for k, v in derr.items())
+              + f" tracer pos={pos:.2e} counts equal={counts_equal}", flush=True)
+        assert derr["Nu"] <= 1e-6 and derr["vrms"] <= 1e-6 and derr["Cavg"] <= 1e-6
+        assert err["SF"] <= 1e-5 and err["u"] <= 1e-5 and err["w"] <= 1e-5
+        assert pos <= 1e-9

@@ -310,19 +310,21 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
     const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
     const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
     const bool tb = d.tb && level >= 2 && ctx->mode != TC_MODE_FP64;
+    // sweeps per launch (4 measured faster than 8 on every level at 2048x1024)
+    const int K = 4;
     int n = 1;
     while (n <= Niter) {
         if (tb) {
-            const int nk = std::min(tb_max_sweeps(), Niter - n + 1);
+            const int nk = std::min(tb_max_sweeps(K), Niter - n + 1);
             double temp[8], cc[8];
             for (int s = 0; s < nk; ++s) cheb_factors(n + s, Niter, dir, temp[s], cc[s]);
             if (f32) {
-                launch_sweep_tb32(ctx->stream, use_ratio, L, rhs32, L.err32[L.cur32], L.err32[1 - L.cur32], d.F,
+                launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, L.err32[L.cur32], L.err32[1 - L.cur32], d.F,
                                   d.h[level], temp, cc, nk, zsg);
                 L.cur32 = 1 - L.cur32;
             } else {
-                launch_sweep_tb64(ctx->stream, use_ratio, L, view.rhs, L.err[L.cur], L.err[1 - L.cur], d.F, d.h[level],
-                                  temp, cc, nk, zsg);
+                launch_sweep_tb64(ctx->stream, use_ratio, K, L, view.rhs, L.err[L.cur], L.err[1 - L.cur], d.F,
+                                  d.h[level], temp, cc, nk, zsg);
                 L.cur = 1 - L.cur;
             }
             n += nk;

@@ -101,11 +101,13 @@ void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* Row
 void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
 
 // --- tc_tb.hip ------------------------------------------------------------------
-int tb_max_sweeps();
-void launch_sweep_tb64(hipStream_t st, bool use_ratio, const Level& L, const double* rhs, const double* S0, double* E,
-                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg);
-void launch_sweep_tb32(hipStream_t st, bool use_ratio, const Level& L, const float* rhs, const float* S0, float* E,
-                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg);
+int tb_max_sweeps(int K);
+void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
+                       double* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
+                       double zsg);
+void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
+                       float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
+                       double zsg);
 
 // --- tc_fields.hip -------------------------------------------------------------
 void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,

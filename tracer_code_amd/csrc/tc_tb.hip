@@ -1,10 +1,10 @@
 // Temporally blocked Jacobi-Richardson sweeps for the correction levels of
-// the defect-correction cycle.  One launch performs up to TB_K consecutive
+// the defect-correction cycle.  One launch performs up to K consecutive
 // sweeps of a visit (iterate_stream, iterative_solver.f90:871-912, with the
 // ghost part of enforceBCs_stream :965-989).
 //
-// Each workgroup stages its TB_T x TB_T tile plus a halo of 2*TB_K cells in
-// LDS, one cell per thread, and sweeps the staged region TB_K times.  The
+// Each workgroup stages its TB_T x TB_T tile plus a halo of 2K cells in
+// LDS, one to three cells per thread, and sweeps the staged region K times.  The
 // band of still-valid cells shrinks by the stencil reach of 2 per sweep, so
 // the tile itself stays exact.  It then writes only its tile and the tile's
 // ghost mirrors.
@@ -19,16 +19,22 @@
 
 namespace tc {
 
-constexpr int TB_T = 16, TB_K = 4;
-constexpr int TB_H = 2 * TB_K;            // halo width
-constexpr int TB_R = TB_T + 2 * TB_H;     // staged region 32 x 32
-constexpr int TB_CELLS = TB_R * TB_R;
-constexpr int TB_THREADS = TB_CELLS;      // one thread per staged cell (1024)
+// Tile TB_T x TB_T; K sweeps per launch need a halo of 2K cells, so the
+// staged region is (TB_T + 4K)^2: 32^2 for K=4, 48^2 for K=8.
+constexpr int TB_T = 16, TB_KMAX = 8;
+constexpr int TB_THREADS = 1024;
+template <int K>
+struct TBGeom {
+    static constexpr int H = 2 * K;
+    static constexpr int R = TB_T + 2 * H;
+    static constexpr int CELLS = R * R;
+    static constexpr int NPT = (CELLS + TB_THREADS - 1) / TB_THREADS;  // staged cells per thread
+};
 
 template <typename T>
 struct SweepFactors {
-    T temp[TB_K];
-    T cconst[TB_K];
+    T temp[TB_KMAX];
+    T cconst[TB_KMAX];
 };
 
 // FD tables and reciprocal spacings of one level in the arithmetic type.
@@ -94,13 +100,13 @@ __device__ __forceinline__ void tb_weights(const Coefs<T>& C, const Tables<T>& T
 
 // Mirror writes of enforceBCs_stream for interior (i,k) into the staged
 // region with global origin (gi0, gk0); only cells inside it.
-template <typename T>
+template <int R, typename T>
 __device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int i, int k, int gi0, int gk0, T v, T zsg)
 {
     const int nx = g.nx, nz = g.nz;
     auto put = [&](int ii, int kk, T x) {
         const int li = ii - gi0, lk = kk - gk0;
-        if (li >= 0 && li < TB_R && lk >= 0 && lk < TB_R) B[lk * TB_R + li] = x;
+        if (li >= 0 && li < R && lk >= 0 && lk < R) B[lk * R + li] = x;
     };
     const bool lo_i = i <= SPAN, hi_i = i >= nx - SPAN, lo_k = k <= SPAN, hi_k = k >= nz - SPAN;
     if (lo_i) put(-i, k, -v);
@@ -114,50 +120,71 @@ __device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int 
 }
 
 #pragma clang fp contract(fast)
-template <typename T, bool RATIO>
+template <typename T, bool RATIO, int K>
 __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
                                                         Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg)
 {
-    __shared__ T buf[2][TB_CELLS];
-    const int l = threadIdx.x;
-    const int li = l % TB_R, lk = l / TB_R;
-    const int gi0 = 1 + blockIdx.x * TB_T - TB_H, gk0 = 1 + blockIdx.y * TB_T - TB_H;  // staged region origin
-    const int i = gi0 + li, k = gk0 + lk;
-    T v = T(0);
-    if (i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
-    buf[0][l] = v;
-    buf[1][l] = v;
-    // sweep s updates the cells at least 2(s+1) from the region edge
-    const int m = min(min(li, TB_R - 1 - li), min(lk, TB_R - 1 - lk));
-    const bool interior = i >= 1 && i <= g.nx - 1 && k >= 1 && k <= g.nz - 1;
-    const int band = (interior && m >= 2) ? m / 2 : 0;
-    W13<T> w;
-    if (band > 0) tb_weights<T, RATIO>(C, Tb, g.at(i, k), w);
+    using G = TBGeom<K>;
+    constexpr int R = G::R, H = G::H, NPT = G::NPT;
+    __shared__ T buf[2][G::CELLS];
+    const int gi0 = 1 + blockIdx.x * TB_T - H, gk0 = 1 + blockIdx.y * TB_T - H;  // staged region origin
+    W13<T> w[NPT];
+    int band[NPT];
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+        const int l = threadIdx.x + u * TB_THREADS;
+        band[u] = 0;
+        if (l >= G::CELLS) continue;
+        const int li = l % R, lk = l / R;
+        const int i = gi0 + li, k = gk0 + lk;
+        T v = T(0);
+        if (i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
+        buf[0][l] = v;
+        buf[1][l] = v;
+        // sweep s updates the cells at least 2(s+1) from the region edge
+        const int m = min(min(li, R - 1 - li), min(lk, R - 1 - lk));
+        const bool interior = i >= 1 && i <= g.nx - 1 && k >= 1 && k <= g.nz - 1;
+        if (interior && m >= 2) {
+            band[u] = m / 2;
+            tb_weights<T, RATIO>(C, Tb, g.at(i, k), w[u]);
+        }
+    }
     __syncthreads();
     int cur = 0;
     for (int s = 0; s < nk; ++s) {
-        if (band > s) {
-            const T* __restrict__ A = buf[cur];
-            T* __restrict__ B = buf[1 - cur];
+        const T* __restrict__ A = buf[cur];
+        T* __restrict__ B = buf[1 - cur];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) {
+            if (band[u] <= s) continue;
+            const int l = threadIdx.x + u * TB_THREADS;
+            const W13<T>& q = w[u];
             const T s0 = A[l];
-            T acc = w.c * s0;
-            acc += w.xm1 * A[l - 1] + w.xp1 * A[l + 1] + w.xm2 * A[l - 2] + w.xp2 * A[l + 2];
-            acc += w.zm1 * A[l - TB_R] + w.zp1 * A[l + TB_R] + w.zm2 * A[l - 2 * TB_R] + w.zp2 * A[l + 2 * TB_R];
-            acc += w.mm * A[l - TB_R - 1] + w.pm * A[l - TB_R + 1] + w.mp * A[l + TB_R - 1] + w.pp * A[l + TB_R + 1];
+            T acc = q.c * s0;
+            acc += q.xm1 * A[l - 1] + q.xp1 * A[l + 1] + q.xm2 * A[l - 2] + q.xp2 * A[l + 2];
+            acc += q.zm1 * A[l - R] + q.zp1 * A[l + R] + q.zm2 * A[l - 2 * R] + q.zp2 * A[l + 2 * R];
+            acc += q.mm * A[l - R - 1] + q.pm * A[l - R + 1] + q.mp * A[l + R - 1] + q.pp * A[l + R + 1];
             T cc = sf.cconst[s];
-            if constexpr (RATIO) cc = T(2) / (-w.lb + T(1) + (w.lb + T(1)) * sf.temp[s]);
-            const T nv = s0 + cc * (w.rd - acc);
+            if constexpr (RATIO) cc = T(2) / (-q.lb + T(1) + (q.lb + T(1)) * sf.temp[s]);
+            const T nv = s0 + cc * (q.rd - acc);
             B[l] = nv;
-            tb_ghosts(B, g, i, k, gi0, gk0, nv, zsg);
+            tb_ghosts<R>(B, g, gi0 + l % R, gk0 + l / R, gi0, gk0, nv, zsg);
         }
         __syncthreads();
         cur = 1 - cur;
     }
     // the tile (interior cells) and its ghost mirrors
-    if (li >= TB_H && li < TB_H + TB_T && lk >= TB_H && lk < TB_H + TB_T && interior) {
-        const T r = buf[cur][l];
-        Eout[g.at(i, k)] = r;
-        stream_ghosts(Eout, g, i, k, r, zsg);
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+        const int l = threadIdx.x + u * TB_THREADS;
+        if (l >= G::CELLS) continue;
+        const int li = l % R, lk = l / R, i = gi0 + li, k = gk0 + lk;
+        if (li >= H && li < H + TB_T && lk >= H && lk < H + TB_T && i >= 1 && i <= g.nx - 1 && k >= 1 &&
+            k <= g.nz - 1) {
+            const T r = buf[cur][l];
+            Eout[g.at(i, k)] = r;
+            stream_ghosts(Eout, g, i, k, r, zsg);
+        }
     }
 }
 #pragma clang fp contract(off)
@@ -175,28 +202,40 @@ static Tables<T> make_tables(const FD& F, const Spacing& h)
 
 static inline dim3 tb_grid(const Grid& g) { return dim3((g.nx - 1 + TB_T - 1) / TB_T, (g.nz - 1 + TB_T - 1) / TB_T); }
 
-int tb_max_sweeps() { return TB_K; }
+int tb_max_sweeps(int K) { return K; }
 
-void launch_sweep_tb64(hipStream_t st, bool use_ratio, const Level& L, const double* rhs, const double* S0, double* E,
-                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg)
+template <typename T>
+static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
+                      const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg)
+{
+    const dim3 gr = tb_grid(g), b(TB_THREADS);
+    if (K == 8) {
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+    } else {
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+    }
+}
+
+void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
+                       double* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
+                       double zsg)
 {
     Coefs<double> C{L.vis, L.visx, L.visz, L.visxx, L.viszz, nullptr, L.visxz, rhs, L.dtstream, L.ratio, nullptr};
     SweepFactors<double> sf{};
     for (int s = 0; s < nk; ++s) { sf.temp[s] = temp[s]; sf.cconst[s] = cconst[s]; }
-    const Tables<double> tb = make_tables<double>(F, h);
-    if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<double, true>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, zsg);
-    else hipLaunchKernelGGL((k_sweep_tb<double, false>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, zsg);
+    tb_launch<double>(st, use_ratio, K, L.g, S0, E, C, make_tables<double>(F, h), sf, nk, zsg);
 }
 
-void launch_sweep_tb32(hipStream_t st, bool use_ratio, const Level& L, const float* rhs, const float* S0, float* E,
-                       const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk, double zsg)
+void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
+                       float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
+                       double zsg)
 {
     Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, rhs, L.dts32, nullptr, L.lb32};
     SweepFactors<float> sf{};
     for (int s = 0; s < nk; ++s) { sf.temp[s] = (float)temp[s]; sf.cconst[s] = (float)cconst[s]; }
-    const Tables<float> tb = make_tables<float>(F, h);
-    if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<float, true>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, (float)zsg);
-    else hipLaunchKernelGGL((k_sweep_tb<float, false>), tb_grid(L.g), dim3(TB_THREADS), 0, st, S0, E, C, tb, L.g, sf, nk, (float)zsg);
+    tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg);
 }
 
 }  // namespace tc
