@@ -222,8 +222,38 @@ int tc_set_profiling(tc_ctx* ctx, int enable);
 int tc_kernel_timing(tc_ctx* ctx, int which, double* avg_us, int64_t* count);
 
 /* ---- learned initial guess (no counterpart in the reference) ----------- */
-/* Enables/disables the convolutional warm start used as the multigrid
- * initial guess (the hook is error(1,:,:)=SF, iterative_solver.f90:87). */
+/*
+ * A 4-level U-Net (channels 16/32/64/128, 3x3 convolutions, GELU; BF16
+ * weights and activations, FP32 accumulation on MFMA) predicts the increment
+ * of the streamfunction from (T, C, log10 eta, psi_prev, psi_prev - psi_prev2)
+ * on the H x W = nz x nx node grid (psi_prev, psi_prev2: the last two
+ * solutions).  The hook is the warm start error(1,:,:)=SF of
+ * multigrid (iterative_solver.f90:87): in the defect-correction modes the
+ * solver evaluates the exact residual of the guess and of the plain warm
+ * start and iterates from the better one, to the same tolerance.
+ */
+typedef struct tc_nn tc_nn;
+int tc_nn_create(int32_t H, int32_t W, int32_t trainable, tc_nn** out);
+int tc_nn_destroy(tc_nn* nn);
+const char* tc_nn_last_error(const tc_nn* nn);
+int tc_nn_param_count(const tc_nn* nn, int64_t* count);
+/* parameters: per layer W[cout][9][cin] then b[cout]; last the 1x1 head w[16], b */
+int tc_nn_set_weights(tc_nn* nn, const float* w, int64_t count);
+int tc_nn_get_weights(tc_nn* nn, float* w, int64_t count);
+int tc_nn_get_grads(tc_nn* nn, float* g, int64_t count);
+/* x: H*W*5 (pixel k*W+i, channel fastest); target, mask: H*W (may be NULL) */
+int tc_nn_set_sample(tc_nn* nn, const float* x, const float* target, const float* mask);
+int tc_nn_forward(tc_nn* nn, float* out);
+int tc_nn_backward(tc_nn* nn, double* loss);
+int tc_nn_adam(tc_nn* nn, float lr, float grad_scale);
+/* Solver side: attach a network (H = nz, W = nx), record training samples
+ * (inputs before / increments after each solve) for the next `capacity`
+ * solves, train on them, and use the network as the initial guess. */
+int tc_learn_attach(tc_ctx* ctx, tc_nn* nn);
+int tc_learn_record(tc_ctx* ctx, int32_t capacity);
+int tc_learn_samples(const tc_ctx* ctx, int32_t* count);
+int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_t seed, double* mean_loss);
+/* Enables/disables the learned guess in the defect-correction modes. */
 int tc_guess_enable(tc_ctx* ctx, int enable);
 
 #ifdef __cplusplus

@@ -1,0 +1,105 @@
+"""GPU checks of the learned-guess U-Net (BF16 on MFMA) against the CPU
+reference oracle/unet.py, and of its use inside the solver.
+
+* forward output and parameter gradients equal the reference evaluated with
+  the same bfloat16 roundings (tolerances below), and stay close to the
+  plain float64 reference;
+* one Adam step equals the textbook update;
+* in a defect-correction solve the guess never changes the answer: the
+  solve still ends at the tolerance, and the solver reports whether the
+  guess or the plain warm start was used.
+"""
+import numpy as np
+import pytest
+
+from oracle import unet
+from tracer_code_amd import Solver, config, ic, learn
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+@pytest.mark.parametrize("H,W", [(16, 16), (32, 64)])
+def test_forward_and_gradients_match_reference(H, W):
+    th = unet.init_params(3)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (H, W, 5))
+    t = rng.standard_normal((H, W))
+    m = np.ones((H, W))
+    m[0, :] = 0
+    m[:, 0] = 0
+    print(f"[nn grad check {H}x{W}]", flush=True)
+    with learn.Network(H, W) as net:
+        net.set_weights(th.astype(np.float32))
+        net.set_sample(x, t, m)
+        out = net.forward()
+        loss = net.backward()
+        g = net.get_grads()
+    lb, ob, gb = unet.UNetRef(th.astype(np.float32), bf=True).loss_and_grad(x, t, m)
+    l64, o64, g64 = unet.UNetRef(th.astype(np.float32)).loss_and_grad(x, t, m)
+    e_out, e_out64 = rel(out, ob), rel(out, o64)
+    e_g, e_g64 = rel(g, gb), rel(g, g64)
+    print(f"out rel err vs bf16-ref {e_out:.2e} vs fp64-ref {e_out64:.2e}; loss {loss:.6e} ref {lb:.6e}; "
+          f"grad rel err vs bf16-ref {e_g:.2e} vs fp64-ref {e_g64:.2e}", flush=True)
+    assert e_out <= 1e-2 and e_out64 <= 5e-2
+    assert abs(loss - lb) <= 2e-2 * abs(lb)
+    assert e_g <= 3e-2 and e_g64 <= 1e-1
+    # layer by layer: every block of parameters carries the gradient
+    o = 0
+    for ci, co, _ in unet.LAYERS:
+        n = co * 9 * ci + co
+        assert rel(g[o:o + n], gb[o:o + n]) <= 6e-2, (ci, co)
+        o += n
+
+
+def test_adam_step():
+    H, W = 16, 16
+    th = unet.init_params(5).astype(np.float32)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (H, W, 5))
+    t = rng.standard_normal((H, W))
+    m = np.ones((H, W))
+    with learn.Network(H, W) as net:
+        net.set_weights(th)
+        net.set_sample(x, t, m)
+        net.forward()
+        net.backward()
+        g = net.get_grads().astype(np.float64)
+        net.adam(1e-3, 0.5)
+        w1 = net.get_weights().astype(np.float64)
+    gs = 0.5 * g
+    mm = 0.1 * gs
+    vv = 0.001 * gs * gs
+    want = th - 1e-3 * (mm / 0.1) / (np.sqrt(vv / 0.001) + 1e-8)
+    assert np.max(np.abs(w1 - want)) <= 1e-6
+
+
+def test_guess_in_solver_keeps_tolerance():
+    params, cfg = config.preset_params("cfg0", nx=64, nz=64, tpc=4)
+    T, r, s, ty = ic.workload(cfg, seed=11)
+    print("[learned guess 64x64]", flush=True)
+    with Solver(params, len(r)) as g, learn.Network(params.nz, params.nx) as net:
+        net.set_weights(learn.init_weights(0))
+        g.initialize(T, r, s, ty)
+        g.set_mode("mixed")
+        g.learn_attach(net)
+        g.learn_record(64)
+        g.time_step(4)
+        n = g.learn_samples()
+        assert n >= 8
+        loss0 = g.learn_train(1, 1e-3, 2, seed=1)
+        loss1 = g.learn_train(6, 1e-3, 2, seed=2)
+        print(f"samples {n}, loss after 1 epoch {loss0:.3e}, after 7 {loss1:.3e}", flush=True)
+        assert np.isfinite(loss1) and loss1 < loss0
+        g.guess_enable(True)
+        used = 0
+        for _ in range(3):
+            g.time_step(1)
+            st = g.stats()
+            used += st.guess_used
+            assert st.residual_last <= params.tolerance * st.res_ref_last
+        print(f"guess used in {used} of the last 3 steps' final solves", flush=True)
+        g.learn_attach(None)

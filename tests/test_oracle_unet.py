@@ -1,0 +1,46 @@
+"""Pins of the CPU U-Net reference (oracle/unet.py) that the GPU network is
+checked against: its hand-written backward pass equals central finite
+differences of its forward loss, and its layer table and parameter count
+equal the library's network definition (tracer_code_amd/learn.py)."""
+import numpy as np
+
+from oracle import unet
+from tracer_code_amd import learn
+
+
+def test_layer_table_and_count_match_product():
+    assert [(ci, co) for ci, co, _ in unet.LAYERS] == learn.LAYERS
+    assert unet.param_count() == learn.param_count() == 487585
+
+
+def test_backward_matches_finite_differences():
+    th = unet.init_params(1)
+    rng = np.random.default_rng(2)
+    H, W = 16, 16
+    x = rng.standard_normal((H, W, 5))
+    t = rng.standard_normal((H, W))
+    m = np.ones((H, W))
+    m[0, :] = 0
+    m[:, 0] = 0
+    _, _, g = unet.UNetRef(th).loss_and_grad(x, t, m)
+    # one index per layer's weights and biases, and the head
+    idx, o = [], 0
+    for ci, co, _ in unet.LAYERS:
+        idx += [o + 3, o + co * 9 * ci + 1]
+        o += co * 9 * ci + co
+    idx += [o + 2, o + 16]
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(th)
+        e[i] = h
+        lp, _, _ = unet.UNetRef(th + e).loss_and_grad(x, t, m)
+        lm, _, _ = unet.UNetRef(th - e).loss_and_grad(x, t, m)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-8, (i, fd, g[i])
+
+
+def test_bf16_rounding():
+    # 8 significant bits: ties go to even, the rest to nearest
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 3 * 2 ** -9, -3.0, 0.0])
+    r = unet.bf16(x)
+    assert list(r) == [1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -3.0, 0.0]

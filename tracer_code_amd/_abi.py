@@ -105,6 +105,22 @@ SIGNATURES = {
     "tc_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_kernel_timing": (ctypes.c_int, [_VP, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "tc_derivative_eigenvalues": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "tc_nn_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(_VP)]),
+    "tc_nn_destroy": (ctypes.c_int, [_VP]),
+    "tc_nn_last_error": (ctypes.c_char_p, [_VP]),
+    "tc_nn_param_count": (ctypes.c_int, [_VP, _P(ctypes.c_int64)]),
+    "tc_nn_set_weights": (ctypes.c_int, [_VP, _VP, ctypes.c_int64]),
+    "tc_nn_get_weights": (ctypes.c_int, [_VP, _VP, ctypes.c_int64]),
+    "tc_nn_get_grads": (ctypes.c_int, [_VP, _VP, ctypes.c_int64]),
+    "tc_nn_set_sample": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "tc_nn_forward": (ctypes.c_int, [_VP, _VP]),
+    "tc_nn_backward": (ctypes.c_int, [_VP, _P(ctypes.c_double)]),
+    "tc_nn_adam": (ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float]),
+    "tc_learn_attach": (ctypes.c_int, [_VP, _VP]),
+    "tc_learn_record": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_learn_samples": (ctypes.c_int, [_VP, _P(ctypes.c_int32)]),
+    "tc_learn_train": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.c_uint64,
+                                      _P(ctypes.c_double)]),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

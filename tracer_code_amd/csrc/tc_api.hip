@@ -12,6 +12,7 @@
 #include <new>
 
 #include "tc_kernels.h"
+#include "tc_nn.h"
 
 using namespace tc;
 
@@ -58,6 +59,7 @@ struct tc_dev {
     double* d_scal = nullptr;
     double* h_scal = nullptr;
     double* partialA = nullptr;
+    int learn_slot = -1;       // recorded sample of the solve in progress
     double* partialB = nullptr;
     int max_partial = 0;
     bool coarse_ready = false;
@@ -440,6 +442,9 @@ int stop_test(tc_ctx* ctx, tc_dev& d, int q, bool& done)
     return 0;
 }
 
+int learn_pre_solve(tc_ctx* ctx, tc_dev& d, bool allow_guess, int* slot, bool* have_guess);
+int learn_post_solve(tc_ctx* ctx, int slot, const double* psi);
+
 int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_out)
 {
     TC_TRY(wait_coarse(ctx, DEV(ctx)));
@@ -451,7 +456,11 @@ int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_o
         ctx->err = "DGBTRF: singular coarse-grid matrix";
         status = TC_ERR_SINGULAR;
     }
-    if (status == 0) TC_TRY(copy_plane(ctx, ctx->SF, psi, ctx->lv[1].g));
+    if (status == 0) {
+        TC_TRY(copy_plane(ctx, ctx->SF, psi, ctx->lv[1].g));
+        TC_TRY(learn_post_solve(ctx, DEV(ctx).learn_slot, ctx->SF));
+    }
+    DEV(ctx).learn_slot = -1;
     ctx->stats.q_last = q;
     ctx->stats.residual_last = ctx->residual_mag[1];
     ctx->stats.res_ref_last = ctx->res_ref[1];
@@ -473,6 +482,10 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
     L1.cur = 0;
     TC_TRY(copy_plane(ctx, L1.err[0], ctx->SF, L1.g));
     TC_TRY(solve_setup(ctx, d));
+    {
+        bool hg = false;
+        TC_TRY(learn_pre_solve(ctx, d, false, &d.learn_slot, &hg));
+    }
     bool first_sweep = true;
     const int* multi = P.multi;
     int q = 1;
@@ -524,6 +537,57 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
         }
     }
     return finish_solve(ctx, status, q, L1.err[L1.cur], q_out);
+}
+
+// ---------------------------------------------------------------------------
+// learned initial guess: inputs before a solve, targets and history after
+// ---------------------------------------------------------------------------
+int learn_pre_solve(tc_ctx* ctx, tc_dev& d, bool allow_guess, int* slot, bool* have_guess)
+{
+    *slot = -1;
+    *have_guess = false;
+    if (!ctx->nn || ctx->nhist < 1) return 0;
+    UNet& u = ctx->nn->net;
+    const Grid& g = ctx->lv[1].g;
+    const size_t np = (size_t)u.npx[0];
+    const double* C = (ctx->p.comp == 1 && ctx->p.ntype > 0) ? ctx->Cbuoy[0] : nullptr;
+    launch_nn_scales(ctx->stream, ctx->psi_h1, ctx->Tbuoy, ctx->T_h1, g, ctx->nn_part, ctx->nn_scales);
+    if (ctx->rec_n < ctx->rec_cap) {
+        *slot = ctx->rec_n;
+        launch_nn_input(ctx->stream, ctx->Tbuoy, C, ctx->lv[1].vis, ctx->psi_h1, ctx->T_h1, g, u.W, u.H,
+                        ctx->nn_scales, ctx->rec_x + (size_t)*slot * np * 8);
+        TC_HIP(hipMemcpyAsync(ctx->rec_scales + 3 * *slot, ctx->nn_scales, 3 * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (allow_guess && ctx->guess) {
+        launch_nn_input(ctx->stream, ctx->Tbuoy, C, ctx->lv[1].vis, ctx->psi_h1, ctx->T_h1, g, u.W, u.H,
+                        ctx->nn_scales, u.x0);
+        u.forward(ctx->stream);
+        const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
+        launch_nn_apply(ctx->stream, u.out, ctx->psi_h1, ctx->guess_psi, g, u.W, ctx->nn_scales, zsg);
+        *have_guess = true;
+    }
+    return 0;
+}
+
+int learn_post_solve(tc_ctx* ctx, int slot, const double* psi)
+{
+    if (!ctx->nn) return 0;
+    UNet& u = ctx->nn->net;
+    const Grid& g = ctx->lv[1].g;
+    if (slot >= 0) {
+        const size_t np = (size_t)u.npx[0];
+        // the scales of the inputs (max |psi_prev - psi_prev2|) normalise the target
+        TC_HIP(hipMemcpyAsync(ctx->nn_scales, ctx->rec_scales + 3 * slot, 3 * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        launch_nn_target(ctx->stream, psi, ctx->psi_h1, g, u.W, u.H, ctx->nn_scales, ctx->rec_t + (size_t)slot * np,
+                         ctx->rec_mask);
+        ctx->rec_n++;
+    }
+    TC_TRY(copy_plane(ctx, ctx->psi_h1, psi, g));
+    TC_TRY(copy_plane(ctx, ctx->T_h1, ctx->Tbuoy, g));
+    ctx->nhist = 1;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -611,10 +675,33 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
     L1.cur32 = 0;
     if (f32) TC_HIP(hipMemsetAsync(L1.err32[0], 0, L1.g.elems * sizeof(float), ctx->stream));
     else TC_HIP(hipMemsetAsync(L1.err[0], 0, plane_bytes(L1.g), ctx->stream));
-    {
-        int nb = launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res,
-                                    f32 ? L1.res32 : nullptr, d.F, d.h[1], P.ivis, d.partialA);
-        (void)nb;
+    // initial defect; with the learned guess, iterate from whichever of the
+    // guess and the plain warm start has the smaller residual
+    bool have_guess = false;
+    TC_TRY(learn_pre_solve(ctx, d, true, &d.learn_slot, &have_guess));
+    auto defect = [&](const double* hi, double* mag) -> int {
+        const int nb = launch_residual_dd(ctx->stream, hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr,
+                                          d.F, d.h[1], P.ivis, d.partialA);
+        if (mag) {
+            launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
+            TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+            *mag = d.h_scal[S_RESID1];
+        }
+        return 0;
+    };
+    ctx->stats.guess_used = 0;
+    if (have_guess) {
+        double m_plain = 0.0, m_guess = 0.0;
+        TC_TRY(defect(ctx->psi_hi, &m_plain));
+        TC_TRY(defect(ctx->guess_psi, &m_guess));
+        if (m_guess < m_plain) {
+            TC_TRY(copy_plane(ctx, ctx->psi_hi, ctx->guess_psi, L1.g));
+            ctx->stats.guess_used = 1;
+        } else {
+            TC_TRY(defect(ctx->psi_hi, nullptr));
+        }
+    } else {
+        TC_TRY(defect(ctx->psi_hi, nullptr));
     }
     const int* multi = P.multi;
     int q = 1;
@@ -1001,11 +1088,13 @@ void free_ctx(tc_ctx* ctx)
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
                    ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
-                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo};
+                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->psi_h1, ctx->T_h1,
+                   ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi};
     for (double* p : b) F(p);
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
     F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag);
+    F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
@@ -1501,7 +1590,118 @@ int tc_kernel_timing(tc_ctx* ctx, int which, double* avg_us, int64_t* count)
 int tc_guess_enable(tc_ctx* ctx, int enable)
 {
     if (!ctx) return TC_ERR_ARG;
+    if (enable && !ctx->nn) {
+        ctx->err = "no network attached (tc_learn_attach)";
+        return TC_ERR_ARG;
+    }
     ctx->guess = enable ? 1 : 0;
+    return 0;
+}
+
+int tc_learn_attach(tc_ctx* ctx, tc_nn* nn)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (nn && (nn->net.H != ctx->p.nz || nn->net.W != ctx->p.nx)) {
+        ctx->err = "network grid must be H = nz, W = nx";
+        return TC_ERR_ARG;
+    }
+    ctx->nn = nn;
+    ctx->nhist = 0;
+    ctx->guess = 0;
+    if (!nn) return 0;
+    const Grid& g = ctx->lv[1].g;
+    if (!ctx->psi_h1) {
+        TC_TRY(dalloc(ctx, &ctx->psi_h1, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->T_h1, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->guess_psi, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->nn_scales, 3));
+        const size_t nb = (size_t)((g.nx - 1 + 63) / 64) * (size_t)((g.nz - 1 + 3) / 4);
+        TC_TRY(dalloc(ctx, &ctx->nn_part, 2 * nb));
+    }
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_learn_record(tc_ctx* ctx, int32_t capacity)
+{
+    if (!ctx || capacity < 0) return TC_ERR_ARG;
+    if (!ctx->nn) {
+        ctx->err = "no network attached (tc_learn_attach)";
+        return TC_ERR_ARG;
+    }
+    const size_t np = (size_t)ctx->p.nx * (size_t)ctx->p.nz;
+    auto F = [](void* q) { if (q) (void)hipFree(q); };
+    F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask); F(ctx->rec_scales);
+    ctx->rec_x = nullptr; ctx->rec_t = nullptr; ctx->rec_mask = nullptr; ctx->rec_scales = nullptr;
+    ctx->rec_cap = 0;
+    ctx->rec_n = 0;
+    if (capacity == 0) return 0;
+    TC_HIP(hipMalloc((void**)&ctx->rec_x, (size_t)capacity * np * 8 * sizeof(__bf16)));
+    TC_HIP(hipMalloc((void**)&ctx->rec_t, (size_t)capacity * np * sizeof(float)));
+    TC_HIP(hipMalloc((void**)&ctx->rec_mask, np * sizeof(float)));
+    TC_HIP(hipMalloc((void**)&ctx->rec_scales, (size_t)capacity * 3 * sizeof(double)));
+    ctx->rec_cap = capacity;
+    return 0;
+}
+
+int tc_learn_samples(const tc_ctx* ctx, int32_t* count)
+{
+    if (!ctx || !count) return TC_ERR_ARG;
+    *count = ctx->rec_n;
+    return 0;
+}
+
+// Adam on the recorded samples: each epoch visits them in a seeded random
+// order, one forward + backward per sample, an update every `batch` samples.
+int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_t seed, double* mean_loss)
+{
+    if (!ctx || !ctx->nn || !ctx->nn->net.trainable || epochs < 1 || batch < 1) return TC_ERR_ARG;
+    if (ctx->rec_n < 1) {
+        ctx->err = "no recorded samples";
+        return TC_ERR_ARG;
+    }
+    UNet& u = ctx->nn->net;
+    const size_t np = (size_t)u.npx[0];
+    // mask: interior nodes, the same for every sample
+    {
+        std::vector<float> m(np);
+        for (int k = 0; k < u.H; ++k)
+            for (int i = 0; i < u.W; ++i) m[(size_t)k * u.W + i] = (i >= 1 && k >= 1) ? 1.0f : 0.0f;
+        TC_HIP(hipMemcpyAsync(u.mask, m.data(), np * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        u.inv_n = (float)(1.0 / ((double)(u.W - 1) * (double)(u.H - 1)));
+    }
+    std::vector<int> order(ctx->rec_n);
+    uint64_t st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+    double sum = 0.0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int q = 0; q < ctx->rec_n; ++q) order[q] = q;
+        for (int q = ctx->rec_n - 1; q > 0; --q) {
+            st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+            const int r = (int)((st >> 33) % (uint64_t)(q + 1));
+            std::swap(order[q], order[r]);
+        }
+        TC_HIP(hipMemsetAsync(u.loss_d, 0, sizeof(double), ctx->stream));
+        int inb = 0;
+        for (int q = 0; q < ctx->rec_n; ++q) {
+            const int sidx = order[q];
+            TC_HIP(hipMemcpyAsync(u.x0, ctx->rec_x + (size_t)sidx * np * 8, np * 8 * sizeof(__bf16),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+            TC_HIP(hipMemcpyAsync(u.target, ctx->rec_t + (size_t)sidx * np, np * sizeof(float), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+            u.forward(ctx->stream);
+            u.backward(ctx->stream);
+            if (++inb == batch || q == ctx->rec_n - 1) {
+                u.adam(ctx->stream, lr, 1.0f / (float)inb);
+                inb = 0;
+            }
+        }
+        double l = 0.0;
+        TC_HIP(hipMemcpyAsync(&l, u.loss_d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        sum = l / (double)ctx->rec_n;
+    }
+    if (mean_loss) *mean_loss = sum;
     return 0;
 }
 

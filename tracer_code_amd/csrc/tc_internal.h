@@ -104,8 +104,18 @@ struct tc_ctx {
     double time = 0.0, dt = 0.0;
     int tstep = 0;
     tc_stats stats{};
-    // learned guess
-    void* guess_net = nullptr;
+    // learned guess (tc_nn.hip): attached network, solution history, recorded samples
+    struct tc_nn* nn = nullptr;
+    double *psi_h1 = nullptr, *T_h1 = nullptr;  // solution and buoyancy temperature of the last solve
+    int nhist = 0;
+    int rec_cap = 0, rec_n = 0;
+    __bf16* rec_x = nullptr;      // rec_cap x [nz*nx*8] inputs
+    float* rec_t = nullptr;       // rec_cap x [nz*nx] targets
+    float* rec_mask = nullptr;    // [nz*nx]
+    double* nn_scales = nullptr;  // [3] scales of the current solve: max|psi_prev|, increment scale, max|dT|
+    double* rec_scales = nullptr; // rec_cap x [3] scales of the recorded samples
+    double* nn_part = nullptr;    // block partials of the scale reduction
+    double* guess_psi = nullptr;  // candidate initial guess
     // implementation-private host bookkeeping (tc_api.hip)
     void* impl = nullptr;
     // kernel timing (tc_set_profiling)

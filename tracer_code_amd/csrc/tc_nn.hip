@@ -1,0 +1,849 @@
+// Learned initial guess for the multigrid solve: a 4-level U-Net (channels
+// 16/32/64/128, 3x3 convolutions, GELU) on the node grid, BF16 weights and
+// activations with FP32 accumulation on the MFMA units (v_mfma_f32_16x16x32_bf16),
+// and hand-written forward, backward and Adam kernels.
+//
+// Inputs per node (i,k), i < nx, k < nz: T, C, log10(eta)/5, psi_prev/s
+// and (T - T_prev)/dT, where psi_prev and T_prev are the solution and the
+// buoyancy temperature of the previous solve, s = max |psi_prev| and
+// dT = max |T - T_prev|; output: the increment psi - psi_prev divided by
+// s*dT.  The increment responds (almost linearly) to the change of the
+// buoyancy field, which differs irregularly between the RK stages.  There is no
+// counterpart in the reference: the hook is the warm start
+// error(1,:,:) = SF of multigrid (iterative_solver.f90:87).
+//
+// Layout: activations are NHWC, pixel p = k*W + i, channels padded to a
+// multiple of 8 (16-byte fragments).  A convolution is an implicit GEMM
+// out[p][co] = sum_{tap,ci} X[p+off(tap)][ci] W[co][tap*CINP+ci] with zero
+// padding; one wavefront computes a 16-pixel x 16-channel tile.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "tc_kernels.h"
+#include "tc_nn.h"
+
+namespace tc {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float gelu_f(float z)
+{
+    const float a = 0.7978845608028654f, b = 0.044715f;
+    return 0.5f * z * (1.0f + tanhf(a * (z + b * z * z * z)));
+}
+__device__ __forceinline__ float gelu_grad(float z)
+{
+    const float a = 0.7978845608028654f, b = 0.044715f;
+    const float u = a * (z + b * z * z * z);
+    const float t = tanhf(u);
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * a * (1.0f + 3.0f * b * z * z);
+}
+
+// ---------------------------------------------------------------------------
+// Convolution as implicit GEMM.  in: NHWC with channel stride cs_in, the
+// layer reading channels [off_in, off_in + CINP); wk: [cout][KP] bf16 with
+// k = tap*CINP + ci (tap = (dy+1)*3 + (dx+1)), zero past 9*CINP.
+// mode 0: z = acc + bias -> zout (bf16), y = gelu(z) -> out (bf16)
+// mode 1: out32[p][off_out + co] += acc (FP32 gradient accumulation)
+// mode 2: y = acc + bias -> out (bf16), no activation
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+    const __bf16* in;
+    int cs_in, off_in, cinp;
+    const __bf16* wk;
+    int kp, cout;
+    const float* bias;
+    __bf16* out;
+    __bf16* zout;
+    float* out32;
+    int cs_out, off_out;
+    int H, W;
+    int mode;
+};
+
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int npix = a.H * a.W;
+    const int p0 = (blockIdx.x * 4 + wv) * 16;
+    const int cb = blockIdx.y;
+    if (p0 >= npix) return;
+    const int row = lane & 15, grp = lane >> 4;
+    const int p = p0 + row;
+    const int py = p / a.W, px = p - py * a.W;
+    const int co = cb * 16 + row;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = grp * 8; k0 < a.kp; k0 += 32) {
+        bf16x8 fa, fb;
+        const int tap = k0 / a.cinp, ci0 = k0 - tap * a.cinp;
+        bool ok = tap < 9 && p < npix;
+        int q = 0;
+        if (ok) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const int yy = py + dy, xx = px + dx;
+            ok = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            q = yy * a.W + xx;
+        }
+        if (ok) fa = *reinterpret_cast<const bf16x8*>(a.in + (size_t)q * a.cs_in + a.off_in + ci0);
+        else
+            for (int j = 0; j < 8; ++j) fa[j] = (__bf16)0.0f;
+        fb = *reinterpret_cast<const bf16x8*>(a.wk + (size_t)co * a.kp + k0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+    }
+    // lane holds D[pixel p0 + grp*4 + i][channel cb*16 + row]
+    const int c = cb * 16 + row;
+    for (int i = 0; i < 4; ++i) {
+        const int pp = p0 + grp * 4 + i;
+        if (pp >= npix) continue;
+        const size_t o = (size_t)pp * a.cs_out + a.off_out + c;
+        if (a.mode == 1) {
+            a.out32[o] += acc[i];
+        } else {
+            const float z = acc[i] + (a.bias ? a.bias[c] : 0.0f);
+            if (a.mode == 0) {
+                a.zout[(size_t)pp * a.cout + c] = (__bf16)z;
+                a.out[o] = (__bf16)gelu_f(z);
+            } else {
+                a.out[o] = (__bf16)z;
+            }
+        }
+    }
+}
+
+// Weight gradient: partial[split][co][k] = sum over the split's pixels of
+// dz[p][co] * X[p+off(tap)][ci]; as an MFMA product the rows are output
+// channels, the columns k, the reduction runs over 32 pixels at a time.
+struct WGradArgs {
+    const __bf16* dz;  // [npix][cout]
+    const __bf16* in;
+    int cs_in, off_in, cinp;
+    int kp, cout;
+    float* partial;    // [nsplit][cout][kp]
+    int H, W, pix_per_split;
+};
+
+__global__ __launch_bounds__(64) void k_conv3x3_wgrad(WGradArgs a)
+{
+    const int lane = threadIdx.x;
+    const int cb = blockIdx.x, kb = blockIdx.y, split = blockIdx.z;
+    const int npix = a.H * a.W;
+    const int row = lane & 15, grp = lane >> 4;
+    const int co = cb * 16 + row;   // A row
+    const int kcol = kb * 16 + row; // B column
+    const int tap = kcol / a.cinp, ci = kcol - tap * a.cinp;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int pbeg = split * a.pix_per_split;
+    int pend = pbeg + a.pix_per_split;
+    if (pend > npix) pend = npix;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int pb = pbeg; pb < pend; pb += 32) {
+        bf16x8 fa, fb;
+        for (int j = 0; j < 8; ++j) {
+            const int p = pb + grp * 8 + j;
+            __bf16 va = (__bf16)0.0f, vb = (__bf16)0.0f;
+            if (p < pend) {
+                va = a.dz[(size_t)p * a.cout + co];
+                if (tap < 9) {
+                    const int py = p / a.W, px = p - py * a.W;
+                    const int yy = py + dy, xx = px + dx;
+                    if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                        vb = a.in[(size_t)(yy * a.W + xx) * a.cs_in + a.off_in + ci];
+                }
+            }
+            fa[j] = va;
+            fb[j] = vb;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb, acc, 0, 0, 0);
+    }
+    // lane holds D[co = cb*16 + grp*4 + i][k = kb*16 + row]
+    for (int i = 0; i < 4; ++i)
+        a.partial[((size_t)split * a.cout + cb * 16 + grp * 4 + i) * a.kp + kcol] = acc[i];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_splits(const float* __restrict__ partial, int nsplit, size_t n,
+                                                       float* __restrict__ out)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.0f;
+    for (int q = 0; q < nsplit; ++q) s += partial[(size_t)q * n + e];
+    out[e] += s;
+}
+
+// bias gradient: db[c] += sum_p dz[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const __bf16* __restrict__ dz, int npix, int cout,
+                                                   float* __restrict__ db)
+{
+    const int c = blockIdx.x;
+    float s = 0.0f;
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) s += (float)dz[(size_t)p * cout + c];
+    s = (float)block_sum((double)s);
+    if (threadIdx.x == 0) db[c] += s;
+}
+
+// dz = dy * gelu'(z) (bf16), and clear dy
+__global__ __launch_bounds__(256) void k_gelu_back(float* __restrict__ dy, int cs, int off, const __bf16* __restrict__ z,
+                                                   __bf16* __restrict__ dz, int npix, int cout)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)npix * cout) return;
+    const size_t p = e / cout, c = e - p * cout;
+    const size_t o = p * cs + off + c;
+    dz[e] = (__bf16)(dy[o] * gelu_grad((float)z[e]));
+    dy[o] = 0.0f;
+}
+
+// 2x2 average pool of channels [off, off+C) (stride cs) into [0,C) (stride C)
+__global__ __launch_bounds__(256) void k_pool(const __bf16* __restrict__ in, int cs, int off, __bf16* __restrict__ out,
+                                              int C, int H, int W)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Ho * Wo * C) return;
+    const int c = (int)(e % C);
+    const size_t po = e / C;
+    const int y = (int)(po / Wo), x = (int)(po % Wo);
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) s += (float)in[(size_t)((2 * y + a) * W + 2 * x + b) * cs + off + c];
+    out[e] = (__bf16)(0.25f * s);
+}
+// dIn[fine][off+c] += 0.25 * dOut[coarse][c]
+__global__ __launch_bounds__(256) void k_pool_back(float* __restrict__ din, int cs, int off,
+                                                   const float* __restrict__ dout, int C, int H, int W)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)H * W * C) return;
+    const int c = (int)(e % C);
+    const size_t p = e / C;
+    const int y = (int)(p / W), x = (int)(p % W);
+    din[p * cs + off + c] += 0.25f * dout[(size_t)((y / 2) * (W / 2) + x / 2) * C + c];
+}
+// nearest 2x upsampling of [0,C) (stride C) at H/2 x W/2 into [off, off+C) (stride cs)
+__global__ __launch_bounds__(256) void k_up(const __bf16* __restrict__ in, __bf16* __restrict__ out, int cs, int off,
+                                            int C, int H, int W)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)H * W * C) return;
+    const int c = (int)(e % C);
+    const size_t p = e / C;
+    const int y = (int)(p / W), x = (int)(p % W);
+    out[p * cs + off + c] = in[(size_t)((y / 2) * (W / 2) + x / 2) * C + c];
+}
+// dIn[coarse][c] += sum of the 4 fine dOut[.][off+c]
+__global__ __launch_bounds__(256) void k_up_back(const float* __restrict__ dout, int cs, int off, float* __restrict__ din,
+                                                 int C, int H, int W)
+{
+    const int Hc = H / 2, Wc = W / 2;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Hc * Wc * C) return;
+    const int c = (int)(e % C);
+    const size_t pc = e / C;
+    const int y = (int)(pc / Wc), x = (int)(pc % Wc);
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const size_t o = (size_t)((2 * y + a) * W + 2 * x + b) * cs + off + c;
+            s += dout[o];
+        }
+    din[e] += s;
+}
+
+// 1x1 output layer 16 -> 1 (FP32): out[p] = sum_c w[c] y[p][c] + b
+__global__ __launch_bounds__(256) void k_head(const __bf16* __restrict__ y, const float* __restrict__ w,
+                                              const float* __restrict__ b, float* __restrict__ out, int npix)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float s = b[0];
+    for (int c = 0; c < 16; ++c) s += w[c] * (float)y[(size_t)p * 16 + c];
+    out[p] = s;
+}
+
+// MSE loss over the mask, d(out) and the head's gradients.  Per-block
+// partial sums of the loss and of the head gradients (17 values).
+__global__ __launch_bounds__(256) void k_head_back(const float* __restrict__ out, const float* __restrict__ target,
+                                                   const float* __restrict__ mask, const __bf16* __restrict__ y,
+                                                   const float* __restrict__ w, float* __restrict__ dy, int npix,
+                                                   float inv_n, double* __restrict__ part)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0, g[17];
+    for (int c = 0; c < 17; ++c) g[c] = 0.0;
+    if (p < npix) {
+        const float m = mask[p];
+        const float d = (out[p] - target[p]) * m;
+        l = (double)d * d * inv_n;
+        const float dout = 2.0f * d * inv_n;
+        for (int c = 0; c < 16; ++c) {
+            const float yc = (float)y[(size_t)p * 16 + c];
+            g[c] = (double)dout * yc;
+            dy[(size_t)p * 16 + c] += dout * w[c];
+        }
+        g[16] = dout;
+    }
+    l = block_sum(l);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * 18] = l;
+    for (int c = 0; c < 17; ++c) {
+        const double s = block_sum(g[c]);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * 18 + 1 + c] = s;
+    }
+}
+__global__ __launch_bounds__(256) void k_head_reduce(const double* __restrict__ part, int nb, float* __restrict__ gw,
+                                                     float* __restrict__ gb, double* __restrict__ loss)
+{
+    const int c = threadIdx.x;
+    if (c >= 18) return;
+    double s = 0.0;
+    for (int q = 0; q < nb; ++q) s += part[(size_t)q * 18 + c];
+    if (c == 0) *loss += s;
+    else if (c <= 16) gw[c - 1] += (float)s;
+    else gb[0] += (float)s;
+}
+
+// Adam (bias-corrected) on FP32 master weights
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2, float scale)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gg = g[e] * scale;
+    const float mm = b1 * m[e] + (1.0f - b1) * gg;
+    const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    w[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    g[e] = 0.0f;
+}
+
+// pack FP32 master weights W[co][tap][ci] (ci < cin) into the forward layout
+// wk[co][tap*cinp + ci] and the backward-data layout
+// wt[ci][tap'*coutp + co] with tap' = 8 - tap (zero padded, bf16)
+__global__ __launch_bounds__(256) void k_pack(const float* __restrict__ w, int cin, int cinp, int cout, int kp,
+                                              int kpt, __bf16* __restrict__ wk, __bf16* __restrict__ wt, int cinp16)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n1 = (size_t)cout * kp;
+    if (e < n1) {
+        const int co = (int)(e / kp), k = (int)(e % kp);
+        const int tap = k / cinp, ci = k % cinp;
+        float v = 0.0f;
+        if (tap < 9 && ci < cin) v = w[((size_t)co * 9 + tap) * cin + ci];
+        wk[e] = (__bf16)v;
+    }
+    const size_t n2 = (size_t)cinp16 * kpt;
+    if (e < n2) {
+        const int ci = (int)(e / kpt), k = (int)(e % kpt);
+        const int tp = k / cout, co = k % cout;
+        float v = 0.0f;
+        if (tp < 9 && ci < cin) v = w[((size_t)co * 9 + (8 - tp)) * cin + ci];
+        wt[e] = (__bf16)v;
+    }
+}
+
+// gradient of the packed layout back to W[co][tap][ci] (accumulated)
+__global__ __launch_bounds__(256) void k_unpack_grad(const float* __restrict__ gk, int cin, int cinp, int cout, int kp,
+                                                     float* __restrict__ g)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)cout * 9 * cin) return;
+    const int ci = (int)(e % cin);
+    const size_t r = e / cin;
+    const int tap = (int)(r % 9), co = (int)(r / 9);
+    g[e] += gk[(size_t)co * kp + tap * cinp + ci];
+}
+
+// input assembly: node (i,k) of the solver plane -> pixel k*W+i (8 channels)
+// scales[0] = s = max |psi_prev|, scales[2] = dT = max |T - T_prev| (1 if
+// zero), scales[1] = s * dT: the scale of the predicted increment
+__global__ __launch_bounds__(256) void k_nn_input(const double* __restrict__ T, const double* __restrict__ C,
+                                                  const double* __restrict__ vis, const double* __restrict__ p1,
+                                                  const double* __restrict__ Tp, Grid g, int W, int H,
+                                                  const double* __restrict__ scales, __bf16* __restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= W || k >= H) return;
+    const double inv_s = 1.0 / scales[0], inv_dt = 1.0 / scales[2];
+    const size_t c = g.at(i, k);
+    const size_t o = ((size_t)k * W + i) * 8;
+    x[o + 0] = (__bf16)(float)T[c];
+    x[o + 1] = (__bf16)(float)(C ? C[c] : 0.0);
+    x[o + 2] = (__bf16)(float)(log10(vis[c]) * 0.2);
+    x[o + 3] = (__bf16)(float)(p1[c] * inv_s);
+    x[o + 4] = (__bf16)(float)((T[c] - Tp[c]) * inv_dt);
+    for (int j = 5; j < 8; ++j) x[o + j] = (__bf16)0.0f;
+}
+// training target: (psi - psi_prev) / sd on interior nodes, mask 1 there
+__global__ __launch_bounds__(256) void k_nn_target(const double* __restrict__ psi, const double* __restrict__ p1, Grid g,
+                                                   int W, int H, const double* __restrict__ scales,
+                                                   float* __restrict__ t, float* __restrict__ mask)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= W || k >= H) return;
+    const double inv_sd = 1.0 / scales[1];
+    const size_t c = g.at(i, k);
+    const bool in = i >= 1 && k >= 1;
+    t[(size_t)k * W + i] = in ? (float)((psi[c] - p1[c]) * inv_sd) : 0.0f;
+    mask[(size_t)k * W + i] = in ? 1.0f : 0.0f;
+}
+// guess: psi = psi_prev + sd * out on the interior, with enforceBCs_stream mirrors
+__global__ __launch_bounds__(256) void k_nn_apply(const float* __restrict__ out, const double* __restrict__ p1,
+                                                  double* __restrict__ psi, Grid g, int W,
+                                                  const double* __restrict__ scales, double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const double sd = scales[1];
+    const size_t c = g.at(i, k);
+    const double v = p1[c] + sd * (double)out[(size_t)k * W + i];
+    psi[c] = v;
+    stream_ghosts(psi, g, i, k, v, zsg);
+}
+
+// max |psi_prev| and max |T - T_prev| over the interior: per-block
+// partials, then the final pass writes the scales (guarded against zero)
+__global__ __launch_bounds__(256) void k_nn_scale_part(const double* __restrict__ p1, const double* __restrict__ T,
+                                                       const double* __restrict__ Tp, Grid g, double* __restrict__ part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    double a = 0.0, b = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        a = fabs(p1[c]);
+        b = fabs(T[c] - Tp[c]);
+    }
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        part[2 * (blockIdx.y * gridDim.x + blockIdx.x)] = a;
+        part[2 * (blockIdx.y * gridDim.x + blockIdx.x) + 1] = b;
+    }
+}
+__global__ __launch_bounds__(256) void k_nn_scale_final(const double* __restrict__ part, int n, double* __restrict__ sc)
+{
+    double a = 0.0, b = 0.0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        a = fmax(a, part[2 * t]);
+        b = fmax(b, part[2 * t + 1]);
+    }
+    a = block_max(a);
+    b = block_max(b);
+    if (threadIdx.x == 0) {
+        const double s = a > 0.0 ? a : 1.0, dT = b > 0.0 ? b : 1.0;
+        sc[0] = s;
+        sc[1] = s * dT;
+        sc[2] = dT;
+    }
+}
+
+void launch_nn_input(hipStream_t st, const double* T, const double* C, const double* vis, const double* p1,
+                     const double* Tp, const Grid& g, int W, int H, const double* scales, __bf16* x)
+{
+    hipLaunchKernelGGL(k_nn_input, dim3((W + 255) / 256, H), dim3(256), 0, st, T, C, vis, p1, Tp, g, W, H, scales, x);
+}
+void launch_nn_target(hipStream_t st, const double* psi, const double* p1, const Grid& g, int W, int H,
+                      const double* scales, float* t, float* mask)
+{
+    hipLaunchKernelGGL(k_nn_target, dim3((W + 255) / 256, H), dim3(256), 0, st, psi, p1, g, W, H, scales, t, mask);
+}
+void launch_nn_apply(hipStream_t st, const float* out, const double* p1, double* psi, const Grid& g, int W,
+                     const double* scales, double zsg)
+{
+    hipLaunchKernelGGL(k_nn_apply, dim3((g.nx - 1 + 255) / 256, g.nz - 1), dim3(256), 0, st, out, p1, psi, g, W, scales,
+                       zsg);
+}
+void launch_nn_scales(hipStream_t st, const double* p1, const double* T, const double* Tp, const Grid& g,
+                      double* partial, double* scales)
+{
+    dim3 b(64, 4);
+    dim3 gr((g.nx - 1 + 63) / 64, (g.nz - 1 + 3) / 4);
+    hipLaunchKernelGGL(k_nn_scale_part, gr, b, 0, st, p1, T, Tp, g, partial);
+    hipLaunchKernelGGL(k_nn_scale_final, dim3(1), dim3(256), 0, st, partial, (int)(gr.x * gr.y), scales);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(size_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void UNet::layer_defs()
+{
+    // name-free list in forward order: (cin, cout, level)
+    const int c1 = 16, c2 = 32, c3 = 64, c4 = 128;
+    L.clear();
+    auto add = [&](int cin, int cout, int lev) { L.push_back(Layer{cin, cout, lev}); };
+    add(5, c1, 0); add(c1, c1, 0);            // e1a e1b
+    add(c1, c2, 1); add(c2, c2, 1);           // e2a e2b
+    add(c2, c3, 2); add(c3, c3, 2);           // e3a e3b
+    add(c3, c4, 3); add(c4, c4, 3);           // b4a b4b
+    add(c4 + c3, c3, 2); add(c3, c3, 2);      // d3a d3b
+    add(c3 + c2, c2, 1); add(c2, c2, 1);      // d2a d2b
+    add(c2 + c1, c1, 0); add(c1, c1, 0);      // d1a d1b
+}
+
+static inline int up8(int c) { return (c + 7) / 8 * 8; }
+static inline int up16(int c) { return (c + 15) / 16 * 16; }
+static inline int up32(int c) { return (c + 31) / 32 * 32; }
+
+size_t UNet::param_count() const
+{
+    size_t n = 0;
+    for (const Layer& l : L) n += (size_t)l.cout * 9 * l.cin + l.cout;
+    return n + 16 + 1;
+}
+
+int UNet::alloc(int H_, int W_, bool train)
+{
+    layer_defs();
+    H = H_;
+    W = W_;
+    if (H % 8 || W % 8 || H < 8 || W < 8) return TC_ERR_ARG;
+    auto dmal = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) return TC_ERR_NOMEM;
+        if (hipMemset(*p, 0, bytes) != hipSuccess) return TC_ERR_HIP;
+        owned.push_back(*p);
+        return 0;
+    };
+    nparam = param_count();
+    TC_NN_TRY(dmal((void**)&w32, nparam * sizeof(float)));
+    if (train) {
+        TC_NN_TRY(dmal((void**)&g32, nparam * sizeof(float)));
+        TC_NN_TRY(dmal((void**)&m32, nparam * sizeof(float)));
+        TC_NN_TRY(dmal((void**)&v32, nparam * sizeof(float)));
+    }
+    // per-layer packed weights
+    size_t off = 0;
+    for (Layer& l : L) {
+        l.cinp = up8(l.cin);
+        l.kp = up32(9 * l.cinp);
+        l.kpt = up32(9 * l.cout);
+        l.cinp16 = up16(l.cin);
+        l.woff = off;
+        off += (size_t)l.cout * 9 * l.cin;
+        l.boff = off;
+        off += l.cout;
+        TC_NN_TRY(dmal((void**)&l.wk, (size_t)l.cout * l.kp * sizeof(__bf16)));
+        TC_NN_TRY(dmal((void**)&l.wt, (size_t)l.cinp16 * l.kpt * sizeof(__bf16)));
+    }
+    head_off = off;
+    // activations: per level the pixel count
+    for (int lev = 0; lev < 4; ++lev) npx[lev] = (H >> lev) * (W >> lev);
+    // buffers (bf16): x0 (8), a1 (16), s1 (48: [up 32 | skip 16]), p1 (16),
+    // a2 (32), s2 (96: [up 64 | skip 32]), p2 (32), a3 (64), s3 (192: [up 128 | skip 64]),
+    // p3 (64), a4 (128), b4 (128), d3a (64), d3b (64), d2a (32), d2b (32), d1a (16), d1b (16)
+    struct B { __bf16** p; int lev; int ch; };
+    B bufs[] = {{&x0, 0, 8}, {&a1, 0, 16}, {&s1, 0, 48}, {&p1, 1, 16}, {&a2, 1, 32}, {&s2, 1, 96}, {&p2, 2, 32},
+                {&a3, 2, 64}, {&s3, 2, 192}, {&p3, 3, 64}, {&a4, 3, 128}, {&b4, 3, 128}, {&d3a, 2, 64},
+                {&d3b, 2, 64}, {&d2a, 1, 32}, {&d2b, 1, 32}, {&d1a, 0, 16}, {&d1b, 0, 16}};
+    for (B& b : bufs) TC_NN_TRY(dmal((void**)b.p, (size_t)npx[b.lev] * b.ch * sizeof(__bf16)));
+    for (Layer& l : L) TC_NN_TRY(dmal((void**)&l.z, (size_t)npx[l.lev] * l.cout * sizeof(__bf16)));
+    TC_NN_TRY(dmal((void**)&out, (size_t)npx[0] * sizeof(float)));
+    if (train) {
+        // FP32 gradients of the activation buffers with the same layouts
+        struct G { float** p; int lev; int ch; };
+        G gb[] = {{&ga1, 0, 16}, {&gs1, 0, 48}, {&gp1, 1, 16}, {&ga2, 1, 32}, {&gs2, 1, 96}, {&gp2, 2, 32},
+                  {&ga3, 2, 64}, {&gs3, 2, 192}, {&gp3, 3, 64}, {&ga4, 3, 128}, {&gb4, 3, 128}, {&gd3a, 2, 64},
+                  {&gd3b, 2, 64}, {&gd2a, 1, 32}, {&gd2b, 1, 32}, {&gd1a, 0, 16}, {&gd1b, 0, 16}, {&gx0, 0, 8}};
+        for (G& b : gb) TC_NN_TRY(dmal((void**)b.p, (size_t)npx[b.lev] * b.ch * sizeof(float)));
+        TC_NN_TRY(dmal((void**)&dz, (size_t)npx[0] * 192 * sizeof(__bf16)));
+        size_t maxpart = 0;
+        for (Layer& l : L) {
+            const int pps = 2048;
+            l.nsplit = (npx[l.lev] + pps - 1) / pps;
+            maxpart = std::max(maxpart, (size_t)l.nsplit * l.cout * l.kp);
+        }
+        TC_NN_TRY(dmal((void**)&wpart, maxpart * sizeof(float)));
+        TC_NN_TRY(dmal((void**)&gw32, (size_t)2 * 192 * 9 * 128 * sizeof(float)));
+        TC_NN_TRY(dmal((void**)&hpart, (size_t)nblk(npx[0]) * 18 * sizeof(double)));
+        TC_NN_TRY(dmal((void**)&loss_d, sizeof(double)));
+    }
+    TC_NN_TRY(dmal((void**)&target, (size_t)npx[0] * sizeof(float)));
+    TC_NN_TRY(dmal((void**)&mask, (size_t)npx[0] * sizeof(float)));
+    trainable = train;
+    return 0;
+}
+
+void UNet::release()
+{
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+}
+
+void UNet::pack(hipStream_t st)
+{
+    for (Layer& l : L) {
+        const size_t n = std::max((size_t)l.cout * l.kp, (size_t)l.cinp16 * l.kpt);
+        hipLaunchKernelGGL(k_pack, dim3(nblk(n)), dim3(256), 0, st, w32 + l.woff, l.cin, l.cinp, l.cout, l.kp, l.kpt,
+                           l.wk, l.wt, l.cinp16);
+    }
+}
+
+void UNet::conv(hipStream_t st, const Layer& l, const __bf16* in, int cs_in, int off_in, __bf16* o, int cs_out,
+                int off_out)
+{
+    ConvArgs a{in, cs_in, off_in, l.cinp, l.wk, l.kp, l.cout, w32 + l.boff, o, l.z, nullptr, cs_out, off_out,
+               H >> l.lev, W >> l.lev, 0};
+    const int tiles = (npx[l.lev] + 15) / 16;
+    hipLaunchKernelGGL(k_conv3x3, dim3((tiles + 3) / 4, l.cout / 16), dim3(256), 0, st, a);
+}
+
+void UNet::forward(hipStream_t st)
+{
+    auto pool = [&](const __bf16* in, int cs, int off, __bf16* o, int C, int lev) {
+        const int h = H >> lev, w = W >> lev;
+        hipLaunchKernelGGL(k_pool, dim3(nblk((size_t)(h / 2) * (w / 2) * C)), dim3(256), 0, st, in, cs, off, o, C, h, w);
+    };
+    auto up = [&](const __bf16* in, __bf16* o, int cs, int off, int C, int lev) {
+        const int h = H >> lev, w = W >> lev;
+        hipLaunchKernelGGL(k_up, dim3(nblk((size_t)h * w * C)), dim3(256), 0, st, in, o, cs, off, C, h, w);
+    };
+    conv(st, L[0], x0, 8, 0, a1, 16, 0);
+    conv(st, L[1], a1, 16, 0, s1, 48, 32);
+    pool(s1, 48, 32, p1, 16, 0);
+    conv(st, L[2], p1, 16, 0, a2, 32, 0);
+    conv(st, L[3], a2, 32, 0, s2, 96, 64);
+    pool(s2, 96, 64, p2, 32, 1);
+    conv(st, L[4], p2, 32, 0, a3, 64, 0);
+    conv(st, L[5], a3, 64, 0, s3, 192, 128);
+    pool(s3, 192, 128, p3, 64, 2);
+    conv(st, L[6], p3, 64, 0, a4, 128, 0);
+    conv(st, L[7], a4, 128, 0, b4, 128, 0);
+    up(b4, s3, 192, 0, 128, 2);
+    conv(st, L[8], s3, 192, 0, d3a, 64, 0);
+    conv(st, L[9], d3a, 64, 0, d3b, 64, 0);
+    up(d3b, s2, 96, 0, 64, 1);
+    conv(st, L[10], s2, 96, 0, d2a, 32, 0);
+    conv(st, L[11], d2a, 32, 0, d2b, 32, 0);
+    up(d2b, s1, 48, 0, 32, 0);
+    conv(st, L[12], s1, 48, 0, d1a, 16, 0);
+    conv(st, L[13], d1a, 16, 0, d1b, 16, 0);
+    hipLaunchKernelGGL(k_head, dim3(nblk(npx[0])), dim3(256), 0, st, d1b, w32 + head_off, w32 + head_off + 16, out,
+                       npx[0]);
+}
+
+// backward of one conv layer: dy (FP32, layer output layout) -> dz -> weight,
+// bias and input gradients (the input gradient is accumulated into gin)
+void UNet::conv_back(hipStream_t st, const Layer& l, float* gy, int cs_y, int off_y, const __bf16* in, int cs_in,
+                     int off_in, float* gin, int cs_gin, int off_gin)
+{
+    const int np = npx[l.lev];
+    hipLaunchKernelGGL(k_gelu_back, dim3(nblk((size_t)np * l.cout)), dim3(256), 0, st, gy, cs_y, off_y, l.z, dz, np,
+                       l.cout);
+    // weight gradient
+    const int pps = 2048;
+    WGradArgs wa{dz, in, cs_in, off_in, l.cinp, l.kp, l.cout, wpart, H >> l.lev, W >> l.lev, pps};
+    hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(l.cout / 16, l.kp / 16, l.nsplit), dim3(64), 0, st, wa);
+    const size_t n = (size_t)l.cout * l.kp;
+    hipMemsetAsync(gw32, 0, n * sizeof(float), st);
+    hipLaunchKernelGGL(k_reduce_splits, dim3(nblk(n)), dim3(256), 0, st, wpart, l.nsplit, n, gw32);
+    hipLaunchKernelGGL(k_unpack_grad, dim3(nblk((size_t)l.cout * 9 * l.cin)), dim3(256), 0, st, gw32, l.cin, l.cinp,
+                       l.cout, l.kp, g32 + l.woff);
+    hipLaunchKernelGGL(k_bias_grad, dim3(l.cout), dim3(256), 0, st, dz, np, l.cout, g32 + l.boff);
+    // input gradient: conv of dz with the flipped, transposed weights
+    if (gin) {
+        ConvArgs a{dz, l.cout, 0, l.cout, l.wt, l.kpt, l.cinp16, nullptr, nullptr, nullptr, gin, cs_gin, off_gin,
+                   H >> l.lev, W >> l.lev, 1};
+        const int tiles = (np + 15) / 16;
+        hipLaunchKernelGGL(k_conv3x3, dim3((tiles + 3) / 4, l.cinp16 / 16), dim3(256), 0, st, a);
+    }
+}
+
+double UNet::backward(hipStream_t st)
+{
+    auto poolb = [&](float* din, int cs, int off, const float* dout, int C, int lev) {
+        const int h = H >> lev, w = W >> lev;
+        hipLaunchKernelGGL(k_pool_back, dim3(nblk((size_t)h * w * C)), dim3(256), 0, st, din, cs, off, dout, C, h, w);
+    };
+    auto upb = [&](const float* dout, int cs, int off, float* din, int C, int lev) {
+        const int h = H >> lev, w = W >> lev;
+        hipLaunchKernelGGL(k_up_back, dim3(nblk((size_t)(h / 2) * (w / 2) * C)), dim3(256), 0, st, dout, cs, off, din,
+                           C, h, w);
+    };
+    const int nb = (int)nblk(npx[0]);
+    hipLaunchKernelGGL(k_head_back, dim3(nb), dim3(256), 0, st, out, target, mask, d1b, w32 + head_off, gd1b, npx[0],
+                       inv_n, hpart);
+    hipLaunchKernelGGL(k_head_reduce, dim3(1), dim3(64), 0, st, hpart, nb, g32 + head_off, g32 + head_off + 16, loss_d);
+    // decoder level 1
+    conv_back(st, L[13], gd1b, 16, 0, d1a, 16, 0, gd1a, 16, 0);
+    conv_back(st, L[12], gd1a, 16, 0, s1, 48, 0, gs1, 48, 0);
+    upb(gs1, 48, 0, gd2b, 32, 0);
+    conv_back(st, L[11], gd2b, 32, 0, d2a, 32, 0, gd2a, 32, 0);
+    conv_back(st, L[10], gd2a, 32, 0, s2, 96, 0, gs2, 96, 0);
+    upb(gs2, 96, 0, gd3b, 64, 1);
+    conv_back(st, L[9], gd3b, 64, 0, d3a, 64, 0, gd3a, 64, 0);
+    conv_back(st, L[8], gd3a, 64, 0, s3, 192, 0, gs3, 192, 0);
+    upb(gs3, 192, 0, gb4, 128, 2);
+    conv_back(st, L[7], gb4, 128, 0, a4, 128, 0, ga4, 128, 0);
+    conv_back(st, L[6], ga4, 128, 0, p3, 64, 0, gp3, 64, 0);
+    poolb(gs3, 192, 128, gp3, 64, 2);
+    hipMemsetAsync(gp3, 0, (size_t)npx[3] * 64 * sizeof(float), st);
+    conv_back(st, L[5], gs3, 192, 128, a3, 64, 0, ga3, 64, 0);
+    conv_back(st, L[4], ga3, 64, 0, p2, 32, 0, gp2, 32, 0);
+    poolb(gs2, 96, 64, gp2, 32, 1);
+    hipMemsetAsync(gp2, 0, (size_t)npx[2] * 32 * sizeof(float), st);
+    conv_back(st, L[3], gs2, 96, 64, a2, 32, 0, ga2, 32, 0);
+    conv_back(st, L[2], ga2, 32, 0, p1, 16, 0, gp1, 16, 0);
+    poolb(gs1, 48, 32, gp1, 16, 0);
+    hipMemsetAsync(gp1, 0, (size_t)npx[1] * 16 * sizeof(float), st);
+    conv_back(st, L[1], gs1, 48, 32, a1, 16, 0, ga1, 16, 0);
+    conv_back(st, L[0], ga1, 16, 0, x0, 8, 0, nullptr, 0, 0);
+    // the upsampled halves of the concat gradients were consumed: clear them
+    hipMemsetAsync(gs1, 0, (size_t)npx[0] * 48 * sizeof(float), st);
+    hipMemsetAsync(gs2, 0, (size_t)npx[1] * 96 * sizeof(float), st);
+    hipMemsetAsync(gs3, 0, (size_t)npx[2] * 192 * sizeof(float), st);
+    return 0.0;
+}
+
+void UNet::adam(hipStream_t st, float lr, float scale)
+{
+    ++step;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nparam)), dim3(256), 0, st, w32, g32, m32, v32, nparam, lr, b1, b2, eps, c1,
+                       c2, scale);
+    pack(st);
+}
+
+}  // namespace tc
+
+// ===========================================================================
+// C-ABI of the network on its own (training from a host driver, tests)
+// ===========================================================================
+namespace {
+int nn_check(tc_nn* n, hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        n->err = std::string(what) + ": " + hipGetErrorString(e);
+        return TC_ERR_HIP;
+    }
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int tc_nn_create(int32_t H, int32_t W, int32_t trainable, tc_nn** out)
+{
+    if (!out) return TC_ERR_ARG;
+    *out = nullptr;
+    tc_nn* n = new (std::nothrow) tc_nn();
+    if (!n) return TC_ERR_NOMEM;
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete n;
+        return TC_ERR_HIP;
+    }
+    const int st = n->net.alloc(H, W, trainable != 0);
+    if (st) {
+        n->net.release();
+        (void)hipStreamDestroy(n->stream);
+        delete n;
+        return st;
+    }
+    *out = n;
+    return 0;
+}
+
+int tc_nn_destroy(tc_nn* n)
+{
+    if (!n) return TC_ERR_ARG;
+    (void)hipStreamSynchronize(n->stream);
+    n->net.release();
+    (void)hipStreamDestroy(n->stream);
+    delete n;
+    return 0;
+}
+
+int tc_nn_param_count(const tc_nn* n, int64_t* count)
+{
+    if (!n || !count) return TC_ERR_ARG;
+    *count = (int64_t)n->net.nparam;
+    return 0;
+}
+
+int tc_nn_set_weights(tc_nn* n, const float* w, int64_t count)
+{
+    if (!n || !w || count != (int64_t)n->net.nparam) return TC_ERR_ARG;
+    TC_NN_TRY(nn_check(n, hipMemcpyAsync(n->net.w32, w, count * sizeof(float), hipMemcpyHostToDevice, n->stream), "weights"));
+    n->net.pack(n->stream);
+    if (n->net.trainable) {
+        TC_NN_TRY(nn_check(n, hipMemsetAsync(n->net.m32, 0, count * sizeof(float), n->stream), "adam m"));
+        TC_NN_TRY(nn_check(n, hipMemsetAsync(n->net.v32, 0, count * sizeof(float), n->stream), "adam v"));
+        TC_NN_TRY(nn_check(n, hipMemsetAsync(n->net.g32, 0, count * sizeof(float), n->stream), "grads"));
+        n->net.step = 0;
+    }
+    return nn_check(n, hipStreamSynchronize(n->stream), "sync");
+}
+
+int tc_nn_get_weights(tc_nn* n, float* w, int64_t count)
+{
+    if (!n || !w || count != (int64_t)n->net.nparam) return TC_ERR_ARG;
+    TC_NN_TRY(nn_check(n, hipMemcpyAsync(w, n->net.w32, count * sizeof(float), hipMemcpyDeviceToHost, n->stream), "weights"));
+    return nn_check(n, hipStreamSynchronize(n->stream), "sync");
+}
+
+int tc_nn_get_grads(tc_nn* n, float* g, int64_t count)
+{
+    if (!n || !g || !n->net.trainable || count != (int64_t)n->net.nparam) return TC_ERR_ARG;
+    TC_NN_TRY(nn_check(n, hipMemcpyAsync(g, n->net.g32, count * sizeof(float), hipMemcpyDeviceToHost, n->stream), "grads"));
+    return nn_check(n, hipStreamSynchronize(n->stream), "sync");
+}
+
+// x: H*W*5 values, pixel-major (p = k*W + i), channel fastest; target/mask: H*W
+int tc_nn_set_sample(tc_nn* n, const float* x, const float* target, const float* mask)
+{
+    if (!n || !x) return TC_ERR_ARG;
+    tc::UNet& u = n->net;
+    const size_t np = (size_t)u.npx[0];
+    n->hx.assign(np * 8, (__bf16)0.0f);
+    for (size_t p = 0; p < np; ++p)
+        for (int c = 0; c < 5; ++c) n->hx[p * 8 + c] = (__bf16)x[p * 5 + c];
+    TC_NN_TRY(nn_check(n, hipMemcpyAsync(u.x0, n->hx.data(), np * 8 * sizeof(__bf16), hipMemcpyHostToDevice, n->stream), "x"));
+    if (target) TC_NN_TRY(nn_check(n, hipMemcpyAsync(u.target, target, np * sizeof(float), hipMemcpyHostToDevice, n->stream), "t"));
+    if (mask) {
+        TC_NN_TRY(nn_check(n, hipMemcpyAsync(u.mask, mask, np * sizeof(float), hipMemcpyHostToDevice, n->stream), "m"));
+        double cnt = 0.0;
+        for (size_t p = 0; p < np; ++p) cnt += mask[p];
+        u.inv_n = cnt > 0.0 ? (float)(1.0 / cnt) : 0.0f;
+    }
+    return nn_check(n, hipStreamSynchronize(n->stream), "sync");
+}
+
+int tc_nn_forward(tc_nn* n, float* out)
+{
+    if (!n) return TC_ERR_ARG;
+    n->net.forward(n->stream);
+    if (out)
+        TC_NN_TRY(nn_check(n, hipMemcpyAsync(out, n->net.out, (size_t)n->net.npx[0] * sizeof(float),
+                                             hipMemcpyDeviceToHost, n->stream), "out"));
+    return nn_check(n, hipStreamSynchronize(n->stream), "forward");
+}
+
+// After tc_nn_forward: accumulates the parameter gradients of the masked
+// MSE loss and returns the loss of this sample.
+int tc_nn_backward(tc_nn* n, double* loss)
+{
+    if (!n || !n->net.trainable) return TC_ERR_ARG;
+    TC_NN_TRY(nn_check(n, hipMemsetAsync(n->net.loss_d, 0, sizeof(double), n->stream), "loss"));
+    n->net.backward(n->stream);
+    double l = 0.0;
+    TC_NN_TRY(nn_check(n, hipMemcpyAsync(&l, n->net.loss_d, sizeof(double), hipMemcpyDeviceToHost, n->stream), "loss"));
+    TC_NN_TRY(nn_check(n, hipStreamSynchronize(n->stream), "backward"));
+    if (loss) *loss = l;
+    return 0;
+}
+
+int tc_nn_adam(tc_nn* n, float lr, float grad_scale)
+{
+    if (!n || !n->net.trainable) return TC_ERR_ARG;
+    n->net.adam(n->stream, lr, grad_scale);
+    return nn_check(n, hipStreamSynchronize(n->stream), "adam");
+}
+
+const char* tc_nn_last_error(const tc_nn* n) { return n ? n->err.c_str() : "null network"; }
+
+}  // extern "C"

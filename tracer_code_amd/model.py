@@ -187,6 +187,29 @@ class Solver:
         self._check(self.lib.tc_stability_RK4(self.h, ctypes.byref(dt)), "stability_RK4")
         return dt.value
 
+    # -- learned initial guess ----------------------------------------------
+    def learn_attach(self, net):
+        """Attach a tracer_code_amd.learn.Network (H = nz, W = nx) or None."""
+        self._check(self.lib.tc_learn_attach(self.h, net.h if net is not None else None), "learn_attach")
+        self._net = net
+
+    def learn_record(self, capacity):
+        self._check(self.lib.tc_learn_record(self.h, int(capacity)), "learn_record")
+
+    def learn_samples(self):
+        n = ctypes.c_int32()
+        self._check(self.lib.tc_learn_samples(self.h, ctypes.byref(n)), "learn_samples")
+        return n.value
+
+    def learn_train(self, epochs, lr, batch=1, seed=0):
+        loss = ctypes.c_double()
+        self._check(self.lib.tc_learn_train(self.h, int(epochs), float(lr), int(batch), int(seed),
+                                            ctypes.byref(loss)), "learn_train")
+        return loss.value
+
+    def guess_enable(self, enable):
+        self._check(self.lib.tc_guess_enable(self.h, 1 if enable else 0), "guess_enable")
+
     def set_profiling(self, enable):
         self._check(self.lib.tc_set_profiling(self.h, int(bool(enable))), "set_profiling")
 
