@@ -135,7 +135,7 @@ typedef struct tc_stats {
     int32_t n_eta;           /* smoother_vis passes at level 1 of the last solve */
     int32_t n_temp;          /* smoother_time_T iterations of the last call */
     int32_t n_comp[TC_MAX_TYPES];
-    int32_t guess_used;      /* learned guess accepted in the last solve (1) or rejected (0) */
+    int32_t guess_used;      /* start of the last solve: 0 plain, 1 extrapolation, 2 extrapolation + network */
     int32_t pad;
     double residual_last;    /* residual_mag(1) of the last solve */
     double res_ref_last;     /* res_ref(1) of the last solve */
@@ -149,6 +149,10 @@ typedef struct tc_stats {
     int64_t c_iters;         /* smoother_time iterations (cumulative) */
     int64_t t_iters;         /* smoother_time_T iterations (cumulative) */
     int64_t mg_cycles;       /* multigrid cycles (cumulative) */
+    int64_t guesses_used[3]; /* solves started from each TC_GUESS_* candidate (cumulative) */
+    /* Last solve: residual_mag(1) after the first level-1 smoothing from the
+     * plain start, the extrapolation and the learned guess (0: not judged). */
+    double guess_res[3];
 } tc_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -246,15 +250,26 @@ int tc_nn_set_sample(tc_nn* nn, const float* x, const float* target, const float
 int tc_nn_forward(tc_nn* nn, float* out);
 int tc_nn_backward(tc_nn* nn, double* loss);
 int tc_nn_adam(tc_nn* nn, float lr, float grad_scale);
-/* Solver side: attach a network (H = nz, W = nx), record training samples
- * (inputs before / increments after each solve) for the next `capacity`
- * solves, train on them, and use the network as the initial guess. */
+/* Solver side: attach a network whose H x W are the nz x nx of a multigrid
+ * level L < grids (the network then works on level L: inputs restricted
+ * there, its predicted increment prolonged to the fine grid), record
+ * training samples (inputs before / increments after each solve) for the
+ * next `capacity` solves, train on them, and use the network as the
+ * initial guess. */
 int tc_learn_attach(tc_ctx* ctx, tc_nn* nn);
 int tc_learn_record(tc_ctx* ctx, int32_t capacity);
 int tc_learn_samples(const tc_ctx* ctx, int32_t* count);
 int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_t seed, double* mean_loss);
-/* Enables/disables the learned guess in the defect-correction modes. */
-int tc_guess_enable(tc_ctx* ctx, int enable);
+/* Initial guess of the defect-correction modes' solves inside tc_time_step.
+ * TC_GUESS_PLAIN: the last solution (the reference's warm start).
+ * TC_GUESS_EXTRAP: also the stage's extrapolation in time from its solutions
+ * in the last two steps.  TC_GUESS_LEARNED: also the extrapolation plus the
+ * attached network's predicted correction of it.  Every candidate is
+ * judged by the residual after the cycle's first level-1 smoothing and the
+ * smallest wins; the plain start is always a candidate, and every solve
+ * iterates to the same tolerance. */
+enum { TC_GUESS_PLAIN = 0, TC_GUESS_EXTRAP = 1, TC_GUESS_LEARNED = 2 };
+int tc_guess_enable(tc_ctx* ctx, int mode);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,7 @@ the plain float64 network that finite differences pin.
 """
 import numpy as np
 
-LAYERS = [(5, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (64, 128, 3), (128, 128, 3),
+LAYERS = [(6, 16, 0), (16, 16, 0), (16, 32, 1), (32, 32, 1), (32, 64, 2), (64, 64, 2), (64, 128, 3), (128, 128, 3),
           (192, 64, 2), (64, 64, 2), (96, 32, 1), (32, 32, 1), (48, 16, 0), (16, 16, 0)]
 
 
@@ -131,7 +131,7 @@ class UNetRef:
         return self.r(zf), self.r(gelu(zf))
 
     def forward(self, x):
-        """x (H, W, 5) -> output (H, W); keeps what backward needs."""
+        """x (H, W, 6) -> output (H, W); keeps what backward needs."""
         r = self.r
         s = {"x0": r(x)}
         s["z0"], s["a1"] = self._layer(0, s["x0"])

@@ -26,7 +26,7 @@ def rel(a, b):
 def test_forward_and_gradients_match_reference(H, W):
     th = unet.init_params(3)
     rng = np.random.default_rng(4)
-    x = rng.uniform(-1, 1, (H, W, 5))
+    x = rng.uniform(-1, 1, (H, W, 6))
     t = rng.standard_normal((H, W))
     m = np.ones((H, W))
     m[0, :] = 0
@@ -59,7 +59,7 @@ def test_adam_step():
     H, W = 16, 16
     th = unet.init_params(5).astype(np.float32)
     rng = np.random.default_rng(6)
-    x = rng.uniform(-1, 1, (H, W, 5))
+    x = rng.uniform(-1, 1, (H, W, 6))
     t = rng.standard_normal((H, W))
     m = np.ones((H, W))
     with learn.Network(H, W) as net:
@@ -77,29 +77,35 @@ def test_adam_step():
     assert np.max(np.abs(w1 - want)) <= 1e-6
 
 
-def test_guess_in_solver_keeps_tolerance():
+@pytest.mark.parametrize("level", [1, 3])
+def test_guess_in_solver_keeps_tolerance(level):
     params, cfg = config.preset_params("cfg0", nx=64, nz=64, tpc=4)
     T, r, s, ty = ic.workload(cfg, seed=11)
-    print("[learned guess 64x64]", flush=True)
-    with Solver(params, len(r)) as g, learn.Network(params.nz, params.nx) as net:
+    print(f"[learned guess 64x64, network on level {level}]", flush=True)
+    with Solver(params, len(r)) as g, learn.Network(*learn.level_shape(params.nx, params.nz, level)) as net:
         net.set_weights(learn.init_weights(0))
         g.initialize(T, r, s, ty)
         g.set_mode("mixed")
         g.learn_attach(net)
         g.learn_record(64)
-        g.time_step(4)
+        # samples need two earlier solves of the same RK stage and one
+        # measured extrapolation error: steps 4.. give 4 each
+        g.time_step(6)
         n = g.learn_samples()
-        assert n >= 8
-        loss0 = g.learn_train(1, 1e-3, 2, seed=1)
-        loss1 = g.learn_train(6, 1e-3, 2, seed=2)
-        print(f"samples {n}, loss after 1 epoch {loss0:.3e}, after 7 {loss1:.3e}", flush=True)
+        assert n == 12
+        loss0 = g.learn_train(1, 0.0, 2, seed=1)   # lr 0: the untrained network's loss
+        loss1 = g.learn_train(20, 1e-3, 2, seed=2)
+        print(f"samples {n}, loss untrained {loss0:.3e}, after 20 epochs {loss1:.3e}", flush=True)
         assert np.isfinite(loss1) and loss1 < loss0
-        g.guess_enable(True)
-        used = 0
+        g.guess_enable("learned")
+        st0 = g.stats()
         for _ in range(3):
             g.time_step(1)
             st = g.stats()
-            used += st.guess_used
             assert st.residual_last <= params.tolerance * st.res_ref_last
-        print(f"guess used in {used} of the last 3 steps' final solves", flush=True)
+            # the last solve judged all three starts
+            assert st.guess_res[0] > 0 and st.guess_res[1] > 0 and st.guess_res[2] > 0
+        starts = [st.guesses_used[i] - st0.guesses_used[i] for i in range(3)]
+        print(f"starts over the last 3 steps (plain, extrapolation, learned): {starts}", flush=True)
+        assert sum(starts) == 12
         g.learn_attach(None)

@@ -10,14 +10,14 @@ from tracer_code_amd import learn
 
 def test_layer_table_and_count_match_product():
     assert [(ci, co) for ci, co, _ in unet.LAYERS] == learn.LAYERS
-    assert unet.param_count() == learn.param_count() == 487585
+    assert unet.param_count() == learn.param_count() == 487729
 
 
 def test_backward_matches_finite_differences():
     th = unet.init_params(1)
     rng = np.random.default_rng(2)
     H, W = 16, 16
-    x = rng.standard_normal((H, W, 5))
+    x = rng.standard_normal((H, W, 6))
     t = rng.standard_normal((H, W))
     m = np.ones((H, W))
     m[0, :] = 0

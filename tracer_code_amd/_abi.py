@@ -24,6 +24,7 @@ TC_ERR_NOMEM = -4
 TC_MODE_FP64 = 0
 TC_MODE_MIXED = 1
 TC_MODE_DC64 = 2
+TC_GUESS_PLAIN, TC_GUESS_EXTRAP, TC_GUESS_LEARNED = 0, 1, 2
 
 # field ids
 TC_F_T, TC_F_SF, TC_F_TVIS, TC_F_TBUOY, TC_F_CNEW, TC_F_CVIS, TC_F_CBUOY = range(7)
@@ -71,7 +72,8 @@ class TcStats(ctypes.Structure):
         ("dt_last", ctypes.c_double), ("time", ctypes.c_double),
         ("sweeps_exec", ctypes.c_int64 * TC_MAX_LEVELS), ("residuals_exec", ctypes.c_int64 * TC_MAX_LEVELS),
         ("coarse_solves", ctypes.c_int64), ("vis_iters", ctypes.c_int64), ("c_iters", ctypes.c_int64),
-        ("t_iters", ctypes.c_int64), ("mg_cycles", ctypes.c_int64),
+        ("t_iters", ctypes.c_int64), ("mg_cycles", ctypes.c_int64), ("guesses_used", ctypes.c_int64 * 3),
+        ("guess_res", ctypes.c_double * 3),
     ]
 
 

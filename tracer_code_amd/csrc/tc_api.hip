@@ -442,8 +442,8 @@ int stop_test(tc_ctx* ctx, tc_dev& d, int q, bool& done)
     return 0;
 }
 
-int learn_pre_solve(tc_ctx* ctx, tc_dev& d, bool allow_guess, int* slot, bool* have_guess);
-int learn_post_solve(tc_ctx* ctx, int slot, const double* psi);
+int guess_pre_solve(tc_ctx* ctx, bool dc, int* slot, bool* nn_guess);
+int guess_post_solve(tc_ctx* ctx, int slot, const double* psi);
 
 int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_out)
 {
@@ -458,7 +458,7 @@ int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_o
     }
     if (status == 0) {
         TC_TRY(copy_plane(ctx, ctx->SF, psi, ctx->lv[1].g));
-        TC_TRY(learn_post_solve(ctx, DEV(ctx).learn_slot, ctx->SF));
+        TC_TRY(guess_post_solve(ctx, DEV(ctx).learn_slot, ctx->SF));
     }
     DEV(ctx).learn_slot = -1;
     ctx->stats.q_last = q;
@@ -484,7 +484,7 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
     TC_TRY(solve_setup(ctx, d));
     {
         bool hg = false;
-        TC_TRY(learn_pre_solve(ctx, d, false, &d.learn_slot, &hg));
+        TC_TRY(guess_pre_solve(ctx, false, &d.learn_slot, &hg));
     }
     bool first_sweep = true;
     const int* multi = P.multi;
@@ -542,51 +542,161 @@ int run_multigrid(tc_ctx* ctx, int32_t* q_out)
 // ---------------------------------------------------------------------------
 // learned initial guess: inputs before a solve, targets and history after
 // ---------------------------------------------------------------------------
-int learn_pre_solve(tc_ctx* ctx, tc_dev& d, bool allow_guess, int* slot, bool* have_guess)
+// f (level-1 plane) -> level m by the cycle's restriction (full weighting);
+// the intermediate levels use the nn_ra/nn_rb scratch planes, and each
+// result's boundary nodes are zero.  m = 1 copies.
+int restrict_to(tc_ctx* ctx, const double* f, int m, double* dst)
 {
-    *slot = -1;
-    *have_guess = false;
-    if (!ctx->nn || ctx->nhist < 1) return 0;
-    UNet& u = ctx->nn->net;
-    const Grid& g = ctx->lv[1].g;
-    const size_t np = (size_t)u.npx[0];
-    const double* C = (ctx->p.comp == 1 && ctx->p.ntype > 0) ? ctx->Cbuoy[0] : nullptr;
-    launch_nn_scales(ctx->stream, ctx->psi_h1, ctx->Tbuoy, ctx->T_h1, g, ctx->nn_part, ctx->nn_scales);
-    if (ctx->rec_n < ctx->rec_cap) {
-        *slot = ctx->rec_n;
-        launch_nn_input(ctx->stream, ctx->Tbuoy, C, ctx->lv[1].vis, ctx->psi_h1, ctx->T_h1, g, u.W, u.H,
-                        ctx->nn_scales, ctx->rec_x + (size_t)*slot * np * 8);
-        TC_HIP(hipMemcpyAsync(ctx->rec_scales + 3 * *slot, ctx->nn_scales, 3 * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-    }
-    if (allow_guess && ctx->guess) {
-        launch_nn_input(ctx->stream, ctx->Tbuoy, C, ctx->lv[1].vis, ctx->psi_h1, ctx->T_h1, g, u.W, u.H,
-                        ctx->nn_scales, u.x0);
-        u.forward(ctx->stream);
-        const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
-        launch_nn_apply(ctx->stream, u.out, ctx->psi_h1, ctx->guess_psi, g, u.W, ctx->nn_scales, zsg);
-        *have_guess = true;
+    if (m == 1) return copy_plane(ctx, dst, f, ctx->lv[1].g);
+    double* bufs[2] = {ctx->nn_ra, ctx->nn_rb};
+    const double* in = f;
+    for (int L = 2; L <= m; ++L) {
+        double* out = (L == m) ? dst : bufs[L & 1];
+        TC_HIP(hipMemsetAsync(out, 0, plane_bytes(ctx->lv[L].g), ctx->stream));
+        launch_restrict(ctx->stream, in, ctx->lv[L - 1].g, out, ctx->lv[L].g);
+        in = out;
     }
     return 0;
 }
 
-int learn_post_solve(tc_ctx* ctx, int slot, const double* psi)
+// fine += P_2 ... P_m e (e on level m, zero boundary), by the cycle's
+// prolongation with the streamfunction ghost mirrors at every level.
+int prolong_add(tc_ctx* ctx, const double* e, int m, double* fine, double zsg)
 {
+    double* bufs[2] = {ctx->nn_ra, ctx->nn_rb};
+    const double* in = e;
+    for (int L = m - 1; L >= 2; --L) {
+        double* out = bufs[L & 1];
+        TC_HIP(hipMemsetAsync(out, 0, plane_bytes(ctx->lv[L].g), ctx->stream));
+        launch_prolong(ctx->stream, out, ctx->lv[L].g, in, ctx->lv[L + 1].g, zsg);
+        in = out;
+    }
+    launch_prolong(ctx->stream, fine, ctx->lv[1].g, in, ctx->lv[2].g, zsg);
+    return 0;
+}
+
+// Candidate initial guesses of a solve inside the time step, built before
+// it (stage s = solve_stage, with solutions of stage s in the last two
+// steps):
+//   ext_psi   = psi_1 + c (psi_1 - psi_2), c = (tau - tau_1) / (tau_1 - tau_2):
+//               the stage's linear extrapolation in time;
+//   guess_psi = ext_psi + the network's predicted correction of it.
+// The network works on multigrid level m = nn_level (its H x W are that
+// level's nz x nx): the inputs are restricted there, and its correction is
+// prolonged to the fine grid by the cycle's own transfer operators, so it
+// arrives as smooth as a coarse-grid correction.  Inputs: T, C, log10 eta,
+// psi_1/s, (psi_1 - psi_2)/sd, (T - T_1)/dT (psi_i, T_1 of stage s);
+// output: (psi - ext)/e, e = max |psi - ext| of the stage's last solve.
+// Also records the network's training inputs (the target follows the solve).
+int guess_pre_solve(tc_ctx* ctx, bool dc, int* slot, bool* nn_guess)
+{
+    *slot = -1;
+    *nn_guess = false;
+    ctx->have_ext = false;
+    const int s = ctx->solve_stage;
+    if (s < 0 || !ctx->sh1[s] || ctx->scount[s] < 2) return 0;
+    const bool want_ext = dc && ctx->guess >= TC_GUESS_EXTRAP;
+    if (!want_ext && !ctx->nn) return 0;
+    const Grid& g = ctx->lv[1].g;
+    const double den = ctx->tau1[s] - ctx->tau2[s];
+    const double c = (den != 0.0) ? (ctx->solve_tau - ctx->tau1[s]) / den : 1.0;
+    launch_extrap(ctx->stream, ctx->sh1[s], ctx->sh2[s], c, ctx->ext_psi, g.elems);
+    ctx->have_ext = true;
     if (!ctx->nn) return 0;
     UNet& u = ctx->nn->net;
-    const Grid& g = ctx->lv[1].g;
-    if (slot >= 0) {
-        const size_t np = (size_t)u.npx[0];
-        // the scales of the inputs (max |psi_prev - psi_prev2|) normalise the target
-        TC_HIP(hipMemcpyAsync(ctx->nn_scales, ctx->rec_scales + 3 * slot, 3 * sizeof(double), hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        launch_nn_target(ctx->stream, psi, ctx->psi_h1, g, u.W, u.H, ctx->nn_scales, ctx->rec_t + (size_t)slot * np,
-                         ctx->rec_mask);
-        ctx->rec_n++;
+    const int m = ctx->nn_level;
+    const Grid& gc = ctx->lv[m].g;
+    const size_t np = (size_t)u.npx[0];
+    TC_TRY(restrict_to(ctx, ctx->ext_psi, m, ctx->nn_ext));
+    const bool record = ctx->rec_n < ctx->rec_cap && ctx->ecount[s] >= 1;
+    const bool predict = dc && ctx->guess == TC_GUESS_LEARNED && ctx->ecount[s] >= 1;
+    if (!record && !predict) return 0;
+    const double* C = nullptr;
+    TC_TRY(restrict_to(ctx, ctx->Tbuoy, m, ctx->nn_T));
+    if (ctx->p.comp == 1 && ctx->p.ntype > 0) {
+        TC_TRY(restrict_to(ctx, ctx->Cbuoy[0], m, ctx->nn_C));
+        C = ctx->nn_C;
     }
-    TC_TRY(copy_plane(ctx, ctx->psi_h1, psi, g));
-    TC_TRY(copy_plane(ctx, ctx->T_h1, ctx->Tbuoy, g));
-    ctx->nhist = 1;
+    TC_TRY(restrict_to(ctx, ctx->sh1[s], m, ctx->nn_p1));
+    TC_TRY(restrict_to(ctx, ctx->sh2[s], m, ctx->nn_p2));
+    launch_nn_scales(ctx->stream, ctx->nn_p1, ctx->nn_p2, ctx->nn_T, ctx->Tc1[s], gc, ctx->nn_part, ctx->nn_scales);
+    // output scale: the stage's last extrapolation error
+    TC_HIP(hipMemcpyAsync(ctx->nn_scales + 1, ctx->stage_err + NN_NSCALE * s + 3, sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    if (record) {
+        *slot = ctx->rec_n;
+        launch_nn_input(ctx->stream, ctx->nn_T, C, ctx->lv[m].vis, ctx->nn_p1, ctx->nn_p2, ctx->Tc1[s], gc, u.W, u.H,
+                        ctx->nn_scales, ctx->rec_x + (size_t)*slot * np * 8);
+        TC_HIP(hipMemcpyAsync(ctx->rec_scales + NN_NSCALE * *slot, ctx->nn_scales, NN_NSCALE * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (predict) {
+        launch_nn_input(ctx->stream, ctx->nn_T, C, ctx->lv[m].vis, ctx->nn_p1, ctx->nn_p2, ctx->Tc1[s], gc, u.W, u.H,
+                        ctx->nn_scales, u.x0);
+        u.forward(ctx->stream);
+        const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
+        if (m == 1) {
+            launch_nn_apply(ctx->stream, u.out, ctx->ext_psi, ctx->guess_psi, g, u.W, ctx->nn_scales, zsg);
+        } else {
+            TC_HIP(hipMemsetAsync(ctx->nn_inc, 0, plane_bytes(gc), ctx->stream));
+            launch_nn_apply(ctx->stream, u.out, nullptr, ctx->nn_inc, gc, u.W, ctx->nn_scales, zsg);
+            TC_TRY(copy_plane(ctx, ctx->guess_psi, ctx->ext_psi, g));
+            TC_TRY(prolong_add(ctx, ctx->nn_inc, m, ctx->guess_psi, zsg));
+        }
+        *nn_guess = true;
+    }
+    return 0;
+}
+
+// After a solve inside the time step: the training target, the stage's
+// extrapolation error (the next output scale) and the history.
+int guess_post_solve(tc_ctx* ctx, int slot, const double* psi)
+{
+    const int s = ctx->solve_stage;
+    if (s < 0 || !ctx->sh1[s]) return 0;
+    if (ctx->nn) {
+        UNet& u = ctx->nn->net;
+        const int m = ctx->nn_level;
+        const Grid& gc = ctx->lv[m].g;
+        if (ctx->have_ext) {
+            TC_TRY(restrict_to(ctx, psi, m, ctx->nn_psi));
+            if (slot >= 0) {
+                const size_t np = (size_t)u.npx[0];
+                TC_HIP(hipMemcpyAsync(ctx->nn_scales, ctx->rec_scales + NN_NSCALE * slot, NN_NSCALE * sizeof(double),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+                launch_nn_target(ctx->stream, ctx->nn_psi, ctx->nn_ext, gc, u.W, u.H, ctx->nn_scales,
+                                 ctx->rec_t + (size_t)slot * np, ctx->rec_mask);
+                ctx->rec_n++;
+            }
+            // [3] = max |R psi - R ext| (the T arguments only fill [2])
+            launch_nn_scales(ctx->stream, ctx->nn_psi, ctx->nn_ext, ctx->nn_psi, ctx->nn_psi, gc, ctx->nn_part,
+                             ctx->stage_err + NN_NSCALE * s);
+            ctx->ecount[s]++;
+        }
+        TC_TRY(restrict_to(ctx, ctx->Tbuoy, m, ctx->Tc1[s]));
+    }
+    std::swap(ctx->sh1[s], ctx->sh2[s]);
+    TC_TRY(copy_plane(ctx, ctx->sh1[s], psi, ctx->lv[1].g));
+    ctx->tau2[s] = ctx->tau1[s];
+    ctx->tau1[s] = ctx->solve_tau;
+    if (ctx->scount[s] < 2) ctx->scount[s]++;
+    return 0;
+}
+
+// fine-grid history and candidate planes (TC_GUESS_EXTRAP and up, or a network)
+int guess_alloc(tc_ctx* ctx)
+{
+    if (ctx->ext_psi) return 0;
+    const Grid& g = ctx->lv[1].g;
+    for (int s = 0; s < 4; ++s) {
+        TC_TRY(dalloc(ctx, &ctx->sh1[s], g.elems));
+        TC_TRY(dalloc(ctx, &ctx->sh2[s], g.elems));
+        ctx->scount[s] = 0;
+    }
+    TC_TRY(dalloc(ctx, &ctx->ext_psi, g.elems));
+    TC_TRY(dalloc(ctx, &ctx->guess_psi, g.elems));
+    TC_TRY(dalloc(ctx, &ctx->guess_lo, g.elems));
+    TC_TRY(dalloc(ctx, &ctx->stage_err, 4 * NN_NSCALE));
     return 0;
 }
 
@@ -675,10 +785,14 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
     L1.cur32 = 0;
     if (f32) TC_HIP(hipMemsetAsync(L1.err32[0], 0, L1.g.elems * sizeof(float), ctx->stream));
     else TC_HIP(hipMemsetAsync(L1.err[0], 0, plane_bytes(L1.g), ctx->stream));
-    // initial defect; with the learned guess, iterate from whichever of the
-    // guess and the plain warm start has the smaller residual
-    bool have_guess = false;
-    TC_TRY(learn_pre_solve(ctx, d, true, &d.learn_slot, &have_guess));
+    // initial defect.  With more than one candidate start (the plain warm
+    // start, the extrapolation, the extrapolation plus the network's
+    // correction), each takes the cycle's first level-1 smoothing and the
+    // one with the smallest residual after it continues: the smoother
+    // removes the rough part of any start's error at once, what remains is
+    // what the cycles must do.
+    bool nn_guess = false;
+    TC_TRY(guess_pre_solve(ctx, true, &d.learn_slot, &nn_guess));
     auto defect = [&](const double* hi, double* mag) -> int {
         const int nb = launch_residual_dd(ctx->stream, hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr,
                                           d.F, d.h[1], P.ivis, d.partialA);
@@ -690,25 +804,63 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
         return 0;
     };
     ctx->stats.guess_used = 0;
-    if (have_guess) {
-        double m_plain = 0.0, m_guess = 0.0;
-        TC_TRY(defect(ctx->psi_hi, &m_plain));
-        TC_TRY(defect(ctx->guess_psi, &m_guess));
-        if (m_guess < m_plain) {
-            TC_TRY(copy_plane(ctx, ctx->psi_hi, ctx->guess_psi, L1.g));
-            ctx->stats.guess_used = 1;
-        } else {
-            TC_TRY(defect(ctx->psi_hi, nullptr));
-        }
-    } else {
-        TC_TRY(defect(ctx->psi_hi, nullptr));
-    }
+    for (int c = 0; c < 3; ++c) ctx->stats.guess_res[c] = 0.0;
     const int* multi = P.multi;
+    TC_TRY(defect(ctx->psi_hi, nullptr));
+    double** cand[3] = {nullptr, (ctx->have_ext && ctx->guess >= TC_GUESS_EXTRAP) ? &ctx->ext_psi : nullptr,
+                        nn_guess ? &ctx->guess_psi : nullptr};
+    const bool judge = cand[1] || cand[2];
+    if (judge) {
+        const int dir1 = (multi[2] > multi[1]) ? 0 : 1;
+        const double norm = (double)(L1.g.nx - 1) * (double)(L1.g.nz - 1);
+        auto first_visit = [&](double* mag) -> int {
+            TC_TRY(dc_iterate(ctx, d, 1, dir1, f32));
+            TC_TRY(dc_absorb_residual(ctx, d, f32));
+            TC_TRY(fetch_scalars(ctx, d, S_COUNT));
+            *mag = d.h_scal[S_RESID1];
+            return 0;
+        };
+        double best = 0.0;
+        TC_TRY(first_visit(&best));
+        ctx->stats.guess_res[0] = best / norm;
+        bool stale = false;  // the residual planes belong to a start that lost
+        for (int c = 1; c < 3; ++c) {
+            if (!cand[c]) continue;
+            std::swap(ctx->psi_hi, *cand[c]);
+            std::swap(ctx->psi_lo, ctx->guess_lo);
+            TC_HIP(hipMemsetAsync(ctx->psi_lo, 0, plane_bytes(L1.g), ctx->stream));
+            TC_TRY(defect(ctx->psi_hi, nullptr));
+            double mag = 0.0;
+            TC_TRY(first_visit(&mag));
+            ctx->stats.guess_res[c] = mag / norm;
+            if (mag < best) {
+                best = mag;
+                ctx->stats.guess_used = c;
+                stale = false;
+            } else {
+                std::swap(ctx->psi_hi, *cand[c]);
+                std::swap(ctx->psi_lo, ctx->guess_lo);
+                stale = true;
+            }
+        }
+        if (stale) {
+            double mag = 0.0;
+            TC_TRY(defect(ctx->psi_hi, &mag));
+        }
+    }
+    if (ctx->solve_stage >= 0) ctx->stats.guesses_used[ctx->stats.guess_used]++;
+    const bool first_visit_done = judge;
     int q = 1;
     int status = 0;
     for (;;) {
         bool done = false;
         for (int p = 1; p <= P.nsteps; ++p) {
+            if (first_visit_done && q == 1 && p == 1) {
+                // level 1's first visit ran while judging the guess
+                status = stop_test(ctx, d, q, done);
+                if (status || done) break;
+                continue;
+            }
             const int level = multi[p];
             const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
             Level& L = ctx->lv[level];
@@ -987,6 +1139,8 @@ int energy_time(tc_ctx* ctx, tc_dev& d)
         if (P.RK_update == 1) {
             TC_TRY(tracers_to_corners(ctx, d, TC_TR_X1, 1, nullptr));
             TC_TRY(smoother_time_T(ctx, d, ctx->T1, nullptr));
+            ctx->solve_stage = 1;
+            ctx->solve_tau = ctx->time + 0.5 * dt;
             TC_TRY(run_multigrid(ctx, nullptr));
             TC_TRY(velocities(ctx, d));
         }
@@ -998,6 +1152,8 @@ int energy_time(tc_ctx* ctx, tc_dev& d)
             TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
             TC_TRY(tracers_to_corners(ctx, d, TC_TR_X2, 1, nullptr));
             TC_TRY(smoother_time_T(ctx, d, ctx->T2, nullptr));
+            ctx->solve_stage = 2;
+            ctx->solve_tau = ctx->time + 0.5 * dt;
             TC_TRY(run_multigrid(ctx, nullptr));
             TC_TRY(velocities(ctx, d));
         }
@@ -1009,12 +1165,15 @@ int energy_time(tc_ctx* ctx, tc_dev& d)
             TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
             TC_TRY(tracers_to_corners(ctx, d, TC_TR_X1, 1, nullptr));
             TC_TRY(smoother_time_T(ctx, d, ctx->T1, nullptr));
+            ctx->solve_stage = 3;
+            ctx->solve_tau = ctx->time + dt;
             TC_TRY(run_multigrid(ctx, nullptr));
             TC_TRY(velocities(ctx, d));
         }
         TC_TRY(energy_stage(ctx, d, 4, dt));
         TC_TRY(tracer_stage(ctx, d, 4, dt));
     }
+    ctx->solve_stage = -1;
     TC_TRY(copy_C(ctx, ctx->Cnew, ctx->C0));
     TC_TRY(tracers_to_corners(ctx, d, TC_TR_X, 1, nullptr));
     TC_TRY(smoother_time_T(ctx, d, ctx->T, nullptr));
@@ -1088,10 +1247,12 @@ void free_ctx(tc_ctx* ctx)
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
                    ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
-                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->psi_h1, ctx->T_h1,
-                   ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi};
+                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->ext_psi, ctx->stage_err,
+                   ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi, ctx->guess_lo, ctx->nn_T,
+                   ctx->nn_C, ctx->nn_p1, ctx->nn_p2, ctx->nn_ext, ctx->nn_psi, ctx->nn_inc, ctx->nn_ra, ctx->nn_rb};
     for (double* p : b) F(p);
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
+    for (int t = 0; t < 4; ++t) { F(ctx->sh1[t]); F(ctx->sh2[t]); F(ctx->Tc1[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
     F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag);
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
@@ -1549,12 +1710,15 @@ int tc_time_step(tc_ctx* ctx, int nsteps)
     if (!ctx || nsteps < 0) return TC_ERR_ARG;
     tc_dev& d = DEV(ctx);
     for (int s = 0; s < nsteps; ++s) {
+        ctx->solve_stage = 0;
+        ctx->solve_tau = ctx->time;
         TC_TRY(run_multigrid(ctx, nullptr));
         TC_TRY(velocities(ctx, d));
         launch_strain(ctx->stream, ctx->u, ctx->w, ctx->strain, ctx->g, d.F, ctx->lv[1].dr, ctx->lv[1].ds);
         TC_TRY(energy_time(ctx, d));
         ctx->time += ctx->dt;
         ctx->tstep++;
+        ctx->solve_stage = -1;
     }
     TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1587,36 +1751,51 @@ int tc_kernel_timing(tc_ctx* ctx, int which, double* avg_us, int64_t* count)
     return 0;
 }
 
-int tc_guess_enable(tc_ctx* ctx, int enable)
+int tc_guess_enable(tc_ctx* ctx, int mode)
 {
-    if (!ctx) return TC_ERR_ARG;
-    if (enable && !ctx->nn) {
+    if (!ctx || mode < TC_GUESS_PLAIN || mode > TC_GUESS_LEARNED) return TC_ERR_ARG;
+    if (mode == TC_GUESS_LEARNED && !ctx->nn) {
         ctx->err = "no network attached (tc_learn_attach)";
         return TC_ERR_ARG;
     }
-    ctx->guess = enable ? 1 : 0;
+    if (mode != TC_GUESS_PLAIN) TC_TRY(guess_alloc(ctx));
+    ctx->guess = mode;
+    TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
 int tc_learn_attach(tc_ctx* ctx, tc_nn* nn)
 {
     if (!ctx) return TC_ERR_ARG;
-    if (nn && (nn->net.H != ctx->p.nz || nn->net.W != ctx->p.nx)) {
-        ctx->err = "network grid must be H = nz, W = nx";
+    int m = 0;
+    if (nn)
+        for (int L = 1; L < ctx->grids; ++L)
+            if (nn->net.H == ctx->lv[L].g.nz && nn->net.W == ctx->lv[L].g.nx) { m = L; break; }
+    if (nn && !m) {
+        ctx->err = "network grid must be H = nz, W = nx of a multigrid level below the coarsest";
+        return TC_ERR_ARG;
+    }
+    if (nn && ctx->nn_level && m != ctx->nn_level) {
+        ctx->err = "a network of another level was attached before";
         return TC_ERR_ARG;
     }
     ctx->nn = nn;
-    ctx->nhist = 0;
-    ctx->guess = 0;
-    if (!nn) return 0;
-    const Grid& g = ctx->lv[1].g;
-    if (!ctx->psi_h1) {
-        TC_TRY(dalloc(ctx, &ctx->psi_h1, g.elems));
-        TC_TRY(dalloc(ctx, &ctx->T_h1, g.elems));
-        TC_TRY(dalloc(ctx, &ctx->guess_psi, g.elems));
-        TC_TRY(dalloc(ctx, &ctx->nn_scales, 3));
-        const size_t nb = (size_t)((g.nx - 1 + 63) / 64) * (size_t)((g.nz - 1 + 3) / 4);
-        TC_TRY(dalloc(ctx, &ctx->nn_part, 2 * nb));
+    for (int s = 0; s < 4; ++s) ctx->ecount[s] = 0;
+    if (!nn) {
+        if (ctx->guess == TC_GUESS_LEARNED) ctx->guess = TC_GUESS_EXTRAP;
+        return 0;
+    }
+    ctx->nn_level = m;
+    const Grid& gc = ctx->lv[m].g;
+    TC_TRY(guess_alloc(ctx));
+    if (!ctx->nn_T) {
+        double** coarse[] = {&ctx->nn_T, &ctx->nn_C, &ctx->nn_p1, &ctx->nn_p2, &ctx->nn_ext, &ctx->nn_psi,
+                             &ctx->nn_inc, &ctx->Tc1[0], &ctx->Tc1[1], &ctx->Tc1[2], &ctx->Tc1[3]};
+        for (double** q : coarse) TC_TRY(dalloc(ctx, q, gc.elems));
+        TC_TRY(dalloc(ctx, &ctx->nn_ra, ctx->lv[2].g.elems));
+        TC_TRY(dalloc(ctx, &ctx->nn_rb, ctx->lv[2].g.elems));
+        TC_TRY(dalloc(ctx, &ctx->nn_scales, NN_NSCALE));
+        TC_TRY(dalloc(ctx, &ctx->nn_part, 3 * (size_t)nn_scale_blocks(gc)));
     }
     TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1639,7 +1818,7 @@ int tc_learn_record(tc_ctx* ctx, int32_t capacity)
     TC_HIP(hipMalloc((void**)&ctx->rec_x, (size_t)capacity * np * 8 * sizeof(__bf16)));
     TC_HIP(hipMalloc((void**)&ctx->rec_t, (size_t)capacity * np * sizeof(float)));
     TC_HIP(hipMalloc((void**)&ctx->rec_mask, np * sizeof(float)));
-    TC_HIP(hipMalloc((void**)&ctx->rec_scales, (size_t)capacity * 3 * sizeof(double)));
+    TC_HIP(hipMalloc((void**)&ctx->rec_scales, (size_t)capacity * NN_NSCALE * sizeof(double)));
     ctx->rec_cap = capacity;
     return 0;
 }

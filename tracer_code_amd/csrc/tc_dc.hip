@@ -301,8 +301,23 @@ __global__ __launch_bounds__(256) void k_coeffs_to_f32(const double* __restrict_
     }
 }
 
+// out = a + c * (a - b) over whole planes: linear extrapolation in time of
+// two streamfunctions; the ghost mirrors of enforceBCs_stream are linear,
+// so the result keeps them
+__global__ __launch_bounds__(256) void k_extrap(const double* __restrict__ a, const double* __restrict__ b, double c,
+                                                double* __restrict__ out, size_t n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = fma(c, a[e] - b[e], a[e]);
+}
+
 // ---- launchers -------------------------------------------------------------
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n)
+{
+    hipLaunchKernelGGL(k_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, c, out, n);
+}
 
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial)

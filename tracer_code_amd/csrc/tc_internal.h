@@ -66,7 +66,9 @@ struct tc_ctx {
     tc_params p;
     int64_t ntr = 0;
     int mode = TC_MODE_FP64;
-    int guess = 0;
+    int guess = 0;                // initial guess of the defect-correction modes (TC_GUESS_*)
+    int solve_stage = -1;         // RK stage of the running solve within the time step (0..3), -1 outside one
+    double solve_tau = 0.0;       // model time the stage's buoyancy field stands for
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;   // coarse LU / inverse overlap stream
     hipEvent_t ev_inv_ready = nullptr, ev_visc_ready = nullptr;
@@ -104,18 +106,34 @@ struct tc_ctx {
     double time = 0.0, dt = 0.0;
     int tstep = 0;
     tc_stats stats{};
-    // learned guess (tc_nn.hip): attached network, solution history, recorded samples
+    // initial-guess history: per RK stage, the solutions of that stage in
+    // the last two time steps (fine grid) and the model times they stand for
+    double* sh1[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* sh2[4] = {nullptr, nullptr, nullptr, nullptr};
+    double tau1[4] = {0, 0, 0, 0}, tau2[4] = {0, 0, 0, 0};
+    int scount[4] = {0, 0, 0, 0};
+    double* ext_psi = nullptr;    // the extrapolated guess of the running solve
+    bool have_ext = false;
+    int ecount[4] = {0, 0, 0, 0}; // per stage: extrapolation errors measured with the network attached
+    double* stage_err = nullptr;  // 4 x [NN_NSCALE]; [3] = the stage's last max |R psi - R ext|
+    // learned correction (tc_nn.hip): attached network and recorded samples
     struct tc_nn* nn = nullptr;
-    double *psi_h1 = nullptr, *T_h1 = nullptr;  // solution and buoyancy temperature of the last solve
-    int nhist = 0;
+    int nn_level = 0;             // multigrid level the network works on (its H x W = that level's nz x nx)
+    double* Tc1[4] = {nullptr, nullptr, nullptr, nullptr};  // per stage: buoyancy T of its last solve (level nn_level)
+    double *nn_T = nullptr, *nn_C = nullptr;       // this solve's T and C restricted to level nn_level
+    double *nn_p1 = nullptr, *nn_p2 = nullptr;     // the stage's last two solutions restricted
+    double *nn_ext = nullptr, *nn_psi = nullptr;   // restricted extrapolation and solution
+    double* nn_inc = nullptr;                      // predicted correction (level nn_level)
+    double *nn_ra = nullptr, *nn_rb = nullptr;     // restriction/prolongation scratch (level-2 sized)
     int rec_cap = 0, rec_n = 0;
-    __bf16* rec_x = nullptr;      // rec_cap x [nz*nx*8] inputs
-    float* rec_t = nullptr;       // rec_cap x [nz*nx] targets
-    float* rec_mask = nullptr;    // [nz*nx]
-    double* nn_scales = nullptr;  // [3] scales of the current solve: max|psi_prev|, increment scale, max|dT|
-    double* rec_scales = nullptr; // rec_cap x [3] scales of the recorded samples
+    __bf16* rec_x = nullptr;      // rec_cap x [H*W*8] inputs (the network's grid)
+    float* rec_t = nullptr;       // rec_cap x [H*W] targets
+    float* rec_mask = nullptr;    // [H*W]
+    double* nn_scales = nullptr;  // [4] scales of the current solve (tc_nn.h NN_NSCALE)
+    double* rec_scales = nullptr; // rec_cap x [4] scales of the recorded samples
     double* nn_part = nullptr;    // block partials of the scale reduction
-    double* guess_psi = nullptr;  // candidate initial guess
+    double* guess_psi = nullptr;  // candidate start being judged (high part; after a swap, the other start)
+    double* guess_lo = nullptr;   // its low part
     // implementation-private host bookkeeping (tc_api.hip)
     void* impl = nullptr;
     // kernel timing (tc_set_profiling)

@@ -12,6 +12,9 @@
 
 namespace tc {
 
+constexpr int NN_IN = 6;          // input channels (stored padded to 8 per pixel)
+constexpr int NN_NSCALE = 4;      // per-solve scales: max|psi_prev|, s*dT, dT, max|psi_prev - psi_prev2|
+
 struct Layer {
     int cin, cout, lev;           // input/output channels, pyramid level (0 = full resolution)
     int cinp = 0, kp = 0;         // padded input channels (x8), padded GEMM depth 9*cinp (x32)
@@ -60,13 +63,14 @@ struct UNet {
 
 // kernels shared with the solver integration (tc_api.hip)
 void launch_nn_input(hipStream_t st, const double* T, const double* C, const double* vis, const double* p1,
-                     const double* Tp, const Grid& g, int W, int H, const double* scales, __bf16* x);
+                     const double* p2, const double* Tp, const Grid& g, int W, int H, const double* scales, __bf16* x);
 void launch_nn_target(hipStream_t st, const double* psi, const double* p1, const Grid& g, int W, int H,
                       const double* scales, float* t, float* mask);
 void launch_nn_apply(hipStream_t st, const float* out, const double* p1, double* psi, const Grid& g, int W,
                      const double* scales, double zsg);
-void launch_nn_scales(hipStream_t st, const double* p1, const double* T, const double* Tp, const Grid& g,
-                      double* partial, double* scales);
+int nn_scale_blocks(const Grid& g);  // partials of launch_nn_scales: 3 doubles per block
+void launch_nn_scales(hipStream_t st, const double* p1, const double* p2, const double* T, const double* Tp,
+                      const Grid& g, double* partial, double* scales);
 
 }  // namespace tc
 

@@ -3,9 +3,10 @@
 // activations with FP32 accumulation on the MFMA units (v_mfma_f32_16x16x32_bf16),
 // and hand-written forward, backward and Adam kernels.
 //
-// Inputs per node (i,k), i < nx, k < nz: T, C, log10(eta)/5, psi_prev/s
-// and (T - T_prev)/dT, where psi_prev and T_prev are the solution and the
-// buoyancy temperature of the previous solve, s = max |psi_prev| and
+// Inputs per node (i,k), i < nx, k < nz: T, C, log10(eta)/5, psi_prev/s,
+// (psi_prev - psi_prev2)/sd and (T - T_prev)/dT, where psi_prev, psi_prev2
+// are the solutions of the last two solves, T_prev the buoyancy temperature
+// of the last one, s = max |psi_prev|, sd = max |psi_prev - psi_prev2| and
 // dT = max |T - T_prev|; output: the increment psi - psi_prev divided by
 // s*dT.  The increment responds (almost linearly) to the change of the
 // buoyancy field, which differs irregularly between the RK stages.  There is no
@@ -359,26 +360,30 @@ __global__ __launch_bounds__(256) void k_unpack_grad(const float* __restrict__ g
 }
 
 // input assembly: node (i,k) of the solver plane -> pixel k*W+i (8 channels)
-// scales[0] = s = max |psi_prev|, scales[2] = dT = max |T - T_prev| (1 if
-// zero), scales[1] = s * dT: the scale of the predicted increment
+// scales[0] = s = max |psi_prev|, scales[2] = dT = max |T - T_prev|,
+// scales[3] = sd = max |psi_prev - psi_prev2| (each 1 if zero),
+// scales[1] = s * dT: the scale of the predicted increment
 __global__ __launch_bounds__(256) void k_nn_input(const double* __restrict__ T, const double* __restrict__ C,
                                                   const double* __restrict__ vis, const double* __restrict__ p1,
-                                                  const double* __restrict__ Tp, Grid g, int W, int H,
-                                                  const double* __restrict__ scales, __bf16* __restrict__ x)
+                                                  const double* __restrict__ p2, const double* __restrict__ Tp, Grid g,
+                                                  int W, int H, const double* __restrict__ scales,
+                                                  __bf16* __restrict__ x)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (i >= W || k >= H) return;
-    const double inv_s = 1.0 / scales[0], inv_dt = 1.0 / scales[2];
+    const double inv_s = 1.0 / scales[0], inv_dt = 1.0 / scales[2], inv_sd = 1.0 / scales[3];
     const size_t c = g.at(i, k);
     const size_t o = ((size_t)k * W + i) * 8;
     x[o + 0] = (__bf16)(float)T[c];
     x[o + 1] = (__bf16)(float)(C ? C[c] : 0.0);
     x[o + 2] = (__bf16)(float)(log10(vis[c]) * 0.2);
     x[o + 3] = (__bf16)(float)(p1[c] * inv_s);
-    x[o + 4] = (__bf16)(float)((T[c] - Tp[c]) * inv_dt);
-    for (int j = 5; j < 8; ++j) x[o + j] = (__bf16)0.0f;
+    x[o + 4] = (__bf16)(float)((p1[c] - p2[c]) * inv_sd);
+    x[o + 5] = (__bf16)(float)((T[c] - Tp[c]) * inv_dt);
+    x[o + 6] = (__bf16)0.0f;
+    x[o + 7] = (__bf16)0.0f;
 }
-// training target: (psi - psi_prev) / sd on interior nodes, mask 1 there
+// training target: (psi - psi_prev) / (s*dT) on interior nodes, mask 1 there
 __global__ __launch_bounds__(256) void k_nn_target(const double* __restrict__ psi, const double* __restrict__ p1, Grid g,
                                                    int W, int H, const double* __restrict__ scales,
                                                    float* __restrict__ t, float* __restrict__ mask)
@@ -391,7 +396,8 @@ __global__ __launch_bounds__(256) void k_nn_target(const double* __restrict__ ps
     t[(size_t)k * W + i] = in ? (float)((psi[c] - p1[c]) * inv_sd) : 0.0f;
     mask[(size_t)k * W + i] = in ? 1.0f : 0.0f;
 }
-// guess: psi = psi_prev + sd * out on the interior, with enforceBCs_stream mirrors
+// guess: psi = psi_prev + s*dT * out on the interior, with enforceBCs_stream
+// mirrors (p1 null: the increment alone)
 __global__ __launch_bounds__(256) void k_nn_apply(const float* __restrict__ out, const double* __restrict__ p1,
                                                   double* __restrict__ psi, Grid g, int W,
                                                   const double* __restrict__ scales, double zsg)
@@ -400,51 +406,61 @@ __global__ __launch_bounds__(256) void k_nn_apply(const float* __restrict__ out,
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const double sd = scales[1];
     const size_t c = g.at(i, k);
-    const double v = p1[c] + sd * (double)out[(size_t)k * W + i];
+    const double v = (p1 ? p1[c] : 0.0) + sd * (double)out[(size_t)k * W + i];
     psi[c] = v;
     stream_ghosts(psi, g, i, k, v, zsg);
 }
 
-// max |psi_prev| and max |T - T_prev| over the interior: per-block
-// partials, then the final pass writes the scales (guarded against zero)
-__global__ __launch_bounds__(256) void k_nn_scale_part(const double* __restrict__ p1, const double* __restrict__ T,
-                                                       const double* __restrict__ Tp, Grid g, double* __restrict__ part)
+// max |psi_prev|, max |T - T_prev| and max |psi_prev - psi_prev2| over the
+// interior: per-block partials (3 per block), then the final pass writes the
+// scales (guarded against zero)
+__global__ __launch_bounds__(256) void k_nn_scale_part(const double* __restrict__ p1, const double* __restrict__ p2,
+                                                       const double* __restrict__ T, const double* __restrict__ Tp,
+                                                       Grid g, double* __restrict__ part)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, k = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, e = 0.0;
     if (i <= g.nx - 1 && k <= g.nz - 1) {
         const size_t c = g.at(i, k);
         a = fabs(p1[c]);
         b = fabs(T[c] - Tp[c]);
+        e = fabs(p1[c] - p2[c]);
     }
     a = block_max(a);
     b = block_max(b);
+    e = block_max(e);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        part[2 * (blockIdx.y * gridDim.x + blockIdx.x)] = a;
-        part[2 * (blockIdx.y * gridDim.x + blockIdx.x) + 1] = b;
+        double* q = part + 3 * (blockIdx.y * gridDim.x + blockIdx.x);
+        q[0] = a;
+        q[1] = b;
+        q[2] = e;
     }
 }
 __global__ __launch_bounds__(256) void k_nn_scale_final(const double* __restrict__ part, int n, double* __restrict__ sc)
 {
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, e = 0.0;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        a = fmax(a, part[2 * t]);
-        b = fmax(b, part[2 * t + 1]);
+        a = fmax(a, part[3 * t]);
+        b = fmax(b, part[3 * t + 1]);
+        e = fmax(e, part[3 * t + 2]);
     }
     a = block_max(a);
     b = block_max(b);
+    e = block_max(e);
     if (threadIdx.x == 0) {
         const double s = a > 0.0 ? a : 1.0, dT = b > 0.0 ? b : 1.0;
         sc[0] = s;
         sc[1] = s * dT;
         sc[2] = dT;
+        sc[3] = e > 0.0 ? e : 1.0;
     }
 }
 
 void launch_nn_input(hipStream_t st, const double* T, const double* C, const double* vis, const double* p1,
-                     const double* Tp, const Grid& g, int W, int H, const double* scales, __bf16* x)
+                     const double* p2, const double* Tp, const Grid& g, int W, int H, const double* scales, __bf16* x)
 {
-    hipLaunchKernelGGL(k_nn_input, dim3((W + 255) / 256, H), dim3(256), 0, st, T, C, vis, p1, Tp, g, W, H, scales, x);
+    hipLaunchKernelGGL(k_nn_input, dim3((W + 255) / 256, H), dim3(256), 0, st, T, C, vis, p1, p2, Tp, g, W, H, scales,
+                       x);
 }
 void launch_nn_target(hipStream_t st, const double* psi, const double* p1, const Grid& g, int W, int H,
                       const double* scales, float* t, float* mask)
@@ -457,12 +473,14 @@ void launch_nn_apply(hipStream_t st, const float* out, const double* p1, double*
     hipLaunchKernelGGL(k_nn_apply, dim3((g.nx - 1 + 255) / 256, g.nz - 1), dim3(256), 0, st, out, p1, psi, g, W, scales,
                        zsg);
 }
-void launch_nn_scales(hipStream_t st, const double* p1, const double* T, const double* Tp, const Grid& g,
-                      double* partial, double* scales)
+int nn_scale_blocks(const Grid& g) { return ((g.nx - 1 + 63) / 64) * ((g.nz - 1 + 3) / 4); }
+
+void launch_nn_scales(hipStream_t st, const double* p1, const double* p2, const double* T, const double* Tp,
+                      const Grid& g, double* partial, double* scales)
 {
     dim3 b(64, 4);
     dim3 gr((g.nx - 1 + 63) / 64, (g.nz - 1 + 3) / 4);
-    hipLaunchKernelGGL(k_nn_scale_part, gr, b, 0, st, p1, T, Tp, g, partial);
+    hipLaunchKernelGGL(k_nn_scale_part, gr, b, 0, st, p1, p2, T, Tp, g, partial);
     hipLaunchKernelGGL(k_nn_scale_final, dim3(1), dim3(256), 0, st, partial, (int)(gr.x * gr.y), scales);
 }
 
@@ -477,7 +495,7 @@ void UNet::layer_defs()
     const int c1 = 16, c2 = 32, c3 = 64, c4 = 128;
     L.clear();
     auto add = [&](int cin, int cout, int lev) { L.push_back(Layer{cin, cout, lev}); };
-    add(5, c1, 0); add(c1, c1, 0);            // e1a e1b
+    add(NN_IN, c1, 0); add(c1, c1, 0);        // e1a e1b
     add(c1, c2, 1); add(c2, c2, 1);           // e2a e2b
     add(c2, c3, 2); add(c3, c3, 2);           // e3a e3b
     add(c3, c4, 3); add(c4, c4, 3);           // b4a b4b
@@ -793,7 +811,7 @@ int tc_nn_get_grads(tc_nn* n, float* g, int64_t count)
     return nn_check(n, hipStreamSynchronize(n->stream), "sync");
 }
 
-// x: H*W*5 values, pixel-major (p = k*W + i), channel fastest; target/mask: H*W
+// x: H*W*NN_IN values, pixel-major (p = k*W + i), channel fastest; target/mask: H*W
 int tc_nn_set_sample(tc_nn* n, const float* x, const float* target, const float* mask)
 {
     if (!n || !x) return TC_ERR_ARG;
@@ -801,7 +819,7 @@ int tc_nn_set_sample(tc_nn* n, const float* x, const float* target, const float*
     const size_t np = (size_t)u.npx[0];
     n->hx.assign(np * 8, (__bf16)0.0f);
     for (size_t p = 0; p < np; ++p)
-        for (int c = 0; c < 5; ++c) n->hx[p * 8 + c] = (__bf16)x[p * 5 + c];
+        for (int c = 0; c < tc::NN_IN; ++c) n->hx[p * 8 + c] = (__bf16)x[p * tc::NN_IN + c];
     TC_NN_TRY(nn_check(n, hipMemcpyAsync(u.x0, n->hx.data(), np * 8 * sizeof(__bf16), hipMemcpyHostToDevice, n->stream), "x"));
     if (target) TC_NN_TRY(nn_check(n, hipMemcpyAsync(u.target, target, np * sizeof(float), hipMemcpyHostToDevice, n->stream), "t"));
     if (mask) {

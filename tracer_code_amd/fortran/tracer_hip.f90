@@ -43,7 +43,8 @@ module tracer_hip
     integer(c_int32_t) :: q_last, n_eta, n_temp, n_comp(TC_MAX_TYPES), guess_used, pad
     real(c_double) :: residual_last, res_ref_last, eig_fine, dt_last, time
     integer(c_int64_t) :: sweeps_exec(TC_MAX_LEVELS), residuals_exec(TC_MAX_LEVELS)
-    integer(c_int64_t) :: coarse_solves, vis_iters, c_iters, t_iters, mg_cycles
+    integer(c_int64_t) :: coarse_solves, vis_iters, c_iters, t_iters, mg_cycles, guesses_used(3)
+    real(c_double) :: guess_res(3)
   end type tc_stats
 
   interface

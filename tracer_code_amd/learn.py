@@ -1,16 +1,18 @@
 """The learned initial guess: network handle, weights and the train/evaluate
 workflow.
 
-    python -m tracer_code_amd.learn train --config cfg2 --nx 512 --nz 256 \\
-        --seeds 4 --steps 12 --epochs 8 --out tracer_code_amd/weights/unet_cfg2_512x256.npz
-    python -m tracer_code_amd.learn evaluate --config cfg2 --nx 512 --nz 256 \\
-        --weights tracer_code_amd/weights/unet_cfg2_512x256.npz --seed 100 --steps 6
+    python -m tracer_code_amd.learn train --config cfg1 --nx 768 --nz 256 --level 4 \\
+        --seeds 20 --steps 25 --epochs 30 --ra-range 3e5 3e6 --out tracer_code_amd/weights/unet_cfg1_L4.npz
+    python -m tracer_code_amd.learn evaluate --config cfg1 --nx 768 --nz 256 \\
+        --weights tracer_code_amd/weights/unet_cfg1_L4.npz --seed 100 --steps 6 --ra 1e6
 
 The network (a 4-level U-Net, channels 16/32/64/128, BF16 on the MFMA
 units) runs and trains entirely in the HIP library (tc_nn_*, tc_learn_*);
-this module only drives it.  Training data are the solver's own solves:
-before each solve the inputs (T, C, log10 eta, the two previous solutions)
-are recorded on the device, after it the increment of the solution.
+this module only drives it.  It works on one multigrid level (`--level`,
+its grid is that level's): before each solve the inputs (T, C, log10 eta,
+the two previous solutions, the change of T, all restricted to the level)
+are recorded on the device, after it the increment of the restricted
+solution; as a guess, the predicted increment is prolonged to the fine grid.
 """
 import argparse
 import ctypes
@@ -26,7 +28,7 @@ from . import _abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 # (cin, cout) per 3x3 layer in forward order; the 1x1 head 16 -> 1 follows
-LAYERS = [(5, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (192, 64), (64, 64),
+LAYERS = [(6, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 128), (128, 128), (192, 64), (64, 64),
           (96, 32), (32, 32), (48, 16), (16, 16)]
 
 
@@ -103,7 +105,7 @@ class Network:
 
     def set_sample(self, x, target=None, mask=None):
         x = np.ascontiguousarray(x, dtype=np.float32)
-        assert x.shape == (self.H, self.W, 5)
+        assert x.shape == (self.H, self.W, 6)
         t = None if target is None else np.ascontiguousarray(target, dtype=np.float32)
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
         self._check(self.lib.tc_nn_set_sample(self.h, x.ctypes.data, None if t is None else t.ctypes.data,
@@ -123,11 +125,22 @@ class Network:
         self._check(self.lib.tc_nn_adam(self.h, float(lr), float(grad_scale)), "adam")
 
 
-def _solver(config, nx, nz, seed, tpc=None):
+def level_shape(nx, nz, level):
+    """(H, W) of the network on multigrid level `level` (1 = the fine grid)."""
+    return nz >> (level - 1), nx >> (level - 1)
+
+
+def _solver(config, nx, nz, seed, tpc=None, ra=None):
+    """A Solver on the named workload; `ra` replaces RaT and scales RaC with
+    it (the buoyancy ratio stays)."""
     from . import Solver, config as cfgmod, ic
     ov = dict(nx=nx, nz=nz)
     if tpc:
         ov["tpc"] = tpc
+    if ra:
+        base = cfgmod.PRESETS[config]
+        ov["RaT"] = float(ra)
+        ov["RaC"] = tuple(c * float(ra) / base["RaT"] for c in base["RaC"])
     params, cfg = cfgmod.preset_params(config, **ov)
     T, r, s, ty = ic.workload(cfg, seed=seed)
     g = Solver(params, len(r))
@@ -135,68 +148,103 @@ def _solver(config, nx, nz, seed, tpc=None):
     return params, g
 
 
-def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tpc=None, log=print):
+def draw_ra(ra_range, seed):
+    """Ra log-uniform in ra_range (None: the preset's)."""
+    if not ra_range:
+        return None
+    rng = np.random.default_rng(1000 + seed)
+    lo, hi = np.log(ra_range[0]), np.log(ra_range[1])
+    return float(np.exp(rng.uniform(lo, hi)))
+
+
+def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tpc=None, ra_range=None, level=4,
+          log=print):
     """Record the solves of `steps` time steps from each of `seeds` seeded
-    starts, train the network on them on the GPU and save the weights."""
-    net = Network(nz, nx, trainable=True)
+    starts (Ra drawn per seed from ra_range), then train the network on all
+    of them on the GPU -- each epoch visits every run's samples in a fresh
+    shuffled order, with a cosine learning-rate decay to lr/20 -- and save
+    the weights."""
+    net = Network(*level_shape(nx, nz, level), trainable=True)
     net.set_weights(init_weights(0))
     total = 0
     t0 = time.perf_counter()
-    loss = None
-    solvers = []
+    solvers, ras = [], []
     for sd in range(seeds):
-        params, g = _solver(config, nx, nz, seed=sd, tpc=tpc)
+        ra = draw_ra(ra_range, sd)
+        _, g = _solver(config, nx, nz, seed=sd, tpc=tpc, ra=ra)
         g.set_mode(mode)
         g.learn_attach(net)
         g.learn_record(4 * steps)
         g.time_step(steps)
         n = g.learn_samples()
         total += n
-        log(f"[learn] seed {sd}: {n} samples ({time.perf_counter() - t0:.1f}s)")
-        loss = g.learn_train(epochs, lr, batch, seed=sd)
-        log(f"[learn] seed {sd}: trained {epochs} epochs, mean loss {loss:.4e} ({time.perf_counter() - t0:.1f}s)")
+        solvers.append(g)
+        ras.append(ra)
+        log(f"[learn] run {sd} (Ra {ra if ra else 'preset'}): {n} samples ({time.perf_counter() - t0:.1f}s)")
+    losses = []
+    for ep in range(epochs):
+        lr_ep = lr * (0.05 + 0.95 * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs - 1, 1))))
+        order = np.random.default_rng(ep).permutation(len(solvers))
+        ls = [solvers[j].learn_train(1, lr_ep, batch, seed=ep * 1000 + int(j)) for j in order]
+        losses.append(float(np.mean(ls)))
+        log(f"[learn] epoch {ep}: lr {lr_ep:.2e}, mean loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f}s)")
+    for g in solvers:
         g.learn_record(0)
         g.learn_attach(None)
         g.close()
     w = net.get_weights()
     net.close()
     meta = dict(config=config, nx=nx, nz=nz, seeds=seeds, steps=steps, epochs=epochs, lr=lr, batch=batch,
-                samples=total, final_loss=loss, mode=mode)
+                samples=total, losses=losses, mode=mode, tpc=tpc, ra=ras, level=level)
     if out:
         os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
         save_weights(out, w, meta)
     return w, meta
 
 
-def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=2):
-    """Multigrid cycles per solve and time per step with and without the
-    learned guess from the same seeded state (`warm` steps first build the
-    solution history the guess needs)."""
+def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=3, ra=None, level=4):
+    """Multigrid cycles and time per step from the same seeded state with
+    each initial-guess mode: "plain" (the reference's warm start), "extrap"
+    (plain or the RK stage's extrapolation in time) and "learned" (also the
+    extrapolation plus the network's correction).  `warm` steps first build
+    the per-stage history.  Reports which start won how often and the median
+    residual after the first level-1 smoothing of each candidate relative to
+    the plain start (over the solves where both were judged)."""
     res = {}
-    for use in (False, True):
-        params, g = _solver(config, nx, nz, seed=seed, tpc=tpc)
+    for gm in ("plain", "extrap", "learned"):
+        params, g = _solver(config, nx, nz, seed=seed, tpc=tpc, ra=ra)
         g.set_mode(mode)
-        net = Network(nz, nx, trainable=False)
-        net.set_weights(weights)
-        g.learn_attach(net)
+        net = None
+        if gm == "learned":
+            net = Network(*level_shape(nx, nz, level), trainable=False)
+            net.set_weights(weights)
+            g.learn_attach(net)
+        g.guess_enable(gm)
         g.time_step(warm)
-        g.guess_enable(use)
         st0 = g.stats()
         g.sync()
         t0 = time.perf_counter()
-        used = 0
+        rel = {1: [], 2: []}
         for _ in range(steps):
             g.time_step(1)
-            used += g.stats().guess_used
+            st = g.stats()
+            for c in (1, 2):
+                if st.guess_res[0] > 0 and st.guess_res[c] > 0:
+                    rel[c].append(st.guess_res[c] / st.guess_res[0])
         g.sync()
         el = time.perf_counter() - t0
         st1 = g.stats()
-        cyc = st1.mg_cycles - st0.mg_cycles
-        res["learned" if use else "plain"] = dict(cycles_per_step=cyc / steps, ms_per_step=1e3 * el / steps,
-                                                  last_guess_used=used)
-        g.learn_attach(None)
+        r = dict(cycles_per_step=(st1.mg_cycles - st0.mg_cycles) / steps, ms_per_step=1e3 * el / steps,
+                 starts={k: int(st1.guesses_used[i] - st0.guesses_used[i])
+                         for i, k in enumerate(("plain", "extrap", "learned"))})
+        for c, k in ((1, "extrap"), (2, "learned")):
+            if rel[c]:
+                r[f"median_{k}_vs_plain_first_smoothing"] = float(np.median(rel[c]))
+        res[gm] = r
+        if net is not None:
+            g.learn_attach(None)
+            net.close()
         g.close()
-        net.close()
     return res
 
 
@@ -214,16 +262,20 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--mode", default="mixed")
+    ap.add_argument("--ra-range", type=float, nargs=2, default=None)
+    ap.add_argument("--ra", type=float, default=None)
+    ap.add_argument("--level", type=int, default=4)
     ap.add_argument("--out", default=None)
     ap.add_argument("--weights", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         _, meta = train(a.config, a.nx, a.nz, a.seeds, a.steps, a.epochs, a.lr, a.batch, a.out, a.mode, a.tpc,
-                        log=lambda m: print(m, file=sys.stderr, flush=True))
+                        a.ra_range, a.level, log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(meta))
     else:
-        w, _ = load_weights(a.weights)
-        print(json.dumps(evaluate(a.config, a.nx, a.nz, w, a.seed, a.steps, a.mode, a.tpc)))
+        w, meta = load_weights(a.weights)
+        print(json.dumps(evaluate(a.config, a.nx, a.nz, w, a.seed, a.steps, a.mode, a.tpc, ra=a.ra,
+                                  level=meta.get("level", a.level))))
     return 0
 
 

@@ -207,8 +207,18 @@ class Solver:
                                             ctypes.byref(loss)), "learn_train")
         return loss.value
 
-    def guess_enable(self, enable):
-        self._check(self.lib.tc_guess_enable(self.h, 1 if enable else 0), "guess_enable")
+    GUESS_MODES = {"plain": _abi.TC_GUESS_PLAIN, "extrap": _abi.TC_GUESS_EXTRAP, "learned": _abi.TC_GUESS_LEARNED}
+
+    def guess_enable(self, mode):
+        """Initial-guess candidates of the defect-correction solves inside
+        time_step: "plain" (the last solution only), "extrap" (also the RK
+        stage's extrapolation in time), "learned" (also the extrapolation
+        plus the attached network's correction); True means "learned"."""
+        if mode is True:
+            mode = "learned"
+        elif mode is False:
+            mode = "plain"
+        self._check(self.lib.tc_guess_enable(self.h, self.GUESS_MODES[mode]), "guess_enable")
 
     def set_profiling(self, enable):
         self._check(self.lib.tc_set_profiling(self.h, int(bool(enable))), "set_profiling")
