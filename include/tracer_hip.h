@@ -56,6 +56,11 @@ enum {
     TC_MODE_MIXED = 1, /* defect correction: double-double psi and fine residual, FP32 W-cycle corrections */
     TC_MODE_DC64 = 2   /* defect correction as MIXED with FP64 W-cycle corrections */
 };
+/* In both defect-correction modes the correction cycle's coarsest-grid
+ * solve uses the inverse of the previous solve's coarse operator while the
+ * current one is factorised on a side stream (ready for the next solve); the
+ * first solve waits for its own.  The outer residual is exact, so every solve
+ * still ends at the same tolerance. */
 
 /* Field identifiers for tc_put_field / tc_get_field. */
 enum {

@@ -371,18 +371,44 @@ int iterate_stream(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool& 
     return 0;
 }
 
-int coarse_setup(tc_ctx* ctx, tc_dev& d)
+// Assembly, DGBEQU/DGBTF2 and the explicit inverse of the coarsest
+// operator on the side stream.  lagged (defect-correction modes): the result
+// goes to the *_next buffers for the next solve, and this solve uses the
+// previous solve's (swapped in here); the first solve waits for its own.
+int coarse_setup(tc_ctx* ctx, tc_dev& d, bool lagged)
 {
     Level& L = ctx->lv[ctx->grids];
     const tc_params& P = ctx->p;
+    bool to_next = false;
+    if (lagged && d.coarse_ready) {
+        if (ctx->next_pending) {
+            std::swap(ctx->Ainv, ctx->Ainv_next);
+            std::swap(ctx->RowB, ctx->RowB_next);
+            std::swap(ctx->d_info, ctx->d_info_next);
+            std::swap(ctx->ev_inv_ready, ctx->ev_inv_next);
+        }
+        if (!ctx->Ainv_next) {
+            TC_TRY(dalloc(ctx, &ctx->Ainv_next, (size_t)ctx->ngc * ctx->ngc));
+            TC_TRY(dalloc(ctx, &ctx->RowB_next, ctx->ngc));
+            TC_HIP(hipMalloc((void**)&ctx->d_info_next, sizeof(int32_t)));
+            TC_HIP(hipMemsetAsync(ctx->d_info_next, 0, sizeof(int32_t), ctx->stream));
+        }
+        to_next = true;
+    }
+    double* Ainv = to_next ? ctx->Ainv_next : ctx->Ainv;
+    double* RowB = to_next ? ctx->RowB_next : ctx->RowB;
+    int32_t* info = to_next ? ctx->d_info_next : ctx->d_info;
+    // the side stream starts after this solve's viscosity and after every
+    // earlier use of the buffers it overwrites (stream order of the main stream)
     TC_HIP(hipEventRecord(ctx->ev_visc_ready, ctx->stream));
     TC_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_visc_ready, 0));
     TC_HIP(hipMemsetAsync(ctx->Mc, 0, sizeof(double) * (size_t)ctx->ldab * ctx->ngc, ctx->side));
-    launch_coarse_assemble(ctx->side, ctx->Mc, ctx->RowB, L, d.F, d.h[ctx->grids], P.ivis, P.Vbc, ctx->kl, ctx->ku,
+    launch_coarse_assemble(ctx->side, ctx->Mc, RowB, L, d.F, d.h[ctx->grids], P.ivis, P.Vbc, ctx->kl, ctx->ku,
                            ctx->ngc);
-    launch_dgbtf2(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->d_info);
-    launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, ctx->Ainv, ctx->Ur);
-    TC_HIP(hipEventRecord(ctx->ev_inv_ready, ctx->side));
+    launch_dgbtf2(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, info);
+    launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, Ainv, ctx->Ur);
+    TC_HIP(hipEventRecord(to_next ? ctx->ev_inv_next : ctx->ev_inv_ready, ctx->side));
+    ctx->next_pending = to_next;
     d.coarse_ready = true;
     return 0;
 }
@@ -405,7 +431,7 @@ int solve_setup(tc_ctx* ctx, tc_dev& d)
     TC_TRY(fetch_scalars(ctx, d, S_COUNT));
     ctx->res_ref[1] = d.h_scal[S_RESREF1] / (double)(L1.g.nx - 2) / (double)(L1.g.nz - 2);
     for (int L = 1; L <= G; ++L) TC_TRY(compute_viscosity(ctx, d, L));
-    if (ctx->time == 0.0 || P.ivis > 0 || !d.coarse_ready) TC_TRY(coarse_setup(ctx, d));
+    if (ctx->time == 0.0 || P.ivis > 0 || !d.coarse_ready) TC_TRY(coarse_setup(ctx, d, ctx->mode != TC_MODE_FP64));
     for (int L = 1; L <= G; ++L) {
         EigScalars e = eig_scalars(P, ctx->lv[L]);
         launch_local_dt(ctx->stream, ctx->lv[L], e, P.courant_stream);
@@ -1246,7 +1272,7 @@ void free_ctx(tc_ctx* ctx)
     }
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
-                   ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
+                   ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->RowB_next, ctx->Ainv_next, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
                    ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->ext_psi, ctx->stage_err,
                    ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi, ctx->guess_lo, ctx->nn_T,
                    ctx->nn_C, ctx->nn_p1, ctx->nn_p2, ctx->nn_ext, ctx->nn_psi, ctx->nn_inc, ctx->nn_ra, ctx->nn_rb};
@@ -1254,7 +1280,7 @@ void free_ctx(tc_ctx* ctx)
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
     for (int t = 0; t < 4; ++t) { F(ctx->sh1[t]); F(ctx->sh2[t]); F(ctx->Tc1[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
-    F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag);
+    F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag); F(ctx->d_info_next);
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
@@ -1266,6 +1292,7 @@ void free_ctx(tc_ctx* ctx)
     }
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
     if (ctx->ev_inv_ready) (void)hipEventDestroy(ctx->ev_inv_ready);
+    if (ctx->ev_inv_next) (void)hipEventDestroy(ctx->ev_inv_next);
     if (ctx->ev_visc_ready) (void)hipEventDestroy(ctx->ev_visc_ready);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -1290,6 +1317,7 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
     TC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     TC_HIP(hipEventCreateWithFlags(&ctx->ev_inv_ready, hipEventDisableTiming));
+    TC_HIP(hipEventCreateWithFlags(&ctx->ev_inv_next, hipEventDisableTiming));
     TC_HIP(hipEventCreateWithFlags(&ctx->ev_visc_ready, hipEventDisableTiming));
     // FD tables (order 2): Drs(i,k)=D1(k)*D1(i) ... (main.f90:448-459)
     for (int j = 0; j < 5; ++j) { d->F.D1[j] = p->D1[j]; d->F.D2[j] = p->D2[j]; d->F.D3[j] = p->D3[j]; d->F.D4[j] = p->D4[j]; }

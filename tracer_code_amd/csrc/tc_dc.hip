@@ -94,8 +94,9 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
                                                      float* __restrict__ r32, Grid g, FD F, Spacing h, int ivis,
                                                      double* __restrict__ partial)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     double a = 0.0;
     if (i <= g.nx - 1 && k <= g.nz - 1) {
         const size_t c = g.at(i, k);
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
     }
     if (partial) {
         a = block_sum(a);
-        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[tile.y * gridDim.x + tile.x] = a;
     }
 }
 
@@ -189,8 +190,9 @@ __global__ __launch_bounds__(256) void k_sweep32(const float* __restrict__ S0, f
                                                  const float* __restrict__ dts, const float* __restrict__ lb32, Grid g,
                                                  FD32 F, H32 h, float temp, float cconst, float zsg)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const size_t c = g.at(i, k);
     float t = stream_space32<float>(F, h, S0, c, g.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);
@@ -210,8 +212,9 @@ __global__ __launch_bounds__(256) void k_residual32(const float* __restrict__ S,
                                                     const float* __restrict__ vxz, const float* __restrict__ rhs,
                                                     Grid g, FD32 F, H32 h)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const size_t c = g.at(i, k);
     R[c] = (float)stream_space32<double>(F, h, S, c, g.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);

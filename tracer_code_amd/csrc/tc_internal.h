@@ -15,7 +15,11 @@
 namespace tc {
 
 constexpr int SPAN = 2;       // ghost width of every grid field (order/2+1)
-constexpr int ORG_PAD = 8;    // elements before column i=0 so that i=0 is 64-byte aligned
+// elements before column i=0 of each row: with the row pitch a multiple of
+// 32 elements, the first interior column i=1 starts a 128-byte cache line
+// (FP32) or 256 bytes (FP64), so a 64-wide row segment of a launch touches
+// the fewest lines and no line is shared by two workgroups of one row
+constexpr int ORG_PAD = 31;
 
 // A ghosted 2-D plane f(-2:nx+2,-2:nz+2) in device memory.  Element (i,k)
 // sits at data[org + k*ld + i]; ld is a multiple of 32 doubles.
@@ -26,6 +30,21 @@ struct Grid {
     size_t elems = 0;     // allocation size in elements
     __host__ __device__ inline size_t at(int i, int k) const { return (size_t)org + (size_t)((ptrdiff_t)k * ld + i); }
 };
+
+#ifdef __HIPCC__
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with
+// its own L2.  xcd_tile() maps a 2-D launch's linear workgroup id so that
+// XCD x takes the x-th contiguous band of the tile grid (row-major): tiles
+// that share stencil halos then mostly share an L2.  Identity when 8 does
+// not divide the number of workgroups.  Returns the tile (x, y).
+__device__ inline int2 xcd_tile()
+{
+    const int gx = (int)gridDim.x, n = (int)(gridDim.x * gridDim.y);
+    int w = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    if ((n & 7) == 0) w = (w & 7) * (n >> 3) + (w >> 3);
+    return make_int2(w % gx, w / gx);
+}
+#endif
 
 inline Grid make_grid(int nx, int nz)
 {
@@ -72,6 +91,7 @@ struct tc_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;   // coarse LU / inverse overlap stream
     hipEvent_t ev_inv_ready = nullptr, ev_visc_ready = nullptr;
+    hipEvent_t ev_inv_next = nullptr;  // the lagged factorisation in progress (defect-correction modes)
     std::string err;
     // hierarchy
     int grids = 0;
@@ -96,6 +116,11 @@ struct tc_ctx {
     double* Ur = nullptr;  // row-major copy of the U band (explicit inverse)
     int32_t* ipiv = nullptr;
     int32_t* d_info = nullptr;
+    // defect-correction modes: the factorisation of this solve's coarse
+    // operator is computed into these while the solve uses the previous one
+    double *RowB_next = nullptr, *Ainv_next = nullptr;
+    int32_t* d_info_next = nullptr;
+    bool next_pending = false;
     // tracers (SoA)
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;

@@ -23,8 +23,9 @@ __global__ __launch_bounds__(256) void k_sweep(const double* __restrict__ S0, do
                                                const double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
                                                double temp, double cconst, double zsg)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const size_t c = g.at(i, k);
     double t = stream_space<POW2>(F, h, ivis, S0, c, g.ld, vis[c], vx[c], vz[c], vxx[c], vzz[c], vxz[c], rhs[c]);
@@ -78,8 +79,9 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ S, 
                                                   const double* __restrict__ rhs, Grid g, FD F, Spacing h, int ivis,
                                                   double* __restrict__ partial)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     double a = 0.0;
     if (i <= g.nx - 1 && k <= g.nz - 1) {
         const size_t c = g.at(i, k);
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ S, 
     }
     if (partial) {
         a = block_sum(a);
-        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[tile.y * gridDim.x + tile.x] = a;
     }
 }
 
