@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}
+DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
 
 
@@ -89,6 +90,10 @@ def main():
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--mode", default="mixed", choices=["mixed", "dc64", "fp64"])
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cycle", default="W", choices=["W", "F", "V"])
+    ap.add_argument("--nmulti", type=int, default=None, help="smoother sweeps per level visit (default: the preset's)")
+    ap.add_argument("--dc-sweeps", type=int, default=None,
+                    help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -98,7 +103,10 @@ def main():
 
     from tracer_code_amd import Solver, _abi, config, ic
 
-    params, cfg = config.preset_params(args.config)
+    ov = dict(cycle=args.cycle)
+    if args.nmulti:
+        ov["Nmulti"] = args.nmulti
+    params, cfg = config.preset_params(args.config, **ov)
     t_setup = time.perf_counter()
     T, r, s, ty = ic.workload(cfg, seed=args.seed)
     ntr = len(r)
@@ -108,6 +116,18 @@ def main():
     solver.set_mode(args.mode)
     solver.initialize(T, r, s, ty)
     del T, r, s, ty
+    tune = None
+    dc_sweeps = params.Nmulti
+    if args.mode != "fp64":
+        if args.dc_sweeps is None:
+            # untimed: one real time step per candidate
+            t0 = time.perf_counter()
+            dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+            print(f"[bench] autotuned sweeps per visit: {dc_sweeps} ({time.perf_counter() - t0:.1f}s; "
+                  f"ms/step {tune})", file=sys.stderr, flush=True)
+        else:
+            dc_sweeps = args.dc_sweeps
+            solver.set_dc_sweeps(dc_sweeps)
     for w in range(args.warmup):
         t0 = time.perf_counter()
         solver.time_step(1)
@@ -159,6 +179,11 @@ def main():
             "workload": f"{params.nx}x{params.nz} aspect {params.aspect:g}, Ra={params.RaT:g}, "
                         f"{ntr} tracers, multigrid to {params.tolerance:g} relative residual",
             "mode": args.mode,
+            "cycle": args.cycle,
+            "sweeps_per_visit": dc_sweeps,
+            "reference_sweeps_per_visit": params.Nmulti,
+            "autotune_ms_per_step": tune,
+            "dc_retries": after.dc_retries,
             "levels": params.grids,
             "mg_cycles_per_step": cycles_per_step,
             "sweeps_per_step": sweeps_per_step,

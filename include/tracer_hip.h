@@ -158,6 +158,7 @@ typedef struct tc_stats {
     /* Last solve: residual_mag(1) after the first level-1 smoothing from the
      * plain start, the extrapolation and the learned guess (0: not judged). */
     double guess_res[3];
+    int64_t dc_retries;      /* defect-correction solves redone with Nmulti sweeps (cumulative) */
 } tc_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -167,6 +168,20 @@ int tc_init(const tc_params* p, int64_t ntr, tc_ctx** out);
 int tc_finalize(tc_ctx* ctx);
 const char* tc_last_error(const tc_ctx* ctx);
 int tc_set_mode(tc_ctx* ctx, int mode);
+/* Jacobi-Richardson sweeps per level visit of the defect-correction modes'
+ * correction cycle (0: the parameters' Nmulti, the reference's value).  A
+ * solve that with it diverges or exceeds TC_DC_FAST_MAX_CYCLES cycles is
+ * redone with Nmulti (counted in tc_stats.dc_retries). */
+#define TC_DC_FAST_MAX_CYCLES 200
+int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps);
+/* Autotuner of the defect-correction sweep count: runs steps_each time
+ * steps (real steps of the run) with each of the n candidates, and keeps
+ * the largest candidate whose wall time per step is within 3% of the
+ * fastest among those that needed no retry (0, i.e. Nmulti, if all did).
+ * ms_per_step[n] (may be NULL) receives the measured times, *best the
+ * choice, which is also set as with tc_set_dc_sweeps. */
+int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int32_t steps_each, int32_t* best,
+                          double* ms_per_step);
 int tc_get_stats(const tc_ctx* ctx, tc_stats* out);
 int tc_sync(tc_ctx* ctx);
 

@@ -69,3 +69,25 @@ def test_full_size_reaches_tolerance(mode):
         g.set_mode(mode)
         g.multigrid()
         check_dc_solution(params, g, f"2048x1024 {mode}")
+
+
+def test_dc_sweeps_retry_and_autotune():
+    """A sweep count too small to converge is caught (divergence or the cycle
+    cap) and the solve is redone with Nmulti; the autotuner keeps one of its
+    candidates and every solve still meets the tolerance."""
+    params, g, _ = solver_for("cfg0", nx=64, nz=64, tpc=4)
+    with g:
+        g.set_mode("mixed")
+        g.set_dc_sweeps(1)
+        g.time_step(1)
+        st = g.stats()
+        print(f"1 sweep per visit: retries {st.dc_retries}, q_last {st.q_last}", flush=True)
+        assert st.dc_retries >= 1
+        check_dc_solution(params, g, "after retry")
+        r0 = st.dc_retries
+        best, ms = g.autotune_dc_sweeps([32, 48, 70], 1)
+        print(f"autotune: best {best}, ms/step {ms}", flush=True)
+        assert best in (32, 48, 70) and set(ms) == {32, 48, 70}
+        g.time_step(1)
+        check_dc_solution(params, g, f"{best} sweeps per visit")
+        assert g.stats().dc_retries == r0

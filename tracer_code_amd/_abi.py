@@ -73,7 +73,7 @@ class TcStats(ctypes.Structure):
         ("sweeps_exec", ctypes.c_int64 * TC_MAX_LEVELS), ("residuals_exec", ctypes.c_int64 * TC_MAX_LEVELS),
         ("coarse_solves", ctypes.c_int64), ("vis_iters", ctypes.c_int64), ("c_iters", ctypes.c_int64),
         ("t_iters", ctypes.c_int64), ("mg_cycles", ctypes.c_int64), ("guesses_used", ctypes.c_int64 * 3),
-        ("guess_res", ctypes.c_double * 3),
+        ("guess_res", ctypes.c_double * 3), ("dc_retries", ctypes.c_int64),
     ]
 
 
@@ -85,6 +85,8 @@ SIGNATURES = {
     "tc_finalize": (ctypes.c_int, [_VP]),
     "tc_last_error": (ctypes.c_char_p, [_VP]),
     "tc_set_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "tc_set_dc_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     "tc_get_stats": (ctypes.c_int, [_VP, _P(TcStats)]),
     "tc_sync": (ctypes.c_int, [_VP]),
     "tc_put_field": (ctypes.c_int, [_VP, ctypes.c_int, _VP]),

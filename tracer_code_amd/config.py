@@ -63,9 +63,30 @@ def optimize_grid(nx, nz, grids, n_points_min=4):
     return nx, nz, grids
 
 
-def multigrid_setup(grids):
+def cycle_sequence(grids, cycle="W"):
+    """Level visit sequence of one cycle.  "W": the reference's cycles
+    (multigrid_cycle.f90); "V": down to the coarsest and back up to level 2;
+    "F": down, then up one more level at a time with a descent to the
+    coarsest after each, ending at level 2.  Every sequence starts at level 1
+    and ends at level 2, as the reference's do (the next cycle's level-1
+    visit smooths after and before)."""
+    if cycle == "W":
+        return _CYCLES[grids]
+    down = list(range(1, grids + 1))
+    if cycle == "V":
+        return tuple(down + list(range(grids - 1, 1, -1)))
+    if cycle == "F":
+        seq = down[:]
+        for top in range(grids - 1, 1, -1):
+            seq += list(range(grids - 1, top - 1, -1)) + list(range(top + 1, grids + 1))
+        seq += list(range(grids - 1, 1, -1))
+        return tuple(seq)
+    raise ValueError(f"unknown cycle {cycle!r}")
+
+
+def multigrid_setup(grids, cycle="W"):
     """multigrid_setup (multigrid_cycle.f90:1-99): (nsteps, multi(0:51))."""
-    seq = _CYCLES[grids]
+    seq = cycle_sequence(grids, cycle)
     multi = [0] * _abi.TC_MAX_MULTI
     for p, lv in enumerate(seq, start=1):
         multi[p] = lv
@@ -221,10 +242,10 @@ def make_params(nx, nz, aspect=1.0, grids=6, mingrid=4, RaT=1e5, comp=1, ntype=1
                 visT=100.0, visP=1.0, visC=(1.0,), delta_visT=100.0, delta_visC=(2.0,), conduct_factor=(1.0,),
                 H=0.0, Htype=(0.0,), tolerance=1e-8, Nmulti=70, vis_iter=12, max_cycles=1000, eig_ratio=4.0,
                 eig_ratio_min=4.0, courant=0.99, courantT=0.99, courantC=0.99, courant_stream=0.99, torder=4,
-                RK_update=1, Vbc=0, Tbot=0, order=2, fixes=0, lib=None, **_ignored):
+                RK_update=1, Vbc=0, Tbot=0, order=2, fixes=0, cycle="W", lib=None, **_ignored):
     """tc_params for one run (order=2, iterate_order=1, uniform grid)."""
     nx, nz, grids = optimize_grid(nx, nz, grids, mingrid)
-    nsteps, multi = multigrid_setup(grids)
+    nsteps, multi = multigrid_setup(grids, cycle)
     D1, D2, D3, D4, Dint = fd_coefficients(order)
     base = derivative_eigenvalues(D1, D2, D3, D4, lib=lib)
     pad = lambda seq, fill: (list(seq) + [fill] * _abi.TC_MAX_TYPES)[:_abi.TC_MAX_TYPES]

@@ -5,6 +5,7 @@
 // (default priority) factors and inverts the coarsest-grid matrix while the
 // first W-cycle descent runs.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -760,7 +761,7 @@ int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
     Level view = L;
     if (level == 1) view.rhs = L.res;
     const float* rhs32 = (level == 1) ? L.res32 : L.rhs32;
-    TC_TRY(run_sweeps(ctx, d, level, P.Nmulti, dir, f32, view, rhs32, nullptr));
+    TC_TRY(run_sweeps(ctx, d, level, ctx->dc_sweeps_active, dir, f32, view, rhs32, nullptr));
     if (level > 1) {
         if (f32) launch_residual32(ctx->stream, L, L.err32[L.cur32], L.res32, L.rhs32, d.F, d.h[level]);
         else launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis, nullptr);
@@ -792,7 +793,7 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
 // from a zero correction; the coarsest grid is solved in FP64.  In exact
 // arithmetic one cycle is the reference's cycle with zeroed coarse
 // corrections; the stop test reads the double-double residual of psi.
-int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
+int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
 {
     tc_dev& d = DEV(ctx);
     const tc_params& P = ctx->p;
@@ -933,13 +934,32 @@ int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
         ctx->stats.mg_cycles++;
         if (status || done) break;
         q++;
-        if (q > P.max_cycles) {
+        if (q > max_cycles) {
             ctx->err = "Multigrid Convergence Problem: Too Many Multigrid Cycles Needed";
             status = TC_ERR_MAX_CYCLES;
             break;
         }
     }
     return finish_solve(ctx, status, q, ctx->psi_hi, q_out);
+}
+
+// A defect-correction solve with the tuned sweep count, redone with the
+// parameters' Nmulti if it diverges or stalls (SF is only written by a
+// solve that converged, so the retry starts from the same state).
+int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
+{
+    const tc_params& P = ctx->p;
+    const int fast = ctx->dc_sweeps;
+    if (fast <= 0 || fast == P.Nmulti) {
+        ctx->dc_sweeps_active = P.Nmulti;
+        return dc_solve(ctx, q_out, P.max_cycles);
+    }
+    ctx->dc_sweeps_active = fast;
+    const int st = dc_solve(ctx, q_out, std::min(P.max_cycles, TC_DC_FAST_MAX_CYCLES));
+    if (st != TC_ERR_RESID_BLOWUP && st != TC_ERR_MAX_CYCLES) return st;
+    ctx->stats.dc_retries++;
+    ctx->dc_sweeps_active = P.Nmulti;
+    return dc_solve(ctx, q_out, P.max_cycles);
 }
 
 // ---------------------------------------------------------------------------
@@ -1487,6 +1507,44 @@ int tc_set_mode(tc_ctx* ctx, int mode)
     if (!ctx) return TC_ERR_ARG;
     if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED && mode != TC_MODE_DC64) return TC_ERR_ARG;
     ctx->mode = mode;
+    return 0;
+}
+
+int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps)
+{
+    if (!ctx || sweeps < 0 || sweeps > 10000) return TC_ERR_ARG;
+    ctx->dc_sweeps = sweeps;
+    return 0;
+}
+
+int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int32_t steps_each, int32_t* best,
+                          double* ms_per_step)
+{
+    if (!ctx || !candidates || n < 1 || steps_each < 1 || !best) return TC_ERR_ARG;
+    for (int j = 0; j < n; ++j)
+        if (candidates[j] < 1 || candidates[j] > 10000) return TC_ERR_ARG;
+    std::vector<double> ms(n);
+    std::vector<bool> ok(n);
+    for (int j = 0; j < n; ++j) {
+        ctx->dc_sweeps = candidates[j];
+        const int64_t r0 = ctx->stats.dc_retries;
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        TC_TRY(tc_time_step(ctx, steps_each));
+        ms[j] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps_each;
+        ok[j] = ctx->stats.dc_retries == r0;
+        if (ms_per_step) ms_per_step[j] = ms[j];
+    }
+    // the largest count within 3% of the fastest: a margin from the counts
+    // at which the Chebyshev-weighted sweeps stop converging
+    double fastest = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (ok[j] && (fastest == 0.0 || ms[j] < fastest)) fastest = ms[j];
+    int pick = -1;
+    for (int j = 0; j < n; ++j)
+        if (ok[j] && ms[j] <= 1.03 * fastest && (pick < 0 || candidates[j] > candidates[pick])) pick = j;
+    ctx->dc_sweeps = pick >= 0 ? candidates[pick] : 0;
+    *best = ctx->dc_sweeps;
     return 0;
 }
 

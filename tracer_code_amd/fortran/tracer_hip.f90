@@ -45,6 +45,7 @@ module tracer_hip
     integer(c_int64_t) :: sweeps_exec(TC_MAX_LEVELS), residuals_exec(TC_MAX_LEVELS)
     integer(c_int64_t) :: coarse_solves, vis_iters, c_iters, t_iters, mg_cycles, guesses_used(3)
     real(c_double) :: guess_res(3)
+    integer(c_int64_t) :: dc_retries
   end type tc_stats
 
   interface

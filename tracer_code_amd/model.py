@@ -207,6 +207,20 @@ class Solver:
                                             ctypes.byref(loss)), "learn_train")
         return loss.value
 
+    def set_dc_sweeps(self, sweeps):
+        """Sweeps per level visit of the defect-correction cycle (0: Nmulti)."""
+        self._check(self.lib.tc_set_dc_sweeps(self.h, int(sweeps)), "set_dc_sweeps")
+
+    def autotune_dc_sweeps(self, candidates, steps_each=1):
+        """Runs steps_each time steps with each candidate sweep count and
+        keeps the best; returns (best, {candidate: ms per step})."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = ctypes.c_int32()
+        self._check(self.lib.tc_autotune_dc_sweeps(self.h, c.ctypes.data, len(c), int(steps_each), ctypes.byref(best),
+                                                   ms.ctypes.data), "autotune_dc_sweeps")
+        return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
     GUESS_MODES = {"plain": _abi.TC_GUESS_PLAIN, "extrap": _abi.TC_GUESS_EXTRAP, "learned": _abi.TC_GUESS_LEARNED}
 
     def guess_enable(self, mode):
