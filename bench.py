@@ -91,9 +91,14 @@ def main():
     ap.add_argument("--mode", default="mixed", choices=["mixed", "dc64", "fp64"])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cycle", default="W", choices=["W", "F", "V"])
+    ap.add_argument("--tolerance", type=float, default=None,
+                    help="relative residual of every solve (default: the preset's 1e-8)")
     ap.add_argument("--nmulti", type=int, default=None, help="smoother sweeps per level visit (default: the preset's)")
     ap.add_argument("--dc-sweeps", type=int, default=None,
                     help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
+    ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
+                    help="initial-guess candidates of the defect-correction solves (learned: with the committed "
+                         "network of this workload, if there is one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -104,6 +109,8 @@ def main():
     from tracer_code_amd import Solver, _abi, config, ic
 
     ov = dict(cycle=args.cycle)
+    if args.tolerance:
+        ov["tolerance"] = args.tolerance
     if args.nmulti:
         ov["Nmulti"] = args.nmulti
     params, cfg = config.preset_params(args.config, **ov)
@@ -116,6 +123,20 @@ def main():
     solver.set_mode(args.mode)
     solver.initialize(T, r, s, ty)
     del T, r, s, ty
+    net = None
+    guess = args.guess if args.mode != "fp64" else "plain"
+    if guess == "learned":
+        from tracer_code_amd import learn
+        wpath = os.path.join(ROOT, "tracer_code_amd", "weights", f"unet_{args.config}_L4.npz")
+        if os.path.exists(wpath):
+            w, meta = learn.load_weights(wpath)
+            net = learn.Network(*learn.level_shape(params.nx, params.nz, meta["level"]), trainable=False)
+            net.set_weights(w)
+            solver.learn_attach(net)
+        else:
+            guess = "extrap"
+    if args.mode != "fp64":
+        solver.guess_enable(guess)
     tune = None
     dc_sweeps = params.Nmulti
     if args.mode != "fp64":
@@ -184,6 +205,9 @@ def main():
             "reference_sweeps_per_visit": params.Nmulti,
             "autotune_ms_per_step": tune,
             "dc_retries": after.dc_retries,
+            "guess": guess,
+            "starts": {k: after.guesses_used[i] - before.guesses_used[i]
+                       for i, k in enumerate(("plain", "extrap", "learned"))},
             "levels": params.grids,
             "mg_cycles_per_step": cycles_per_step,
             "sweeps_per_step": sweeps_per_step,
@@ -201,6 +225,9 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    if net is not None:
+        solver.learn_attach(None)
+        net.close()
     solver.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(params, sweeps_per_step)

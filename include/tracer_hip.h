@@ -238,8 +238,10 @@ int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const doub
 /* ---- measurement hooks (no counterpart in the reference) --------------- */
 /* Kernel ids for tc_kernel_timing. */
 enum { TC_K_SWEEP_L1 = 0, TC_K_COUNT };
-/* enable=1: bracket every launch of the timed kernels with HIP events on the
- * library stream (the stream the kernels run on); enable=0 stops. */
+/* enable=1: time the level-1 sweeps with HIP events on the library stream
+ * (the stream the kernels run on): one event pair around each visit's
+ * back-to-back launches (around each launch where the reference form's wall
+ * reset follows the first sweep); enable=0 stops. */
 int tc_set_profiling(tc_ctx* ctx, int enable);
 /* Average device duration (microseconds) and launch count of kernel `which`
  * since profiling was enabled. */

@@ -109,3 +109,25 @@ def test_guess_in_solver_keeps_tolerance(level):
         print(f"starts over the last 3 steps (plain, extrapolation, learned): {starts}", flush=True)
         assert sum(starts) == 12
         g.learn_attach(None)
+
+
+def test_trained_network_gain_on_held_out_run():
+    """What the trained network gains: the committed weights (768x256 grid,
+    network on multigrid level 4, trained on 720 solves of 20 seeded runs
+    with Ra log-uniform in [3e5, 3e6]) against the plain warm start and the
+    extrapolation alone, on a held-out seed.  Every start is judged by the
+    residual after the first level-1 smoothing, so the learned form can only
+    be chosen where it is better there; the test reports the cycles and
+    times of the three forms and checks that the learned form needs no more
+    cycles than the plain one (up to 2%).  Every solve meets the tolerance
+    (a solve that does not raises)."""
+    import os
+    path = os.path.join(os.path.dirname(learn.__file__), "weights", "unet_cfg1_L4.npz")
+    w, meta = learn.load_weights(path)
+    print(f"[trained network: {meta['samples']} samples, level {meta['level']}]", flush=True)
+    res = learn.evaluate("cfg1", 768, 256, w, seed=100, steps=2, tpc=10, ra=1e6, level=meta["level"], warm=3)
+    for k, v in res.items():
+        print(f"{k}: {v}", flush=True)
+    assert res["learned"]["cycles_per_step"] <= 1.02 * res["plain"]["cycles_per_step"]
+    assert res["extrap"]["cycles_per_step"] <= 1.02 * res["plain"]["cycles_per_step"]
+    assert sum(res["learned"]["starts"].values()) == 8

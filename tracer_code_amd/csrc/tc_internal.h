@@ -166,6 +166,7 @@ struct tc_ctx {
     // kernel timing (tc_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_cnt;    // launches bracketed by each event pair
     int prof_used = 0;
     double prof_ms[TC_K_COUNT] = {0};
     int64_t prof_n[TC_K_COUNT] = {0};

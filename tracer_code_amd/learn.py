@@ -158,7 +158,7 @@ def draw_ra(ra_range, seed):
 
 
 def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tpc=None, ra_range=None, level=4,
-          log=print):
+          dc_sweeps=0, log=print):
     """Record the solves of `steps` time steps from each of `seeds` seeded
     starts (Ra drawn per seed from ra_range), then train the network on all
     of them on the GPU -- each epoch visits every run's samples in a fresh
@@ -173,6 +173,7 @@ def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tp
         ra = draw_ra(ra_range, sd)
         _, g = _solver(config, nx, nz, seed=sd, tpc=tpc, ra=ra)
         g.set_mode(mode)
+        g.set_dc_sweeps(dc_sweeps)
         g.learn_attach(net)
         g.learn_record(4 * steps)
         g.time_step(steps)
@@ -195,14 +196,14 @@ def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tp
     w = net.get_weights()
     net.close()
     meta = dict(config=config, nx=nx, nz=nz, seeds=seeds, steps=steps, epochs=epochs, lr=lr, batch=batch,
-                samples=total, losses=losses, mode=mode, tpc=tpc, ra=ras, level=level)
+                samples=total, losses=losses, mode=mode, tpc=tpc, ra=ras, level=level, dc_sweeps=dc_sweeps)
     if out:
         os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
         save_weights(out, w, meta)
     return w, meta
 
 
-def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=3, ra=None, level=4):
+def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=3, ra=None, level=4, dc_sweeps=0):
     """Multigrid cycles and time per step from the same seeded state with
     each initial-guess mode: "plain" (the reference's warm start), "extrap"
     (plain or the RK stage's extrapolation in time) and "learned" (also the
@@ -214,6 +215,7 @@ def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=
     for gm in ("plain", "extrap", "learned"):
         params, g = _solver(config, nx, nz, seed=seed, tpc=tpc, ra=ra)
         g.set_mode(mode)
+        g.set_dc_sweeps(dc_sweeps)
         net = None
         if gm == "learned":
             net = Network(*level_shape(nx, nz, level), trainable=False)
@@ -265,17 +267,18 @@ def main(argv=None):
     ap.add_argument("--ra-range", type=float, nargs=2, default=None)
     ap.add_argument("--ra", type=float, default=None)
     ap.add_argument("--level", type=int, default=4)
+    ap.add_argument("--dc-sweeps", type=int, default=0)
     ap.add_argument("--out", default=None)
     ap.add_argument("--weights", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "train":
         _, meta = train(a.config, a.nx, a.nz, a.seeds, a.steps, a.epochs, a.lr, a.batch, a.out, a.mode, a.tpc,
-                        a.ra_range, a.level, log=lambda m: print(m, file=sys.stderr, flush=True))
+                        a.ra_range, a.level, a.dc_sweeps, log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(meta))
     else:
         w, meta = load_weights(a.weights)
         print(json.dumps(evaluate(a.config, a.nx, a.nz, w, a.seed, a.steps, a.mode, a.tpc, ra=a.ra,
-                                  level=meta.get("level", a.level))))
+                                  level=meta.get("level", a.level), dc_sweeps=a.dc_sweeps)))
     return 0
 
 
