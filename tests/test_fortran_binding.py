@@ -1,7 +1,8 @@
 """The Fortran ISO_C_BINDING interface (tracer_code_amd/fortran/tracer_hip.f90).
 
 * CPU: flang's layout of the interoperable types equals the C layout
-  (sizes and field offsets of tc_params / tc_stats).
+  (sizes and field offsets of tc_params / tc_stats), and the module has an
+  interface for every entry point of include/tracer_hip.h.
 * GPU: a Fortran program that calls the C-ABI in the order of the
   reference's start-up and first solve gives bit-identical SF, u, w and one
   time step's SF and tracers to the Python mirror of the same calls.
@@ -16,6 +17,17 @@ import pytest
 from tracer_code_amd import Solver, _abi, build, config, ic
 
 FB = build.FBUILD
+
+
+def test_module_binds_every_entry_point():
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "tracer_hip.h")).read()
+    mod = open(os.path.join(root, "tracer_code_amd", "fortran", "tracer_hip.f90")).read()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(tc_\w+)\(", hdr, re.M))
+    bound = set(re.findall(r'bind\(C, name="(tc_\w+)"\)', mod))
+    assert declared and not (declared - bound), sorted(declared - bound)
+    assert not (bound - declared), sorted(bound - declared)
 
 
 def need(prog):

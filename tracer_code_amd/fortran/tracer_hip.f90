@@ -12,6 +12,8 @@ module tracer_hip
                                TC_ERR_TRACER_DOMAIN = 3, TC_ERR_SINGULAR = 4, TC_ERR_ARG = -1, &
                                TC_ERR_HIP = -2, TC_ERR_UNSUPPORTED = -3, TC_ERR_NOMEM = -4
   integer(c_int), parameter :: TC_MODE_FP64 = 0, TC_MODE_MIXED = 1, TC_MODE_DC64 = 2
+  integer(c_int), parameter :: TC_GUESS_PLAIN = 0, TC_GUESS_EXTRAP = 1, TC_GUESS_LEARNED = 2
+  integer(c_int), parameter :: TC_DC_FAST_MAX_CYCLES = 200
   integer(c_int), parameter :: TC_F_T = 0, TC_F_SF = 1, TC_F_TVIS = 2, TC_F_TBUOY = 3, TC_F_CNEW = 4, &
                                TC_F_CVIS = 5, TC_F_CBUOY = 6, TC_F_U = 7, TC_F_W = 8, TC_F_STRAIN = 9, &
                                TC_F_VIS = 10, TC_F_CONDUCT = 11, TC_F_NEMPTY = 12, TC_F_COUNT_TYPE = 13, &
@@ -180,10 +182,101 @@ module tracer_hip
       real(c_double), intent(out) :: avg_us
       integer(c_int64_t), intent(out) :: count
     end function
-    integer(c_int) function tc_guess_enable(ctx, enable) bind(C, name="tc_guess_enable")
+    integer(c_int) function tc_guess_enable(ctx, mode) bind(C, name="tc_guess_enable")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), value :: enable
+      integer(c_int), value :: mode
+    end function
+    integer(c_int) function tc_set_dc_sweeps(ctx, sweeps) bind(C, name="tc_set_dc_sweeps")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sweeps
+    end function
+    integer(c_int) function tc_autotune_dc_sweeps(ctx, candidates, n, steps_each, best, ms_per_step) &
+        bind(C, name="tc_autotune_dc_sweeps")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: candidates(*)
+      integer(c_int32_t), value :: n, steps_each
+      integer(c_int32_t), intent(out) :: best
+      real(c_double), intent(out) :: ms_per_step(*)
+    end function
+    ! learned initial guess: the network handle (tc_nn*) is a c_ptr
+    integer(c_int) function tc_nn_create(H, W, trainable, nn) bind(C, name="tc_nn_create")
+      import :: c_int, c_int32_t, c_ptr
+      integer(c_int32_t), value :: H, W, trainable
+      type(c_ptr), intent(out) :: nn
+    end function
+    integer(c_int) function tc_nn_destroy(nn) bind(C, name="tc_nn_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nn
+    end function
+    type(c_ptr) function tc_nn_last_error(nn) bind(C, name="tc_nn_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: nn
+    end function
+    integer(c_int) function tc_nn_param_count(nn, count) bind(C, name="tc_nn_param_count")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: nn
+      integer(c_int64_t), intent(out) :: count
+    end function
+    integer(c_int) function tc_nn_set_weights(nn, w, count) bind(C, name="tc_nn_set_weights")
+      import :: c_int, c_int64_t, c_float, c_ptr
+      type(c_ptr), value :: nn
+      real(c_float), intent(in) :: w(*)
+      integer(c_int64_t), value :: count
+    end function
+    integer(c_int) function tc_nn_get_weights(nn, w, count) bind(C, name="tc_nn_get_weights")
+      import :: c_int, c_int64_t, c_float, c_ptr
+      type(c_ptr), value :: nn
+      real(c_float), intent(out) :: w(*)
+      integer(c_int64_t), value :: count
+    end function
+    integer(c_int) function tc_nn_get_grads(nn, g, count) bind(C, name="tc_nn_get_grads")
+      import :: c_int, c_int64_t, c_float, c_ptr
+      type(c_ptr), value :: nn
+      real(c_float), intent(out) :: g(*)
+      integer(c_int64_t), value :: count
+    end function
+    integer(c_int) function tc_nn_set_sample(nn, x, target, mask) bind(C, name="tc_nn_set_sample")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nn, x, target, mask
+    end function
+    integer(c_int) function tc_nn_forward(nn, out) bind(C, name="tc_nn_forward")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: nn, out
+    end function
+    integer(c_int) function tc_nn_backward(nn, loss) bind(C, name="tc_nn_backward")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: nn
+      real(c_double), intent(out) :: loss
+    end function
+    integer(c_int) function tc_nn_adam(nn, lr, grad_scale) bind(C, name="tc_nn_adam")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: nn
+      real(c_float), value :: lr, grad_scale
+    end function
+    integer(c_int) function tc_learn_attach(ctx, nn) bind(C, name="tc_learn_attach")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, nn
+    end function
+    integer(c_int) function tc_learn_record(ctx, capacity) bind(C, name="tc_learn_record")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: capacity
+    end function
+    integer(c_int) function tc_learn_samples(ctx, count) bind(C, name="tc_learn_samples")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: count
+    end function
+    integer(c_int) function tc_learn_train(ctx, epochs, lr, batch, seed, mean_loss) bind(C, name="tc_learn_train")
+      import :: c_int, c_int32_t, c_int64_t, c_float, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: epochs, batch
+      real(c_float), value :: lr
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: mean_loss
     end function
   end interface
 
