@@ -69,6 +69,7 @@ struct tc_dev {
     bool tb = true;            // temporally blocked sweeps on levels >= 2
     bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
     int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
+    unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (two slot sets)
     int32_t* h_ctl = nullptr;  // pinned mirror
     std::vector<double> hbuf;  // host staging plane
 };
@@ -770,9 +771,10 @@ int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
     if (level == 1) view.rhs = L.res;
     const float* rhs32 = (level == 1) ? L.res32 : L.rhs32;
     TC_TRY(run_sweeps(ctx, d, level, ctx->dc_sweeps_active, dir, f32, view, rhs32, nullptr));
-    if (level > 1) {
-        if (f32) launch_residual32(ctx->stream, L, L.err32[L.cur32], L.res32, L.rhs32, d.F, d.h[level]);
-        else launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis, nullptr);
+    // FP64: compute_residual now; FP32: fused into the restriction of the
+    // next descent (k_residual_restrict32), the only reader of the residual
+    if (level > 1 && !f32) {
+        launch_residual(ctx->stream, d.pow2[level], L, L.err[L.cur], L.res, d.F, d.h[level], P.ivis, nullptr);
         ctx->stats.residuals_exec[level - 1]++;
     }
     return 0;
@@ -902,7 +904,13 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             if (multi[p - 1] < multi[p]) {
                 Level& Lf = ctx->lv[level - 1];
                 if (f32) {
-                    launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
+                    if (level - 1 == 1) {
+                        launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
+                    } else {
+                        launch_residual_restrict32(ctx->stream, Lf, Lf.err32[Lf.cur32], Lf.rhs32, d.F, d.h[level - 1],
+                                                   L.rhs32, L.g);
+                        ctx->stats.residuals_exec[level - 2]++;
+                    }
                     TC_HIP(hipMemsetAsync(L.err32[L.cur32], 0, L.g.elems * sizeof(float), ctx->stream));
                 } else {
                     launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
@@ -993,18 +1001,18 @@ int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double 
                 int* iters)
 {
     const tc_params& P = ctx->p;
-    launch_zero_i32(ctx->stream, d.ctl, 4);
+    // ctl: [0] stop flag, [1] stop index; fslots: the iterations' maxima
+    launch_zero_i32(ctx->stream, d.ctl, 2);
+    launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(d.fslots), 2 * filter_slot_words());
+    unsigned long long* slots = d.fslots;
     int m = 0, batch = 8;
     for (;;) {
         for (int b = 0; b < batch; ++b, ++m) {
             double* cur = (m % 2 == 0) ? A : B;
             double* nxt = (m % 2 == 0) ? B : A;
-            const int nb = launch_smoothness_stop(ctx->stream, cur, ctx->g, d.partialA, d.ctl);
-            launch_stop_test(ctx->stream, d.partialA, nb, delta, m, d.ctl);
-            launch_filter_step(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2,
-                               cfac(m), courant, d.ctl);
-            if (comp) launch_bcs_comp(ctx->stream, nxt, ctx->g, d.ctl);
-            else launch_bcs_T(ctx->stream, nxt, ctx->g, P.Tbot, d.ctl);
+            launch_smoothness_max(ctx->stream, cur, ctx->g, slots + (size_t)(m & 1) * (filter_slot_words() / 2), d.ctl);
+            launch_filter_step_bcs(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2,
+                                   ctx->lv[1].ds2, cfac(m), courant, P.Tbot, slots, m, delta, d.ctl);
         }
         TC_HIP(hipMemcpyAsync(d.h_ctl, d.ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TC_HIP(hipStreamSynchronize(ctx->stream));
@@ -1313,7 +1321,7 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
-        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl);
+        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         delete d;
@@ -1449,6 +1457,7 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_TRY(dalloc(ctx, &d->partialB, nb));
         TC_TRY(dalloc(ctx, &d->d_scal, 16));
         TC_TRY(dalloc_i(ctx, &d->ctl, 4));
+        TC_HIP(hipMalloc((void**)&d->fslots, filter_slot_words() * sizeof(unsigned long long)));
         TC_HIP(hipHostMalloc((void**)&d->h_ctl, 4 * sizeof(int32_t), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_scal, 16 * sizeof(double), hipHostMallocDefault));
     }

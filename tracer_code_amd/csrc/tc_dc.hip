@@ -234,6 +234,35 @@ __global__ __launch_bounds__(256) void k_restrict32(const float* __restrict__ in
     out[go.at(i, k)] = corners + sides + in[gi.at(ii, kk)] * 0.25f;
 }
 
+// compute_residual of fine level Lf fused with restrict into level Lf+1:
+// one thread per coarse interior point evaluates the 8 fine residuals the
+// restriction weights (the reference's corner set, iterative_solver.f90:
+// 604-609, reads (ii-1,kk+1) twice and not (ii+1,kk-1)), each rounded to FP32
+// exactly as k_residual32 stores it, and combines them in k_restrict32's
+// order -- the same values as the two kernels, without the residual plane.
+__global__ __launch_bounds__(256) void k_residual_restrict32(const float* __restrict__ S, const float* __restrict__ vis,
+                                                             const float* __restrict__ vx, const float* __restrict__ vz,
+                                                             const float* __restrict__ a1,
+                                                             const float* __restrict__ vxz,
+                                                             const float* __restrict__ rhs, Grid gi, FD32 F, H32 h,
+                                                             float* __restrict__ out, Grid go)
+{
+    const int2 tile = xcd_tile();
+    const int i = tile.x * blockDim.x + threadIdx.x + 1;
+    const int k = tile.y * blockDim.y + threadIdx.y + 1;
+    if (i > go.nx - 1 || k > go.nz - 1) return;
+    const int ii = 2 * i, kk = 2 * k;
+    auto r = [&](int a, int b) {
+        const size_t c = gi.at(a, b);
+        return (float)stream_space32<double>(F, h, S, c, gi.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);
+    };
+    const float rmm = r(ii - 1, kk - 1), rmp = r(ii - 1, kk + 1), rpp = r(ii + 1, kk + 1);
+    const float rm0 = r(ii - 1, kk), r0m = r(ii, kk - 1), rp0 = r(ii + 1, kk), r0p = r(ii, kk + 1), r00 = r(ii, kk);
+    float corners = (rmm + rmp + rpp + rmp) * (1.0f / 16.0f);
+    float sides = (rm0 + r0m + rp0 + r0p) * (1.0f / 8.0f);
+    out[go.at(i, k)] = corners + sides + r00 * 0.25f;
+}
+
 __global__ __launch_bounds__(256) void k_prolong32(float* __restrict__ Ef, Grid gf, const float* __restrict__ Ec,
                                                    Grid gc, float zsg)
 {
@@ -376,6 +405,14 @@ void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* o
 {
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_restrict32, grid2(go.nx - 1, go.nz - 1, b), b, 0, st, in, gi, out, go);
+}
+
+void launch_residual_restrict32(hipStream_t st, const Level& Lf, const float* S, const float* rhs, const FD& F,
+                                const Spacing& h, float* out, const Grid& go)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_residual_restrict32, grid2(go.nx - 1, go.nz - 1, b), b, 0, st, S, Lf.vis32, Lf.visx32,
+                       Lf.visz32, Lf.a132, Lf.visxz32, rhs, Lf.g, fd32_from(F), h32_from(h), out, go);
 }
 
 void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg)

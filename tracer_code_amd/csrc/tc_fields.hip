@@ -172,6 +172,149 @@ __global__ __launch_bounds__(256) void k_filter_step(const double* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused filter iteration: two launches instead of four.
+//   k_smoothness_max: the smoothness measure of the current field, as
+//     k_smoothness, reduced by atomicMax on the bit patterns (the measure
+//     is >= 0, whose IEEE order is the integer order) into FILT_SLOTS
+//     maxima on separate cache lines, block b into slot b % FILT_SLOTS, so
+//     that the atomics of different blocks rarely meet on one address.
+//   k_filter_step_bcs: the stop test on that maximum (the whole grid reads
+//     the same value, so every block takes the same branch; block (0,0)
+//     records it), then the filter step with the boundary conditions folded
+//     in: each cell computes its final value after the step and the
+//     reference's BC routine, evaluating the step at the interior cell a
+//     ghost mirrors.
+// Iteration m uses the slot set m%2; the step kernel clears the other set
+// for the next iteration's reduction.
+// ---------------------------------------------------------------------------
+constexpr int FILT_SLOTS = 64, FILT_STRIDE = 16;  // 16 x 8 B = one 128-byte line per slot
+constexpr int FILT_ROWS = 4;                      // rows per thread of k_smoothness_max
+
+__global__ __launch_bounds__(256) void k_smoothness_max(const double* __restrict__ T, Grid g,
+                                                        unsigned long long* __restrict__ slots,
+                                                        const int32_t* __restrict__ stop)
+{
+    if (stop[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0;
+    for (int r = 0; r < FILT_ROWS; ++r) {
+        const int k = (blockIdx.y * blockDim.y + threadIdx.y) * FILT_ROWS + r + 1;
+        if (i <= g.nx - 1 && k <= g.nz - 1) {
+            const size_t c = g.at(i, k);
+            const double t0 = T[c];
+            double a1 = fabs(T[g.at(i, k - 1)] - t0);
+            double a2 = fabs(T[g.at(i + 1, k - 1)] - t0);
+            double a3 = fabs(T[g.at(i + 1, k)] - t0);
+            double a4 = fabs(T[g.at(i, k + 1)] - t0);
+            double a5 = fabs(T[g.at(i + 1, k + 1)] - t0);
+            double b = a1;
+            if (a2 > b) b = a2;
+            if (a3 > b) b = a3;
+            if (a4 > b) b = a4;
+            if (a5 > b) b = a5;
+            if (b > a || b != b) a = b;
+        }
+    }
+    a = block_max(a);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const int s = (int)((blockIdx.y * gridDim.x + blockIdx.x) % FILT_SLOTS);
+        atomicMax(slots + s * FILT_STRIDE, (unsigned long long)__double_as_longlong(a));
+    }
+}
+
+// the step at a node 0..nx x 0..nz (k_filter_step's formula)
+template <bool COMP>
+__device__ __forceinline__ double filter_new(const double* __restrict__ T0, const double* __restrict__ dts,
+                                             const Grid& g, const FD& F, int i, int k, double dr2, double ds2,
+                                             double cfac, double courant)
+{
+    const size_t c = g.at(i, k);
+    double Trr = dot3(F.D2, T0, c, 1) / dr2;
+    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
+    double temp = Trr + Tss;
+    double d = dts[c] * courant;
+    if constexpr (COMP) {
+        double v = T0[c] + cfac * d * temp;
+        if (v < 0.0) v = 0.0;
+        if (v > 1.0) v = 1.0;
+        return v;
+    } else {
+        return T0[c] + d * temp;
+    }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_filter_step_bcs(const double* __restrict__ T0, double* __restrict__ T,
+                                                         const double* __restrict__ dts, Grid g, FD F, double dr2,
+                                                         double ds2, double cfac, double courant, int Tbot,
+                                                         unsigned long long* __restrict__ slots, int m, double delta,
+                                                         int32_t* __restrict__ ctl)
+{
+    if (ctl[0]) return;
+    // the iteration's maximum over the slot set (first wave, one slot per lane)
+    __shared__ double s_max;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const unsigned long long* set = slots + (size_t)(m & 1) * FILT_SLOTS * FILT_STRIDE;
+    if (tid < 64) {
+        unsigned long long v = set[tid * FILT_STRIDE];
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long w = __shfl_xor(v, o, 64);
+            v = (w > v) ? w : v;
+        }
+        if (tid == 0) s_max = __longlong_as_double((long long)v);
+    }
+    __syncthreads();
+    const double smax = s_max;
+    const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+    if (smax < delta) {
+        if (first && tid == 0) {
+            ctl[0] = 1;
+            ctl[1] = m;
+        }
+        return;
+    }
+    if (first && tid < FILT_SLOTS) slots[((size_t)((m + 1) & 1) * FILT_SLOTS + tid) * FILT_STRIDE] = 0ull;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
+    if (i > g.nx + SPAN || k > g.nz + SPAN) return;
+    const int nx = g.nx, nz = g.nz;
+    // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
+    const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant); };
+    double v;
+    if constexpr (COMP) {
+        // enforceBCs_comp (composition.f90:792-803) in its interleaved
+        // order: column +-1, row +-1, column +-2, row +-2; the first row
+        // mirror still reads the old ghost columns +-2 (T0, which the step
+        // copies there)
+        if (k >= 0 && k <= nz) {
+            v = W(im, k);
+        } else {
+            const int ks = (k < 0) ? -k : 2 * nz - k;  // mirrored row
+            const bool row1 = (k == -1 || k == nz + 1);
+            if (row1 && (i == -2 || i == nx + 2)) v = T0[g.at(i, ks)];
+            else v = W(im, ks);
+        }
+    } else {
+        // enforceBCs (T): side mirrors on rows 0..nz, then the bottom/top
+        // ghost rows from the mirrored rows, then the wall values
+        if (k == nz) {
+            v = 0.0;
+        } else if (k == 0 && Tbot == 0) {
+            v = 1.0;
+        } else if (k >= 0 && k < nz) {
+            v = W(im, k);
+        } else if (k < 0) {
+            const double x = W(im, -k);
+            v = (Tbot == 0) ? -(x - 1.0) + 1.0 : x;
+        } else {
+            v = -W(im, 2 * nz - k);
+        }
+    }
+    T[g.at(i, k)] = v;
+}
+
 // Stop test of the filter loops: max of the block maxima of k_smoothness;
 // when it is below delta the flag is set and the iteration index recorded.
 __global__ __launch_bounds__(256) void k_stop_test(const double* __restrict__ partial, int n, double delta, int m,
@@ -561,6 +704,29 @@ void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, 
         hipLaunchKernelGGL((k_filter_step<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
     else
         hipLaunchKernelGGL((k_filter_step<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
+}
+
+void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
+                           const int32_t* stop)
+{
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_smoothness_max, grid2(g.nx, (g.nz - 1 + FILT_ROWS - 1) / FILT_ROWS, b), b, 0, st, T0, g,
+                       slots, stop);
+}
+size_t filter_slot_words() { return 2 * (size_t)FILT_SLOTS * FILT_STRIDE; }
+
+void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                            const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
+                            unsigned long long* slots, int m, double delta, int32_t* ctl)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1 + 2 * SPAN, g.nz + 1 + 2 * SPAN, b);
+    if (comp)
+        hipLaunchKernelGGL((k_filter_step_bcs<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
+                           slots, m, delta, ctl);
+    else
+        hipLaunchKernelGGL((k_filter_step_bcs<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
+                           slots, m, delta, ctl);
 }
 
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n)

@@ -96,6 +96,8 @@ void launch_sweep32(hipStream_t st, bool use_ratio, const Level& L, const float*
 void launch_residual32(hipStream_t st, const Level& L, const float* S, float* R, const float* rhs, const FD& F,
                        const Spacing& h);
 void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* out, const Grid& go);
+void launch_residual_restrict32(hipStream_t st, const Level& Lf, const float* S, const float* rhs, const FD& F,
+                                const Spacing& h, float* out, const Grid& go);
 void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg);
 void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
                            float* E, int ngc, double zsg);
@@ -127,6 +129,12 @@ void launch_stop_test(hipStream_t st, const double* partial, int n, double delta
 void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
                         const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop);
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
+void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
+                           const int32_t* stop);
+size_t filter_slot_words();  // u64 words of the two slot sets of the fused filter loop
+void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                            const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
+                            unsigned long long* slots, int m, double delta, int32_t* ctl);
 void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
                 const Grid& g, double dr, double ds, double aspect, int32_t* flag);
 void launch_ratio(hipStream_t st, const CntArr& cnt, int32_t* nempty, const PtrArr& Cnew, const Grid& g, int ntype,

@@ -19,14 +19,16 @@
 
 namespace tc {
 
-// Tile TB_T x TB_T; K sweeps per launch need a halo of 2K cells, so the
-// staged region is (TB_T + 4K)^2: 32^2 for K=4, 48^2 for K=8.
-constexpr int TB_T = 16, TB_KMAX = 8;
+// Tile TT x TT; K sweeps per launch need a halo of 2K cells, so the
+// staged region is (TT + 4K)^2: 32^2 for TT=16, K=4 (4x the tile), 48^2 for
+// TT=32, K=4 (2.25x) -- the larger tile stages less halo per updated cell
+// and serves levels with enough tiles to fill the chip.
+constexpr int TB_KMAX = 8;
 constexpr int TB_THREADS = 1024;
-template <int K>
+template <int TT, int K>
 struct TBGeom {
     static constexpr int H = 2 * K;
-    static constexpr int R = TB_T + 2 * H;
+    static constexpr int R = TT + 2 * H;
     static constexpr int CELLS = R * R;
     static constexpr int NPT = (CELLS + TB_THREADS - 1) / TB_THREADS;  // staged cells per thread
 };
@@ -120,14 +122,15 @@ __device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int 
 }
 
 #pragma clang fp contract(fast)
-template <typename T, bool RATIO, int K>
+template <typename T, bool RATIO, int K, int TT>
 __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
                                                         Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg)
 {
-    using G = TBGeom<K>;
+    using G = TBGeom<TT, K>;
     constexpr int R = G::R, H = G::H, NPT = G::NPT;
     __shared__ T buf[2][G::CELLS];
-    const int gi0 = 1 + blockIdx.x * TB_T - H, gk0 = 1 + blockIdx.y * TB_T - H;  // staged region origin
+    const int2 tile = xcd_tile();
+    const int gi0 = 1 + tile.x * TT - H, gk0 = 1 + tile.y * TT - H;  // staged region origin
     W13<T> w[NPT];
     int band[NPT];
 #pragma unroll
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
         const int l = threadIdx.x + u * TB_THREADS;
         if (l >= G::CELLS) continue;
         const int li = l % R, lk = l / R, i = gi0 + li, k = gk0 + lk;
-        if (li >= H && li < H + TB_T && lk >= H && lk < H + TB_T && i >= 1 && i <= g.nx - 1 && k >= 1 &&
+        if (li >= H && li < H + TT && lk >= H && lk < H + TT && i >= 1 && i <= g.nx - 1 && k >= 1 &&
             k <= g.nz - 1) {
             const T r = buf[cur][l];
             Eout[g.at(i, k)] = r;
@@ -200,22 +203,46 @@ static Tables<T> make_tables(const FD& F, const Spacing& h)
     return t;
 }
 
-static inline dim3 tb_grid(const Grid& g) { return dim3((g.nx - 1 + TB_T - 1) / TB_T, (g.nz - 1 + TB_T - 1) / TB_T); }
+static inline dim3 tb_grid(const Grid& g, int TT) { return dim3((g.nx - 1 + TT - 1) / TT, (g.nz - 1 + TT - 1) / TT); }
 
 int tb_max_sweeps(int K) { return K; }
+
+// the 32x32 tile where it still gives at least 2 workgroups per CU (256 CUs)
+// and the arithmetic is FP32 (three staged cells of FP64 weights per thread
+// would spill)
+template <typename T>
+static int tb_tile(const Grid& g)
+{
+    if (sizeof(T) != 4) return 16;
+    const dim3 g32 = tb_grid(g, 32);
+    return (g32.x * g32.y >= 512) ? 32 : 16;
+}
+
+template <typename T, int TT>
+static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
+                        const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg)
+{
+    const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
+    if (K == 8) {
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+    } else {
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+    }
+}
 
 template <typename T>
 static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
                       const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg)
 {
-    const dim3 gr = tb_grid(g), b(TB_THREADS);
-    if (K == 8) {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
-    } else {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+    if constexpr (sizeof(T) == 4) {
+        if (tb_tile<T>(g) == 32 && K == 4) {
+            tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg);
+            return;
+        }
     }
+    tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg);
 }
 
 void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
