@@ -376,18 +376,38 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ r, const
                                              const int32_t* __restrict__ ty, int64_t n, CntArr cnt, Grid g, double dr,
                                              double ds, double aspect, int32_t* __restrict__ flag)
 {
-    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= n) return;
-    const double rv = r[id], sv = s[id];
-    // the reference's test (r<=0 or r>=aspect or s<=0 or s>=1), written so
-    // that a NaN position is also rejected before it can form an index
-    if (!(rv > 0.0 && rv < aspect && sv > 0.0 && sv < 1.0)) {
-        atomicOr(flag, 1);
-        return;
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // key of the (type, node) counter this tracer adds to; none = ~0
+    uint32_t key = ~0u;
+    int t = 0;
+    size_t node = 0;
+    if (id < n) {
+        const double rv = r[id], sv = s[id];
+        // the reference's test (r<=0 or r>=aspect or s<=0 or s>=1), written so
+        // that a NaN position is also rejected before it can form an index
+        if (!(rv > 0.0 && rv < aspect && sv > 0.0 && sv < 1.0)) {
+            atomicOr(flag, 1);
+        } else {
+            const int i = (int)round(rv / dr);
+            const int k = (int)round(sv / ds);
+            node = g.at(i, k);
+            t = ty[id];
+            key = (uint32_t)node * (uint32_t)(TC_MAX_TYPES + 1) + (uint32_t)t;
+        }
     }
-    const int i = (int)round(rv / dr);
-    const int k = (int)round(sv / ds);
-    atomicAdd(&cnt.v[ty[id]][g.at(i, k)], 1);
+    // wavefront segmented reduction: consecutive lanes with the same key form
+    // a run (tracers are stored node by node, and move less than a cell per
+    // step, so runs of ~tpc are the rule); the run's first lane adds the
+    // run's length, one atomic per run instead of one per tracer
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const bool boundary = lane == 0 || prev != key;
+    const unsigned long long starts = __ballot(boundary);
+    if (boundary && key != ~0u) {
+        const unsigned long long later = (lane == 63) ? 0ull : (starts >> (lane + 1));
+        const int len = later ? __builtin_ctzll(later) + 1 : 64 - lane;
+        atomicAdd(&cnt.v[t][node], len);
+    }
 }
 
 // Mirrored ghost counts (composition.f90:745-754), nempty (:756) and the
