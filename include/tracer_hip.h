@@ -235,6 +235,16 @@ int tc_time_step(tc_ctx* ctx, int nsteps);
 int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
                               double base[7]);
 
+/* ---- tracer order (no counterpart in the reference) --------------------- */
+/* Reorders the device tracer arrays by the node each tracer is binned to
+ * (row-major node index; the order carries no meaning in the reference).
+ * tc_get_tracers keeps returning the caller's order; tc_put_tracers sets
+ * the caller's order again. */
+int tc_sort_tracers(tc_ctx* ctx);
+/* Sort automatically at the start of every `interval`-th tc_time_step step
+ * (0: never, the default). */
+int tc_set_tracer_sort(tc_ctx* ctx, int32_t interval);
+
 /* ---- measurement hooks (no counterpart in the reference) --------------- */
 /* Kernel ids for tc_kernel_timing. */
 enum { TC_K_SWEEP_L1 = 0, TC_K_COUNT };

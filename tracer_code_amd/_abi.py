@@ -86,6 +86,8 @@ SIGNATURES = {
     "tc_last_error": (ctypes.c_char_p, [_VP]),
     "tc_set_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_set_dc_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_sort_tracers": (ctypes.c_int, [_VP]),
+    "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     "tc_get_stats": (ctypes.c_int, [_VP, _P(TcStats)]),
     "tc_sync": (ctypes.c_int, [_VP]),

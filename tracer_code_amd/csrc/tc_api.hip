@@ -1317,6 +1317,8 @@ void free_ctx(tc_ctx* ctx)
     for (int t = 0; t < 4; ++t) { F(ctx->sh1[t]); F(ctx->sh2[t]); F(ctx->Tc1[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
     F(ctx->nempty); F(ctx->ipiv); F(ctx->d_info); F(ctx->ttype); F(ctx->d_flag); F(ctx->d_info_next);
+    F(ctx->tr_id); F(ctx->tr_id_spare); F(ctx->spare_d); F(ctx->spare_i); F(ctx->sort_tmp);
+    for (int b = 0; b < 2; ++b) { F(ctx->sort_k[b]); F(ctx->sort_v[b]); }
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
@@ -1686,6 +1688,71 @@ int tc_get_field(tc_ctx* ctx, int field, void* host)
     return TC_ERR_ARG;
 }
 
+int sort_tracers(tc_ctx* ctx)
+{
+    const int64_t n = ctx->ntr;
+    if (n <= 0) return 0;
+    if (n >= ((int64_t)1 << 31)) {
+        ctx->err = "tracer sort supports fewer than 2^31 tracers";
+        return TC_ERR_UNSUPPORTED;
+    }
+    const tc_params& P = ctx->p;
+    if (!ctx->spare_d) {
+        for (int b = 0; b < 2; ++b) {
+            TC_HIP(hipMalloc((void**)&ctx->sort_k[b], (size_t)n * sizeof(uint32_t)));
+            TC_HIP(hipMalloc((void**)&ctx->sort_v[b], (size_t)n * sizeof(uint32_t)));
+        }
+        TC_HIP(hipMalloc((void**)&ctx->spare_d, (size_t)n * sizeof(double)));
+        TC_HIP(hipMalloc((void**)&ctx->spare_i, (size_t)n * sizeof(int32_t)));
+        TC_HIP(hipMalloc((void**)&ctx->tr_id_spare, (size_t)n * sizeof(int64_t)));
+    }
+    const Grid& g = ctx->g;
+    const uint32_t bad = (uint32_t)(g.nx + 1) * (uint32_t)(g.nz + 1);
+    int bits = 1;
+    while (bits < 32 && ((uint64_t)1 << bits) <= bad) ++bits;
+    size_t need = 0;
+    TC_TRY(sort_pairs(ctx->stream, nullptr, &need, ctx->sort_k[0], ctx->sort_k[1], ctx->sort_v[0], ctx->sort_v[1], n,
+                      bits));
+    if (need > ctx->sort_tmp_bytes) {
+        if (ctx->sort_tmp) TC_HIP(hipFree(ctx->sort_tmp));
+        TC_HIP(hipMalloc(&ctx->sort_tmp, need));
+        ctx->sort_tmp_bytes = need;
+    }
+    launch_tracer_keys(ctx->stream, ctx->rtr, ctx->str, n, g.nx, ctx->lv[1].dr, ctx->lv[1].ds, P.aspect, bad,
+                       ctx->sort_k[0], ctx->sort_v[0]);
+    size_t bytes = ctx->sort_tmp_bytes;
+    TC_TRY(sort_pairs(ctx->stream, ctx->sort_tmp, &bytes, ctx->sort_k[0], ctx->sort_k[1], ctx->sort_v[0],
+                      ctx->sort_v[1], n, bits));
+    const uint32_t* perm = ctx->sort_v[1];
+    double** arrays[] = {&ctx->rtr, &ctx->str, &ctx->rtr0, &ctx->str0, &ctx->rtr1, &ctx->str1, &ctx->rtr2, &ctx->str2};
+    for (double** a : arrays) {
+        launch_gather_d(ctx->stream, *a, perm, ctx->spare_d, n);
+        std::swap(*a, ctx->spare_d);
+    }
+    launch_gather_i32(ctx->stream, ctx->ttype, perm, ctx->spare_i, n);
+    std::swap(ctx->ttype, ctx->spare_i);
+    launch_gather_id(ctx->stream, ctx->tr_id, perm, ctx->tr_id_spare, n);
+    if (!ctx->tr_id) TC_HIP(hipMalloc((void**)&ctx->tr_id, (size_t)n * sizeof(int64_t)));
+    std::swap(ctx->tr_id, ctx->tr_id_spare);
+    ctx->sorts++;
+    return 0;
+}
+
+int tc_sort_tracers(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    TC_TRY(sort_tracers(ctx));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_set_tracer_sort(tc_ctx* ctx, int32_t interval)
+{
+    if (!ctx || interval < 0) return TC_ERR_ARG;
+    ctx->sort_interval = interval;
+    return 0;
+}
+
 int tc_put_tracers(tc_ctx* ctx, const double* rtr, const double* str, const int32_t* ttype)
 {
     if (!ctx || !rtr || !str || !ttype) return TC_ERR_ARG;
@@ -1700,6 +1767,11 @@ int tc_put_tracers(tc_ctx* ctx, const double* rtr, const double* str, const int3
     TC_HIP(hipMemcpyAsync(ctx->str, str, nb, hipMemcpyHostToDevice, ctx->stream));
     TC_HIP(hipMemcpyAsync(ctx->ttype, ttype, (size_t)ctx->ntr * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     TC_HIP(hipStreamSynchronize(ctx->stream));
+    // the caller's order again
+    if (ctx->tr_id) {
+        TC_HIP(hipFree(ctx->tr_id));
+        ctx->tr_id = nullptr;
+    }
     return 0;
 }
 
@@ -1710,8 +1782,16 @@ int tc_get_tracers(tc_ctx* ctx, int which, double* r, double* s)
     double *dr, *ds;
     tracer_set(ctx, which, &dr, &ds);
     const size_t nb = (size_t)ctx->ntr * sizeof(double);
-    TC_HIP(hipMemcpyAsync(r, dr, nb, hipMemcpyDeviceToHost, ctx->stream));
-    TC_HIP(hipMemcpyAsync(s, ds, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (ctx->tr_id) {
+        // node-sorted on the device: back to the caller's order
+        launch_scatter_d(ctx->stream, dr, ctx->tr_id, ctx->spare_d, ctx->ntr);
+        TC_HIP(hipMemcpyAsync(r, ctx->spare_d, nb, hipMemcpyDeviceToHost, ctx->stream));
+        launch_scatter_d(ctx->stream, ds, ctx->tr_id, ctx->spare_d, ctx->ntr);
+        TC_HIP(hipMemcpyAsync(s, ctx->spare_d, nb, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        TC_HIP(hipMemcpyAsync(r, dr, nb, hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipMemcpyAsync(s, ds, nb, hipMemcpyDeviceToHost, ctx->stream));
+    }
     TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -1813,6 +1893,7 @@ int tc_time_step(tc_ctx* ctx, int nsteps)
     if (!ctx || nsteps < 0) return TC_ERR_ARG;
     tc_dev& d = DEV(ctx);
     for (int s = 0; s < nsteps; ++s) {
+        if (ctx->sort_interval > 0 && ctx->tstep % ctx->sort_interval == 0) TC_TRY(sort_tracers(ctx));
         ctx->solve_stage = 0;
         ctx->solve_tau = ctx->time;
         TC_TRY(run_multigrid(ctx, nullptr));

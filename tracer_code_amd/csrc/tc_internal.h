@@ -127,6 +127,16 @@ struct tc_ctx {
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;
     int32_t* ttype = nullptr;
+    // node-sorted tracer order (tc_sort_tracers): caller index of each slot
+    // (null: the caller's order), sort scratch, and the automatic interval
+    int64_t *tr_id = nullptr, *tr_id_spare = nullptr;
+    uint32_t *sort_k[2] = {nullptr, nullptr}, *sort_v[2] = {nullptr, nullptr};
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    double* spare_d = nullptr;
+    int32_t* spare_i = nullptr;
+    int sort_interval = 0;
+    int64_t sorts = 0;
     int32_t* d_flag = nullptr;   // tracer-out-of-domain flag
     int32_t* h_flag = nullptr;   // pinned
     // model time

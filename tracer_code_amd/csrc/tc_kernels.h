@@ -135,6 +135,16 @@ size_t filter_slot_words();  // u64 words of the two slot sets of the fused filt
 void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
                             const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
                             unsigned long long* slots, int m, double delta, int32_t* ctl);
+// --- tc_sort.hip ----------------------------------------------------------------
+void launch_tracer_keys(hipStream_t st, const double* r, const double* s, int64_t n, int nx, double dr, double ds,
+                        double aspect, uint32_t bad, uint32_t* key, uint32_t* val);
+void launch_gather_d(hipStream_t st, const double* in, const uint32_t* idx, double* out, int64_t n);
+void launch_gather_i32(hipStream_t st, const int32_t* in, const uint32_t* idx, int32_t* out, int64_t n);
+void launch_gather_id(hipStream_t st, const int64_t* id, const uint32_t* idx, int64_t* out, int64_t n);
+void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, double* out, int64_t n);
+int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
+               const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
+
 void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
                 const Grid& g, double dr, double ds, double aspect, int32_t* flag);
 void launch_ratio(hipStream_t st, const CntArr& cnt, int32_t* nempty, const PtrArr& Cnew, const Grid& g, int ntype,

@@ -1,0 +1,95 @@
+// Tracer state sorted by node.  The tracer arrays are structure-of-arrays;
+// sorting them by the node each tracer is binned to (tracers_to_corners'
+// nearest node, composition.f90:704-715) keeps the tracers of one node
+// together: the RK stages then read the velocity of neighbouring cells with
+// neighbouring lanes, and binning finds long runs of one (node, type) key
+// per wavefront.  The order carries no meaning in the reference; tr_id maps
+// each slot back to the caller's tracer index for the boundary copies.
+#include <hipcub/hipcub.hpp>
+
+#include "tc_kernels.h"
+
+namespace tc {
+
+// key = the binning node (row-major k * (nx+1) + i), out-of-domain or NaN
+// positions sort last; val = the slot
+__global__ __launch_bounds__(256) void k_tracer_keys(const double* __restrict__ r, const double* __restrict__ s,
+                                                     int64_t n, int nx, double dr, double ds, double aspect,
+                                                     uint32_t bad, uint32_t* __restrict__ key,
+                                                     uint32_t* __restrict__ val)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double rv = r[j], sv = s[j];
+    uint32_t kk = bad;
+    if (rv > 0.0 && rv < aspect && sv > 0.0 && sv < 1.0) {
+        const int i = (int)round(rv / dr), k = (int)round(sv / ds);
+        kk = (uint32_t)k * (uint32_t)(nx + 1) + (uint32_t)i;
+    }
+    key[j] = kk;
+    val[j] = (uint32_t)j;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const T* __restrict__ in, const uint32_t* __restrict__ idx,
+                                                T* __restrict__ out, int64_t n)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = in[idx[j]];
+}
+
+// new id of slot j: the id of the slot it came from (identity if none yet)
+__global__ __launch_bounds__(256) void k_gather_id(const int64_t* __restrict__ id, const uint32_t* __restrict__ idx,
+                                                   int64_t* __restrict__ out, int64_t n)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = id ? id[idx[j]] : (int64_t)idx[j];
+}
+
+// caller order: out[id[j]] = in[j]
+__global__ __launch_bounds__(256) void k_scatter_d(const double* __restrict__ in, const int64_t* __restrict__ id,
+                                                   double* __restrict__ out, int64_t n)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[id[j]] = in[j];
+}
+
+static inline unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_tracer_keys(hipStream_t st, const double* r, const double* s, int64_t n, int nx, double dr, double ds,
+                        double aspect, uint32_t bad, uint32_t* key, uint32_t* val)
+{
+    hipLaunchKernelGGL(k_tracer_keys, dim3(blocks(n)), dim3(256), 0, st, r, s, n, nx, dr, ds, aspect, bad, key, val);
+}
+
+void launch_gather_d(hipStream_t st, const double* in, const uint32_t* idx, double* out, int64_t n)
+{
+    hipLaunchKernelGGL(k_gather<double>, dim3(blocks(n)), dim3(256), 0, st, in, idx, out, n);
+}
+
+void launch_gather_i32(hipStream_t st, const int32_t* in, const uint32_t* idx, int32_t* out, int64_t n)
+{
+    hipLaunchKernelGGL(k_gather<int32_t>, dim3(blocks(n)), dim3(256), 0, st, in, idx, out, n);
+}
+
+void launch_gather_id(hipStream_t st, const int64_t* id, const uint32_t* idx, int64_t* out, int64_t n)
+{
+    hipLaunchKernelGGL(k_gather_id, dim3(blocks(n)), dim3(256), 0, st, id, idx, out, n);
+}
+
+void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, double* out, int64_t n)
+{
+    hipLaunchKernelGGL(k_scatter_d, dim3(blocks(n)), dim3(256), 0, st, in, id, out, n);
+}
+
+// LSD radix sort of (key, slot) pairs on the key's low `bits` bits; with
+// temp == nullptr only sets *temp_bytes
+int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
+               const uint32_t* vin, uint32_t* vout, int64_t n, int bits)
+{
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, (int)n, 0, bits,
+                                                           st);
+    return e == hipSuccess ? 0 : TC_ERR_HIP;
+}
+
+}  // namespace tc

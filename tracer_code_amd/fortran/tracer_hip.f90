@@ -187,6 +187,15 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function
+    integer(c_int) function tc_sort_tracers(ctx) bind(C, name="tc_sort_tracers")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function tc_set_tracer_sort(ctx, interval) bind(C, name="tc_set_tracer_sort")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: interval
+    end function
     integer(c_int) function tc_set_dc_sweeps(ctx, sweeps) bind(C, name="tc_set_dc_sweeps")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

@@ -207,6 +207,14 @@ class Solver:
                                             ctypes.byref(loss)), "learn_train")
         return loss.value
 
+    def sort_tracers(self):
+        """Reorder the device tracers by node (get_tracers keeps the caller's order)."""
+        self._check(self.lib.tc_sort_tracers(self.h), "sort_tracers")
+
+    def set_tracer_sort(self, interval):
+        """Sort the tracers at the start of every interval-th time step (0: never)."""
+        self._check(self.lib.tc_set_tracer_sort(self.h, int(interval)), "set_tracer_sort")
+
     def set_dc_sweeps(self, sweeps):
         """Sweeps per level visit of the defect-correction cycle (0: Nmulti)."""
         self._check(self.lib.tc_set_dc_sweeps(self.h, int(sweeps)), "set_dc_sweeps")
