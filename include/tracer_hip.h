@@ -235,6 +235,16 @@ int tc_time_step(tc_ctx* ctx, int nsteps);
 int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
                               double base[7]);
 
+/* ---- ensembles (no counterpart in the reference) ------------------------ */
+/* Runs nsteps time steps (tc_time_step) on each of n independent contexts
+ * -- separate cases, any grids and parameters -- concurrently: `threads`
+ * host threads (0: one per context) take the contexts from a shared queue,
+ * each context on its own stream, so the GPU overlaps the cases' launches
+ * (a single case leaves most of the chip idle on its coarse levels).  Each
+ * case's results are those of running it alone.  Returns 0 or the first
+ * nonzero status, with the index of that context in *failed (-1 if none). */
+int tc_ensemble_step(tc_ctx* const* ctxs, int32_t n, int32_t nsteps, int32_t threads, int32_t* failed);
+
 /* ---- tracer order (no counterpart in the reference) --------------------- */
 /* Reorders the device tracer arrays by the node each tracer is binned to
  * (row-major node index; the order carries no meaning in the reference).

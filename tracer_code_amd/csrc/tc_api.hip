@@ -5,12 +5,14 @@
 // (default priority) factors and inverts the coarsest-grid matrix while the
 // first W-cycle descent runs.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
 #include <new>
+#include <thread>
 
 #include "tc_kernels.h"
 #include "tc_nn.h"
@@ -1905,6 +1907,42 @@ int tc_time_step(tc_ctx* ctx, int nsteps)
         ctx->solve_stage = -1;
     }
     TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_ensemble_step(tc_ctx* const* ctxs, int32_t n, int32_t nsteps, int32_t threads, int32_t* failed)
+{
+    if (!ctxs || n < 0 || nsteps < 0 || threads < 0) return TC_ERR_ARG;
+    for (int j = 0; j < n; ++j) {
+        if (!ctxs[j]) return TC_ERR_ARG;
+        for (int q = 0; q < j; ++q)
+            if (ctxs[q] == ctxs[j]) return TC_ERR_ARG;  // a context is driven by one thread at a time
+    }
+    if (failed) *failed = -1;
+    if (n == 0) return 0;
+    const int nt = std::min<int>(threads > 0 ? threads : n, n);
+    std::atomic<int> next{0};
+    std::vector<int> status(n, 0);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TC_ERR_HIP;  // the workers use the caller's device
+    auto worker = [&]() {
+        (void)hipSetDevice(dev);
+        for (;;) {
+            const int j = next.fetch_add(1);
+            if (j >= n) break;
+            status[j] = tc_time_step(ctxs[j], nsteps);
+        }
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (int j = 0; j < n; ++j)
+        if (status[j] != 0) {
+            if (failed) *failed = j;
+            return status[j];
+        }
     return 0;
 }
 

@@ -14,6 +14,23 @@
 
 namespace tc {
 
+// 1/h^n of the stencil terms as double-double reciprocals: hi = fl(1/x),
+// lo = fl((1 - hi*x) / x) with the residual 1 - hi*x exact by FMA
+enum { IH_R2, IH_S2, IH_R3, IH_S3, IH_R4, IH_S4, IH_RS, IH_RS2, IH_R2S, IH_R2S2, IH_COUNT };
+struct InvSpacing {
+    double hi[IH_COUNT], lo[IH_COUNT];
+};
+static InvSpacing inv_spacing(const Spacing& h)
+{
+    const double x[IH_COUNT] = {h.dr2, h.ds2, h.dr3, h.ds3, h.dr4, h.ds4, h.drds, h.drds2, h.dr2ds, h.dr2ds2};
+    InvSpacing r;
+    for (int q = 0; q < IH_COUNT; ++q) {
+        r.hi[q] = 1.0 / x[q];
+        r.lo[q] = std::fma(-r.hi[q], x[q], 1.0) / x[q];
+    }
+    return r;
+}
+
 // ---- double-double helpers (Knuth TwoSum, FMA TwoProd) ---------------------
 struct dd {
     double hi, lo;
@@ -47,51 +64,70 @@ __device__ __forceinline__ dd dd_mul_d(dd a, double b)
     return quick_two_sum(p, e);
 }
 __device__ __forceinline__ dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
-// a / b for a double b (one Newton correction)
-__device__ __forceinline__ dd dd_div_d(dd a, double b)
+// Compensated stencil sums.  A finite difference of psi = hi + lo cancels
+// to ~h^4 of |psi| (the FP64 floor of the reference form); here every tap's
+// product w*hi is split exactly (FMA TwoProd), the products are summed with
+// TwoSum, and the rounding errors, the product errors and the w*lo terms
+// are gathered in a second double.  The result (s, e) is the difference to
+// ~eps^2 * sum|w*hi|, i.e. double-double accuracy, at a third of the
+// instructions of a double-double accumulation.
+struct CSum {
+    double s = 0.0, e = 0.0;
+};
+__device__ __forceinline__ void csum_tap(CSum& a, double w, double h, double l)
 {
-    double q1 = a.hi / b;
-    dd p = {q1 * b, fma(q1, b, -(q1 * b))};
-    dd r = dd_add(a, dd_neg(p));
-    double q2 = r.hi / b;
-    return quick_two_sum(q1, q2);
+    const double p = w * h;
+    const double pe = fma(w, h, -p);
+    const double s = a.s + p;
+    const double bb = s - a.s;
+    const double t = (a.s - (s - bb)) + (p - bb);
+    a.s = s;
+    a.e = fma(w, l, a.e + (t + pe));
 }
-
-// sum_j D(j) S(c + j*st) in double-double, S = hi + lo
-__device__ __forceinline__ dd dd_dot(const double* D, int j0, int j1, const double* __restrict__ hi,
-                                     const double* __restrict__ lo, size_t c, ptrdiff_t st)
+__device__ __forceinline__ dd csum_dot(const double* D, int j0, int j1, const double* __restrict__ hi,
+                                       const double* __restrict__ lo, size_t c, ptrdiff_t st)
 {
-    dd acc = {0.0, 0.0};
+    CSum a;
     for (int j = j0; j <= j1; ++j) {
         const double w = D[j + 2];
         if (w == 0.0) continue;
         const size_t q = c + j * st;
-        acc = dd_add(acc, dd_mul_d({hi[q], lo[q]}, w));
+        csum_tap(a, w, hi[q], lo[q]);
     }
-    return acc;
+    return quick_two_sum(a.s, a.e);
 }
-__device__ __forceinline__ dd dd_sum9(const double* M, const double* __restrict__ hi, const double* __restrict__ lo,
-                                      size_t c, ptrdiff_t ld)
+__device__ __forceinline__ dd csum_9(const double* M, const double* __restrict__ hi, const double* __restrict__ lo,
+                                     size_t c, ptrdiff_t ld)
 {
-    dd acc = {0.0, 0.0};
+    CSum a;
     for (int dk = -1; dk <= 1; ++dk)
         for (int di = -1; di <= 1; ++di) {
             const double w = M[(dk + 1) * 3 + (di + 1)];
             if (w == 0.0) continue;
             const size_t q = c + dk * ld + di;
-            acc = dd_add(acc, dd_mul_d({hi[q], lo[q]}, w));
+            csum_tap(a, w, hi[q], lo[q]);
         }
-    return acc;
+    return quick_two_sum(a.s, a.e);
+}
+// a * b for double-doubles (to ~eps^2)
+__device__ __forceinline__ dd dd_mul(dd a, dd b)
+{
+    const double p = a.hi * b.hi;
+    double e = fma(a.hi, b.hi, -p);
+    e = fma(a.hi, b.lo, e);
+    e = fma(a.lo, b.hi, e);
+    return quick_two_sum(p, e);
 }
 
 // Fine-grid residual r = RHS - L(hi+lo) in double-double; writes r (FP64)
-// and its FP32 rounding, and per-block sums of |r|.
+// and its FP32 rounding, and per-block sums of |r|.  The 1/h^n factors are
+// double-double reciprocals (InvSpacing).
 __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ hi, const double* __restrict__ lo,
                                                      const double* __restrict__ vis, const double* __restrict__ vx,
                                                      const double* __restrict__ vz, const double* __restrict__ vxx,
                                                      const double* __restrict__ vzz, const double* __restrict__ vxz,
                                                      const double* __restrict__ rhs, double* __restrict__ r64,
-                                                     float* __restrict__ r32, Grid g, FD F, Spacing h, int ivis,
+                                                     float* __restrict__ r32, Grid g, FD F, InvSpacing ih, int ivis,
                                                      double* __restrict__ partial)
 {
     const int2 tile = xcd_tile();
@@ -101,20 +137,21 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
     if (i <= g.nx - 1 && k <= g.nz - 1) {
         const size_t c = g.at(i, k);
         const ptrdiff_t ld = g.ld;
-        dd Srr = dd_div_d(dd_dot(F.D2, -1, 1, hi, lo, c, 1), h.dr2);
-        dd Sss = dd_div_d(dd_dot(F.D2, -1, 1, hi, lo, c, ld), h.ds2);
-        dd Srrr = dd_div_d(dd_dot(F.D3, -2, 2, hi, lo, c, 1), h.dr3);
-        dd Ssss = dd_div_d(dd_dot(F.D3, -2, 2, hi, lo, c, ld), h.ds3);
-        dd Srrrr = dd_div_d(dd_dot(F.D4, -2, 2, hi, lo, c, 1), h.dr4);
-        dd Sssss = dd_div_d(dd_dot(F.D4, -2, 2, hi, lo, c, ld), h.ds4);
-        dd Srs = dd_div_d(dd_sum9(F.Drs, hi, lo, c, ld), h.drds);
-        dd Srss = dd_div_d(dd_sum9(F.Drss, hi, lo, c, ld), h.drds2);
-        dd Srrs = dd_div_d(dd_sum9(F.Drrs, hi, lo, c, ld), h.dr2ds);
-        dd Srrss = dd_div_d(dd_sum9(F.Drrss, hi, lo, c, ld), h.dr2ds2);
+        auto sc = [&](dd x, int q) { return dd_mul(x, dd{ih.hi[q], ih.lo[q]}); };
         const double v = vis[c];
+        dd Srrrr = sc(csum_dot(F.D4, -2, 2, hi, lo, c, 1), IH_R4);
+        dd Sssss = sc(csum_dot(F.D4, -2, 2, hi, lo, c, ld), IH_S4);
+        dd Srrss = sc(csum_9(F.Drrss, hi, lo, c, ld), IH_R2S2);
         dd sum = dd_mul_d(dd_add(Srrrr, Sssss), v);
         sum = dd_add(sum, dd_mul_d(Srrss, 2.0 * v));
         if (ivis != 0) {
+            dd Srr = sc(csum_dot(F.D2, -1, 1, hi, lo, c, 1), IH_R2);
+            dd Sss = sc(csum_dot(F.D2, -1, 1, hi, lo, c, ld), IH_S2);
+            dd Srrr = sc(csum_dot(F.D3, -2, 2, hi, lo, c, 1), IH_R3);
+            dd Ssss = sc(csum_dot(F.D3, -2, 2, hi, lo, c, ld), IH_S3);
+            dd Srs = sc(csum_9(F.Drs, hi, lo, c, ld), IH_RS);
+            dd Srss = sc(csum_9(F.Drss, hi, lo, c, ld), IH_RS2);
+            dd Srrs = sc(csum_9(F.Drrs, hi, lo, c, ld), IH_R2S);
             sum = dd_add(sum, dd_mul_d(dd_add(Srr, dd_neg(Sss)), vxx[c] - vzz[c]));
             sum = dd_add(sum, dd_mul_d(Srs, 4.0 * vxz[c]));
             sum = dd_add(sum, dd_mul_d(dd_add(Srrr, Srss), 2.0 * vx[c]));
@@ -357,7 +394,7 @@ int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const
     dim3 b(64, 4);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     hipLaunchKernelGGL(k_residual_dd, gr, b, 0, st, hi, lo, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz, rhs, r64,
-                       r32, L.g, F, h, ivis, partial);
+                       r32, L.g, F, inv_spacing(h), ivis, partial);
     return (int)(gr.x * gr.y);
 }
 

@@ -187,6 +187,12 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function
+    integer(c_int) function tc_ensemble_step(ctxs, n, nsteps, threads, failed) bind(C, name="tc_ensemble_step")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: n, nsteps, threads
+      integer(c_int32_t), intent(out) :: failed
+    end function
     integer(c_int) function tc_sort_tracers(ctx) bind(C, name="tc_sort_tracers")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
