@@ -818,8 +818,12 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     launch_bc_stream(ctx->stream, ctx->psi_hi, L1.g, P.Vbc);
     TC_HIP(hipMemsetAsync(ctx->psi_lo, 0, plane_bytes(L1.g), ctx->stream));
     TC_TRY(solve_setup(ctx, d));
-    if (f32)
+    if (f32) {
         for (int L = 1; L <= G; ++L) launch_coeffs_to_f32(ctx->stream, ctx->lv[L], L == ctx->ratio_level, P.ivis);
+        if (d.tb)
+            for (int L = 2; L < G; ++L)
+                if (tb_uses_w13(ctx->lv[L])) launch_tb_weights32(ctx->stream, ctx->lv[L], d.F, d.h[L]);
+    }
     L1.cur = 0;
     L1.cur32 = 0;
     if (f32) TC_HIP(hipMemsetAsync(L1.err32[0], 0, L1.g.elems * sizeof(float), ctx->stream));
@@ -1305,7 +1309,7 @@ void free_ctx(tc_ctx* ctx)
                        l.dtvis, l.visf, l.visxf, l.viszf, l.visxxf, l.viszzf, l.visxzf, l.vtmp, l.res};
         for (double* p : a) F(p);
         float* a32[] = {l.err32[0], l.err32[1], l.rhs32, l.res32, l.vis32, l.visx32, l.visz32, l.a132, l.visxz32,
-                        l.dts32, l.lb32};
+                        l.dts32, l.lb32, l.w13};
         for (float* p : a32) F(p);
     }
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,

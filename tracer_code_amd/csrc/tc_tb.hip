@@ -122,9 +122,29 @@ __device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int 
 }
 
 #pragma clang fp contract(fast)
-template <typename T, bool RATIO, int K, int TT>
+// The 13 dt-scaled tap weights of every interior cell of a level, once per
+// solve (the operator's coefficients are fixed during a solve): plane q of
+// W holds tap q in W13 order (c, xm1, xp1, xm2, xp2, zm1, zp1, zm2, zp2, mm,
+// pm, mp, pp).  The same arithmetic as tb_weights, so the sweeps that read
+// them give the same values.
+__global__ __launch_bounds__(256) void k_tb_weights32(Coefs<float> C, Tables<float> Tb, Grid g,
+                                                      float* __restrict__ W)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k), n = g.elems;
+    W13<float> w;
+    tb_weights<float, false>(C, Tb, c, w);
+    const float v[13] = {w.c, w.xm1, w.xp1, w.xm2, w.xp2, w.zm1, w.zp1, w.zm2, w.zp2, w.mm, w.pm, w.mp, w.pp};
+#pragma unroll
+    for (int q = 0; q < 13; ++q) W[q * n + c] = v[q];
+}
+
+template <typename T, bool RATIO, int K, int TT, bool PRE = false>
 __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
-                                                        Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg)
+                                                        Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg,
+                                                        const T* __restrict__ W)
 {
     using G = TBGeom<TT, K>;
     constexpr int R = G::R, H = G::H, NPT = G::NPT;
@@ -149,7 +169,19 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
         const bool interior = i >= 1 && i <= g.nx - 1 && k >= 1 && k <= g.nz - 1;
         if (interior && m >= 2) {
             band[u] = m / 2;
-            tb_weights<T, RATIO>(C, Tb, g.at(i, k), w[u]);
+            const size_t c = g.at(i, k);
+            if constexpr (PRE) {
+                const size_t n = g.elems;
+                W13<T>& q = w[u];
+                q.c = W[c]; q.xm1 = W[n + c]; q.xp1 = W[2 * n + c]; q.xm2 = W[3 * n + c]; q.xp2 = W[4 * n + c];
+                q.zm1 = W[5 * n + c]; q.zp1 = W[6 * n + c]; q.zm2 = W[7 * n + c]; q.zp2 = W[8 * n + c];
+                q.mm = W[9 * n + c]; q.pm = W[10 * n + c]; q.mp = W[11 * n + c]; q.pp = W[12 * n + c];
+                q.rd = C.dts[c] * C.rhs[c];
+                q.lb = T(-2);
+                if constexpr (RATIO) q.lb = C.lb[c];
+            } else {
+                tb_weights<T, RATIO>(C, Tb, c, w[u]);
+            }
         }
     }
     __syncthreads();
@@ -220,21 +252,30 @@ static int tb_tile(const Grid& g)
 
 template <typename T, int TT>
 static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
-                        const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg)
+                        const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr)
 {
     const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
+    if constexpr (sizeof(T) == 4 && TT == 16) {
+        if (W && K == 4) {
+            if (use_ratio)
+                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+            else
+                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+            return;
+        }
+    }
     if (K == 8) {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
     } else {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
     }
 }
 
 template <typename T>
 static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
-                      const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg)
+                      const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr)
 {
     if constexpr (sizeof(T) == 4) {
         if (tb_tile<T>(g) == 32 && K == 4) {
@@ -242,7 +283,28 @@ static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, cons
             return;
         }
     }
-    tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg);
+    tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
+}
+
+// precomputed weights where the launches are latency-bound (16x16 tiles):
+// they save each launch the weight arithmetic; the 32x32-tile level keeps
+// computing them (it is bound by the staged bytes, and 13 planes are more
+// bytes than the 6 coefficient planes)
+bool tb_uses_w13(const Level& L) { return tb_tile<float>(L.g) == 16; }
+
+void launch_tb_weights32(hipStream_t st, Level& L, const FD& F, const Spacing& h)
+{
+    if (!L.w13) {
+        if (hipMalloc((void**)&L.w13, 13 * L.g.elems * sizeof(float)) != hipSuccess) {
+            L.w13 = nullptr;
+            return;
+        }
+    }
+    // rhs: any plane (tb_weights also forms dts*rhs, unused here)
+    Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, L.rhs32, L.dts32, nullptr, nullptr};
+    dim3 b(64, 4);
+    dim3 gr((L.g.nx - 1 + 63) / 64, (L.g.nz - 1 + 3) / 4);
+    hipLaunchKernelGGL(k_tb_weights32, gr, b, 0, st, C, make_tables<float>(F, h), L.g, L.w13);
 }
 
 void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
@@ -262,7 +324,8 @@ void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, co
     Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, rhs, L.dts32, nullptr, L.lb32};
     SweepFactors<float> sf{};
     for (int s = 0; s < nk; ++s) { sf.temp[s] = (float)temp[s]; sf.cconst[s] = (float)cconst[s]; }
-    tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg);
+    tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg,
+                     tb_uses_w13(L) ? L.w13 : nullptr);
 }
 
 }  // namespace tc
