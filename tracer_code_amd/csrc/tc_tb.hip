@@ -255,7 +255,7 @@ static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, co
                         const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr)
 {
     const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
-    if constexpr (sizeof(T) == 4 && TT == 16) {
+    if constexpr (sizeof(T) == 4) {
         if (W && K == 4) {
             if (use_ratio)
                 hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
@@ -279,18 +279,17 @@ static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, cons
 {
     if constexpr (sizeof(T) == 4) {
         if (tb_tile<T>(g) == 32 && K == 4) {
-            tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg);
+            tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
             return;
         }
     }
     tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
 }
 
-// precomputed weights where the launches are latency-bound (16x16 tiles):
-// they save each launch the weight arithmetic; the 32x32-tile level keeps
-// computing them (it is bound by the staged bytes, and 13 planes are more
-// bytes than the 6 coefficient planes)
-bool tb_uses_w13(const Level& L) { return tb_tile<float>(L.g) == 16; }
+// precomputed weights on every FP32 level: they save each launch the weight
+// arithmetic of its staged cells (three per thread with 32x32 tiles), which
+// bounds the launches more than the extra planes they read
+bool tb_uses_w13(const Level& L) { (void)L; return true; }
 
 void launch_tb_weights32(hipStream_t st, Level& L, const FD& F, const Spacing& h)
 {
