@@ -309,7 +309,8 @@ int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_
  * attached network's predicted correction of it.  Every candidate is
  * judged by the residual after the cycle's first level-1 smoothing and the
  * smallest wins; the plain start is always a candidate, and every solve
- * iterates to the same tolerance. */
+ * iterates to the same tolerance.  A candidate that lost its last 8
+ * judgements is built and judged only every 8th solve until it wins. */
 enum { TC_GUESS_PLAIN = 0, TC_GUESS_EXTRAP = 1, TC_GUESS_LEARNED = 2 };
 int tc_guess_enable(tc_ctx* ctx, int mode);
 

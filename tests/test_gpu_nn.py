@@ -99,15 +99,20 @@ def test_guess_in_solver_keeps_tolerance(level):
         assert np.isfinite(loss1) and loss1 < loss0
         g.guess_enable("learned")
         st0 = g.stats()
+        judged = 0
         for _ in range(3):
             g.time_step(1)
             st = g.stats()
             assert st.residual_last <= params.tolerance * st.res_ref_last
-            # the last solve judged all three starts
-            assert st.guess_res[0] > 0 and st.guess_res[1] > 0 and st.guess_res[2] > 0
+            # the learned mode judges the plain start and the learned one
+            assert st.guess_res[1] == 0
+            judged += st.guess_res[2] > 0
+            if st.guess_res[2] > 0:
+                assert st.guess_res[0] > 0
         starts = [st.guesses_used[i] - st0.guesses_used[i] for i in range(3)]
-        print(f"starts over the last 3 steps (plain, extrapolation, learned): {starts}", flush=True)
-        assert sum(starts) == 12
+        print(f"starts over the last 3 steps (plain, extrapolation, learned): {starts}; last solves judged: {judged}",
+              flush=True)
+        assert sum(starts) == 12 and starts[1] == 0 and judged >= 1
         g.learn_attach(None)
 
 

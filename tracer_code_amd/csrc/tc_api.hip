@@ -626,6 +626,16 @@ int prolong_add(tc_ctx* ctx, const double* e, int m, double* fine, double zsg)
 // psi_1/s, (psi_1 - psi_2)/sd, (T - T_1)/dT (psi_i, T_1 of stage s);
 // output: (psi - ext)/e, e = max |psi - ext| of the stage's last solve.
 // Also records the network's training inputs (the target follows the solve).
+// A candidate that lost its last GUESS_BACKOFF judgements is built and
+// judged only every GUESS_BACKOFF-th solve until it wins again: a start that
+// does not help then costs little (its first visit, the network's forward
+// pass), and one that starts helping is picked up within that many solves.
+constexpr int GUESS_BACKOFF = 8;
+bool guess_due(const tc_ctx* ctx, int c)
+{
+    return ctx->guess_losses[c] < GUESS_BACKOFF || ctx->guess_solves % GUESS_BACKOFF == 0;
+}
+
 int guess_pre_solve(tc_ctx* ctx, bool dc, int* slot, bool* nn_guess)
 {
     *slot = -1;
@@ -647,7 +657,7 @@ int guess_pre_solve(tc_ctx* ctx, bool dc, int* slot, bool* nn_guess)
     const size_t np = (size_t)u.npx[0];
     TC_TRY(restrict_to(ctx, ctx->ext_psi, m, ctx->nn_ext));
     const bool record = ctx->rec_n < ctx->rec_cap && ctx->ecount[s] >= 1;
-    const bool predict = dc && ctx->guess == TC_GUESS_LEARNED && ctx->ecount[s] >= 1;
+    const bool predict = dc && ctx->guess == TC_GUESS_LEARNED && ctx->ecount[s] >= 1 && guess_due(ctx, 2);
     if (!record && !predict) return 0;
     const double* C = nullptr;
     TC_TRY(restrict_to(ctx, ctx->Tbuoy, m, ctx->nn_T));
@@ -850,7 +860,10 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     for (int c = 0; c < 3; ++c) ctx->stats.guess_res[c] = 0.0;
     const int* multi = P.multi;
     TC_TRY(defect(ctx->psi_hi, nullptr));
-    double** cand[3] = {nullptr, (ctx->have_ext && ctx->guess >= TC_GUESS_EXTRAP) ? &ctx->ext_psi : nullptr,
+    // TC_GUESS_EXTRAP judges the extrapolation, TC_GUESS_LEARNED the
+    // extrapolation plus the network's correction (the network's base)
+    double** cand[3] = {nullptr,
+                        (ctx->have_ext && ctx->guess == TC_GUESS_EXTRAP && guess_due(ctx, 1)) ? &ctx->ext_psi : nullptr,
                         nn_guess ? &ctx->guess_psi : nullptr};
     const bool judge = cand[1] || cand[2];
     if (judge) {
@@ -890,8 +903,13 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             double mag = 0.0;
             TC_TRY(defect(ctx->psi_hi, &mag));
         }
+        for (int c = 1; c < 3; ++c)
+            if (cand[c]) ctx->guess_losses[c] = (ctx->stats.guess_used == c) ? 0 : ctx->guess_losses[c] + 1;
     }
-    if (ctx->solve_stage >= 0) ctx->stats.guesses_used[ctx->stats.guess_used]++;
+    if (ctx->solve_stage >= 0) {
+        ctx->stats.guesses_used[ctx->stats.guess_used]++;
+        if (ctx->guess != TC_GUESS_PLAIN) ctx->guess_solves++;
+    }
     const bool first_visit_done = judge;
     int q = 1;
     int status = 0;
@@ -1987,6 +2005,7 @@ int tc_guess_enable(tc_ctx* ctx, int mode)
     }
     if (mode != TC_GUESS_PLAIN) TC_TRY(guess_alloc(ctx));
     ctx->guess = mode;
+    for (int c = 0; c < 3; ++c) ctx->guess_losses[c] = 0;
     TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }

@@ -275,9 +275,10 @@ __global__ __launch_bounds__(256) void k_filter_step_bcs(const double* __restric
         return;
     }
     if (first && tid < FILT_SLOTS) slots[((size_t)((m + 1) & 1) * FILT_SLOTS + tid) * FILT_STRIDE] = 0ull;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
+    // x from i = -ORG_PAD, so that each row segment starts a cache line
+    const int i = blockIdx.x * blockDim.x + threadIdx.x - ORG_PAD;
     const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
-    if (i > g.nx + SPAN || k > g.nz + SPAN) return;
+    if (i < -SPAN || i > g.nx + SPAN || k > g.nz + SPAN) return;
     const int nx = g.nx, nz = g.nz;
     // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
     const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
@@ -740,7 +741,7 @@ void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double*
                             unsigned long long* slots, int m, double delta, int32_t* ctl)
 {
     dim3 b(64, 4);
-    dim3 gr = grid2(g.nx + 1 + 2 * SPAN, g.nz + 1 + 2 * SPAN, b);
+    dim3 gr = grid2(g.nx + 1 + SPAN + ORG_PAD, g.nz + 1 + 2 * SPAN, b);
     if (comp)
         hipLaunchKernelGGL((k_filter_step_bcs<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
                            slots, m, delta, ctl);

@@ -150,6 +150,8 @@ struct tc_ctx {
     double* sh2[4] = {nullptr, nullptr, nullptr, nullptr};
     double tau1[4] = {0, 0, 0, 0}, tau2[4] = {0, 0, 0, 0};
     int scount[4] = {0, 0, 0, 0};
+    int guess_losses[3] = {0, 0, 0};  // consecutive lost judgements of each candidate
+    int64_t guess_solves = 0;         // solves inside time steps with candidates enabled
     double* ext_psi = nullptr;    // the extrapolated guess of the running solve
     bool have_ext = false;
     int ecount[4] = {0, 0, 0, 0}; // per stage: extrapolation errors measured with the network attached
