@@ -235,6 +235,15 @@ int tc_time_step(tc_ctx* ctx, int nsteps);
 int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
                               double base[7]);
 
+/* ---- diagnostics -------------------------------------------------------- */
+/* Replaces stats (basics.f90:236-324) on the device fields: recomputes the
+ * strain rate invariant, then out = {vrms, Nu (top), Tavg, Cavg (type 1),
+ * nodes without tracers, min and max tracers per node, Nu at the bottom,
+ * max strain on the top row}, with the trapezoid integrals of integrate2D /
+ * integrate1Dx (basics.f90:341-390).  The sums run in another order than
+ * the reference's loops (equal to rounding); the integers are exact. */
+int tc_diagnostics(tc_ctx* ctx, double out[9]);
+
 /* ---- ensembles (no counterpart in the reference) ------------------------ */
 /* Runs nsteps time steps (tc_time_step) on each of n independent contexts
  * -- separate cases, any grids and parameters -- concurrently: `threads`

@@ -87,6 +87,7 @@ SIGNATURES = {
     "tc_set_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_set_dc_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_sort_tracers": (ctypes.c_int, [_VP]),
+    "tc_diagnostics": (ctypes.c_int, [_VP, _VP]),
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),

@@ -65,6 +65,8 @@ struct tc_dev {
     int learn_slot = -1;       // recorded sample of the solve in progress
     double* partialB = nullptr;
     int max_partial = 0;
+    void* diag = nullptr;           // tc_diagnostics: atomics and results
+    double* diag_part = nullptr;    // tc_diagnostics: 3 partial sums per block
     bool coarse_ready = false;
     bool trace = false;
     bool coarse_lu = false;
@@ -1347,7 +1349,7 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
-        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots);
+        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->diag); F(d->diag_part);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         delete d;
@@ -1843,6 +1845,50 @@ int tc_compute_velocities(tc_ctx* ctx)
     if (!ctx) return TC_ERR_ARG;
     TC_TRY(velocities(ctx, DEV(ctx)));
     TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_diagnostics(tc_ctx* ctx, double out[9])
+{
+    if (!ctx || !out) return TC_ERR_ARG;
+    tc_dev& d = DEV(ctx);
+    const tc_params& P = ctx->p;
+    const Grid& g = ctx->g;
+    // scratch: 3 partial sums per block, the atomics and the 6 results
+    const int nb = diag_blocks(g);
+    if (!d.diag_part) TC_HIP(hipMalloc((void**)&d.diag_part, (size_t)3 * nb * sizeof(double)));
+    double* part = d.diag_part;
+    struct Mem {
+        int32_t ints[4];
+        unsigned long long smax;
+        double res[6];
+    };
+    static_assert(sizeof(Mem) <= 128, "diagnostics scratch");
+    if (!d.diag) TC_HIP(hipMalloc(&d.diag, sizeof(Mem)));
+    Mem init{};
+    init.ints[1] = 0x7fffffff;
+    init.ints[2] = -1;
+    TC_HIP(hipMemcpyAsync(d.diag, &init, sizeof(Mem), hipMemcpyHostToDevice, ctx->stream));
+    Mem* m = static_cast<Mem*>(d.diag);
+    launch_strain(ctx->stream, ctx->u, ctx->w, ctx->strain, g, d.F, ctx->lv[1].dr, ctx->lv[1].ds);
+    const double* C = (P.comp == 1 && P.ntype > 0) ? ctx->Cnew[0] : nullptr;
+    launch_diagnostics(ctx->stream, ctx->u, ctx->w, ctx->T, C, ctx->nempty, ctx->strain, ctx->conduct, g, d.F,
+                       ctx->lv[1].dr, ctx->lv[1].ds, part, m->ints, &m->smax, m->res);
+    Mem h{};
+    TC_HIP(hipMemcpyAsync(&h, d.diag, sizeof(Mem), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    const double a = P.aspect;
+    out[0] = std::sqrt(h.res[0] / a);                         // vrms
+    out[1] = h.res[3] / h.res[4];                             // Nu
+    out[2] = h.res[1] / a;                                    // Tavg
+    out[3] = C ? h.res[2] / a : 0.0;                          // Cavg
+    out[4] = h.ints[0];                                       // empty nodes
+    out[5] = h.ints[1];                                       // min tracers per node
+    out[6] = h.ints[2];                                       // max tracers per node
+    out[7] = h.res[5] / a;                                    // Nu at the bottom
+    double smax;
+    std::memcpy(&smax, &h.smax, sizeof(double));
+    out[8] = smax;                                            // max strain at the surface
     return 0;
 }
 

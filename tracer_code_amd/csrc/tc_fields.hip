@@ -666,6 +666,98 @@ void launch_strain(hipStream_t st, const double* u, const double* w, double* str
     hipLaunchKernelGGL(k_sym_bcs, dim3(1), dim3(256), 0, st, strain, g);
 }
 
+// ---------------------------------------------------------------------------
+// stats (basics.f90:236-324) on the device: trapezoid integrals
+// (integrate2D / integrate1Dx, basics.f90:341-390; uniform grids, metrics 1)
+// as weighted block sums, the tracer-count extremes and the surface strain
+// maximum as integer / bit-pattern atomics, and the two Nusselt integrals
+// along the walls in one workgroup.  The sums run in another order than the
+// reference's row-by-row loops, so they agree to rounding.
+// ---------------------------------------------------------------------------
+// part[3*b + {0,1,2}]: the block's weighted sums of u^2+w^2, T and C;
+// ints[0] += nodes with nempty == 0, ints[1] = min nempty, ints[2] = max
+// nempty; smax: max strain on the top row (bits of a non-negative double)
+__global__ __launch_bounds__(256) void k_diag_part(const double* __restrict__ u, const double* __restrict__ w,
+                                                   const double* __restrict__ T, const double* __restrict__ C,
+                                                   const int32_t* __restrict__ nempty,
+                                                   const double* __restrict__ strain, Grid g, double dr, double ds,
+                                                   double* __restrict__ part, int32_t* __restrict__ ints,
+                                                   unsigned long long* __restrict__ smax)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    double v2 = 0.0, tt = 0.0, cc = 0.0;
+    if (i <= g.nx && k <= g.nz) {
+        const size_t c = g.at(i, k);
+        const double wgt = ((i == 0 || i == g.nx) ? 0.5 : 1.0) * ((k == 0 || k == g.nz) ? 0.5 : 1.0) * dr * ds;
+        v2 = (u[c] * u[c] + w[c] * w[c]) * wgt;
+        tt = T[c] * wgt;
+        cc = C ? C[c] * wgt : 0.0;
+        const int32_t n = nempty[c];
+        if (n == 0) atomicAdd(&ints[0], 1);
+        atomicMin(&ints[1], n);
+        atomicMax(&ints[2], n);
+        if (k == g.nz) atomicMax(smax, (unsigned long long)__double_as_longlong(strain[c]));
+    }
+    v2 = block_sum(v2);
+    tt = block_sum(tt);
+    cc = block_sum(cc);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double* q = part + 3 * (blockIdx.y * gridDim.x + blockIdx.x);
+        q[0] = v2;
+        q[1] = tt;
+        q[2] = cc;
+    }
+}
+
+// out[0..2] = the sums of part (fixed order); out[3..5] = the wall integrals
+// of -conduct*dT/dz at the top, T at the bottom and -conduct*dT/dz at the
+// bottom (integrate1Dx)
+__global__ __launch_bounds__(1024) void k_diag_final(const double* __restrict__ part, int nb,
+                                                     const double* __restrict__ T,
+                                                     const double* __restrict__ conduct, Grid g, FD F, double dr,
+                                                     double ds, double* __restrict__ out)
+{
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, a = 0.0, b = 0.0, e = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        s0 += part[3 * t];
+        s1 += part[3 * t + 1];
+        s2 += part[3 * t + 2];
+    }
+    for (int i = threadIdx.x; i <= g.nx; i += blockDim.x) {
+        const double wi = ((i == 0 || i == g.nx) ? 0.5 : 1.0) * dr;
+        const size_t ct = g.at(i, g.nz), cb = g.at(i, 0);
+        const double Tz_surf = dot3(F.D1, T, ct, g.ld) / ds;
+        const double Tz_bot = dot3(F.D1, T, cb, g.ld) / ds;
+        a += -conduct[ct] * Tz_surf * wi;
+        b += T[cb] * wi;
+        e += -conduct[cb] * Tz_bot * wi;
+    }
+    s0 = block_sum(s0);
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
+    a = block_sum(a);
+    b = block_sum(b);
+    e = block_sum(e);
+    if (threadIdx.x == 0) {
+        out[0] = s0; out[1] = s1; out[2] = s2; out[3] = a; out[4] = b; out[5] = e;
+    }
+}
+
+int diag_blocks(const Grid& g) { return ((g.nx + 1 + 63) / 64) * ((g.nz + 1 + 3) / 4); }
+
+void launch_diagnostics(hipStream_t st, const double* u, const double* w, const double* T, const double* C,
+                        const int32_t* nempty, const double* strain, const double* conduct, const Grid& g,
+                        const FD& F, double dr, double ds, double* part, int32_t* ints, unsigned long long* smax,
+                        double* out)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1, g.nz + 1, b);
+    hipLaunchKernelGGL(k_diag_part, gr, b, 0, st, u, w, T, C, nempty, strain, g, dr, ds, part, ints, smax);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(1024), 0, st, part, (int)(gr.x * gr.y), T, conduct, g, F, dr, ds,
+                       out);
+}
+
 int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial)
 {
     dim3 b(64, 4);

@@ -132,6 +132,11 @@ void launch_stop_test(hipStream_t st, const double* partial, int n, double delta
 void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
                         const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop);
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
+int diag_blocks(const Grid& g);
+void launch_diagnostics(hipStream_t st, const double* u, const double* w, const double* T, const double* C,
+                        const int32_t* nempty, const double* strain, const double* conduct, const Grid& g,
+                        const FD& F, double dr, double ds, double* part, int32_t* ints, unsigned long long* smax,
+                        double* out);
 void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
                            const int32_t* stop);
 size_t filter_slot_words();  // u64 words of the two slot sets of the fused filter loop

@@ -193,6 +193,11 @@ module tracer_hip
       integer(c_int32_t), value :: n, nsteps, threads
       integer(c_int32_t), intent(out) :: failed
     end function
+    integer(c_int) function tc_diagnostics(ctx, out) bind(C, name="tc_diagnostics")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(9)
+    end function
     integer(c_int) function tc_sort_tracers(ctx) bind(C, name="tc_sort_tracers")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -207,6 +207,14 @@ class Solver:
                                             ctypes.byref(loss)), "learn_train")
         return loss.value
 
+    def diagnostics(self):
+        """stats (basics.f90:236-324) on the device: dict of vrms, Nu, Tavg,
+        Cavg, empty, tmin, tmax, Nu_bot, strain_max."""
+        out = np.zeros(9)
+        self._check(self.lib.tc_diagnostics(self.h, _ptr(out)), "diagnostics")
+        keys = ("vrms", "Nu", "Tavg", "Cavg", "empty", "tmin", "tmax", "Nu_bot", "strain_max")
+        return dict(zip(keys, out.tolist()))
+
     def sort_tracers(self):
         """Reorder the device tracers by node (get_tracers keeps the caller's order)."""
         self._check(self.lib.tc_sort_tracers(self.h), "sort_tracers")
