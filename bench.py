@@ -15,13 +15,18 @@ the reference's own cycle, which FP64 rounding keeps above a 1e-8 residual at
 2048x1024 (it stops with "Too Many Multigrid Cycles").
 
 Prints one JSON line.  `roofline` reports the level-1 Jacobi-Richardson sweep
-(the dominant kernel): algorithmic bytes per launch per interior point are
-80 B in FP64 (read S, eta and its 5 derivatives, RHS, dt_stream; write the
-new iterate) and 36 B in FP32 (read S, eta, eta_x, eta_z, eta_xx-eta_zz,
-eta_xz, RHS, dt_stream; write), over the HIP-event-timed average launch
-duration on the library's stream.  `cpu_baseline` times the CPU oracle's
-level-1 sweep on this host and converts it to steps/s with the sweep counts
-the GPU executed per step.
+(the dominant kernel): algorithmic bytes per interior point are the
+reference's arrays read or written once -- S, eta, eta_x, eta_z, eta_xx,
+eta_zz, eta_xz, RHS, dt_stream, ratio read, the new iterate written: 11 x 8 B
+= 88 B in FP64, 44 B in FP32 -- over the HIP-event-timed average launch
+duration on the library's stream.  The kernels themselves move less
+(`kernel_bytes_per_launch`: eta_xx - eta_zz is one plane formed once per
+solve, and the ratio plane is read only on the level where the reference's
+ratio is not the reset value): 80 B / 36 B per point.  `step_roofline`
+sums the same per-point bytes over every sweep of the step plus 304 B per
+tracer for the RK stages and 20 B per tracer per binning.  `cpu_baseline`
+times the CPU oracle's level-1 sweep on this host and converts it to
+steps/s with the sweep counts the GPU executed per step.
 """
 import argparse
 import json
@@ -33,7 +38,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}
+SWEEP_BYTES_PER_POINT = {"fp64": 88.0, "dc64": 88.0, "mixed": 44.0}      # algorithmic
+KERNEL_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}     # what the kernels move
+TRACER_BYTES_PER_STEP = 304.0 + 5 * 20.0  # RK4 stages + five binnings, per tracer
 DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
 
@@ -182,6 +189,14 @@ def main():
         for line in open(tfile):
             if line.startswith("bytes_per_launch"):
                 traffic = float(line.split("=")[1])
+    # whole step: every sweep of every level at the per-point bytes, plus
+    # the tracer stages and binnings (residuals, transfers, filters and the
+    # viscosity set-up are not counted)
+    step_bytes = TRACER_BYTES_PER_STEP * ntr
+    for L, cnt in enumerate(sweeps_per_step, start=1):
+        step_bytes += cnt * (params.nx // 2 ** (L - 1) - 1) * (params.nz // 2 ** (L - 1) - 1) * \
+            SWEEP_BYTES_PER_POINT[args.mode]
+    step_gbs = step_bytes / (elapsed / args.steps) / 1e9
     out = {
         "metric": "steps/s @2048x1024+42M tracers to 1e-8 resid: mixed+learned vs plain FP64",
         "value": steps_per_s,
@@ -223,7 +238,10 @@ def main():
             "avg_launch_us": avg_us,
             "launches": nlaunch,
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "kernel_bytes_per_launch": KERNEL_BYTES_PER_POINT[args.mode] * pts1,
         },
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
     }
     if net is not None:
         solver.learn_attach(None)
