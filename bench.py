@@ -145,6 +145,7 @@ def main():
     if args.mode != "fp64":
         solver.guess_enable(guess)
     tune = None
+    tiles = None
     dc_sweeps = params.Nmulti
     if args.mode != "fp64":
         if args.dc_sweeps is None:
@@ -153,6 +154,11 @@ def main():
             dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
             print(f"[bench] autotuned sweeps per visit: {dc_sweeps} ({time.perf_counter() - t0:.1f}s; "
                   f"ms/step {tune})", file=sys.stderr, flush=True)
+            if args.mode == "mixed":
+                t0 = time.perf_counter()
+                tiles = solver.autotune_tb_tiles(1)
+                print(f"[bench] autotuned tiles by level: {tiles} ({time.perf_counter() - t0:.1f}s)",
+                      file=sys.stderr, flush=True)
         else:
             dc_sweeps = args.dc_sweeps
             solver.set_dc_sweeps(dc_sweeps)
@@ -219,6 +225,7 @@ def main():
             "sweeps_per_visit": dc_sweeps,
             "reference_sweeps_per_visit": params.Nmulti,
             "autotune_ms_per_step": tune,
+            "tb_tiles_by_level": tiles,
             "dc_retries": after.dc_retries,
             "guess": guess,
             "starts": {k: after.guesses_used[i] - before.guesses_used[i]

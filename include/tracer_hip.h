@@ -182,6 +182,15 @@ int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps);
  * choice, which is also set as with tc_set_dc_sweeps. */
 int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int32_t steps_each, int32_t* best,
                           double* ms_per_step);
+/* Tile of the mixed mode's temporally blocked sweeps on `level` (2 ..
+ * grids-1): 16 or 32 (16x16 or 32x32 cells per workgroup, each staged with
+ * an 8-cell halo), 0 = chosen by the level's size. */
+int tc_set_tb_tile(tc_ctx* ctx, int32_t level, int32_t tile);
+/* Autotuner of those tiles: level by level from 2, runs steps_each time
+ * steps with each tile and keeps the faster (ties: the size-based choice);
+ * tiles[TC_MAX_LEVELS] (may be NULL) receives the tile of each level
+ * (0 where the level has no such sweeps). */
+int tc_autotune_tb_tiles(tc_ctx* ctx, int32_t steps_each, int32_t* tiles);
 int tc_get_stats(const tc_ctx* ctx, tc_stats* out);
 int tc_sync(tc_ctx* ctx);
 

@@ -91,3 +91,21 @@ def test_dc_sweeps_retry_and_autotune():
         g.time_step(1)
         check_dc_solution(params, g, f"{best} sweeps per visit")
         assert g.stats().dc_retries == r0
+
+
+def test_tile_choice_and_autotune():
+    """Both tiles of the temporally blocked sweeps give solves at the
+    tolerance; the tile autotuner keeps 16 or 32 on every level it tunes."""
+    params, g, _ = solver_for("cfg2", nx=256, nz=128, tpc=4)
+    with g:
+        g.set_mode("mixed")
+        for tile in (16, 32):
+            for L in range(2, params.grids):
+                g.set_tb_tile(L, tile)
+            g.time_step(1)
+            check_dc_solution(params, g, f"tiles {tile}")
+        tiles = g.autotune_tb_tiles(1)
+        print(f"autotuned tiles {tiles}", flush=True)
+        assert all(t in (16, 32) for t in tiles[1:params.grids - 1])
+        g.time_step(1)
+        check_dc_solution(params, g, "autotuned tiles")

@@ -1562,6 +1562,39 @@ int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps)
     return 0;
 }
 
+int tc_set_tb_tile(tc_ctx* ctx, int32_t level, int32_t tile)
+{
+    if (!ctx || level < 2 || level >= ctx->grids || (tile != 0 && tile != 16 && tile != 32)) return TC_ERR_ARG;
+    ctx->lv[level].tb_tile = tile;
+    return 0;
+}
+
+int tc_autotune_tb_tiles(tc_ctx* ctx, int32_t steps_each, int32_t* tiles)
+{
+    if (!ctx || steps_each < 1) return TC_ERR_ARG;
+    auto step_ms = [&](double* ms) -> int {
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        TC_TRY(tc_time_step(ctx, steps_each));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps_each;
+        return 0;
+    };
+    for (int L = 2; L < ctx->grids; ++L) {
+        Level& lv = ctx->lv[L];
+        const int keep = lv.tb_tile;
+        double ms16 = 0.0, ms32 = 0.0;
+        lv.tb_tile = 16;
+        TC_TRY(step_ms(&ms16));
+        lv.tb_tile = 32;
+        TC_TRY(step_ms(&ms32));
+        lv.tb_tile = (ms32 < 0.98 * ms16) ? 32 : (ms16 < 0.98 * ms32 ? 16 : keep);
+    }
+    if (tiles)
+        for (int L = 0; L < TC_MAX_LEVELS; ++L)
+            tiles[L] = (L + 1 >= 2 && L + 1 < ctx->grids) ? ctx->lv[L + 1].tb_tile : 0;
+    return 0;
+}
+
 int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int32_t steps_each, int32_t* best,
                           double* ms_per_step)
 {

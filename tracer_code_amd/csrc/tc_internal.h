@@ -76,7 +76,8 @@ struct Level {
     float *err32[2] = {nullptr, nullptr};
     float *rhs32 = nullptr, *vis32 = nullptr, *visx32 = nullptr, *visz32 = nullptr, *a132 = nullptr, *visxz32 = nullptr;
     float *dts32 = nullptr, *lb32 = nullptr, *res32 = nullptr;
-    float* w13 = nullptr;  // 13 planes: dt-scaled stencil weights of the FP32 operator, per solve (levels >= 3)
+    float* w13 = nullptr;  // 13 planes: dt-scaled stencil weights of the FP32 operator, per solve
+    int tb_tile = 0;       // tile of the FP32 temporally blocked sweeps: 16, 32, or 0 (by level size)
     int cur32 = 0;
 };
 

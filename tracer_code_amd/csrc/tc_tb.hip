@@ -275,10 +275,11 @@ static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, co
 
 template <typename T>
 static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
-                      const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr)
+                      const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr,
+                      int tile = 0)
 {
     if constexpr (sizeof(T) == 4) {
-        if (tb_tile<T>(g) == 32 && K == 4) {
+        if ((tile ? tile : tb_tile<T>(g)) == 32 && K == 4) {
             tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
             return;
         }
@@ -324,7 +325,7 @@ void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, co
     SweepFactors<float> sf{};
     for (int s = 0; s < nk; ++s) { sf.temp[s] = (float)temp[s]; sf.cconst[s] = (float)cconst[s]; }
     tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg,
-                     tb_uses_w13(L) ? L.w13 : nullptr);
+                     tb_uses_w13(L) ? L.w13 : nullptr, L.tb_tile);
 }
 
 }  // namespace tc

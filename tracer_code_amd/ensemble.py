@@ -44,8 +44,9 @@ def draw_cases(n, seed=0):
     return cases
 
 
-def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4):
-    """(solvers, networks) initialised from each case's seeded inputs."""
+def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None):
+    """(solvers, networks) initialised from each case's seeded inputs; `tune`
+    (list of (sweeps, tiles) per case) applies autotuned settings."""
     solvers, nets = [], []
     for c in cases:
         params = config.make_params(**c)
@@ -61,8 +62,27 @@ def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4):
                 g.learn_attach(net)
                 nets.append(net)
             g.guess_enable(guess)
+            if tune:
+                sweeps, tiles = tune[len(solvers)]
+                g.set_dc_sweeps(sweeps)
+                for L, t in enumerate(tiles, start=1):
+                    if 2 <= L < params.grids and t:
+                        g.set_tb_tile(L, t)
         solvers.append(g)
     return solvers, nets
+
+
+def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70)):
+    """Per case: the sweep count and the tiles the autotuners pick (on a
+    scratch copy of the case, advanced by the tuning steps)."""
+    out = []
+    for c in cases:
+        (g,), _ = make_solvers([c], mode)
+        sweeps, _ = g.autotune_dc_sweeps(candidates, 1)
+        tiles = g.autotune_tb_tiles(1) if mode == "mixed" else [0] * 6
+        out.append((sweeps, tiles))
+        g.close()
+    return out
 
 
 def ensemble_step(solvers, steps, threads=0):
@@ -76,12 +96,12 @@ def ensemble_step(solvers, steps, threads=0):
         raise RuntimeError(f"tc_ensemble_step: status {st} in case {failed.value}: {msg.decode() if msg else ''}")
 
 
-def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, log=print):
+def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print):
     """Throughput of the cases run concurrently and one after another, from
     the same initial states (each after one untimed warm-up step)."""
     out = {}
     for concurrent in (False, True):
-        solvers, nets = make_solvers(cases, mode, guess, weights)
+        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune)
         ensemble_step(solvers, 1, threads)  # warm-up, concurrently in both arms
         for g in solvers:
             g.sync()
@@ -116,12 +136,17 @@ def main(argv=None):
     ap.add_argument("--mode", default="mixed", choices=["fp64", "mixed", "dc64"])
     ap.add_argument("--guess", default="plain", choices=["plain", "extrap", "learned"])
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="tune sweeps and tiles per case first (untimed)")
     a = ap.parse_args(argv)
     cases = draw_cases(a.cases, a.seed)
     w = None
     if a.guess == "learned":
         w, _ = learn.load_weights(os.path.join(os.path.dirname(__file__), "weights", "unet_cfg1_L4.npz"))
-    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, log=lambda m: print(m, file=sys.stderr, flush=True))
+    tune = autotune_cases(cases, a.mode) if (a.autotune and a.mode != "fp64") else None
+    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune,
+              log=lambda m: print(m, file=sys.stderr, flush=True))
+    if tune:
+        res["tuned"] = [{"sweeps": sw, "tiles": t} for sw, t in tune]
     res["cases"] = [{k: c[k] for k in ("nx", "nz", "RaT", "visT", "tpc")} for c in cases]
     print(json.dumps(res))
     return 0

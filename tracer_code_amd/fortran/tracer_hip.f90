@@ -198,6 +198,17 @@ module tracer_hip
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(9)
     end function
+    integer(c_int) function tc_set_tb_tile(ctx, level, tile) bind(C, name="tc_set_tb_tile")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: level, tile
+    end function
+    integer(c_int) function tc_autotune_tb_tiles(ctx, steps_each, tiles) bind(C, name="tc_autotune_tb_tiles")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: steps_each
+      integer(c_int32_t), intent(out) :: tiles(*)
+    end function
     integer(c_int) function tc_sort_tracers(ctx) bind(C, name="tc_sort_tracers")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

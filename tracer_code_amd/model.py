@@ -215,6 +215,16 @@ class Solver:
         keys = ("vrms", "Nu", "Tavg", "Cavg", "empty", "tmin", "tmax", "Nu_bot", "strain_max")
         return dict(zip(keys, out.tolist()))
 
+    def set_tb_tile(self, level, tile):
+        self._check(self.lib.tc_set_tb_tile(self.h, int(level), int(tile)), "set_tb_tile")
+
+    def autotune_tb_tiles(self, steps_each=1):
+        """Level-by-level choice of the temporally blocked tile (16 or 32);
+        returns the tile of each level (index 0 = level 1)."""
+        t = np.zeros(_abi.TC_MAX_LEVELS, dtype=np.int32)
+        self._check(self.lib.tc_autotune_tb_tiles(self.h, int(steps_each), t.ctypes.data), "autotune_tb_tiles")
+        return t.tolist()
+
     def sort_tracers(self):
         """Reorder the device tracers by node (get_tracers keeps the caller's order)."""
         self._check(self.lib.tc_sort_tracers(self.h), "sort_tracers")
