@@ -244,6 +244,22 @@ int tc_time_step(tc_ctx* ctx, int nsteps);
 int tc_derivative_eigenvalues(const double D1[5], const double D2[5], const double D3[5], const double D4[5],
                               double base[7]);
 
+/* ---- restart files ------------------------------------------------------ */
+/* The reference's restart files (print_restart main.f90:301-332, read in
+ * build_grid grid.f90:56-95): Fortran unformatted sequential, one record
+ * framed by 4-byte byte counts.  Trestart holds xg(-3:nx+3), zg(-3:nz+3),
+ * T (if has_T), Cnew(1:ntype_c,...) (if ntype_c > 0) and SF, the grid fields
+ * ghosted (-2:nx+2,-2:nz+2) as at the rest of this interface;
+ * restart_tracers holds rtr, str and Ttype (int32).  A record whose length
+ * does not match the sizes is rejected with TC_ERR_ARG.  Host only. */
+int tc_restart_read(const char* path, int32_t nx, int32_t nz, int32_t has_T, int32_t ntype_c, double* xg,
+                    double* zg, double* T, double* Cnew, double* SF);
+int tc_restart_write(const char* path, int32_t nx, int32_t nz, int32_t has_T, int32_t ntype_c, const double* xg,
+                     const double* zg, const double* T, const double* Cnew, const double* SF);
+int tc_restart_tracers_read(const char* path, int64_t ntr, double* rtr, double* str, int32_t* ttype);
+int tc_restart_tracers_write(const char* path, int64_t ntr, const double* rtr, const double* str,
+                             const int32_t* ttype);
+
 /* ---- diagnostics -------------------------------------------------------- */
 /* Replaces stats (basics.f90:236-324) on the device fields: recomputes the
  * strain rate invariant, then out = {vrms, Nu (top), Tavg, Cavg (type 1),

@@ -122,3 +122,37 @@ def test_time_steps_match(case):
         print(f"after {nsteps} steps: SF {e_sf:.2e} T {e_T:.2e} r {e_r:.2e} s {e_s:.2e}", flush=True)
         assert e_sf <= 1e-9 and e_T <= 1e-9
         assert e_r <= 1e-10 and e_s <= 1e-10
+
+
+def test_reference_restart_state():
+    """From the reference's own data: the Trestart shipped with it (200x200
+    T and SF, read by the restart codec).  The first multigrid solve,
+    warm-started from that SF, and one time step match the oracle with the
+    bars above (cfg0 physics, tracers 4 per node from the seeded generator)."""
+    import os
+
+    from tracer_code_amd import restart
+    f = restart.read_trestart(os.path.join(os.path.dirname(__file__), "golden", "Trestart_200x200"), 200, 200)
+    params, cfg = config.preset_params("cfg0", nx=200, nz=200, tpc=4)
+    assert (params.nx, params.nz) == (200, 200)
+    _, r, s, ty = ic.workload(cfg, seed=1)
+    print(f"[reference Trestart 200x200 grids={params.grids} ntr={len(r)}]", flush=True)
+    o = load_oracle_state(params, f["T"], r, s, ty, SF=f["SF"])
+    with Solver(params, len(r)) as g:
+        g.initialize(f["T"], r, s, ty, SF=f["SF"])
+        qg, resid, resref = g.multigrid()
+        assert o.multigrid() == 0
+        qo = int(o.scalar(oracle.ORS_Q))
+        e_sf = rel_err(g.get("SF"), o.plane(oracle.OR_SF))
+        print(f"solve: q gpu={qg} oracle={qo}, SF rel err {e_sf:.2e}", flush=True)
+        assert qg == qo and e_sf <= 1e-10
+        g.time_step(1)
+        assert o.time_step() == 0
+        e_sf = rel_err(g.get("SF"), o.plane(oracle.OR_SF))
+        e_T = rel_err(g.get("T"), o.plane(oracle.OR_T))
+        rg, sg = g.get_tracers("x")
+        e_r = float(np.max(np.abs(rg - o.tracers(oracle.OR_RTR))))
+        e_s = float(np.max(np.abs(sg - o.tracers(oracle.OR_STR))))
+        print(f"step: SF {e_sf:.2e} T {e_T:.2e} r {e_r:.2e} s {e_s:.2e}", flush=True)
+        assert e_sf <= 1e-9 and e_T <= 1e-9 and e_r <= 1e-10 and e_s <= 1e-10
+    o.close()

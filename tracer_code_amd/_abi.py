@@ -88,6 +88,12 @@ SIGNATURES = {
     "tc_set_dc_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_sort_tracers": (ctypes.c_int, [_VP]),
     "tc_diagnostics": (ctypes.c_int, [_VP, _VP]),
+    "tc_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, _VP, _VP, _VP, _VP, _VP]),
+    "tc_restart_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, _VP, _VP, _VP, _VP, _VP]),
+    "tc_restart_tracers_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
+    "tc_restart_tracers_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),

@@ -193,6 +193,33 @@ module tracer_hip
       integer(c_int32_t), value :: n, nsteps, threads
       integer(c_int32_t), intent(out) :: failed
     end function
+    integer(c_int) function tc_restart_read(path, nx, nz, has_T, ntype_c, xg, zg, T, Cnew, SF) &
+        bind(C, name="tc_restart_read")
+      import :: c_int, c_int32_t, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nx, nz, has_T, ntype_c
+      type(c_ptr), value :: xg, zg, T, Cnew, SF
+    end function
+    integer(c_int) function tc_restart_write(path, nx, nz, has_T, ntype_c, xg, zg, T, Cnew, SF) &
+        bind(C, name="tc_restart_write")
+      import :: c_int, c_int32_t, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: nx, nz, has_T, ntype_c
+      type(c_ptr), value :: xg, zg, T, Cnew, SF
+    end function
+    integer(c_int) function tc_restart_tracers_read(path, ntr, rtr, str, ttype) bind(C, name="tc_restart_tracers_read")
+      import :: c_int, c_int64_t, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: ntr
+      type(c_ptr), value :: rtr, str, ttype
+    end function
+    integer(c_int) function tc_restart_tracers_write(path, ntr, rtr, str, ttype) &
+        bind(C, name="tc_restart_tracers_write")
+      import :: c_int, c_int64_t, c_char, c_ptr
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: ntr
+      type(c_ptr), value :: rtr, str, ttype
+    end function
     integer(c_int) function tc_diagnostics(ctx, out) bind(C, name="tc_diagnostics")
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
