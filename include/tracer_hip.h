@@ -201,6 +201,8 @@ int tc_get_field(tc_ctx* ctx, int field, void* host);
 int tc_put_tracers(tc_ctx* ctx, const double* rtr, const double* str, const int32_t* ttype);
 /* which = TC_TR_X / _X0 / _X1 / _X2; returned in ID order. */
 int tc_get_tracers(tc_ctx* ctx, int which, double* r, double* s);
+/* Ttype of the tracers in ID order (equalize_tracers changes it). */
+int tc_get_tracer_types(tc_ctx* ctx, int32_t* ttype);
 /* Model time and dt (energy.f90 stability_RK4 writes dt). */
 int tc_set_time(tc_ctx* ctx, double time, double dt);
 
@@ -268,6 +270,41 @@ int tc_restart_tracers_write(const char* path, int64_t ntr, const double* rtr, c
  * integrate1Dx (basics.f90:341-390).  The sums run in another order than
  * the reference's loops (equal to rounding); the integers are exact. */
 int tc_diagnostics(tc_ctx* ctx, double out[9]);
+
+/* ---- tracer equalisation ------------------------------------------------ */
+/* Replaces equalize_tracers (composition.f90:83-580): tracers of nodes with
+ * more than their type's share (nexcess > 0) are moved to the nodes that lack
+ * the most (nexcess == minexcess), donors near the walls first, by the
+ * reference's rules (the 3x3 single-type test, the type choice toward the
+ * initial per-type totals Nideal, the lottery passes over shuffled row and
+ * column orders, the second loop, the wall clamps), then the tracers are
+ * binned again (tracers_to_corners without the C filter).  The reference
+ * draws its random numbers from one mzran stream per OpenMP thread
+ * (composition.f90:7-19), so its result depends on the thread count; here
+ * every draw is a counter-based hash of (seed, call, cell or tracer index),
+ * so the result is the same on any device and in any tracer order.
+ * Positions stay inside the receiving node's control volume.  Temperature
+ * tracers (tracer=1) are not supported: the temperature lives on the grid.
+ * Fields of the info record are the columns of equalize.dat
+ * (composition.f90:566-574). */
+typedef struct tc_equalize_info {
+    int32_t did_work;                     /* 0: every nexcess was 0, nothing moved */
+    int32_t passes;                       /* lottery passes of both loops */
+    int32_t tpc_type[TC_MAX_TYPES + 1];   /* tracers per node wanted for each type */
+    int32_t minexcess[TC_MAX_TYPES + 1];  /* after the moves; 2147483647 = no node */
+    int64_t deficit;                      /* sum of nexcess over nodes with nexcess <= 0, after */
+    int64_t moved;                        /* tracers repositioned */
+    int64_t nfree;                        /* donor tracers */
+    int64_t count[TC_MAX_TYPES + 1];      /* tracers of each type after (bookkeeping) */
+} tc_equalize_info;
+
+/* Cinit[ntype]: the initial mean composition of each type (Cavg at time 0,
+ * basics.f90:323); tpc: tracers per node for comp=0.  With in_time_step
+ * set, tc_time_step equalizes after stability_RK4 (energy.f90:133-135). */
+int tc_set_equalize(tc_ctx* ctx, int32_t tpc, const double* Cinit, uint64_t seed, int32_t in_time_step);
+/* One equalisation now of the current tracers; call k (from 0) of a context
+ * draws from stream (seed, k).  info may be NULL. */
+int tc_equalize_tracers(tc_ctx* ctx, tc_equalize_info* info);
 
 /* ---- ensembles (no counterpart in the reference) ------------------------ */
 /* Runs nsteps time steps (tc_time_step) on each of n independent contexts

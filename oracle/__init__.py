@@ -14,7 +14,7 @@ import subprocess
 
 import numpy as np
 
-from tracer_code_amd._abi import TcParams
+from tracer_code_amd._abi import TcEqualizeInfo, TcParams
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -107,6 +107,17 @@ def lib():
                      ("or_initialize_state", [vp])]:
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i
+        u64 = ctypes.c_uint64
+        L.or_equalize_tracers.restype = i
+        L.or_equalize_tracers.argtypes = [vp, i, vp, u64, u64, ctypes.POINTER(TcEqualizeInfo)]
+        L.or_set_equalize.restype = None
+        L.or_set_equalize.argtypes = [vp, i, i, vp, u64]
+        L.or_eq_uniform.restype = d
+        L.or_eq_uniform.argtypes = [u64, u64, ctypes.c_uint, u64, u64]
+        L.or_eq_perm40.restype = u64
+        L.or_eq_perm40.argtypes = [u64, u64, u64]
+        L.or_eq_shuffle.restype = None
+        L.or_eq_shuffle.argtypes = [vp, i, u64, u64, ctypes.c_uint, u64]
         _lib = L
     return _lib
 
@@ -191,6 +202,17 @@ class Oracle:
 
     def set_scalar(self, which, v):
         self.L.or_set_scalar(self.h, which, float(v))
+
+    def equalize_tracers(self, tpc, Cinit, seed, call):
+        """equalize_tracers in its counter-based form; returns (status, info)."""
+        info = TcEqualizeInfo()
+        c = np.ascontiguousarray(np.atleast_1d(np.asarray(Cinit, dtype=np.float64)))
+        st = self.L.or_equalize_tracers(self.h, int(tpc), _p(c), int(seed), int(call), ctypes.byref(info))
+        return st, info
+
+    def set_equalize(self, on, tpc, Cinit, seed):
+        c = np.ascontiguousarray(np.atleast_1d(np.asarray(Cinit, dtype=np.float64)))
+        self.L.or_set_equalize(self.h, int(on), int(tpc), _p(c), int(seed))
 
     def __getattr__(self, name):
         fn = getattr(self.L, "or_" + name)

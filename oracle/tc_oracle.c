@@ -70,6 +70,10 @@ struct or_state {
     int tstep, q, n_eta, n_temp, n_comp[TC_MAX_TYPES + 1], status, smooth_vis;
     int Nvis_store, nn_store;
     int64_t cycles_total, vis_iters, c_iters, t_iters;
+    /* equalize_tracers in the time step */
+    int eq_on, eq_tpc;
+    double eq_Cinit[TC_MAX_TYPES];   /* type 1..ntype at [0..ntype-1] */
+    uint64_t eq_seed, eq_calls;
 };
 
 #define IX(s, i, k) ((size_t)((k) + SPAN) * (size_t)(s)->ld + (size_t)((i) + SPAN))
@@ -2022,6 +2026,10 @@ int or_energy_time(or_state* s)
 {
     int st;
     s->dt = or_stability_RK4(s);
+    if (s->eq_on) {
+        tc_equalize_info info;
+        if ((st = or_equalize_tracers(s, s->eq_tpc, s->eq_Cinit, s->eq_seed, s->eq_calls++, &info))) return st;
+    }
     const double dt = s->dt;
     const int torder = s->p.torder;
     for (int t = 1; t <= s->ntype; ++t) memcpy(s->C0[t], s->Cnew[t], s->plane * sizeof(double));
@@ -2165,4 +2173,341 @@ void or_stats(or_state* s, double out[9])
     free(Tz_surf); free(Tz_bot); free(a); free(b);
     out[0] = vrms; out[1] = Nu; out[2] = Tavg; out[3] = Cavg;
     out[4] = empty; out[5] = tmin; out[6] = tmax; out[7] = Nu_bot; out[8] = strain_max;
+}
+
+/* ------------------------------------------------------------------------ */
+/* equalize_tracers (composition.f90:83-580)                                 */
+/* ------------------------------------------------------------------------ */
+/* The reference draws from one Marsaglia mzran stream per OpenMP thread
+ * (composition.f90:7-19, seeds basics.f90:198-213) and splits its loops by
+ * thread, so its result depends on the thread count.  This restatement keeps
+ * every rule of the routine and replaces each draw by a counter-based one:
+ * u = hash(seed, call, stream, a, b) with the SplitMix64 finalizer (Steele,
+ * Lea and Flood, "Fast splittable pseudorandom number generators", 2014) as
+ * the mixing function.  Streams: per-node tracer shuffles (a = node), donor
+ * placement (a = tracer index), row and column visit orders (a = pass).  The
+ * two donor shuffles (composition.f90:307-315, 353-359) become an order by a
+ * keyed bijection of the tracer index; the per-thread donor lists become one
+ * list per type (a single thread of the reference).  The node index is
+ * c = k*(nx+1) + i, the tracer index the caller's (0-based). */
+
+#define EQ_GOLD 0x9e3779b97f4a7c15ULL
+enum { EQ_S_NODE = 1, EQ_S_PLACE = 2, EQ_S_ROWS = 3, EQ_S_COLS = 4 };
+
+static inline uint64_t eq_mix(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ULL;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebULL;
+    x ^= x >> 31;
+    return x;
+}
+static inline uint64_t eq_key(uint64_t seed, uint64_t call) { return eq_mix(seed + EQ_GOLD * (call + 1)); }
+static inline double eq_u(uint64_t key, unsigned stream, uint64_t a, uint64_t b)
+{
+    uint64_t x = eq_mix(key ^ ((uint64_t)stream << 48) ^ a);
+    x = eq_mix(x + EQ_GOLD * (b + 1));
+    return (double)(x >> 11) * 0x1.0p-53;
+}
+/* a bijection of [0, 2^40): xor, odd multiplications and right xorshifts,
+ * each invertible modulo 2^40 */
+static inline uint64_t eq_p40(uint64_t key, uint64_t id)
+{
+    const uint64_t M = (1ULL << 40) - 1;
+    uint64_t x = (id ^ key) & M;
+    x = (x * 0x9e3779b97f4a7c15ULL) & M;
+    x ^= x >> 20;
+    x = (x * 0xbf58476d1ce4e5b9ULL) & M;
+    x ^= x >> 17;
+    x = (x * 0x94d049bb133111ebULL) & M;
+    x ^= x >> 21;
+    return x;
+}
+/* shuffle_serial (composition.f90:22-33): element j swaps with a uniformly
+ * drawn later element, never itself (Sattolo's algorithm: one cycle) */
+static void eq_shuffle(int32_t* a, int n, uint64_t key, unsigned stream, uint64_t id)
+{
+    for (int j = 0; j + 1 < n; ++j) {
+        const double u = eq_u(key, stream, id, (uint64_t)j);
+        const int r = (int)floor(u * (double)(n - 1 - j)) + j + 1;
+        const int32_t t = a[j];
+        a[j] = a[r];
+        a[r] = t;
+    }
+}
+
+double or_eq_uniform(uint64_t seed, uint64_t call, unsigned stream, uint64_t a, uint64_t b)
+{
+    return eq_u(eq_key(seed, call), stream, a, b);
+}
+uint64_t or_eq_perm40(uint64_t seed, uint64_t call, uint64_t id) { return eq_p40(eq_key(seed, call), id); }
+void or_eq_shuffle(int32_t* a, int n, uint64_t seed, uint64_t call, unsigned stream, uint64_t id)
+{
+    eq_shuffle(a, n, eq_key(seed, call), stream, id);
+}
+
+void or_set_equalize(or_state* s, int on, int tpc, const double* Cinit, uint64_t seed)
+{
+    s->eq_on = on;
+    s->eq_tpc = tpc;
+    for (int t = 0; t < s->ntype; ++t) s->eq_Cinit[t] = Cinit ? Cinit[t] : 0.0;
+    s->eq_seed = seed;
+    s->eq_calls = 0;
+}
+
+/* minexcess of the lottery loop (composition.f90:387-423, mode 0), of the
+ * second loop (:484-500, mode 1: its enriched types test tpc_type(0)), and
+ * after the moves (:547-554, mode 2: no tpc_type test).  Fortran's minval of
+ * an empty selection is huge(0). */
+static void eq_minexcess(const or_state* s, int nt, const int* ne, const int* tpc_type, int mode, int* mx)
+{
+    const int nx = s->nx, nz = s->nz;
+    for (int p = 0; p <= nt; ++p) {
+        const int gate = mode == 2 ? 1 : (mode == 1 ? tpc_type[0] > 0 : tpc_type[p] > 0);
+        const int gate0 = mode == 2 ? 1 : tpc_type[0] > 0;
+        if (!(p == 0 ? gate0 : gate)) { mx[p] = 0; continue; }
+        int m = 2147483647;
+        for (int k = 0; k <= nz; ++k)
+            for (int i = 0; i <= nx; ++i) {
+                const int v = ne[k * (nx + 1) + i];
+                if (v > 0) continue;
+                const size_t c = IX(s, i, k);
+                int sel;
+                if (s->p.comp == 0) sel = 1;
+                else if (p == 0) {
+                    double acc = 0.0;
+                    for (int t = 1; t <= s->ntype; ++t) acc += s->Cnew[t][c];
+                    sel = acc == 0.0;
+                } else sel = s->Cnew[p][c] == 1.0;
+                if (sel && v < m) m = v;
+            }
+        mx[p] = m;
+    }
+}
+
+/* first type of largest count at node c (maxloc, composition.f90:438) */
+static int eq_type_save(const or_state* s, int nt, size_t c)
+{
+    int best = 0;
+    for (int t = 1; t <= nt; ++t)
+        if (s->count[t][c] > s->count[best][c]) best = t;
+    return best;
+}
+
+typedef struct { uint64_t key; int32_t id; } eq_donor;
+static int eq_cmp(const void* a, const void* b)
+{
+    const uint64_t x = ((const eq_donor*)a)->key, y = ((const eq_donor*)b)->key;
+    return x < y ? -1 : (x > y);
+}
+
+int or_equalize_tracers(or_state* s, int tpc, const double* Cinit, uint64_t seed, uint64_t call,
+                        tc_equalize_info* info)
+{
+    const int nx = s->nx, nz = s->nz, nt = s->p.comp == 1 ? s->ntype : 0;
+    const int64_t ntr = s->ntr;
+    const size_t V = (size_t)(nx + 1) * (size_t)(nz + 1);
+    const uint64_t key = eq_key(seed, call);
+    memset(info, 0, sizeof(*info));
+    /* Nideal (composition.f90:103-111) */
+    int64_t Nideal[TC_MAX_TYPES + 1] = {0};
+    if (s->p.comp == 1) {
+        int64_t acc = 0;
+        for (int t = 1; t <= nt; ++t) { Nideal[t] = llround(Cinit[t - 1] * (double)ntr); acc += Nideal[t]; }
+        Nideal[0] = ntr - acc;
+    } else Nideal[0] = ntr;
+    /* tpc_type (composition.f90:114-140) */
+    int tpc_type[TC_MAX_TYPES + 1] = {0};
+    if (s->p.comp == 0) tpc_type[0] = tpc;
+    else
+        for (int t = 0; t <= nt; ++t) {
+            int64_t n = 0, sum = 0;
+            for (int k = 0; k <= nz; ++k)
+                for (int i = 0; i <= nx; ++i) {
+                    const size_t c = IX(s, i, k);
+                    int sel;
+                    if (t == 0) {
+                        double acc = 0.0;
+                        for (int q = 1; q <= s->ntype; ++q) acc += s->Cnew[q][c];
+                        sel = acc == 0.0;
+                    } else sel = s->Cnew[t][c] == 1.0;
+                    if (sel) { n++; sum += s->count[t][c]; }
+                }
+            tpc_type[t] = n > 0 ? (int)lround((double)sum / (double)n) : 0;
+        }
+    for (int t = 0; t <= nt; ++t) info->tpc_type[t] = tpc_type[t];
+    /* nexcess (composition.f90:118-161) */
+    int* ne = (int*)calloc(V, sizeof(int));
+    int64_t nfree = 0;
+    int any = 0;
+    for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i <= nx; ++i) {
+            const size_t c = IX(s, i, k);
+            const int n0 = s->nempty[c];
+            int v = 0;
+            if (s->p.comp == 0) {
+                if (n0 > 0) v = n0 - tpc;
+            } else if (n0 > 0) {
+                for (int t = 0; t <= nt; ++t) {
+                    int all = 1;
+                    for (int dk = -1; dk <= 1 && all; ++dk)
+                        for (int di = -1; di <= 1; ++di)
+                            if (s->count[t][IX(s, i + di, k + dk)] != s->nempty[IX(s, i + di, k + dk)]) { all = 0; break; }
+                    if (all) { v = n0 - tpc_type[t]; break; }
+                }
+            }
+            ne[k * (nx + 1) + i] = v;
+            if (v > 0) nfree += v;
+            if (v != 0) any = 1;
+        }
+    info->nfree = nfree;
+    if (!any) { free(ne); return 0; }   /* composition.f90:166-171 */
+    info->did_work = 1;
+    /* tracers of each node in index order (composition.f90:195-205), then
+     * shuffled (:209-217) */
+    int32_t* cnt = (int32_t*)calloc(V + 1, sizeof(int32_t));
+    int32_t* list = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ntr > 0 ? ntr : 1));
+    int32_t* node = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ntr > 0 ? ntr : 1));
+    for (int64_t id = 0; id < ntr; ++id) {
+        const double r = s->rtr[id], sv = s->str[id];
+        if (!(r > 0.0 && r < s->p.aspect && sv > 0.0 && sv < 1.0)) {   /* composition.f90:709-712 */
+            free(ne); free(cnt); free(list); free(node);
+            s->status = TC_ERR_TRACER_DOMAIN;
+            return s->status;
+        }
+        const int i = (int)lround(r / s->dr), k = (int)lround(sv / s->ds);
+        node[id] = k * (nx + 1) + i;
+        cnt[node[id] + 1]++;
+    }
+    for (size_t c = 0; c < V; ++c) cnt[c + 1] += cnt[c];
+    {
+        int32_t* fill = (int32_t*)malloc(sizeof(int32_t) * V);
+        memcpy(fill, cnt, sizeof(int32_t) * V);
+        for (int64_t id = 0; id < ntr; ++id) list[fill[node[id]]++] = (int32_t)id;
+        free(fill);
+    }
+    /* donors (composition.f90:229-335): wall nodes give their tracers within
+     * dist of a wall first (segment 0), then the rest (segment 1); interior
+     * nodes their first nexcess tracers */
+    eq_donor* dn = (eq_donor*)malloc(sizeof(eq_donor) * (size_t)(nfree > 0 ? nfree : 1));
+    int64_t nd = 0;
+    const double dr_ = s->dist_r, ds_ = s->dist_s, asp = s->p.aspect;
+    for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i <= nx; ++i) {
+            const size_t c = (size_t)k * (nx + 1) + i;
+            int want = ne[c];
+            if (want <= 0) continue;
+            int32_t* L = list + cnt[c];
+            const int n = cnt[c + 1] - cnt[c];
+            eq_shuffle(L, n, key, EQ_S_NODE, c);
+            const int edge = i == 0 || i == nx || k == 0 || k == nz;
+            for (int pass = edge ? 0 : 1; pass < 2 && want > 0; ++pass)
+                for (int q = 0; q < n && want > 0; ++q) {
+                    const int32_t id = L[q];
+                    const double r = s->rtr[id], sv = s->str[id];
+                    const int near = sv < ds_ || sv > 1.0 - ds_ || r < dr_ || r > asp - dr_;
+                    if (edge && near != (pass == 0)) continue;
+                    const uint64_t bucket = (uint64_t)s->Ttype[id] * 2 + (uint64_t)pass;
+                    dn[nd].key = (bucket << 40) | eq_p40(key, (uint64_t)id);
+                    dn[nd].id = id;
+                    nd++;
+                    want--;
+                }
+        }
+    free(cnt); free(list); free(node);
+    if (nd != nfree) {   /* composition.f90:344-347 */
+        free(ne); free(dn);
+        s->status = TC_ERR_ARG;
+        return TC_ERR_ARG;
+    }
+    qsort(dn, (size_t)nd, sizeof(eq_donor), eq_cmp);
+    int64_t nfree_t[TC_MAX_TYPES + 1] = {0}, off[TC_MAX_TYPES + 2] = {0};
+    for (int64_t j = 0; j < nd; ++j) nfree_t[(dn[j].key >> 40) / 2]++;
+    for (int t = 0; t <= nt; ++t) off[t + 1] = off[t] + nfree_t[t];
+    /* lottery (composition.f90:362-463), then the second loop (:466-541) */
+    int32_t* rows = (int32_t*)malloc(sizeof(int32_t) * (nx + 1));
+    int32_t* cols = (int32_t*)malloc(sizeof(int32_t) * (nz + 1));
+    for (int i = 0; i <= nx; ++i) rows[i] = i;
+    for (int k = 0; k <= nz; ++k) cols[k] = k;
+    int64_t used[TC_MAX_TYPES + 1] = {0}, tcount[TC_MAX_TYPES + 1];
+    int mx[TC_MAX_TYPES + 1];
+    uint64_t pass = 0;
+    int escape = 0;
+#define EQ_TCOUNT()                                                                  \
+    do {                                                                             \
+        escape = 1;                                                                  \
+        for (int t = 0; t <= nt; ++t) {                                              \
+            int64_t a = 0;                                                           \
+            for (int k = 0; k <= nz; ++k)                                            \
+                for (int i = 0; i <= nx; ++i) a += s->count[t][IX(s, i, k)];         \
+            tcount[t] = a;                                                           \
+            if (a != Nideal[t]) escape = 0;                                          \
+        }                                                                            \
+    } while (0)
+    EQ_TCOUNT();
+    for (int loop = 0; loop < 2; ++loop)
+        for (;;) {
+            if (loop == 0 && !escape) EQ_TCOUNT();
+            eq_minexcess(s, nt, ne, tpc_type, loop, mx);
+            int allz = 1;
+            int64_t su = 0;
+            for (int t = 0; t <= nt; ++t) { if (mx[t] != 0) allz = 0; su += used[t]; }
+            if (allz || su >= nfree) break;
+            eq_shuffle(rows, nx + 1, key, EQ_S_ROWS, pass);
+            eq_shuffle(cols, nz + 1, key, EQ_S_COLS, pass);
+            pass++;
+            int tg = 0;
+            if (loop == 0 && !escape)
+                for (int p = 0; p <= nt; ++p)
+                    if (tcount[p] > Nideal[p]) { tg = p; break; }
+            int moved = 0;
+            for (int ii = 0; ii <= nx; ++ii)
+                for (int kk = 0; kk <= nz; ++kk) {
+                    const int i = rows[ii], k = cols[kk];
+                    const size_t c = IX(s, i, k), cc = (size_t)k * (nx + 1) + i;
+                    const int ts = eq_type_save(s, nt, c);
+                    const int tt = (loop == 0 && !escape) ? tg : ts;
+                    if (!(ne[cc] == mx[ts] && used[tt] < nfree_t[tt] && ne[cc] < 0)) continue;
+                    const int32_t id = dn[off[tt] + used[tt]].id;
+                    used[tt]++;
+                    moved = 1;
+                    const double pr = (eq_u(key, EQ_S_PLACE, (uint64_t)id, 0) - 0.5) * s->dr;
+                    const double ps = (eq_u(key, EQ_S_PLACE, (uint64_t)id, 1) - 0.5) * s->ds;
+                    double r = s->dr * (double)i + pr, sv = s->ds * (double)k + ps;
+                    if (r <= 0.0) r = dr_;
+                    if (r >= asp) r = asp - dr_;
+                    if (sv <= 0.0) sv = ds_;
+                    if (sv >= 1.0) sv = 1.0 - ds_;
+                    s->rtr[id] = r;
+                    s->str[id] = sv;
+                    if (s->p.comp == 1) s->Ttype[id] = ts;
+                    if (loop == 0) {
+                        s->count[tt][c] -= 1;
+                        s->count[s->Ttype[id]][c] += 1;
+                    }
+                    ne[cc] += 1;
+                }
+            if (!moved) break;
+        }
+#undef EQ_TCOUNT
+    info->passes = (int32_t)pass;
+    eq_minexcess(s, nt, ne, tpc_type, 2, mx);
+    for (int t = 0; t <= nt; ++t) {
+        info->minexcess[t] = mx[t];
+        info->moved += used[t];
+        int64_t a = 0;
+        for (int k = 0; k <= nz; ++k)
+            for (int i = 0; i <= nx; ++i) a += s->count[t][IX(s, i, k)];
+        info->count[t] = a;
+    }
+    for (size_t c = 0; c < V; ++c)
+        if (ne[c] <= 0) info->deficit += ne[c];
+    free(rows); free(cols); free(ne); free(dn);
+    /* tracers_to_corners without the C filter (composition.f90:576-578) */
+    const int sv_save = s->smooth_vis;
+    s->smooth_vis = 0;
+    const int st = or_tracers_to_corners(s, TC_TR_X);
+    s->smooth_vis = sv_save;
+    return st;
 }

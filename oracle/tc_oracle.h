@@ -113,6 +113,16 @@ int  or_initialize_state(or_state* s);
 /* diagnostics (basics.f90:236-324): out[0]=vrms, [1]=Nu, [2]=Tavg, [3]=Cavg(1),
  * [4]=empty_cells, [5]=tracer_min, [6]=tracer_max, [7]=Nu_bot, [8]=strain_max */
 void or_stats(or_state* s, double out[9]);
+/* equalize_tracers (composition.f90:83-580) in its counter-based form */
+int  or_equalize_tracers(or_state* s, int tpc, const double* Cinit, uint64_t seed, uint64_t call,
+                         tc_equalize_info* info);
+/* with on set, or_energy_time equalizes after stability_RK4 (energy.f90:133-135),
+ * call k of the run drawing from stream (seed, k) */
+void or_set_equalize(or_state* s, int on, int tpc, const double* Cinit, uint64_t seed);
+/* the pieces of the counter-based random stream, for the tests */
+double or_eq_uniform(uint64_t seed, uint64_t call, unsigned stream, uint64_t a, uint64_t b);
+uint64_t or_eq_perm40(uint64_t seed, uint64_t call, uint64_t id);
+void or_eq_shuffle(int32_t* a, int n, uint64_t seed, uint64_t call, unsigned stream, uint64_t id);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,15 @@ class TcStats(ctypes.Structure):
     ]
 
 
+class TcEqualizeInfo(ctypes.Structure):
+    _fields_ = [
+        ("did_work", ctypes.c_int32), ("passes", ctypes.c_int32),
+        ("tpc_type", ctypes.c_int32 * (TC_MAX_TYPES + 1)), ("minexcess", ctypes.c_int32 * (TC_MAX_TYPES + 1)),
+        ("deficit", ctypes.c_int64), ("moved", ctypes.c_int64), ("nfree", ctypes.c_int64),
+        ("count", ctypes.c_int64 * (TC_MAX_TYPES + 1)),
+    ]
+
+
 # every C-ABI entry point declared in include/tracer_hip.h: name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 _P = ctypes.POINTER
@@ -94,6 +103,8 @@ SIGNATURES = {
                                         ctypes.c_int32, _VP, _VP, _VP, _VP, _VP]),
     "tc_restart_tracers_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
     "tc_restart_tracers_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
+    "tc_set_equalize": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, ctypes.c_uint64, ctypes.c_int32]),
+    "tc_equalize_tracers": (ctypes.c_int, [_VP, _P(TcEqualizeInfo)]),
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
@@ -105,6 +116,7 @@ SIGNATURES = {
     "tc_get_field": (ctypes.c_int, [_VP, ctypes.c_int, _VP]),
     "tc_put_tracers": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "tc_get_tracers": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP]),
+    "tc_get_tracer_types": (ctypes.c_int, [_VP, _VP]),
     "tc_set_time": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_double]),
     "tc_multigrid": (ctypes.c_int, [_VP, _P(ctypes.c_int32), _P(ctypes.c_double), _P(ctypes.c_double)]),
     "tc_compute_velocities": (ctypes.c_int, [_VP]),

@@ -1212,6 +1212,222 @@ int copy_C(tc_ctx* ctx, double* const* dst, double* const* src)
     return 0;
 }
 
+// buffers of equalize (tracer- and node-sized), allocated at the first call
+int eq_alloc(tc_ctx* ctx)
+{
+    if (ctx->eq_red) return 0;
+    const Grid& g = ctx->g;
+    const int64_t n = std::max<int64_t>(ctx->ntr, 1);
+    const size_t V = (size_t)(g.nx + 1) * (size_t)(g.nz + 1);
+    TC_HIP(hipMalloc((void**)&ctx->eq_ne, V * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_start, V * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_end, V * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_rows, (size_t)(g.nx + 1) * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_cols, (size_t)(g.nz + 1) * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_bcnt, (size_t)eq_pass_blocks(g) * EQ_MAXT * sizeof(int32_t)));
+    for (int b = 0; b < 2; ++b) {
+        TC_HIP(hipMalloc((void**)&ctx->eq_k[b], (size_t)n * sizeof(uint64_t)));
+        TC_HIP(hipMalloc((void**)&ctx->eq_v[b], (size_t)n * sizeof(uint32_t)));
+    }
+    size_t need = 0;
+    TC_TRY(eq_sort(ctx->stream, nullptr, &need, ctx->eq_k[0], ctx->eq_k[1], ctx->eq_v[0], ctx->eq_v[1], n, 64,
+                   nullptr));
+    TC_HIP(hipMalloc(&ctx->eq_tmp, std::max<size_t>(need, 1)));
+    ctx->eq_tmp_bytes = std::max<size_t>(need, 1);
+    TC_HIP(hipHostMalloc((void**)&ctx->eq_hred, sizeof(EqRed), hipHostMallocDefault));
+    TC_HIP(hipMalloc((void**)&ctx->eq_red, sizeof(EqRed)));
+    return 0;
+}
+
+// sort scratch of n items on `bits` key bits, grown when too small
+int eq_temp(tc_ctx* ctx, int64_t n, int bits)
+{
+    size_t need = 0;
+    TC_TRY(eq_sort(ctx->stream, nullptr, &need, ctx->eq_k[0], ctx->eq_k[1], ctx->eq_v[0], ctx->eq_v[1], n, bits,
+                   nullptr));
+    if (need > ctx->eq_tmp_bytes) {
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        TC_HIP(hipFree(ctx->eq_tmp));
+        ctx->eq_tmp = nullptr;
+        TC_HIP(hipMalloc(&ctx->eq_tmp, need));
+        ctx->eq_tmp_bytes = need;
+    }
+    return 0;
+}
+
+int eq_fetch(tc_ctx* ctx)
+{
+    TC_HIP(hipMemcpyAsync(ctx->eq_hred, ctx->eq_red, sizeof(EqRed), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// equalize_tracers (composition.f90:83-580) in its counter-based form: the
+// host runs the reference's loop control on per-pass reductions, the
+// kernels of tc_equalize.hip do the per-node and per-tracer work
+int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
+{
+    const tc_params& P = ctx->p;
+    const Grid& g = ctx->g;
+    const hipStream_t st = ctx->stream;
+    const int nt = P.comp == 1 ? P.ntype : 0;
+    const int64_t n = ctx->ntr;
+    tc_equalize_info info;
+    std::memset(&info, 0, sizeof info);
+    const uint64_t key = eq_stream_key(ctx->eq_seed, ctx->eq_calls++);
+    if (n >= ((int64_t)1 << 31)) {
+        ctx->err = "equalize supports fewer than 2^31 tracers";
+        return TC_ERR_UNSUPPORTED;
+    }
+    if (std::max(g.nx, g.nz) + 1 > 16384) {
+        ctx->err = "equalize supports at most 16384 nodes per row or column";
+        return TC_ERR_UNSUPPORTED;
+    }
+    TC_TRY(eq_alloc(ctx));
+    EqRed& h = *ctx->eq_hred;
+    CntArr cnt{};
+    for (int t = 0; t <= P.ntype; ++t) cnt.v[t] = ctx->count[t];
+    PtrArr cn{};
+    for (int t = 0; t < P.ntype; ++t) cn.v[t] = ctx->Cnew[t];
+    const double dr = ctx->lv[1].dr, ds = ctx->lv[1].ds;
+    const double dist_r = 1e-4 * dr, dist_s = 1e-4 * ds;  // dist_factor*dr (main.f90:58, basics.f90:159-160)
+    // Nideal (composition.f90:103-111)
+    int64_t Nideal[EQ_MAXT] = {};
+    if (P.comp == 1) {
+        int64_t acc = 0;
+        for (int t = 1; t <= nt; ++t) {
+            Nideal[t] = std::llround(ctx->eq_Cinit[t - 1] * (double)n);
+            acc += Nideal[t];
+        }
+        Nideal[0] = n - acc;
+    } else {
+        Nideal[0] = n;
+    }
+    // tpc_type (composition.f90:114-140) and nexcess (:118-161)
+    EqTypes tpc{};
+    launch_eq_clear(st, ctx->eq_red, true);
+    if (P.comp == 0) {
+        tpc.v[0] = ctx->eq_tpc;
+    } else {
+        launch_eq_tpc_stats(st, cnt, cn, g, P.ntype, P.comp, ctx->eq_red);
+        TC_TRY(eq_fetch(ctx));
+        for (int t = 0; t <= nt; ++t)
+            tpc.v[t] = h.sel_n[t] ? (int32_t)std::llround((double)h.sel_sum[t] / (double)h.sel_n[t]) : 0;
+    }
+    for (int t = 0; t <= nt; ++t) info.tpc_type[t] = tpc.v[t];
+    launch_eq_nexcess(st, cnt, ctx->nempty, g, nt, P.comp, tpc, ctx->eq_ne, ctx->eq_red);
+    TC_TRY(eq_fetch(ctx));
+    const int64_t nfree = (int64_t)h.nfree;
+    info.nfree = nfree;
+    if (!h.any) {   // composition.f90:166-171
+        if (out) *out = info;
+        return 0;
+    }
+    info.did_work = 1;
+    // tracers of each node in index order, then the donors
+    launch_eq_node_lists(st, ctx->rtr, ctx->str, ctx->tr_id, n, g, dr, ds, P.aspect, ctx->eq_k[0], ctx->eq_v[0],
+                         ctx->eq_start, ctx->eq_end, ctx->eq_red);
+    TC_TRY(eq_fetch(ctx));
+    if (h.bad) {
+        ctx->err = "Tracer advected outside of domain - stopping";
+        return TC_ERR_TRACER_DOMAIN;
+    }
+    const uint64_t V = (uint64_t)(g.nx + 1) * (uint64_t)(g.nz + 1);
+    int cbits = 1;
+    while (cbits < 32 && ((uint64_t)1 << cbits) < V) ++cbits;
+    int cur = 0;
+    TC_TRY(eq_temp(ctx, n, 32 + cbits));
+    size_t bytes = ctx->eq_tmp_bytes;
+    TC_TRY(eq_sort(st, ctx->eq_tmp, &bytes, ctx->eq_k[0], ctx->eq_k[1], ctx->eq_v[0], ctx->eq_v[1], n, 32 + cbits,
+                   &cur));
+    launch_eq_bounds(st, ctx->eq_k[cur], n, ctx->eq_start, ctx->eq_end);
+    const int o = 1 - cur;
+    launch_eq_select(st, ctx->eq_ne, ctx->eq_start, ctx->eq_end, ctx->eq_v[cur], ctx->rtr, ctx->str, ctx->ttype,
+                     ctx->tr_id, g, dist_r, dist_s, P.aspect, key, ctx->eq_k[o], ctx->eq_v[o], ctx->eq_red);
+    TC_TRY(eq_fetch(ctx));
+    if ((int64_t)h.found != nfree) {   // composition.f90:344-347
+        ctx->err = "equalize: donor tracers found inconsistent with nfree (counts out of date?)";
+        return TC_ERR_ARG;
+    }
+    int dsel = 0;
+    TC_TRY(eq_temp(ctx, nfree, 44));
+    bytes = ctx->eq_tmp_bytes;
+    TC_TRY(eq_sort(st, ctx->eq_tmp, &bytes, ctx->eq_k[o], ctx->eq_k[cur], ctx->eq_v[o], ctx->eq_v[cur], nfree, 44,
+                   &dsel));
+    const uint32_t* donors = dsel == 0 ? ctx->eq_v[o] : ctx->eq_v[cur];
+    int64_t nfree_t[EQ_MAXT] = {}, off[EQ_MAXT] = {}, used[EQ_MAXT] = {};
+    int64_t acc = 0;
+    for (int t = 0; t <= nt; ++t) {
+        nfree_t[t] = (int64_t)h.dcount[t];
+        off[t] = acc;
+        acc += nfree_t[t];
+    }
+    // lottery (composition.f90:362-463), then the second loop (:466-541)
+    launch_eq_iota(st, ctx->eq_rows, g.nx + 1, ctx->eq_cols, g.nz + 1);
+    launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_red);
+    TC_TRY(eq_fetch(ctx));
+    auto balanced = [&]() {
+        for (int t = 0; t <= nt; ++t)
+            if ((int64_t)h.tcount[t] != Nideal[t]) return false;
+        return true;
+    };
+    bool escape = balanced();
+    int loop = 0;
+    uint64_t pass = 0;
+    for (;;) {
+        if (loop == 0 && !escape) escape = balanced();
+        EqTypes mx{};
+        bool allz = true;
+        int64_t su = 0;
+        for (int p = 0; p <= nt; ++p) {
+            const bool gate = loop == 0 ? tpc.v[p] > 0 : tpc.v[0] > 0;
+            mx.v[p] = gate ? h.raw_min[p] : 0;
+            if (mx.v[p] != 0) allz = false;
+            su += used[p];
+        }
+        if (allz || su >= nfree) {
+            if (loop == 0) { loop = 1; continue; }
+            break;
+        }
+        launch_eq_shuffle_orders(st, ctx->eq_rows, g.nx + 1, ctx->eq_cols, g.nz + 1, key, pass);
+        pass++;
+        int tg = -1;
+        if (loop == 0 && !escape)
+            for (int p = 0; p <= nt; ++p)
+                if ((int64_t)h.tcount[p] > Nideal[p]) { tg = p; break; }
+        EqOff first{}, avail{};
+        for (int t = 0; t <= nt; ++t) {
+            first.v[t] = off[t] + used[t];
+            avail.v[t] = nfree_t[t] - used[t];
+        }
+        launch_eq_pass(st, ctx->eq_rows, ctx->eq_cols, ctx->eq_ne, cnt, g, nt, tg, mx, ctx->eq_bcnt, donors, first,
+                       avail, ctx->rtr, ctx->str, ctx->ttype, ctx->tr_id, key, dr, ds, dist_r, dist_s, P.aspect, P.comp,
+                       loop == 0 ? 1 : 0, ctx->eq_red);
+        launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_red);
+        TC_TRY(eq_fetch(ctx));
+        int64_t moved = 0;
+        for (int t = 0; t <= nt; ++t) {
+            const int64_t q = std::min<int64_t>((int64_t)h.qtot[t], avail.v[t]);
+            used[t] += q;
+            moved += q;
+        }
+        if (!moved) {
+            if (loop == 0) { loop = 1; continue; }
+            break;
+        }
+    }
+    info.passes = (int32_t)pass;
+    for (int t = 0; t <= nt; ++t) {
+        info.minexcess[t] = h.raw_min[t];
+        info.count[t] = (int64_t)h.tcount[t];
+        info.moved += used[t];
+    }
+    info.deficit = h.deficit;
+    if (out) *out = info;
+    // tracers_to_corners without the C filter (composition.f90:576-578)
+    return tracers_to_corners(ctx, d, TC_TR_X, 0, nullptr);
+}
+
 // energy_time (energy.f90:117-453) for tracer=0, comp=1, local_time=0
 int energy_time(tc_ctx* ctx, tc_dev& d)
 {
@@ -1219,6 +1435,7 @@ int energy_time(tc_ctx* ctx, tc_dev& d)
     double dt;
     TC_TRY(stability_RK4(ctx, d, &dt));
     ctx->dt = dt;
+    if (ctx->eq_in_step) TC_TRY(equalize(ctx, d, nullptr));
     TC_TRY(copy_C(ctx, ctx->C0, ctx->Cnew));
     TC_TRY(copy_plane(ctx, ctx->T0, ctx->T, ctx->g));
     TC_TRY(energy_stage(ctx, d, 1, dt));
@@ -1346,6 +1563,10 @@ void free_ctx(tc_ctx* ctx)
     F(ctx->tr_id); F(ctx->tr_id_spare); F(ctx->spare_d); F(ctx->spare_i); F(ctx->sort_tmp);
     for (int b = 0; b < 2; ++b) { F(ctx->sort_k[b]); F(ctx->sort_v[b]); }
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
+    F(ctx->eq_ne); F(ctx->eq_start); F(ctx->eq_end); F(ctx->eq_rows); F(ctx->eq_cols); F(ctx->eq_bcnt);
+    F(ctx->eq_tmp); F(ctx->eq_red);
+    for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
+    if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
@@ -1855,6 +2076,21 @@ int tc_get_tracers(tc_ctx* ctx, int which, double* r, double* s)
     return 0;
 }
 
+int tc_get_tracer_types(tc_ctx* ctx, int32_t* ttype)
+{
+    if (!ctx || !ttype) return TC_ERR_ARG;
+    if (ctx->ntr == 0) return 0;
+    const size_t nb = (size_t)ctx->ntr * sizeof(int32_t);
+    if (ctx->tr_id) {
+        launch_scatter_i32(ctx->stream, ctx->ttype, ctx->tr_id, ctx->spare_i, ctx->ntr);
+        TC_HIP(hipMemcpyAsync(ttype, ctx->spare_i, nb, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        TC_HIP(hipMemcpyAsync(ttype, ctx->ttype, nb, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int tc_set_time(tc_ctx* ctx, double time, double dt)
 {
     if (!ctx) return TC_ERR_ARG;
@@ -1923,6 +2159,36 @@ int tc_diagnostics(tc_ctx* ctx, double out[9])
     std::memcpy(&smax, &h.smax, sizeof(double));
     out[8] = smax;                                            // max strain at the surface
     return 0;
+}
+
+int tc_set_equalize(tc_ctx* ctx, int32_t tpc, const double* Cinit, uint64_t seed, int32_t in_time_step)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (ctx->p.comp == 1 && !Cinit) {
+        ctx->err = "tc_set_equalize: Cinit is required with comp=1";
+        return TC_ERR_ARG;
+    }
+    if (ctx->p.comp == 0 && tpc <= 0) {
+        ctx->err = "tc_set_equalize: tpc must be positive with comp=0";
+        return TC_ERR_ARG;
+    }
+    ctx->eq_tpc = tpc;
+    for (int t = 0; t < TC_MAX_TYPES; ++t) ctx->eq_Cinit[t] = (Cinit && t < ctx->p.ntype) ? Cinit[t] : 0.0;
+    ctx->eq_seed = seed;
+    ctx->eq_calls = 0;
+    ctx->eq_on = true;
+    ctx->eq_in_step = in_time_step != 0;
+    return 0;
+}
+
+int tc_equalize_tracers(tc_ctx* ctx, tc_equalize_info* info)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (!ctx->eq_on) {
+        ctx->err = "tc_equalize_tracers: call tc_set_equalize first";
+        return TC_ERR_ARG;
+    }
+    return equalize(ctx, DEV(ctx), info);
 }
 
 int tc_compute_strain_rate_invariant(tc_ctx* ctx)

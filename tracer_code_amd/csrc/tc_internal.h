@@ -81,6 +81,22 @@ struct Level {
     int cur32 = 0;
 };
 
+// equalize_tracers: per-call reductions read back by the host
+constexpr int EQ_MAXT = 8;                 // type slots (>= TC_MAX_TYPES + 1)
+struct EqTypes { int32_t v[EQ_MAXT]; };
+struct EqOff { int64_t v[EQ_MAXT]; };
+struct EqRed {
+    int32_t raw_min[EQ_MAXT];              // min nexcess <= 0 over each type's selection (2147483647: none)
+    int32_t any;                           // some nexcess != 0
+    int32_t bad;                           // a tracer outside the domain
+    unsigned long long tcount[EQ_MAXT];    // sum of the counts of each type over the nodes
+    long long deficit;                     // sum of nexcess <= 0
+    unsigned long long nfree, found;       // sum of nexcess > 0; donors found
+    unsigned long long dcount[EQ_MAXT];    // donors of each type
+    unsigned long long sel_n[EQ_MAXT], sel_sum[EQ_MAXT];  // tpc_type selections: nodes, count sums
+    unsigned long long qtot[EQ_MAXT];      // qualifying nodes of each donor type in the last pass
+};
+
 }  // namespace tc
 
 struct tc_ctx {
@@ -177,6 +193,18 @@ struct tc_ctx {
     double* guess_lo = nullptr;   // its low part
     // implementation-private host bookkeeping (tc_api.hip)
     void* impl = nullptr;
+    // equalize_tracers (tc_set_equalize, tc_equalize_tracers)
+    bool eq_on = false, eq_in_step = false;
+    int32_t eq_tpc = 0;
+    double eq_Cinit[TC_MAX_TYPES] = {};
+    uint64_t eq_seed = 0, eq_calls = 0;
+    int32_t *eq_ne = nullptr, *eq_start = nullptr, *eq_end = nullptr, *eq_rows = nullptr, *eq_cols = nullptr;
+    int32_t* eq_bcnt = nullptr;
+    uint64_t* eq_k[2] = {nullptr, nullptr};
+    uint32_t* eq_v[2] = {nullptr, nullptr};
+    void* eq_tmp = nullptr;
+    size_t eq_tmp_bytes = 0;
+    tc::EqRed *eq_red = nullptr, *eq_hred = nullptr;  // device, pinned host mirror
     // kernel timing (tc_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;

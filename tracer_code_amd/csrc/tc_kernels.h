@@ -150,8 +150,38 @@ void launch_gather_d(hipStream_t st, const double* in, const uint32_t* idx, doub
 void launch_gather_i32(hipStream_t st, const int32_t* in, const uint32_t* idx, int32_t* out, int64_t n);
 void launch_gather_id(hipStream_t st, const int64_t* id, const uint32_t* idx, int64_t* out, int64_t n);
 void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, double* out, int64_t n);
+void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n);
 int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
                const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
+
+// equalize_tracers (tc_equalize.hip)
+uint64_t eq_stream_key(uint64_t seed, uint64_t call);
+void launch_eq_clear(hipStream_t st, EqRed* red, bool all);
+void launch_eq_tpc_stats(hipStream_t st, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int ntype, int comp,
+                         EqRed* red);
+void launch_eq_nexcess(hipStream_t st, const CntArr& cnt, const int32_t* nempty, const Grid& g, int nt, int comp,
+                       const EqTypes& tpc, int32_t* ne, EqRed* red);
+void launch_eq_node_lists(hipStream_t st, const double* r, const double* s, const int64_t* id, int64_t n,
+                          const Grid& g, double dr, double ds, double aspect, uint64_t* key, uint32_t* val,
+                          int32_t* start, int32_t* end, EqRed* red);
+void launch_eq_bounds(hipStream_t st, const uint64_t* key, int64_t n, int32_t* start, int32_t* end);
+void launch_eq_select(hipStream_t st, const int32_t* ne, const int32_t* start, const int32_t* end, uint32_t* list,
+                      const double* r, const double* s, const int32_t* ty, const int64_t* id, const Grid& g,
+                      double dist_r, double dist_s, double aspect, uint64_t key, uint64_t* dkey, uint32_t* dval,
+                      EqRed* red);
+void launch_eq_reduce(hipStream_t st, const int32_t* ne, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int nt,
+                      int ntype, int comp, EqRed* red);
+void launch_eq_iota(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc);
+void launch_eq_shuffle_orders(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc, uint64_t key,
+                              uint64_t pass);
+int eq_pass_blocks(const Grid& g);
+void launch_eq_pass(hipStream_t st, const int32_t* rows, const int32_t* cols, int32_t* ne, const CntArr& cnt,
+                    const Grid& g, int nt, int tg, const EqTypes& mx, int32_t* bcnt, const uint32_t* donors,
+                    const EqOff& first, const EqOff& avail, double* r, double* s, int32_t* ty, const int64_t* id,
+                    uint64_t key, double dr, double ds, double dist_r, double dist_s, double aspect, int comp,
+                    int update_counts, EqRed* red);
+int eq_sort(hipStream_t st, void* temp, size_t* temp_bytes, uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1,
+            int64_t n, int bits, int* cur);
 
 void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t* ty, int64_t n, const CntArr& cnt,
                 const Grid& g, double dr, double ds, double aspect, int32_t* flag);

@@ -47,8 +47,9 @@ __global__ __launch_bounds__(256) void k_gather_id(const int64_t* __restrict__ i
 }
 
 // caller order: out[id[j]] = in[j]
-__global__ __launch_bounds__(256) void k_scatter_d(const double* __restrict__ in, const int64_t* __restrict__ id,
-                                                   double* __restrict__ out, int64_t n)
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(const T* __restrict__ in, const int64_t* __restrict__ id,
+                                                 T* __restrict__ out, int64_t n)
 {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) out[id[j]] = in[j];
@@ -79,7 +80,12 @@ void launch_gather_id(hipStream_t st, const int64_t* id, const uint32_t* idx, in
 
 void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, double* out, int64_t n)
 {
-    hipLaunchKernelGGL(k_scatter_d, dim3(blocks(n)), dim3(256), 0, st, in, id, out, n);
+    hipLaunchKernelGGL(k_scatter<double>, dim3(blocks(n)), dim3(256), 0, st, in, id, out, n);
+}
+
+void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n)
+{
+    hipLaunchKernelGGL(k_scatter<int32_t>, dim3(blocks(n)), dim3(256), 0, st, in, id, out, n);
 }
 
 // LSD radix sort of (key, slot) pairs on the key's low `bits` bits; with

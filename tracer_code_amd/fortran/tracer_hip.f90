@@ -50,6 +50,12 @@ module tracer_hip
     integer(c_int64_t) :: dc_retries
   end type tc_stats
 
+  ! equalize_tracers record: the columns of equalize.dat (composition.f90:566-574)
+  type, bind(C) :: tc_equalize_info
+    integer(c_int32_t) :: did_work, passes, tpc_type(0:TC_MAX_TYPES), minexcess(0:TC_MAX_TYPES)
+    integer(c_int64_t) :: deficit, moved, nfree, count(0:TC_MAX_TYPES)
+  end type tc_equalize_info
+
   interface
     integer(c_int) function tc_init(p, ntr, ctx) bind(C, name="tc_init")
       import :: c_int, c_int64_t, c_ptr, tc_params
@@ -102,6 +108,11 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: which
       real(c_double), intent(out) :: r(*), s(*)
+    end function
+    integer(c_int) function tc_get_tracer_types(ctx, ttype) bind(C, name="tc_get_tracer_types")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: ttype(*)
     end function
     integer(c_int) function tc_set_time(ctx, time, dt) bind(C, name="tc_set_time")
       import :: c_int, c_ptr, c_double
@@ -224,6 +235,18 @@ module tracer_hip
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(9)
+    end function
+    integer(c_int) function tc_set_equalize(ctx, tpc, Cinit, seed, in_time_step) bind(C, name="tc_set_equalize")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: tpc, in_time_step
+      real(c_double), intent(in) :: Cinit(*)
+      integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function tc_equalize_tracers(ctx, info) bind(C, name="tc_equalize_tracers")
+      import :: c_int, c_ptr, tc_equalize_info
+      type(c_ptr), value :: ctx
+      type(tc_equalize_info), intent(out) :: info
     end function
     integer(c_int) function tc_set_tb_tile(ctx, level, tile) bind(C, name="tc_set_tb_tile")
       import :: c_int, c_int32_t, c_ptr

@@ -122,6 +122,27 @@ class Solver:
         self._check(self.lib.tc_get_tracers(self.h, _TRACERS[which], _ptr(r), _ptr(s)), "get tracers")
         return r, s
 
+    def get_tracer_types(self):
+        t = np.empty(self.ntr, dtype=np.int32)
+        self._check(self.lib.tc_get_tracer_types(self.h, _ptr(t)), "get tracer types")
+        return t
+
+    def set_equalize(self, tpc, Cinit=(), seed=0, in_time_step=False):
+        """equalize_tracers settings (composition.f90:83-580): tpc for comp=0,
+        Cinit = the initial mean composition of each type for comp=1; with
+        in_time_step, time_step equalizes after stability_RK4."""
+        c = np.zeros(max(1, self.params.ntype))
+        c[:len(Cinit)] = Cinit
+        self._check(self.lib.tc_set_equalize(self.h, int(tpc), _ptr(c), int(seed), int(bool(in_time_step))),
+                    "set_equalize")
+
+    def equalize_tracers(self):
+        """One equalisation of the current tracers; returns the record whose
+        fields are the columns of equalize.dat (tc_equalize_info)."""
+        info = _abi.TcEqualizeInfo()
+        self._check(self.lib.tc_equalize_tracers(self.h, ctypes.byref(info)), "equalize_tracers")
+        return info
+
     def set_time(self, time, dt=0.0):
         self._check(self.lib.tc_set_time(self.h, float(time), float(dt)), "set_time")
 
