@@ -271,6 +271,23 @@ int tc_restart_tracers_write(const char* path, int64_t ntr, const double* rtr, c
  * the reference's loops (equal to rounding); the integers are exact. */
 int tc_diagnostics(tc_ctx* ctx, double out[9]);
 
+/* One row of stats.dat (basics.f90:236-324) in its column order: the
+ * quantities of tc_diagnostics plus RaT_avg (integrate2D(RaT/conduct/vis),
+ * vis of level 1), the surface mobility, the corner heat fluxes q1..q4 and,
+ * per type, Cavg and the entrainment (the mass above row kent over mass).
+ * kent is entrainment_setup's row (composition.f90:582-590: the last k with
+ * k*ds <= dent); mass[t] is Cavg of type t+1 at the first call (basics.f90:
+ * 250-258); mass = NULL takes this call's Cavg, as the first call does.
+ * Sums run in another order than the reference's loops (equal to rounding). */
+typedef struct tc_stats_row {
+    double time, Tavg, vrms, Nu, Nu_bot, RaT_avg, mobility, strain_max;
+    double q[4];
+    int32_t empty_cells, tracer_min, tracer_max, kent;
+    double Cavg[TC_MAX_TYPES];
+    double ent[TC_MAX_TYPES];
+} tc_stats_row;
+int tc_stats_record(tc_ctx* ctx, int32_t kent, const double* mass, tc_stats_row* out);
+
 /* ---- tracer equalisation ------------------------------------------------ */
 /* Replaces equalize_tracers (composition.f90:83-580): tracers of nodes with
  * more than their type's share (nexcess > 0) are moved to the nodes that lack

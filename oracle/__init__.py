@@ -14,7 +14,7 @@ import subprocess
 
 import numpy as np
 
-from tracer_code_amd._abi import TcEqualizeInfo, TcParams
+from tracer_code_amd._abi import TcEqualizeInfo, TcParams, TcStatsRow
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
@@ -110,6 +110,8 @@ def lib():
         u64 = ctypes.c_uint64
         L.or_equalize_tracers.restype = i
         L.or_equalize_tracers.argtypes = [vp, i, vp, u64, u64, ctypes.POINTER(TcEqualizeInfo)]
+        L.or_stats_record.restype = None
+        L.or_stats_record.argtypes = [vp, i, vp, ctypes.POINTER(TcStatsRow)]
         L.or_set_equalize.restype = None
         L.or_set_equalize.argtypes = [vp, i, i, vp, u64]
         L.or_eq_uniform.restype = d
@@ -209,6 +211,13 @@ class Oracle:
         c = np.ascontiguousarray(np.atleast_1d(np.asarray(Cinit, dtype=np.float64)))
         st = self.L.or_equalize_tracers(self.h, int(tpc), _p(c), int(seed), int(call), ctypes.byref(info))
         return st, info
+
+    def stats_record(self, kent, mass=None):
+        """The stats.dat row (basics.f90:236-324) as a TcStatsRow."""
+        out = TcStatsRow()
+        m = None if mass is None else np.ascontiguousarray(np.atleast_1d(np.asarray(mass, dtype=np.float64)))
+        self.L.or_stats_record(self.h, int(kent), None if m is None else _p(m), ctypes.byref(out))
+        return out
 
     def set_equalize(self, on, tpc, Cinit, seed):
         c = np.ascontiguousarray(np.atleast_1d(np.asarray(Cinit, dtype=np.float64)))

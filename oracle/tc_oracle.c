@@ -2175,6 +2175,55 @@ void or_stats(or_state* s, double out[9])
     out[4] = empty; out[5] = tmin; out[6] = tmax; out[7] = Nu_bot; out[8] = strain_max;
 }
 
+static double g_ratavg(const or_state* s, size_t c, const void* ctx)
+{
+    (void)ctx;
+    return s->p.RaT / s->conduct[c] / s->vis[1][c];
+}
+
+/* integrate1Dz_ent (basics.f90:393-404): the z trapezoid from row kent */
+static double integrate_ent(const or_state* s, const double* f, int kent)
+{
+    double* fintx = (double*)malloc(sizeof(double) * (s->nz + 1));
+    for (int k = 0; k <= s->nz; ++k) fintx[k] = integrate1Dx(s, f + IX(s, 0, k), 1);
+    double fint = 0.0;
+    for (int k = kent; k <= s->nz - 1; ++k)
+        fint = fint + 0.5 * (fintx[k] * s->zs[k + SPAN] + fintx[k + 1] * s->zs[k + 1 + SPAN]) * s->ds;
+    free(fintx);
+    return fint;
+}
+
+void or_stats_record(or_state* s, int kent, const double* mass, tc_stats_row* out)
+{
+    const int nx = s->nx, nz = s->nz;
+    double d[9];
+    or_stats(s, d);
+    memset(out, 0, sizeof(*out));
+    out->time = s->time;
+    out->vrms = d[0]; out->Nu = d[1]; out->Tavg = d[2]; out->Nu_bot = d[7]; out->strain_max = d[8];
+    out->empty_cells = (int32_t)d[4]; out->tracer_min = (int32_t)d[5]; out->tracer_max = (int32_t)d[6];
+    out->kent = kent;
+    out->RaT_avg = integrate2D_fn(s, g_ratavg, NULL) / s->p.aspect;
+    double* top = (double*)malloc(sizeof(double) * (nx + 1));
+    for (int i = 0; i <= nx; ++i) {
+        const size_t c = IX(s, i, nz);
+        top[i] = s->u[c] * s->u[c] + s->w[c] * s->w[c];
+    }
+    out->mobility = sqrt(integrate1Dx(s, top, 1) / s->p.aspect) / d[0];
+    free(top);
+    const int ci[4] = {0, nx, nx, 0}, ck[4] = {nz, nz, 0, 0};
+    for (int j = 0; j < 4; ++j) {
+        const size_t c = IX(s, ci[j], ck[j]);
+        out->q[j] = -s->conduct[c] * dot3(s->D1, s->T + c, s->ld) / s->ds / s->zs[ck[j] + SPAN];
+    }
+    if (s->p.comp == 1)
+        for (int t = 1; t <= s->ntype; ++t) {
+            out->Cavg[t - 1] = integrate2D_fn(s, g_field, s->Cnew[t]) / s->p.aspect;
+            const double m = mass ? mass[t - 1] : out->Cavg[t - 1];
+            out->ent[t - 1] = integrate_ent(s, s->Cnew[t], kent) / s->p.aspect / m;
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* equalize_tracers (composition.f90:83-580)                                 */
 /* ------------------------------------------------------------------------ */

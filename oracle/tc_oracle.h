@@ -113,6 +113,8 @@ int  or_initialize_state(or_state* s);
 /* diagnostics (basics.f90:236-324): out[0]=vrms, [1]=Nu, [2]=Tavg, [3]=Cavg(1),
  * [4]=empty_cells, [5]=tracer_min, [6]=tracer_max, [7]=Nu_bot, [8]=strain_max */
 void or_stats(or_state* s, double out[9]);
+/* the whole stats.dat row (basics.f90:236-324); mass NULL = this call's Cavg */
+void or_stats_record(or_state* s, int kent, const double* mass, tc_stats_row* out);
 /* equalize_tracers (composition.f90:83-580) in its counter-based form */
 int  or_equalize_tracers(or_state* s, int tpc, const double* Cinit, uint64_t seed, uint64_t call,
                          tc_equalize_info* info);

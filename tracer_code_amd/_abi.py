@@ -86,6 +86,17 @@ class TcEqualizeInfo(ctypes.Structure):
     ]
 
 
+class TcStatsRow(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double), ("Tavg", ctypes.c_double), ("vrms", ctypes.c_double), ("Nu", ctypes.c_double),
+        ("Nu_bot", ctypes.c_double), ("RaT_avg", ctypes.c_double), ("mobility", ctypes.c_double),
+        ("strain_max", ctypes.c_double), ("q", ctypes.c_double * 4),
+        ("empty_cells", ctypes.c_int32), ("tracer_min", ctypes.c_int32), ("tracer_max", ctypes.c_int32),
+        ("kent", ctypes.c_int32),
+        ("Cavg", ctypes.c_double * TC_MAX_TYPES), ("ent", ctypes.c_double * TC_MAX_TYPES),
+    ]
+
+
 # every C-ABI entry point declared in include/tracer_hip.h: name -> (restype, argtypes)
 _VP = ctypes.c_void_p
 _P = ctypes.POINTER
@@ -103,6 +114,7 @@ SIGNATURES = {
                                         ctypes.c_int32, _VP, _VP, _VP, _VP, _VP]),
     "tc_restart_tracers_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
     "tc_restart_tracers_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
+    "tc_stats_record": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, _P(TcStatsRow)]),
     "tc_set_equalize": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, ctypes.c_uint64, ctypes.c_int32]),
     "tc_equalize_tracers": (ctypes.c_int, [_VP, _P(TcEqualizeInfo)]),
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),

@@ -282,6 +282,17 @@ def make_params(nx, nz, aspect=1.0, grids=6, mingrid=4, RaT=1e5, comp=1, ntype=1
     return p
 
 
+def entrainment_row(nz, dent=0.1):
+    """kent of entrainment_setup (composition.f90:582-590) on a uniform grid:
+    the last row k with ds*k <= dent (dent = 0.1, main.f90:69)."""
+    ds = 1.0 / nz
+    kent = 0
+    for k in range(nz + 1):
+        if ds * float(k) <= dent:
+            kent = k
+    return kent
+
+
 def preset_params(name, **overrides):
     """tc_params of a named workload; keyword overrides replace preset values
     (e.g. nx/nz for a reduced-size parity case)."""

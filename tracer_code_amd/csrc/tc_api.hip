@@ -67,6 +67,8 @@ struct tc_dev {
     int max_partial = 0;
     void* diag = nullptr;           // tc_diagnostics: atomics and results
     double* diag_part = nullptr;    // tc_diagnostics: 3 partial sums per block
+    double* stats_part = nullptr;   // tc_stats_record: partial sums per block
+    double* stats_out = nullptr;    // tc_stats_record: results
     bool coarse_ready = false;
     bool trace = false;
     bool coarse_lu = false;
@@ -1570,7 +1572,7 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
-        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->diag); F(d->diag_part);
+        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         delete d;
@@ -2158,6 +2160,47 @@ int tc_diagnostics(tc_ctx* ctx, double out[9])
     double smax;
     std::memcpy(&smax, &h.smax, sizeof(double));
     out[8] = smax;                                            // max strain at the surface
+    return 0;
+}
+
+int tc_stats_record(tc_ctx* ctx, int32_t kent, const double* mass, tc_stats_row* out)
+{
+    if (!ctx || !out) return TC_ERR_ARG;
+    const tc_params& P = ctx->p;
+    const Grid& g = ctx->g;
+    if (kent < 0 || kent > g.nz) {
+        ctx->err = "tc_stats_record: kent outside 0..nz";
+        return TC_ERR_ARG;
+    }
+    double d9[9];
+    TC_TRY(tc_diagnostics(ctx, d9));
+    tc_dev& d = DEV(ctx);
+    const int np = stats_np();
+    const int nb = diag_blocks(g);
+    if (!d.stats_part) TC_HIP(hipMalloc((void**)&d.stats_part, (size_t)np * nb * sizeof(double)));
+    if (!d.stats_out) TC_HIP(hipMalloc((void**)&d.stats_out, (size_t)(np + 5) * sizeof(double)));
+    PtrArr cn{};
+    const int nC = P.comp == 1 ? P.ntype : 0;
+    for (int t = 0; t < nC; ++t) cn.v[t] = ctx->Cnew[t];
+    launch_stats_rest(ctx->stream, ctx->u, ctx->w, ctx->T, ctx->conduct, ctx->lv[1].vis, cn, nC, kent, P.RaT, g, d.F,
+                      ctx->lv[1].dr, ctx->lv[1].ds, d.stats_part, d.stats_out);
+    std::vector<double> h((size_t)np + 5);
+    TC_HIP(hipMemcpyAsync(h.data(), d.stats_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    std::memset(out, 0, sizeof(*out));
+    const double a = P.aspect;
+    out->time = ctx->time;
+    out->vrms = d9[0]; out->Nu = d9[1]; out->Tavg = d9[2]; out->Nu_bot = d9[7]; out->strain_max = d9[8];
+    out->empty_cells = (int32_t)d9[4]; out->tracer_min = (int32_t)d9[5]; out->tracer_max = (int32_t)d9[6];
+    out->kent = kent;
+    out->RaT_avg = h[0] / a;
+    out->mobility = std::sqrt(h[np] / a) / d9[0];
+    for (int j = 0; j < 4; ++j) out->q[j] = h[np + 1 + j];
+    for (int t = 0; t < nC; ++t) {
+        out->Cavg[t] = h[1 + t] / a;
+        const double m = mass ? mass[t] : out->Cavg[t];
+        out->ent[t] = h[1 + TC_MAX_TYPES + t] / a / m;
+    }
     return 0;
 }
 

@@ -758,6 +758,80 @@ void launch_diagnostics(hipStream_t st, const double* u, const double* w, const 
                        out);
 }
 
+// The rest of the stats.dat row (basics.f90:236-324): per block, the
+// weighted sums of RaT/conduct/vis, of each Cnew and of each Cnew above row
+// kent (integrate1Dz_ent, basics.f90:393-404: z weights ds/2 at kent and nz,
+// ds between, 0 below kent).  part[STATS_NP*b + {0, 1+t, 1+TC_MAX_TYPES+t}].
+constexpr int STATS_NP = 1 + 2 * TC_MAX_TYPES;
+
+__global__ __launch_bounds__(256) void k_stats_part(const double* __restrict__ conduct, const double* __restrict__ vis,
+                                                    PtrArr Cnew, int nC, int kent, double RaT, Grid g, double dr,
+                                                    double ds, double* __restrict__ part)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    double acc[STATS_NP];
+    for (int j = 0; j < STATS_NP; ++j) acc[j] = 0.0;
+    if (i <= g.nx && k <= g.nz) {
+        const size_t c = g.at(i, k);
+        const double wx = ((i == 0 || i == g.nx) ? 0.5 : 1.0) * dr;
+        const double wz = ((k == 0 || k == g.nz) ? 0.5 : 1.0) * ds;
+        const double we = (kent >= g.nz || k < kent) ? 0.0 : ((k == kent || k == g.nz) ? 0.5 : 1.0) * ds;
+        acc[0] = RaT / conduct[c] / vis[c] * (wx * wz);
+        for (int t = 0; t < nC; ++t) {
+            const double v = Cnew.v[t][c];
+            acc[1 + t] = v * (wx * wz);
+            acc[1 + TC_MAX_TYPES + t] = v * (wx * we);
+        }
+    }
+    for (int j = 0; j < STATS_NP; ++j) acc[j] = block_sum(acc[j]);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double* q = part + STATS_NP * (blockIdx.y * gridDim.x + blockIdx.x);
+        for (int j = 0; j < STATS_NP; ++j) q[j] = acc[j];
+    }
+}
+
+// out[0..STATS_NP-1] = the sums of part; out[STATS_NP] = integrate1Dx of
+// u^2+w^2 on the top row; out[STATS_NP+1..+4] = q1..q4, the heat fluxes at
+// the corners (0,nz), (nx,nz), (nx,0), (0,0)
+__global__ __launch_bounds__(1024) void k_stats_final(const double* __restrict__ part, int nb,
+                                                      const double* __restrict__ u, const double* __restrict__ w,
+                                                      const double* __restrict__ T,
+                                                      const double* __restrict__ conduct, Grid g, FD F, double dr,
+                                                      double ds, double* __restrict__ out)
+{
+    double acc[STATS_NP + 1];
+    for (int j = 0; j <= STATS_NP; ++j) acc[j] = 0.0;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x)
+        for (int j = 0; j < STATS_NP; ++j) acc[j] += part[STATS_NP * t + j];
+    for (int i = threadIdx.x; i <= g.nx; i += blockDim.x) {
+        const size_t c = g.at(i, g.nz);
+        acc[STATS_NP] += (u[c] * u[c] + w[c] * w[c]) * (((i == 0 || i == g.nx) ? 0.5 : 1.0) * dr);
+    }
+    for (int j = 0; j <= STATS_NP; ++j) acc[j] = block_sum(acc[j]);
+    if (threadIdx.x == 0) {
+        for (int j = 0; j <= STATS_NP; ++j) out[j] = acc[j];
+        const int ci[4] = {0, g.nx, g.nx, 0}, ck[4] = {g.nz, g.nz, 0, 0};
+        for (int j = 0; j < 4; ++j) {
+            const size_t c = g.at(ci[j], ck[j]);
+            out[STATS_NP + 1 + j] = -conduct[c] * dot3(F.D1, T, c, g.ld) / ds;
+        }
+    }
+}
+
+int stats_np() { return STATS_NP; }
+
+void launch_stats_rest(hipStream_t st, const double* u, const double* w, const double* T, const double* conduct,
+                       const double* vis, const PtrArr& Cnew, int nC, int kent, double RaT, const Grid& g,
+                       const FD& F, double dr, double ds, double* part, double* out)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1, g.nz + 1, b);
+    hipLaunchKernelGGL(k_stats_part, gr, b, 0, st, conduct, vis, Cnew, nC, kent, RaT, g, dr, ds, part);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(1024), 0, st, part, (int)(gr.x * gr.y), u, w, T, conduct, g, F, dr,
+                       ds, out);
+}
+
 int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial)
 {
     dim3 b(64, 4);

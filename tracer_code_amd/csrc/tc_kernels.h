@@ -133,6 +133,10 @@ void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, 
                         const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop);
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
 int diag_blocks(const Grid& g);
+int stats_np();  // partial sums per block of launch_stats_rest
+void launch_stats_rest(hipStream_t st, const double* u, const double* w, const double* T, const double* conduct,
+                       const double* vis, const PtrArr& Cnew, int nC, int kent, double RaT, const Grid& g,
+                       const FD& F, double dr, double ds, double* part, double* out);
 void launch_diagnostics(hipStream_t st, const double* u, const double* w, const double* T, const double* C,
                         const int32_t* nempty, const double* strain, const double* conduct, const Grid& g,
                         const FD& F, double dr, double ds, double* part, int32_t* ints, unsigned long long* smax,

@@ -50,6 +50,13 @@ module tracer_hip
     integer(c_int64_t) :: dc_retries
   end type tc_stats
 
+  ! one row of stats.dat (basics.f90:236-324)
+  type, bind(C) :: tc_stats_rec
+    real(c_double) :: time, Tavg, vrms, Nu, Nu_bot, RaT_avg, mobility, strain_max, q(4)
+    integer(c_int32_t) :: empty_cells, tracer_min, tracer_max, kent
+    real(c_double) :: Cavg(TC_MAX_TYPES), ent(TC_MAX_TYPES)
+  end type tc_stats_rec
+
   ! equalize_tracers record: the columns of equalize.dat (composition.f90:566-574)
   type, bind(C) :: tc_equalize_info
     integer(c_int32_t) :: did_work, passes, tpc_type(0:TC_MAX_TYPES), minexcess(0:TC_MAX_TYPES)
@@ -235,6 +242,13 @@ module tracer_hip
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(9)
+    end function
+    integer(c_int) function tc_stats_record(ctx, kent, mass, rec) bind(C, name="tc_stats_record")
+      import :: c_int, c_int32_t, c_ptr, tc_stats_rec
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kent
+      type(c_ptr), value :: mass
+      type(tc_stats_rec), intent(out) :: rec
     end function
     integer(c_int) function tc_set_equalize(ctx, tpc, Cinit, seed, in_time_step) bind(C, name="tc_set_equalize")
       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr

@@ -236,6 +236,16 @@ class Solver:
         keys = ("vrms", "Nu", "Tavg", "Cavg", "empty", "tmin", "tmax", "Nu_bot", "strain_max")
         return dict(zip(keys, out.tolist()))
 
+    def stats_record(self, kent, mass=None):
+        """One stats.dat row (basics.f90:236-324) computed on the device, as a
+        TcStatsRow: kent from config.entrainment_row, mass = Cavg of each type
+        at the first call (None: this call's Cavg, as the first call uses)."""
+        out = _abi.TcStatsRow()
+        m = None if mass is None else np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
+        self._check(self.lib.tc_stats_record(self.h, int(kent), None if m is None else _ptr(m), ctypes.byref(out)),
+                    "stats_record")
+        return out
+
     def set_tb_tile(self, level, tile):
         self._check(self.lib.tc_set_tb_tile(self.h, int(level), int(tile)), "set_tb_tile")
 
