@@ -288,6 +288,18 @@ typedef struct tc_stats_row {
 } tc_stats_row;
 int tc_stats_record(tc_ctx* ctx, int32_t kent, const double* mass, tc_stats_row* out);
 
+/* ---- snapshot staging ---------------------------------------------------- */
+/* Replaces the reads of snapshots (main.f90:234-268): copies T, SF, u, w,
+ * strain (recomputed, as stats does), vis, Tvis, nempty, Cnew and Cvis into
+ * device staging planes on the compute stream, then to pinned host memory
+ * on a stream of their own, and returns at once: the next time steps run
+ * while the copy and the host's writing proceed.  A new stage waits for the
+ * previous host copy. */
+int tc_snapshot_stage(tc_ctx* ctx);
+/* Waits for the staged copy and returns one staged field in tc_get_field's
+ * layout (TC_F_T, _SF, _U, _W, _STRAIN, _VIS, _TVIS, _NEMPTY, _CNEW, _CVIS). */
+int tc_snapshot_get(tc_ctx* ctx, int field, void* host);
+
 /* ---- tracer equalisation ------------------------------------------------ */
 /* Replaces equalize_tracers (composition.f90:83-580): tracers of nodes with
  * more than their type's share (nexcess > 0) are moved to the nodes that lack

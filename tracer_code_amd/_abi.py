@@ -115,6 +115,8 @@ SIGNATURES = {
     "tc_restart_tracers_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
     "tc_restart_tracers_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _VP, _VP, _VP]),
     "tc_stats_record": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, _P(TcStatsRow)]),
+    "tc_snapshot_stage": (ctypes.c_int, [_VP]),
+    "tc_snapshot_get": (ctypes.c_int, [_VP, ctypes.c_int, _VP]),
     "tc_set_equalize": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, ctypes.c_uint64, ctypes.c_int32]),
     "tc_equalize_tracers": (ctypes.c_int, [_VP, _P(TcEqualizeInfo)]),
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),

@@ -1569,6 +1569,12 @@ void free_ctx(tc_ctx* ctx)
     F(ctx->eq_tmp); F(ctx->eq_red);
     for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
     if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);
+    if (ctx->snap_stream) (void)hipStreamSynchronize(ctx->snap_stream);
+    F(ctx->snap_dev);
+    if (ctx->snap_host) (void)hipHostFree(ctx->snap_host);
+    if (ctx->ev_snap_d) (void)hipEventDestroy(ctx->ev_snap_d);
+    if (ctx->ev_snap_h) (void)hipEventDestroy(ctx->ev_snap_h);
+    if (ctx->snap_stream) (void)hipStreamDestroy(ctx->snap_stream);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
@@ -2201,6 +2207,103 @@ int tc_stats_record(tc_ctx* ctx, int32_t kent, const double* mass, tc_stats_row*
         const double m = mass ? mass[t] : out->Cavg[t];
         out->ent[t] = h[1 + TC_MAX_TYPES + t] / a / m;
     }
+    return 0;
+}
+
+// snapshot staging slots: T, SF, u, w, strain, vis, Tvis, nempty (int32 in a
+// double-sized slot), Cnew(1..ntype), Cvis(1..ntype)
+enum { SNAP_T = 0, SNAP_SF, SNAP_U, SNAP_W, SNAP_STRAIN, SNAP_VIS, SNAP_TVIS, SNAP_NEMPTY, SNAP_C };
+
+int tc_snapshot_stage(tc_ctx* ctx)
+{
+    if (!ctx) return TC_ERR_ARG;
+    const Grid& g = ctx->g;
+    const int nt = ctx->p.comp == 1 ? ctx->p.ntype : 0;
+    const size_t slots = SNAP_C + 2 * (size_t)nt, pb = plane_bytes(g);
+    if (!ctx->snap_dev) {
+        TC_HIP(hipMalloc((void**)&ctx->snap_dev, slots * pb));
+        TC_HIP(hipHostMalloc((void**)&ctx->snap_host, slots * pb, hipHostMallocDefault));
+        TC_HIP(hipStreamCreateWithFlags(&ctx->snap_stream, hipStreamNonBlocking));
+        TC_HIP(hipEventCreateWithFlags(&ctx->ev_snap_d, hipEventDisableTiming));
+        TC_HIP(hipEventCreateWithFlags(&ctx->ev_snap_h, hipEventDisableTiming));
+    }
+    // the previous stage's host copy must be done before the staging planes are reused
+    if (ctx->snap_staged) TC_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_snap_h, 0));
+    tc_dev& d = DEV(ctx);
+    // strain of the snapshot: compute_strain_rate_invariant of stats (basics.f90:245)
+    launch_strain(ctx->stream, ctx->u, ctx->w, ctx->strain, g, d.F, ctx->lv[1].dr, ctx->lv[1].ds);
+    const void* src[SNAP_C] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->lv[1].vis, ctx->Tvis, ctx->nempty};
+    CopyList L{};
+    for (int j = 0; j < SNAP_C; ++j) {
+        L.src[L.n] = src[j];
+        L.bytes[L.n] = j == SNAP_NEMPTY ? g.elems * sizeof(int32_t) : pb;
+        L.dst_off[L.n] = (size_t)j * pb;
+        L.n++;
+    }
+    for (int t = 0; t < nt; ++t) {
+        L.src[L.n] = ctx->Cnew[t];
+        L.bytes[L.n] = pb;
+        L.dst_off[L.n] = (SNAP_C + (size_t)t) * pb;
+        L.n++;
+        L.src[L.n] = ctx->Cvis[t];
+        L.bytes[L.n] = pb;
+        L.dst_off[L.n] = (SNAP_C + (size_t)nt + t) * pb;
+        L.n++;
+    }
+    static_assert(SNAP_C + 2 * TC_MAX_TYPES <= COPY_MAX, "snapshot planes");
+    launch_copy_list(ctx->stream, L, ctx->snap_dev);
+    TC_HIP(hipEventRecord(ctx->ev_snap_d, ctx->stream));
+    TC_HIP(hipStreamWaitEvent(ctx->snap_stream, ctx->ev_snap_d, 0));
+    TC_HIP(hipMemcpyAsync(ctx->snap_host, ctx->snap_dev, slots * pb, hipMemcpyDeviceToHost, ctx->snap_stream));
+    TC_HIP(hipEventRecord(ctx->ev_snap_h, ctx->snap_stream));
+    ctx->snap_staged = true;
+    return 0;
+}
+
+int tc_snapshot_get(tc_ctx* ctx, int field, void* host)
+{
+    if (!ctx || !host) return TC_ERR_ARG;
+    if (!ctx->snap_staged) {
+        ctx->err = "tc_snapshot_get: nothing staged (call tc_snapshot_stage)";
+        return TC_ERR_ARG;
+    }
+    const Grid& g = ctx->g;
+    const int nt = ctx->p.comp == 1 ? ctx->p.ntype : 0;
+    int slot = -1;
+    switch (field) {
+    case TC_F_T: slot = SNAP_T; break; case TC_F_SF: slot = SNAP_SF; break; case TC_F_U: slot = SNAP_U; break;
+    case TC_F_W: slot = SNAP_W; break; case TC_F_STRAIN: slot = SNAP_STRAIN; break;
+    case TC_F_VIS: slot = SNAP_VIS; break; case TC_F_TVIS: slot = SNAP_TVIS; break;
+    case TC_F_NEMPTY: slot = SNAP_NEMPTY; break;
+    case TC_F_CNEW: case TC_F_CVIS: slot = nt > 0 ? SNAP_C : -1; break;
+    default: break;
+    }
+    if (slot < 0) {
+        ctx->err = "tc_snapshot_get: field not in a snapshot";
+        return TC_ERR_ARG;
+    }
+    TC_HIP(hipEventSynchronize(ctx->ev_snap_h));
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1), h = (size_t)(g.nz + 2 * SPAN + 1);
+    const size_t first = g.at(-SPAN, -SPAN);
+    if (field == TC_F_NEMPTY) {
+        const int32_t* src = reinterpret_cast<const int32_t*>(ctx->snap_host + (size_t)slot * g.elems);
+        int32_t* dst = (int32_t*)host;
+        for (size_t k = 0; k < h; ++k) std::memcpy(dst + k * w, src + first + k * g.ld, w * sizeof(int32_t));
+        return 0;
+    }
+    if (field == TC_F_CNEW || field == TC_F_CVIS) {
+        double* dst = (double*)host;
+        const int base = SNAP_C + (field == TC_F_CVIS ? nt : 0);
+        for (int t = 0; t < nt; ++t) {
+            const double* src = ctx->snap_host + (size_t)(base + t) * g.elems;
+            for (size_t k = 0; k < h; ++k)
+                for (size_t i = 0; i < w; ++i) dst[(k * w + i) * nt + t] = src[first + k * g.ld + i];
+        }
+        return 0;
+    }
+    const double* src = ctx->snap_host + (size_t)slot * g.elems;
+    double* dst = (double*)host;
+    for (size_t k = 0; k < h; ++k) std::memcpy(dst + k * w, src + first + k * g.ld, w * sizeof(double));
     return 0;
 }
 

@@ -205,6 +205,12 @@ struct tc_ctx {
     void* eq_tmp = nullptr;
     size_t eq_tmp_bytes = 0;
     tc::EqRed *eq_red = nullptr, *eq_hred = nullptr;  // device, pinned host mirror
+    // snapshot staging (tc_snapshot_stage / tc_snapshot_get)
+    double* snap_dev = nullptr;      // device staging planes
+    double* snap_host = nullptr;     // pinned host copy
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t ev_snap_d = nullptr, ev_snap_h = nullptr;
+    bool snap_staged = false;
     // kernel timing (tc_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;

@@ -154,6 +154,14 @@ void launch_gather_d(hipStream_t st, const double* in, const uint32_t* idx, doub
 void launch_gather_i32(hipStream_t st, const int32_t* in, const uint32_t* idx, int32_t* out, int64_t n);
 void launch_gather_id(hipStream_t st, const int64_t* id, const uint32_t* idx, int64_t* out, int64_t n);
 void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, double* out, int64_t n);
+constexpr int COPY_MAX = 16;
+struct CopyList {
+    const void* src[COPY_MAX];
+    size_t bytes[COPY_MAX];     // multiples of 4
+    size_t dst_off[COPY_MAX];   // multiples of 16
+    int n;
+};
+void launch_copy_list(hipStream_t st, const CopyList& L, void* dst);
 void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n);
 int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
                const uint32_t* vin, uint32_t* vout, int64_t n, int bits);

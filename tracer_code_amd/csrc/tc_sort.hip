@@ -5,6 +5,8 @@
 // neighbouring lanes, and binning finds long runs of one (node, type) key
 // per wavefront.  The order carries no meaning in the reference; tr_id maps
 // each slot back to the caller's tracer index for the boundary copies.
+#include <algorithm>
+
 #include <hipcub/hipcub.hpp>
 
 #include "tc_kernels.h"
@@ -86,6 +88,32 @@ void launch_scatter_d(hipStream_t st, const double* in, const int64_t* id, doubl
 void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n)
 {
     hipLaunchKernelGGL(k_scatter<int32_t>, dim3(blocks(n)), dim3(256), 0, st, in, id, out, n);
+}
+
+// One launch copying up to COPY_MAX device buffers into one staging block:
+// blockIdx.y picks the buffer, 16-byte words with a 4-byte tail.  A blit
+// kernel streams at HBM rate where queued device-to-device copies may go
+// through the copy engines.
+__global__ __launch_bounds__(256) void k_copy_list(CopyList L, char* __restrict__ dst)
+{
+    const int b = blockIdx.y;
+    const char* src = static_cast<const char*>(L.src[b]);
+    char* out = dst + L.dst_off[b];
+    const size_t n16 = L.bytes[b] / 16;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n16; j += stride)
+        reinterpret_cast<uint4*>(out)[j] = reinterpret_cast<const uint4*>(src)[j];
+    const size_t tail = (L.bytes[b] - n16 * 16) / 4;
+    if (blockIdx.x == 0 && threadIdx.x < tail)
+        reinterpret_cast<uint32_t*>(out + n16 * 16)[threadIdx.x] = reinterpret_cast<const uint32_t*>(src + n16 * 16)[threadIdx.x];
+}
+
+void launch_copy_list(hipStream_t st, const CopyList& L, void* dst)
+{
+    size_t most = 0;
+    for (int j = 0; j < L.n; ++j) most = std::max(most, L.bytes[j] / 16);
+    const unsigned gx = (unsigned)std::min<size_t>(std::max<size_t>((most + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(k_copy_list, dim3(gx, (unsigned)L.n), dim3(256), 0, st, L, static_cast<char*>(dst));
 }
 
 // LSD radix sort of (key, slot) pairs on the key's low `bits` bits; with

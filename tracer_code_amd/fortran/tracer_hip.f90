@@ -250,6 +250,16 @@ module tracer_hip
       type(c_ptr), value :: mass
       type(tc_stats_rec), intent(out) :: rec
     end function
+    integer(c_int) function tc_snapshot_stage(ctx) bind(C, name="tc_snapshot_stage")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function tc_snapshot_get(ctx, field, host) bind(C, name="tc_snapshot_get")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      type(c_ptr), value :: host
+    end function
     integer(c_int) function tc_set_equalize(ctx, tpc, Cinit, seed, in_time_step) bind(C, name="tc_set_equalize")
       import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
       type(c_ptr), value :: ctx

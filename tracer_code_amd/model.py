@@ -246,6 +246,18 @@ class Solver:
                     "stats_record")
         return out
 
+    def snapshot_stage(self):
+        """Stage the snapshot fields (main.f90:234-268) for an asynchronous
+        copy to the host; time steps may run before snapshot_get."""
+        self._check(self.lib.tc_snapshot_stage(self.h), "snapshot_stage")
+
+    def snapshot_get(self, name):
+        """One staged field (as get()), waiting for the copy."""
+        shape, dt = self._shape(name)
+        a = np.empty(shape, dtype=dt)
+        self._check(self.lib.tc_snapshot_get(self.h, _FIELD[name], _ptr(a)), f"snapshot_get {name}")
+        return a
+
     def set_tb_tile(self, level, tile):
         self._check(self.lib.tc_set_tb_tile(self.h, int(level), int(tile)), "set_tb_tile")
 
