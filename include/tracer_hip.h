@@ -174,6 +174,9 @@ int tc_set_mode(tc_ctx* ctx, int mode);
  * redone with Nmulti (counted in tc_stats.dc_retries). */
 #define TC_DC_FAST_MAX_CYCLES 200
 int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps);
+/* Sweeps per visit of one level of the tuned defect-correction cycle (0:
+ * the common count of tc_set_dc_sweeps); the Nmulti retry ignores it. */
+int tc_set_dc_level_sweeps(tc_ctx* ctx, int32_t level, int32_t sweeps);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

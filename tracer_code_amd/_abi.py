@@ -106,6 +106,7 @@ SIGNATURES = {
     "tc_last_error": (ctypes.c_char_p, [_VP]),
     "tc_set_mode": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_set_dc_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_set_dc_level_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32]),
     "tc_sort_tracers": (ctypes.c_int, [_VP]),
     "tc_diagnostics": (ctypes.c_int, [_VP, _VP]),
     "tc_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

@@ -315,7 +315,7 @@ void cheb_factors(int n, int Niter, int dir, double& temp, double& cconst)
 // sweeps per launch (k_sweep_tb, the 13-tap form of the operator) unless
 // TC_TB=0; the reference-form path and level 1 run one sweep per launch.
 int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, const Level& view, const float* rhs32,
-               bool* first_sweep)
+               bool* first_sweep, bool zero_start = false)
 {
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
@@ -337,8 +337,10 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
             double temp[8], cc[8];
             for (int s = 0; s < nk; ++s) cheb_factors(n + s, Niter, dir, temp[s], cc[s]);
             if (f32) {
-                launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, L.err32[L.cur32], L.err32[1 - L.cur32], d.F,
-                                  d.h[level], temp, cc, nk, zsg);
+                // the first launch of a zero-started visit stages zeros instead of reading them
+                const float* in = (zero_start && n == 1) ? nullptr : L.err32[L.cur32];
+                launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, in, L.err32[1 - L.cur32], d.F, d.h[level], temp,
+                                  cc, nk, zsg);
                 L.cur32 = 1 - L.cur32;
             } else {
                 launch_sweep_tb64(ctx->stream, use_ratio, K, L, view.rhs, L.err[L.cur], L.err[1 - L.cur], d.F,
@@ -778,7 +780,14 @@ int dc_alloc(tc_ctx* ctx)
 
 // Jacobi-Richardson sweeps of one visit on a correction level (FP32 or
 // FP64), followed (level > 1) by compute_residual of the level.
-int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
+// blocked FP32 sweeps on this level: its visits can start from a zero
+// correction without a fill (run_sweeps' zero_start)
+bool tb_zero_start(const tc_ctx* ctx, const tc_dev& d, int level, bool f32)
+{
+    return f32 && d.tb && level >= 2 && level < ctx->grids && ctx->mode != TC_MODE_FP64;
+}
+
+int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32, bool zero_start = false)
 {
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
@@ -786,7 +795,10 @@ int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32)
     Level view = L;
     if (level == 1) view.rhs = L.res;
     const float* rhs32 = (level == 1) ? L.res32 : L.rhs32;
-    TC_TRY(run_sweeps(ctx, d, level, ctx->dc_sweeps_active, dir, f32, view, rhs32, nullptr));
+    // a per-level count applies to the tuned cycle, not to the Nmulti retry
+    int n = ctx->dc_sweeps_active;
+    if (ctx->dc_level_sweeps[level] > 0 && ctx->dc_sweeps > 0 && n == ctx->dc_sweeps) n = ctx->dc_level_sweeps[level];
+    TC_TRY(run_sweeps(ctx, d, level, n, dir, f32, view, rhs32, nullptr, zero_start));
     // FP64: compute_residual now; FP32: fused into the restriction of the
     // next descent (k_residual_restrict32), the only reader of the residual
     if (level > 1 && !f32) {
@@ -929,6 +941,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             const int level = multi[p];
             const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
             Level& L = ctx->lv[level];
+            const bool zero_start = multi[p - 1] < multi[p] && tb_zero_start(ctx, d, level, f32);
             if (multi[p - 1] < multi[p]) {
                 Level& Lf = ctx->lv[level - 1];
                 if (f32) {
@@ -939,14 +952,14 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
                                                    L.rhs32, L.g);
                         ctx->stats.residuals_exec[level - 2]++;
                     }
-                    TC_HIP(hipMemsetAsync(L.err32[L.cur32], 0, L.g.elems * sizeof(float), ctx->stream));
+                    if (!zero_start) TC_HIP(hipMemsetAsync(L.err32[L.cur32], 0, L.g.elems * sizeof(float), ctx->stream));
                 } else {
                     launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
                     TC_HIP(hipMemsetAsync(L.err[L.cur], 0, plane_bytes(L.g), ctx->stream));
                 }
             }
             if (level < G) {
-                TC_TRY(dc_iterate(ctx, d, level, dir, f32));
+                TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
             } else {
                 TC_TRY(wait_coarse(ctx, d));
                 if (f32) launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg);
@@ -1781,6 +1794,13 @@ int tc_set_mode(tc_ctx* ctx, int mode)
     if (!ctx) return TC_ERR_ARG;
     if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED && mode != TC_MODE_DC64) return TC_ERR_ARG;
     ctx->mode = mode;
+    return 0;
+}
+
+int tc_set_dc_level_sweeps(tc_ctx* ctx, int32_t level, int32_t sweeps)
+{
+    if (!ctx || level < 1 || level > ctx->p.grids || sweeps < 0) return TC_ERR_ARG;
+    ctx->dc_level_sweeps[level] = sweeps;
     return 0;
 }
 

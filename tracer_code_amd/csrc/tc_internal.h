@@ -106,6 +106,7 @@ struct tc_ctx {
     int guess = 0;                // initial guess of the defect-correction modes (TC_GUESS_*)
     int dc_sweeps = 0;            // sweeps per visit of the defect-correction cycle (0: p.Nmulti)
     int dc_sweeps_active = 0;     // the count the running defect-correction solve uses
+    int dc_level_sweeps[TC_MAX_LEVELS + 1] = {};  // per-level counts of the tuned cycle (0: dc_sweeps)
     int solve_stage = -1;         // RK stage of the running solve within the time step (0..3), -1 outside one
     double solve_tau = 0.0;       // model time the stage's buoyancy field stands for
     hipStream_t stream = nullptr;

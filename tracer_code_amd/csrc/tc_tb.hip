@@ -161,7 +161,8 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
         const int li = l % R, lk = l / R;
         const int i = gi0 + li, k = gk0 + lk;
         T v = T(0);
-        if (i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
+        // Ein == nullptr: a visit that starts from a zero correction
+        if (Ein && i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
         buf[0][l] = v;
         buf[1][l] = v;
         // sweep s updates the cells at least 2(s+1) from the region edge

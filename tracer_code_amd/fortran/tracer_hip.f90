@@ -272,6 +272,11 @@ module tracer_hip
       type(c_ptr), value :: ctx
       type(tc_equalize_info), intent(out) :: info
     end function
+    integer(c_int) function tc_set_dc_level_sweeps(ctx, level, sweeps) bind(C, name="tc_set_dc_level_sweeps")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: level, sweeps
+    end function
     integer(c_int) function tc_set_tb_tile(ctx, level, tile) bind(C, name="tc_set_tb_tile")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

@@ -280,6 +280,10 @@ class Solver:
         """Sweeps per level visit of the defect-correction cycle (0: Nmulti)."""
         self._check(self.lib.tc_set_dc_sweeps(self.h, int(sweeps)), "set_dc_sweeps")
 
+    def set_dc_level_sweeps(self, level, sweeps):
+        """Sweeps per visit of one level of the tuned cycle (0: the common count)."""
+        self._check(self.lib.tc_set_dc_level_sweeps(self.h, int(level), int(sweeps)), "set_dc_level_sweeps")
+
     def autotune_dc_sweeps(self, candidates, steps_each=1):
         """Runs steps_each time steps with each candidate sweep count and
         keeps the best; returns (best, {candidate: ms per step})."""
