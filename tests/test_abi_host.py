@@ -44,6 +44,21 @@ def test_params_struct_layout_matches_c(tmp_path):
     assert int(out[3]) == _abi.TcStats.mg_cycles.offset
 
 
+def test_record_struct_layouts_match_c(tmp_path):
+    prog = tmp_path / "rec.c"
+    prog.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "tracer_hip.h"\n'
+                    'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tc_equalize_info),'
+                    ' offsetof(tc_equalize_info, deficit), offsetof(tc_equalize_info, count),'
+                    ' sizeof(tc_stats_row), offsetof(tc_stats_row, empty_cells), offsetof(tc_stats_row, ent));'
+                    'return 0;}\n')
+    exe = tmp_path / "rec"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    E, R = _abi.TcEqualizeInfo, _abi.TcStatsRow
+    assert out == [ctypes.sizeof(E), E.deficit.offset, E.count.offset, ctypes.sizeof(R), R.empty_cells.offset,
+                   R.ent.offset]
+
+
 def test_init_rejects_unsupported_parameters():
     lib = _abi.load_library()
     p = config.make_params(32, 32)

@@ -1,11 +1,14 @@
 ! Prints the size and selected field offsets of the interoperable types, for
-! comparison with the C compiler's layout of tc_params / tc_stats.
+! comparison with the C compiler's layout of tc_params, tc_stats and the
+! equalize and stats records.
 program tc_layout
   use iso_c_binding
   use tracer_hip
   implicit none
   type(tc_params), target :: p
   type(tc_stats), target :: s
+  type(tc_equalize_info), target :: e
+  type(tc_stats_rec), target :: r
   integer(c_intptr_t) :: b0, b1
   b0 = transfer(c_loc(p), b0)
   write(*, '(a,i0)') 'sizeof_params ', c_sizeof(p)
@@ -19,4 +22,12 @@ program tc_layout
   write(*, '(a,i0)') 'sizeof_stats ', c_sizeof(s)
   b1 = transfer(c_loc(s%residual_last), b1); write(*, '(a,i0)') 'off_residual_last ', b1 - b0
   b1 = transfer(c_loc(s%mg_cycles), b1);     write(*, '(a,i0)') 'off_mg_cycles ', b1 - b0
+  b0 = transfer(c_loc(e), b0)
+  write(*, '(a,i0)') 'sizeof_equalize ', c_sizeof(e)
+  b1 = transfer(c_loc(e%deficit), b1);       write(*, '(a,i0)') 'off_deficit ', b1 - b0
+  b1 = transfer(c_loc(e%count), b1);         write(*, '(a,i0)') 'off_count ', b1 - b0
+  b0 = transfer(c_loc(r), b0)
+  write(*, '(a,i0)') 'sizeof_stats_row ', c_sizeof(r)
+  b1 = transfer(c_loc(r%empty_cells), b1);   write(*, '(a,i0)') 'off_empty_cells ', b1 - b0
+  b1 = transfer(c_loc(r%ent), b1);           write(*, '(a,i0)') 'off_ent ', b1 - b0
 end program tc_layout
