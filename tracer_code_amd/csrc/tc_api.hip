@@ -1240,6 +1240,8 @@ int eq_alloc(tc_ctx* ctx)
     TC_HIP(hipMalloc((void**)&ctx->eq_rows, (size_t)(g.nx + 1) * sizeof(int32_t)));
     TC_HIP(hipMalloc((void**)&ctx->eq_cols, (size_t)(g.nz + 1) * sizeof(int32_t)));
     TC_HIP(hipMalloc((void**)&ctx->eq_bcnt, (size_t)eq_pass_blocks(g) * EQ_MAXT * sizeof(int32_t)));
+    TC_HIP(hipMalloc((void**)&ctx->eq_part, (size_t)eq_pass_blocks(g) * sizeof(EqPart)));
+    TC_HIP(hipHostMalloc((void**)&ctx->eq_hord, (size_t)(g.nx + g.nz + 2) * sizeof(int32_t), hipHostMallocDefault));
     for (int b = 0; b < 2; ++b) {
         TC_HIP(hipMalloc((void**)&ctx->eq_k[b], (size_t)n * sizeof(uint64_t)));
         TC_HIP(hipMalloc((void**)&ctx->eq_v[b], (size_t)n * sizeof(uint32_t)));
@@ -1320,21 +1322,23 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
     }
     // tpc_type (composition.f90:114-140) and nexcess (:118-161)
     EqTypes tpc{};
-    launch_eq_clear(st, ctx->eq_red, true);
+    launch_eq_clear(st, ctx->eq_red);
     if (P.comp == 0) {
         tpc.v[0] = ctx->eq_tpc;
     } else {
-        launch_eq_tpc_stats(st, cnt, cn, g, P.ntype, P.comp, ctx->eq_red);
+        launch_eq_tpc_stats(st, cnt, cn, g, P.ntype, P.comp, ctx->eq_part, ctx->eq_red);
         TC_TRY(eq_fetch(ctx));
-        for (int t = 0; t <= nt; ++t)
-            tpc.v[t] = h.sel_n[t] ? (int32_t)std::llround((double)h.sel_sum[t] / (double)h.sel_n[t]) : 0;
+        for (int t = 0; t <= nt; ++t) {
+            const long long nsel = h.fs[2 * t], sum = h.fs[2 * t + 1];
+            tpc.v[t] = nsel ? (int32_t)std::llround((double)sum / (double)nsel) : 0;
+        }
     }
     for (int t = 0; t <= nt; ++t) info.tpc_type[t] = tpc.v[t];
-    launch_eq_nexcess(st, cnt, ctx->nempty, g, nt, P.comp, tpc, ctx->eq_ne, ctx->eq_red);
+    launch_eq_nexcess(st, cnt, ctx->nempty, g, nt, P.comp, tpc, ctx->eq_ne, ctx->eq_part, ctx->eq_red);
     TC_TRY(eq_fetch(ctx));
-    const int64_t nfree = (int64_t)h.nfree;
+    const int64_t nfree = h.fs[0];
     info.nfree = nfree;
-    if (!h.any) {   // composition.f90:166-171
+    if (h.fs[1] == 0) {   // every nexcess is 0 (composition.f90:166-171)
         if (out) *out = info;
         return 0;
     }
@@ -1378,12 +1382,14 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
         acc += nfree_t[t];
     }
     // lottery (composition.f90:362-463), then the second loop (:466-541)
-    launch_eq_iota(st, ctx->eq_rows, g.nx + 1, ctx->eq_cols, g.nz + 1);
-    launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_red);
+    std::vector<int32_t> rows(g.nx + 1), cols(g.nz + 1);
+    for (int i = 0; i <= g.nx; ++i) rows[i] = i;
+    for (int k = 0; k <= g.nz; ++k) cols[k] = k;
+    launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_part, ctx->eq_red);
     TC_TRY(eq_fetch(ctx));
     auto balanced = [&]() {
         for (int t = 0; t <= nt; ++t)
-            if ((int64_t)h.tcount[t] != Nideal[t]) return false;
+            if (h.fs[t] != Nideal[t]) return false;
         return true;
     };
     bool escape = balanced();
@@ -1396,7 +1402,7 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
         int64_t su = 0;
         for (int p = 0; p <= nt; ++p) {
             const bool gate = loop == 0 ? tpc.v[p] > 0 : tpc.v[0] > 0;
-            mx.v[p] = gate ? h.raw_min[p] : 0;
+            mx.v[p] = gate ? h.fm[p] : 0;
             if (mx.v[p] != 0) allz = false;
             su += used[p];
         }
@@ -1404,12 +1410,18 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
             if (loop == 0) { loop = 1; continue; }
             break;
         }
-        launch_eq_shuffle_orders(st, ctx->eq_rows, g.nx + 1, ctx->eq_cols, g.nz + 1, key, pass);
+        // this pass's visit orders (the previous host copy from eq_hord is done: eq_fetch synchronised)
+        eq_visit_orders(rows.data(), g.nx + 1, cols.data(), g.nz + 1, key, pass);
+        std::memcpy(ctx->eq_hord, rows.data(), rows.size() * sizeof(int32_t));
+        std::memcpy(ctx->eq_hord + rows.size(), cols.data(), cols.size() * sizeof(int32_t));
+        TC_HIP(hipMemcpyAsync(ctx->eq_rows, ctx->eq_hord, rows.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        TC_HIP(hipMemcpyAsync(ctx->eq_cols, ctx->eq_hord + rows.size(), cols.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, st));
         pass++;
         int tg = -1;
         if (loop == 0 && !escape)
             for (int p = 0; p <= nt; ++p)
-                if ((int64_t)h.tcount[p] > Nideal[p]) { tg = p; break; }
+                if (h.fs[p] > Nideal[p]) { tg = p; break; }
         EqOff first{}, avail{};
         for (int t = 0; t <= nt; ++t) {
             first.v[t] = off[t] + used[t];
@@ -1418,7 +1430,7 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
         launch_eq_pass(st, ctx->eq_rows, ctx->eq_cols, ctx->eq_ne, cnt, g, nt, tg, mx, ctx->eq_bcnt, donors, first,
                        avail, ctx->rtr, ctx->str, ctx->ttype, ctx->tr_id, key, dr, ds, dist_r, dist_s, P.aspect, P.comp,
                        loop == 0 ? 1 : 0, ctx->eq_red);
-        launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_red);
+        launch_eq_reduce(st, ctx->eq_ne, cnt, cn, g, nt, P.ntype, P.comp, ctx->eq_part, ctx->eq_red);
         TC_TRY(eq_fetch(ctx));
         int64_t moved = 0;
         for (int t = 0; t <= nt; ++t) {
@@ -1433,11 +1445,11 @@ int equalize(tc_ctx* ctx, tc_dev& d, tc_equalize_info* out)
     }
     info.passes = (int32_t)pass;
     for (int t = 0; t <= nt; ++t) {
-        info.minexcess[t] = h.raw_min[t];
-        info.count[t] = (int64_t)h.tcount[t];
+        info.minexcess[t] = h.fm[t];
+        info.count[t] = h.fs[t];
         info.moved += used[t];
     }
-    info.deficit = h.deficit;
+    info.deficit = h.fs[EQ_MAXT];
     if (out) *out = info;
     // tracers_to_corners without the C filter (composition.f90:576-578)
     return tracers_to_corners(ctx, d, TC_TR_X, 0, nullptr);
@@ -1579,7 +1591,8 @@ void free_ctx(tc_ctx* ctx)
     for (int b = 0; b < 2; ++b) { F(ctx->sort_k[b]); F(ctx->sort_v[b]); }
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     F(ctx->eq_ne); F(ctx->eq_start); F(ctx->eq_end); F(ctx->eq_rows); F(ctx->eq_cols); F(ctx->eq_bcnt);
-    F(ctx->eq_tmp); F(ctx->eq_red);
+    F(ctx->eq_tmp); F(ctx->eq_red); F(ctx->eq_part);
+    if (ctx->eq_hord) (void)hipHostFree(ctx->eq_hord);
     for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
     if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);
     if (ctx->snap_stream) (void)hipStreamSynchronize(ctx->snap_stream);

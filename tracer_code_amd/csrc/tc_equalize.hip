@@ -38,7 +38,7 @@ __host__ __device__ __forceinline__ uint64_t eq_mix(uint64_t x)
 }
 
 // uniform double in [0, 1) with 53 random bits
-__device__ __forceinline__ double eq_u(uint64_t key, unsigned stream, uint64_t a, uint64_t b)
+__host__ __device__ __forceinline__ double eq_u(uint64_t key, unsigned stream, uint64_t a, uint64_t b)
 {
     uint64_t x = eq_mix(key ^ ((uint64_t)stream << 48) ^ a);
     x = eq_mix(x + EQ_GOLD * (b + 1));
@@ -63,7 +63,7 @@ __device__ __forceinline__ uint64_t eq_p40(uint64_t key, uint64_t id)
 // shuffle_serial (composition.f90:22-33): each element swaps with a later
 // one drawn uniformly (Sattolo's algorithm)
 template <typename T>
-__device__ __forceinline__ void eq_shuffle(T* a, int n, uint64_t key, unsigned stream, uint64_t id)
+__host__ __device__ __forceinline__ void eq_shuffle(T* a, int n, uint64_t key, unsigned stream, uint64_t id)
 {
     for (int j = 0; j + 1 < n; ++j) {
         const double u = eq_u(key, stream, id, (uint64_t)j);
@@ -89,6 +89,46 @@ __device__ __forceinline__ int wave_min_i32(int v)
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
     return v;
 }
+
+// workgroup reductions (EQ_BLOCK threads); every thread gets the result
+__device__ __forceinline__ long long eq_block_sum(long long v, long long* lds)
+{
+    v = wave_sum_i64(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    long long t = 0;
+    for (int w = 0; w < EQ_BLOCK / 64; ++w) t += lds[w];
+    return t;
+}
+__device__ __forceinline__ int eq_block_min(int v, long long* lds)
+{
+    v = wave_min_i32(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = 2147483647;
+    for (int w = 0; w < EQ_BLOCK / 64; ++w) t = min(t, (int)lds[w]);
+    return t;
+}
+
+// one workgroup's partial record: sums s[] (0 if unset), minima m[]
+struct EqAcc {
+    long long s[EQ_NS];
+    int m[EQ_NM];
+    __device__ void init()
+    {
+        for (int j = 0; j < EQ_NS; ++j) s[j] = 0;
+        for (int j = 0; j < EQ_NM; ++j) m[j] = 2147483647;
+    }
+    __device__ void store(EqPart* part) const
+    {
+        if (threadIdx.x != 0) return;
+        EqPart& q = part[blockIdx.x];
+        for (int j = 0; j < EQ_NS; ++j) q.s[j] = s[j];
+        for (int j = 0; j < EQ_NM; ++j) q.m[j] = m[j];
+    }
+};
 
 // interior node of a flat index over (nx+1)(nz+1)
 __device__ __forceinline__ bool eq_node(int64_t t, int nx, int nz, int& i, int& k)
@@ -126,45 +166,63 @@ __device__ __forceinline__ int eq_type_save(const CntArr& cnt, size_t c, int nt)
 }  // namespace
 
 // -------------------------------------------------------------------------
-__global__ void k_eq_clear(EqRed* red, int all)
+__global__ void k_eq_clear(EqRed* red)
 {
     const int t = threadIdx.x;
-    if (t < EQ_MAXT) {
-        red->raw_min[t] = 2147483647;
-        red->tcount[t] = 0;
-        if (all) { red->dcount[t] = 0; red->sel_n[t] = 0; red->sel_sum[t] = 0; red->qtot[t] = 0; }
-    }
-    if (t == 0) {
-        red->deficit = 0;
-        if (all) { red->nfree = 0; red->found = 0; red->any = 0; red->bad = 0; }
+    if (t < EQ_MAXT) { red->dcount[t] = 0; red->qtot[t] = 0; }
+    if (t == 0) { red->found = 0; red->bad = 0; }
+}
+
+// the final sums and minima of the per-workgroup partials, in fixed order
+__global__ __launch_bounds__(1024) void k_eq_final(const EqPart* __restrict__ part, int nblk, EqRed* red)
+{
+    __shared__ long long lds[16];
+    for (int j = 0; j < EQ_NS + EQ_NM; ++j) {
+        long long a = j < EQ_NS ? 0 : 2147483647;
+        for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+            a = j < EQ_NS ? a + part[b].s[j] : min(a, (long long)part[b].m[j - EQ_NS]);
+        if (j < EQ_NS) {
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        } else {
+            for (int o = 32; o > 0; o >>= 1) a = min(a, __shfl_xor(a, o, 64));
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long t = lds[0];
+            for (int w = 1; w < (int)(blockDim.x / 64); ++w) t = j < EQ_NS ? t + lds[w] : min(t, lds[w]);
+            if (j < EQ_NS) red->fs[j] = t;
+            else red->fm[j - EQ_NS] = (int32_t)t;
+        }
     }
 }
 
 // tpc_type statistics (composition.f90:114-140): nodes of each mask and
-// the sum of that type's counts over them
+// the sum of that type's counts over them: s[2p], s[2p+1]
 __global__ __launch_bounds__(EQ_BLOCK) void k_eq_tpc_stats(CntArr cnt, PtrArr Cnew, Grid g, int ntype, int comp,
-                                                           EqRed* red)
+                                                           EqPart* part)
 {
+    __shared__ long long lds[EQ_BLOCK / 64];
+    EqAcc acc;
+    acc.init();
     int i = 0, k = 0;
     const bool in = eq_node((int64_t)blockIdx.x * EQ_BLOCK + threadIdx.x, g.nx, g.nz, i, k);
     const size_t c = in ? g.at(i, k) : 0;
-    const int lane = threadIdx.x & 63;
     for (int p = 0; p <= ntype; ++p) {
         const bool sel = in && eq_mask(Cnew, c, p, ntype, comp);
-        const uint64_t n = wave_sum_u64(sel ? 1 : 0);
-        const uint64_t s = wave_sum_u64(sel ? (uint64_t)cnt.v[p][c] : 0);
-        if (lane == 0 && n) {
-            atomicAdd(&red->sel_n[p], (unsigned long long)n);
-            atomicAdd(&red->sel_sum[p], (unsigned long long)s);
-        }
+        acc.s[2 * p] = eq_block_sum(sel ? 1 : 0, lds);
+        acc.s[2 * p + 1] = eq_block_sum(sel ? (long long)cnt.v[p][c] : 0, lds);
     }
+    acc.store(part);
 }
 
 // nexcess (composition.f90:118-161)
 __global__ __launch_bounds__(EQ_BLOCK) void k_eq_nexcess(CntArr cnt, const int32_t* __restrict__ nempty, Grid g,
                                                          int nt, int comp, EqTypes tpc, int32_t* __restrict__ ne,
-                                                         EqRed* red)
+                                                         EqPart* part)
 {
+    __shared__ long long lds[EQ_BLOCK / 64];
     int i = 0, k = 0;
     const int64_t t = (int64_t)blockIdx.x * EQ_BLOCK + threadIdx.x;
     const bool in = eq_node(t, g.nx, g.nz, i, k);
@@ -187,12 +245,11 @@ __global__ __launch_bounds__(EQ_BLOCK) void k_eq_nexcess(CntArr cnt, const int32
         }
         ne[t] = v;
     }
-    const uint64_t pos = wave_sum_u64(v > 0 ? (uint64_t)v : 0);
-    const unsigned long long nz = __ballot(v != 0);
-    if ((threadIdx.x & 63) == 0) {
-        if (pos) atomicAdd(&red->nfree, (unsigned long long)pos);
-        if (nz) atomicOr(&red->any, 1);
-    }
+    EqAcc acc;
+    acc.init();
+    acc.s[0] = eq_block_sum(v > 0 ? v : 0, lds);   // nfree
+    acc.s[1] = eq_block_sum(v != 0 ? 1 : 0, lds);  // nodes with nexcess != 0
+    acc.store(part);
 }
 
 // (node << 32 | tracer index, slot) pairs; node = tracers_to_corners' NINT
@@ -280,51 +337,26 @@ __global__ __launch_bounds__(EQ_BLOCK) void k_eq_select(const int32_t* __restric
         if (per[t]) atomicAdd(&red->dcount[t], (unsigned long long)per[t]);
 }
 
-// minexcess selections before the type gates (composition.f90:387-423),
-// per-type totals (tracer_count, :381-385) and the deficit (:566-574)
+// minexcess selections before the type gates (composition.f90:387-423):
+// m[p]; per-type totals (tracer_count, :381-385): s[p]; the deficit
+// (:566-574): s[EQ_MAXT]
 __global__ __launch_bounds__(EQ_BLOCK) void k_eq_reduce(const int32_t* __restrict__ ne, CntArr cnt, PtrArr Cnew,
-                                                        Grid g, int nt, int ntype, int comp, EqRed* red)
+                                                        Grid g, int nt, int ntype, int comp, EqPart* part)
 {
+    __shared__ long long lds[EQ_BLOCK / 64];
+    EqAcc acc;
+    acc.init();
     int i = 0, k = 0;
     const int64_t t = (int64_t)blockIdx.x * EQ_BLOCK + threadIdx.x;
     const bool in = eq_node(t, g.nx, g.nz, i, k);
     const size_t c = in ? g.at(i, k) : 0;
     const int v = in ? ne[t] : 1;
-    const bool lane0 = (threadIdx.x & 63) == 0;
     for (int p = 0; p <= nt; ++p) {
-        const int m = wave_min_i32((in && v <= 0 && eq_mask(Cnew, c, p, ntype, comp)) ? v : 2147483647);
-        const uint64_t s = wave_sum_u64(in ? (uint64_t)(int64_t)cnt.v[p][c] : 0);
-        if (lane0) {
-            if (m != 2147483647) atomicMin(&red->raw_min[p], m);
-            if (s) atomicAdd(&red->tcount[p], (unsigned long long)s);
-        }
+        acc.m[p] = eq_block_min((in && v <= 0 && eq_mask(Cnew, c, p, ntype, comp)) ? v : 2147483647, lds);
+        acc.s[p] = eq_block_sum(in ? (long long)cnt.v[p][c] : 0, lds);
     }
-    const int64_t d = wave_sum_i64((in && v <= 0) ? (int64_t)v : 0);
-    if (lane0 && d) atomicAdd((unsigned long long*)&red->deficit, (unsigned long long)d);
-}
-
-__global__ void k_eq_iota(int32_t* __restrict__ rows, int nr, int32_t* __restrict__ cols, int nc)
-{
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nr + nc; j += gridDim.x * blockDim.x) {
-        if (j < nr) rows[j] = j;
-        else cols[j - nr] = j - nr;
-    }
-}
-
-// the two visit-order shuffles of a pass (composition.f90:427-429), one
-// workgroup each, sequential in LDS
-__global__ __launch_bounds__(256) void k_eq_shuffle_orders(int32_t* __restrict__ rows, int nr,
-                                                           int32_t* __restrict__ cols, int nc, uint64_t key,
-                                                           uint64_t pass)
-{
-    extern __shared__ int32_t lds[];
-    int32_t* a = blockIdx.x == 0 ? rows : cols;
-    const int n = blockIdx.x == 0 ? nr : nc;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) lds[j] = a[j];
-    __syncthreads();
-    if (threadIdx.x == 0) eq_shuffle(lds, n, key, blockIdx.x == 0 ? EQ_S_ROWS : EQ_S_COLS, pass);
-    __syncthreads();
-    for (int j = threadIdx.x; j < n; j += blockDim.x) a[j] = lds[j];
+    acc.s[EQ_MAXT] = eq_block_sum((in && v <= 0) ? (long long)v : 0, lds);
+    acc.store(part);
 }
 
 namespace {
@@ -469,23 +501,32 @@ uint64_t eq_stream_key(uint64_t seed, uint64_t call) { return eq_mix(seed + EQ_G
 
 static inline unsigned eq_blocks(int64_t n) { return (unsigned)((n + EQ_BLOCK - 1) / EQ_BLOCK); }
 
-void launch_eq_clear(hipStream_t st, EqRed* red, bool all)
+void launch_eq_clear(hipStream_t st, EqRed* red)
 {
-    hipLaunchKernelGGL(k_eq_clear, dim3(1), dim3(64), 0, st, red, all ? 1 : 0);
+    hipLaunchKernelGGL(k_eq_clear, dim3(1), dim3(64), 0, st, red);
+}
+
+static void eq_final(hipStream_t st, const Grid& g, EqPart* part, EqRed* red)
+{
+    const int nblk = (int)eq_blocks((int64_t)(g.nx + 1) * (g.nz + 1));
+    hipLaunchKernelGGL(k_eq_final, dim3(1), dim3(1024), 0, st, part, nblk, red);
 }
 
 void launch_eq_tpc_stats(hipStream_t st, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int ntype, int comp,
-                         EqRed* red)
+                         EqPart* part, EqRed* red)
 {
     const int64_t V = (int64_t)(g.nx + 1) * (g.nz + 1);
-    hipLaunchKernelGGL(k_eq_tpc_stats, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, cnt, Cnew, g, ntype, comp, red);
+    hipLaunchKernelGGL(k_eq_tpc_stats, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, cnt, Cnew, g, ntype, comp, part);
+    eq_final(st, g, part, red);
 }
 
 void launch_eq_nexcess(hipStream_t st, const CntArr& cnt, const int32_t* nempty, const Grid& g, int nt, int comp,
-                       const EqTypes& tpc, int32_t* ne, EqRed* red)
+                       const EqTypes& tpc, int32_t* ne, EqPart* part, EqRed* red)
 {
     const int64_t V = (int64_t)(g.nx + 1) * (g.nz + 1);
-    hipLaunchKernelGGL(k_eq_nexcess, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, cnt, nempty, g, nt, comp, tpc, ne, red);
+    hipLaunchKernelGGL(k_eq_nexcess, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, cnt, nempty, g, nt, comp, tpc, ne,
+                       part);
+    eq_final(st, g, part, red);
 }
 
 void launch_eq_node_lists(hipStream_t st, const double* r, const double* s, const int64_t* id, int64_t n,
@@ -514,23 +555,20 @@ void launch_eq_select(hipStream_t st, const int32_t* ne, const int32_t* start, c
 }
 
 void launch_eq_reduce(hipStream_t st, const int32_t* ne, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int nt,
-                      int ntype, int comp, EqRed* red)
+                      int ntype, int comp, EqPart* part, EqRed* red)
 {
     const int64_t V = (int64_t)(g.nx + 1) * (g.nz + 1);
-    hipLaunchKernelGGL(k_eq_clear, dim3(1), dim3(64), 0, st, red, 0);
-    hipLaunchKernelGGL(k_eq_reduce, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, ne, cnt, Cnew, g, nt, ntype, comp, red);
+    hipLaunchKernelGGL(k_eq_reduce, dim3(eq_blocks(V)), dim3(EQ_BLOCK), 0, st, ne, cnt, Cnew, g, nt, ntype, comp, part);
+    eq_final(st, g, part, red);
 }
 
-void launch_eq_iota(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc)
+// the row and column visit orders of lottery pass `pass`: each pass
+// shuffles the previous orders (composition.f90:427-429); on the host, which
+// runs the pass loop
+void eq_visit_orders(int32_t* rows, int nr, int32_t* cols, int nc, uint64_t key, uint64_t pass)
 {
-    hipLaunchKernelGGL(k_eq_iota, dim3(eq_blocks(nr + nc)), dim3(EQ_BLOCK), 0, st, rows, nr, cols, nc);
-}
-
-void launch_eq_shuffle_orders(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc, uint64_t key,
-                              uint64_t pass)
-{
-    const size_t lds = (size_t)std::max(nr, nc) * sizeof(int32_t);
-    hipLaunchKernelGGL(k_eq_shuffle_orders, dim3(2), dim3(256), lds, st, rows, nr, cols, nc, key, pass);
+    eq_shuffle(rows, nr, key, EQ_S_ROWS, pass);
+    eq_shuffle(cols, nc, key, EQ_S_COLS, pass);
 }
 
 int eq_pass_blocks(const Grid& g) { return (int)eq_blocks((int64_t)(g.nx + 1) * (g.nz + 1)); }

@@ -85,15 +85,17 @@ struct Level {
 constexpr int EQ_MAXT = 8;                 // type slots (>= TC_MAX_TYPES + 1)
 struct EqTypes { int32_t v[EQ_MAXT]; };
 struct EqOff { int64_t v[EQ_MAXT]; };
+constexpr int EQ_NS = 16, EQ_NM = 8;      // sums and minima of one reduction
+struct EqPart {                            // one workgroup's share of a reduction
+    long long s[EQ_NS];
+    int32_t m[EQ_NM];
+};
 struct EqRed {
-    int32_t raw_min[EQ_MAXT];              // min nexcess <= 0 over each type's selection (2147483647: none)
-    int32_t any;                           // some nexcess != 0
-    int32_t bad;                           // a tracer outside the domain
-    unsigned long long tcount[EQ_MAXT];    // sum of the counts of each type over the nodes
-    long long deficit;                     // sum of nexcess <= 0
-    unsigned long long nfree, found;       // sum of nexcess > 0; donors found
+    long long fs[EQ_NS];                   // the last reduction: sums
+    int32_t fm[EQ_NM];                     //                     minima (2147483647: none)
+    int32_t bad, pad;                      // a tracer outside the domain
+    unsigned long long found;              // donors found
     unsigned long long dcount[EQ_MAXT];    // donors of each type
-    unsigned long long sel_n[EQ_MAXT], sel_sum[EQ_MAXT];  // tpc_type selections: nodes, count sums
     unsigned long long qtot[EQ_MAXT];      // qualifying nodes of each donor type in the last pass
 };
 
@@ -201,6 +203,8 @@ struct tc_ctx {
     uint64_t eq_seed = 0, eq_calls = 0;
     int32_t *eq_ne = nullptr, *eq_start = nullptr, *eq_end = nullptr, *eq_rows = nullptr, *eq_cols = nullptr;
     int32_t* eq_bcnt = nullptr;
+    tc::EqPart* eq_part = nullptr;   // per-workgroup partials of the reductions
+    int32_t* eq_hord = nullptr;      // pinned visit orders (rows, then columns)
     uint64_t* eq_k[2] = {nullptr, nullptr};
     uint32_t* eq_v[2] = {nullptr, nullptr};
     void* eq_tmp = nullptr;

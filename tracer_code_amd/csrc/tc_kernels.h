@@ -168,11 +168,11 @@ int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* k
 
 // equalize_tracers (tc_equalize.hip)
 uint64_t eq_stream_key(uint64_t seed, uint64_t call);
-void launch_eq_clear(hipStream_t st, EqRed* red, bool all);
+void launch_eq_clear(hipStream_t st, EqRed* red);
 void launch_eq_tpc_stats(hipStream_t st, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int ntype, int comp,
-                         EqRed* red);
+                         EqPart* part, EqRed* red);
 void launch_eq_nexcess(hipStream_t st, const CntArr& cnt, const int32_t* nempty, const Grid& g, int nt, int comp,
-                       const EqTypes& tpc, int32_t* ne, EqRed* red);
+                       const EqTypes& tpc, int32_t* ne, EqPart* part, EqRed* red);
 void launch_eq_node_lists(hipStream_t st, const double* r, const double* s, const int64_t* id, int64_t n,
                           const Grid& g, double dr, double ds, double aspect, uint64_t* key, uint32_t* val,
                           int32_t* start, int32_t* end, EqRed* red);
@@ -182,10 +182,8 @@ void launch_eq_select(hipStream_t st, const int32_t* ne, const int32_t* start, c
                       double dist_r, double dist_s, double aspect, uint64_t key, uint64_t* dkey, uint32_t* dval,
                       EqRed* red);
 void launch_eq_reduce(hipStream_t st, const int32_t* ne, const CntArr& cnt, const PtrArr& Cnew, const Grid& g, int nt,
-                      int ntype, int comp, EqRed* red);
-void launch_eq_iota(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc);
-void launch_eq_shuffle_orders(hipStream_t st, int32_t* rows, int nr, int32_t* cols, int nc, uint64_t key,
-                              uint64_t pass);
+                      int ntype, int comp, EqPart* part, EqRed* red);
+void eq_visit_orders(int32_t* rows, int nr, int32_t* cols, int nc, uint64_t key, uint64_t pass);
 int eq_pass_blocks(const Grid& g);
 void launch_eq_pass(hipStream_t st, const int32_t* rows, const int32_t* cols, int32_t* ne, const CntArr& cnt,
                     const Grid& g, int nt, int tg, const EqTypes& mx, int32_t* bcnt, const uint32_t* donors,
