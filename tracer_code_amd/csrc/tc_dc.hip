@@ -122,6 +122,9 @@ __device__ __forceinline__ dd dd_mul(dd a, dd b)
 // Fine-grid residual r = RHS - L(hi+lo) in double-double; writes r (FP64)
 // and its FP32 rounding, and per-block sums of |r|.  The 1/h^n factors are
 // double-double reciprocals (InvSpacing).
+constexpr int RD_BX = 64, RD_BY = 4, RD_H = 2;              // block shape (launch_residual_dd), halo
+constexpr int RD_W = RD_BX + 2 * RD_H, RD_CELLS = RD_W * (RD_BY + 2 * RD_H);
+
 __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ hi, const double* __restrict__ lo,
                                                      const double* __restrict__ vis, const double* __restrict__ vx,
                                                      const double* __restrict__ vz, const double* __restrict__ vxx,
@@ -130,28 +133,47 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
                                                      float* __restrict__ r32, Grid g, FD F, InvSpacing ih, int ivis,
                                                      double* __restrict__ partial)
 {
+    // the block's 64x4 points and a 2-cell halo of hi and lo staged in LDS:
+    // the 62 stencil taps of each point read LDS instead of the L1/TA path
+    __shared__ double sh[RD_CELLS], sl[RD_CELLS];
     const int2 tile = xcd_tile();
+    {
+        const int i0 = tile.x * RD_BX + 1 - RD_H, k0 = tile.y * RD_BY + 1 - RD_H;
+        for (int l = threadIdx.y * RD_BX + threadIdx.x; l < RD_CELLS; l += RD_BX * RD_BY) {
+            const int ii = i0 + l % RD_W, kk = k0 + l / RD_W;
+            double h = 0.0, o = 0.0;
+            if (ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN) {
+                const size_t q = g.at(ii, kk);
+                h = hi[q];
+                o = lo[q];
+            }
+            sh[l] = h;
+            sl[l] = o;
+        }
+    }
+    __syncthreads();
     const int i = tile.x * blockDim.x + threadIdx.x + 1;
     const int k = tile.y * blockDim.y + threadIdx.y + 1;
     double a = 0.0;
     if (i <= g.nx - 1 && k <= g.nz - 1) {
         const size_t c = g.at(i, k);
-        const ptrdiff_t ld = g.ld;
+        const size_t cl = (size_t)(threadIdx.y + RD_H) * RD_W + threadIdx.x + RD_H;
+        const ptrdiff_t ld = RD_W;
         auto sc = [&](dd x, int q) { return dd_mul(x, dd{ih.hi[q], ih.lo[q]}); };
         const double v = vis[c];
-        dd Srrrr = sc(csum_dot(F.D4, -2, 2, hi, lo, c, 1), IH_R4);
-        dd Sssss = sc(csum_dot(F.D4, -2, 2, hi, lo, c, ld), IH_S4);
-        dd Srrss = sc(csum_9(F.Drrss, hi, lo, c, ld), IH_R2S2);
+        dd Srrrr = sc(csum_dot(F.D4, -2, 2, sh, sl, cl, 1), IH_R4);
+        dd Sssss = sc(csum_dot(F.D4, -2, 2, sh, sl, cl, ld), IH_S4);
+        dd Srrss = sc(csum_9(F.Drrss, sh, sl, cl, ld), IH_R2S2);
         dd sum = dd_mul_d(dd_add(Srrrr, Sssss), v);
         sum = dd_add(sum, dd_mul_d(Srrss, 2.0 * v));
         if (ivis != 0) {
-            dd Srr = sc(csum_dot(F.D2, -1, 1, hi, lo, c, 1), IH_R2);
-            dd Sss = sc(csum_dot(F.D2, -1, 1, hi, lo, c, ld), IH_S2);
-            dd Srrr = sc(csum_dot(F.D3, -2, 2, hi, lo, c, 1), IH_R3);
-            dd Ssss = sc(csum_dot(F.D3, -2, 2, hi, lo, c, ld), IH_S3);
-            dd Srs = sc(csum_9(F.Drs, hi, lo, c, ld), IH_RS);
-            dd Srss = sc(csum_9(F.Drss, hi, lo, c, ld), IH_RS2);
-            dd Srrs = sc(csum_9(F.Drrs, hi, lo, c, ld), IH_R2S);
+            dd Srr = sc(csum_dot(F.D2, -1, 1, sh, sl, cl, 1), IH_R2);
+            dd Sss = sc(csum_dot(F.D2, -1, 1, sh, sl, cl, ld), IH_S2);
+            dd Srrr = sc(csum_dot(F.D3, -2, 2, sh, sl, cl, 1), IH_R3);
+            dd Ssss = sc(csum_dot(F.D3, -2, 2, sh, sl, cl, ld), IH_S3);
+            dd Srs = sc(csum_9(F.Drs, sh, sl, cl, ld), IH_RS);
+            dd Srss = sc(csum_9(F.Drss, sh, sl, cl, ld), IH_RS2);
+            dd Srrs = sc(csum_9(F.Drrs, sh, sl, cl, ld), IH_R2S);
             sum = dd_add(sum, dd_mul_d(dd_add(Srr, dd_neg(Sss)), vxx[c] - vzz[c]));
             sum = dd_add(sum, dd_mul_d(Srs, 4.0 * vxz[c]));
             sum = dd_add(sum, dd_mul_d(dd_add(Srrr, Srss), 2.0 * vx[c]));
@@ -391,7 +413,7 @@ void launch_extrap(hipStream_t st, const double* a, const double* b, double c, d
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial)
 {
-    dim3 b(64, 4);
+    dim3 b(RD_BX, RD_BY);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     hipLaunchKernelGGL(k_residual_dd, gr, b, 0, st, hi, lo, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz, rhs, r64,
                        r32, L.g, F, inv_spacing(h), ivis, partial);
