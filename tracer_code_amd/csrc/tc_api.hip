@@ -775,6 +775,7 @@ int dc_alloc(tc_ctx* ctx)
     }
     TC_TRY(dalloc(ctx, &ctx->psi_hi, ctx->g.elems));
     TC_TRY(dalloc(ctx, &ctx->psi_lo, ctx->g.elems));
+    TC_TRY(dalloc(ctx, &ctx->coarse_b, (size_t)std::max(ctx->ngc, 1)));
     return 0;
 }
 
@@ -962,7 +963,9 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
                 TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
             } else {
                 TC_TRY(wait_coarse(ctx, d));
-                if (f32) launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg);
+                if (f32)
+                    launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg,
+                                          ctx->coarse_b);
                 else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
                 ctx->stats.coarse_solves++;
             }
@@ -1591,7 +1594,7 @@ void free_ctx(tc_ctx* ctx)
     for (int b = 0; b < 2; ++b) { F(ctx->sort_k[b]); F(ctx->sort_v[b]); }
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     F(ctx->eq_ne); F(ctx->eq_start); F(ctx->eq_end); F(ctx->eq_rows); F(ctx->eq_cols); F(ctx->eq_bcnt);
-    F(ctx->eq_tmp); F(ctx->eq_red); F(ctx->eq_part);
+    F(ctx->eq_tmp); F(ctx->eq_red); F(ctx->eq_part); F(ctx->coarse_b);
     if (ctx->eq_hord) (void)hipHostFree(ctx->eq_hord);
     for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
     if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);

@@ -343,20 +343,28 @@ __global__ __launch_bounds__(256) void k_prolong32(float* __restrict__ Ef, Grid 
 }
 #pragma clang fp contract(off)
 
-// Coarse solve from an FP32 right-hand side with the FP64 inverse, result
-// rounded to FP32.
-__global__ __launch_bounds__(256) void k_coarse_apply32(const double* __restrict__ Ainv, const double* __restrict__ RowB,
-                                                        const float* __restrict__ rhs, float* __restrict__ E, Grid g,
-                                                        int n, float zsg)
+// The equilibrated right-hand side of a coarse solve, b[q] = rhs * RowB at
+// the interior unknowns (q = i*(nz+1) + k) and 0 at the walls, formed once so
+// that the GEMV below streams only the inverse.
+__global__ __launch_bounds__(256) void k_coarse_rhs32(const double* __restrict__ RowB, const float* __restrict__ rhs,
+                                                      Grid g, int n, double* __restrict__ b)
 {
-    const int nxc = g.nx, nzc = g.nz, N = nzc + 1;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int N = g.nz + 1, kq = q % N, iq = q / N;
+    b[q] = (iq > 0 && iq < g.nx && kq > 0 && kq < g.nz) ? (double)rhs[g.at(iq, kq)] * RowB[q] : 0.0;
+}
+
+// Coarse solve from an FP32 right-hand side with the FP64 inverse, result
+// rounded to FP32: one workgroup per unknown, its row of the inverse times b.
+__global__ __launch_bounds__(256) void k_coarse_apply32(const double* __restrict__ Ainv, const double* __restrict__ b,
+                                                        float* __restrict__ E, Grid g, int n, float zsg)
+{
+    const int N = g.nz + 1;
     const int ip = blockIdx.x + 1, kp = blockIdx.y + 1;
     const double* row = Ainv + (size_t)(N * ip + kp) * (size_t)n;
     double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int kq = q % N, iq = q / N;
-        if (iq > 0 && iq < nxc && kq > 0 && kq < nzc) acc += row[q] * ((double)rhs[g.at(iq, kq)] * RowB[q]);
-    }
+    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += row[q] * b[q];
     acc = block_sum(acc);
     if (threadIdx.x == 0) {
         const float v = (float)acc;
@@ -481,9 +489,10 @@ void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec
 }
 
 void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
-                           float* E, int ngc, double zsg)
+                           float* E, int ngc, double zsg, double* b)
 {
-    hipLaunchKernelGGL(k_coarse_apply32, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, RowB, rhs, E, L.g, ngc,
+    hipLaunchKernelGGL(k_coarse_rhs32, dim3((unsigned)((ngc + 255) / 256)), dim3(256), 0, st, RowB, rhs, L.g, ngc, b);
+    hipLaunchKernelGGL(k_coarse_apply32, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, b, E, L.g, ngc,
                        (float)zsg);
 }
 

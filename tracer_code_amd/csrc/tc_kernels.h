@@ -100,7 +100,8 @@ void launch_residual_restrict32(hipStream_t st, const Level& Lf, const float* S,
                                 const Spacing& h, float* out, const Grid& go);
 void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg);
 void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
-                           float* E, int ngc, double zsg);
+                           float* E, int ngc, double zsg,
+                           double* b);
 void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
 
 // --- tc_tb.hip ------------------------------------------------------------------
