@@ -252,6 +252,12 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
         ctx->nn_store = 1;
         if (option == 1) { ctx->nn_store = 1; Niter = 6; lb = -2.0; la = -1.0; }
         n++;
+        // the reference loops until the eigenvalue ratio test passes; a NaN
+        // ratio (or a failed launch) would never pass it
+        if (n > 100000) {
+            ctx->err = "smoother_vis: eigenvalue-ratio test not met after 100000 passes";
+            return TC_ERR_HIP;
+        }
     }
     return 0;
 }
@@ -1040,6 +1046,10 @@ double* field_ptr(tc_ctx* ctx, int which)
 // batches; the stop flag turns the rest of a batch into no-ops, and the
 // host reads the flag (and the stopping index) once per batch.  Returns the
 // number of smoothing iterations; the result ends in A.
+// the filter loops stop on the smoothness test (smoother.f90); a bound far
+// above any count seen (hundreds) turns a loop that cannot stop into an error
+constexpr int FILTER_MAX_ITERS = 1 << 20;
+
 template <typename CFac>
 int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double delta, double courant, CFac cfac,
                 int* iters)
@@ -1058,9 +1068,15 @@ int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double 
             launch_filter_step_bcs(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2,
                                    ctx->lv[1].ds2, cfac(m), courant, P.Tbot, slots, m, delta, d.ctl);
         }
+        // a launch that failed would leave the stop flag unset for ever
+        TC_HIP(hipGetLastError());
         TC_HIP(hipMemcpyAsync(d.h_ctl, d.ctl, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TC_HIP(hipStreamSynchronize(ctx->stream));
         if (d.h_ctl[0]) break;
+        if (m >= FILTER_MAX_ITERS) {
+            ctx->err = "filter loop: no stop after " + std::to_string(m) + " iterations";
+            return TC_ERR_HIP;
+        }
         if (batch < 64) batch *= 2;
     }
     const int mt = d.h_ctl[1];
@@ -1277,6 +1293,7 @@ int eq_temp(tc_ctx* ctx, int64_t n, int bits)
 
 int eq_fetch(tc_ctx* ctx)
 {
+    TC_HIP(hipGetLastError());
     TC_HIP(hipMemcpyAsync(ctx->eq_hred, ctx->eq_red, sizeof(EqRed), hipMemcpyDeviceToHost, ctx->stream));
     TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
