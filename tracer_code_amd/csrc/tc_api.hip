@@ -328,8 +328,8 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
     const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
     const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
     const bool tb = d.tb && level >= 2 && ctx->mode != TC_MODE_FP64;
-    // sweeps per launch (4 measured faster than 8 on every level at 2048x1024)
-    const int K = 4;
+    // sweeps per launch: 4, or 8 on FP32 levels of at most 16 tiles
+    const int K = (tb && f32 && L.tb_tile != 32) ? tb_sweeps_per_launch(L.g) : 4;
     // level-1 timing (tc_set_profiling): one event pair around the visit's
     // back-to-back launches, or around each launch when the first sweep is
     // followed by the wall reset of the reference form

@@ -108,6 +108,7 @@ void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
 int tb_max_sweeps(int K);
 // levels whose temporally blocked sweeps read precomputed weights (Level::w13)
 bool tb_uses_w13(const Level& L);
+int tb_sweeps_per_launch(const Grid& g);
 void launch_tb_weights32(hipStream_t st, Level& L, const FD& F, const Spacing& h);
 void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
                        double* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,

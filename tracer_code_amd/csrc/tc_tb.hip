@@ -240,6 +240,15 @@ static inline dim3 tb_grid(const Grid& g, int TT) { return dim3((g.nx - 1 + TT -
 
 int tb_max_sweeps(int K) { return K; }
 
+// sweeps per launch of a level's FP32 blocked sweeps: 8 where the level has
+// at most 16 tiles of 16x16 (a launch there is all latency: twice the sweeps
+// for ~1.2x the launch time), else 4 (8 measured slower on larger levels)
+int tb_sweeps_per_launch(const Grid& g)
+{
+    const dim3 t = tb_grid(g, 16);
+    return (t.x * t.y <= 16) ? 8 : 4;
+}
+
 // the 32x32 tile where it still gives at least 2 workgroups per CU (256 CUs)
 // and the arithmetic is FP32 (three staged cells of FP64 weights per thread
 // would spill)
@@ -263,6 +272,16 @@ static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, co
             else
                 hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
             return;
+        }
+        if constexpr (TT == 16) {
+            if (W && K == 8) {
+                if (use_ratio)
+                    hipLaunchKernelGGL((k_sweep_tb<T, true, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+                else
+                    hipLaunchKernelGGL((k_sweep_tb<T, false, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg,
+                                       W);
+                return;
+            }
         }
     }
     if (K == 8) {
