@@ -177,6 +177,17 @@ int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps);
 /* Sweeps per visit of one level of the tuned defect-correction cycle (0:
  * the common count of tc_set_dc_sweeps); the Nmulti retry ignores it. */
 int tc_set_dc_level_sweeps(tc_ctx* ctx, int32_t level, int32_t sweeps);
+/* HIP graphs for the FP32 defect-correction cycle (no counterpart in the
+ * reference): the part of a cycle below level 1 (restrictions, the coarse
+ * levels' visits, the coarse solve, the prolongation into psi and the fine
+ * residual) is the same sequence of launches in every cycle of a solve.  With
+ * enable set it is captured once per solve and replayed as one graph launch
+ * in the solve's other cycles (results bit-identical to the launches).  Meant
+ * for many concurrent contexts (tc_ensemble_step), whose host threads then
+ * issue one launch per cycle instead of hundreds.  Off by default. */
+int tc_set_graphs(tc_ctx* ctx, int32_t enable);
+/* Segment graphs captured and replayed so far. */
+int tc_graph_counts(const tc_ctx* ctx, int64_t* captures, int64_t* replays);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

@@ -936,67 +936,153 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     const bool first_visit_done = judge;
     int q = 1;
     int status = 0;
-    for (;;) {
-        bool done = false;
-        for (int p = 1; p <= P.nsteps; ++p) {
-            if (first_visit_done && q == 1 && p == 1) {
-                // level 1's first visit ran while judging the guess
-                status = stop_test(ctx, d, q, done);
-                if (status || done) break;
-                continue;
-            }
-            const int level = multi[p];
-            const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
-            Level& L = ctx->lv[level];
-            const bool zero_start = multi[p - 1] < multi[p] && tb_zero_start(ctx, d, level, f32);
-            if (multi[p - 1] < multi[p]) {
-                Level& Lf = ctx->lv[level - 1];
-                if (f32) {
-                    if (level - 1 == 1) {
-                        launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
-                    } else {
-                        launch_residual_restrict32(ctx->stream, Lf, Lf.err32[Lf.cur32], Lf.rhs32, d.F, d.h[level - 1],
-                                                   L.rhs32, L.g);
-                        ctx->stats.residuals_exec[level - 2]++;
-                    }
-                    if (!zero_start) TC_HIP(hipMemsetAsync(L.err32[L.cur32], 0, L.g.elems * sizeof(float), ctx->stream));
+    // one step p of the cycle schedule: the descent into the level, its
+    // visit, and (level 1) the stop test or the ascent out of it
+    auto step = [&](int p, bool& done, bool& brk) -> int {
+        const int level = multi[p];
+        const int dir = (multi[p + 1] > multi[p]) ? 0 : 1;
+        Level& L = ctx->lv[level];
+        const bool zero_start = multi[p - 1] < multi[p] && tb_zero_start(ctx, d, level, f32);
+        if (multi[p - 1] < multi[p]) {
+            Level& Lf = ctx->lv[level - 1];
+            if (f32) {
+                if (level - 1 == 1) {
+                    launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
                 } else {
-                    launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
-                    TC_HIP(hipMemsetAsync(L.err[L.cur], 0, plane_bytes(L.g), ctx->stream));
+                    launch_residual_restrict32(ctx->stream, Lf, Lf.err32[Lf.cur32], Lf.rhs32, d.F, d.h[level - 1],
+                                               L.rhs32, L.g);
+                    ctx->stats.residuals_exec[level - 2]++;
                 }
-            }
-            if (level < G) {
-                TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
+                // a fill kernel, not a memset: this can be recorded in a graph
+                if (!zero_start)
+                    launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(L.err32[L.cur32]), L.g.elems);
             } else {
-                TC_TRY(wait_coarse(ctx, d));
-                if (f32)
-                    launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg,
-                                          ctx->coarse_b);
-                else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
-                ctx->stats.coarse_solves++;
-            }
-            if (level == 1) {
-                TC_TRY(dc_absorb_residual(ctx, d, f32));
-                status = stop_test(ctx, d, q, done);
-                if (status || done) break;
-            }
-            if (multi[p + 1] < multi[p]) {
-                Level& Lf = ctx->lv[level - 1];
-                if (level == 2) {
-                    // fold the coarse-grid correction into psi; level 1 then
-                    // post-smooths a zero correction against the new defect
-                    launch_prolong_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf.g, f32 ? L.err32[L.cur32] : nullptr,
-                                      f32 ? nullptr : L.err[L.cur], L.g, zsg);
-                    launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res,
-                                       f32 ? Lf.res32 : nullptr, d.F, d.h[1], P.ivis, nullptr);
-                    ctx->stats.residuals_exec[0]++;
-                } else if (f32) {
-                    launch_prolong32(ctx->stream, Lf.err32[Lf.cur32], Lf.g, L.err32[L.cur32], L.g, zsg);
-                } else {
-                    launch_prolong(ctx->stream, Lf.err[Lf.cur], Lf.g, L.err[L.cur], L.g, zsg);
-                }
+                launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
+                TC_HIP(hipMemsetAsync(L.err[L.cur], 0, plane_bytes(L.g), ctx->stream));
             }
         }
+        if (level < G) {
+            TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
+        } else {
+            TC_TRY(wait_coarse(ctx, d));
+            if (f32)
+                launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg,
+                                      ctx->coarse_b);
+            else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
+            ctx->stats.coarse_solves++;
+        }
+        if (level == 1) {
+            TC_TRY(dc_absorb_residual(ctx, d, f32));
+            status = stop_test(ctx, d, q, done);
+            if (status || done) {
+                brk = true;
+                return 0;
+            }
+        }
+        if (multi[p + 1] < multi[p]) {
+            Level& Lf = ctx->lv[level - 1];
+            if (level == 2) {
+                // fold the coarse-grid correction into psi; level 1 then
+                // post-smooths a zero correction against the new defect
+                launch_prolong_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf.g, f32 ? L.err32[L.cur32] : nullptr,
+                                  f32 ? nullptr : L.err[L.cur], L.g, zsg);
+                launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res, f32 ? Lf.res32 : nullptr,
+                                   d.F, d.h[1], P.ivis, nullptr);
+                ctx->stats.residuals_exec[0]++;
+            } else if (f32) {
+                launch_prolong32(ctx->stream, Lf.err32[Lf.cur32], Lf.g, L.err32[L.cur32], L.g, zsg);
+            } else {
+                launch_prolong(ctx->stream, Lf.err[Lf.cur], Lf.g, L.err[L.cur], L.g, zsg);
+            }
+        }
+        return 0;
+    };
+    // the coarse segment p = 2..nsteps: no level-1 visit and no host
+    // synchronisation, the same launches in every cycle of the solve
+    bool seg_graph = ctx->graphs && f32 && multi[1] == 1;
+    for (int p = 2; p <= P.nsteps && seg_graph; ++p)
+        if (multi[p] < 2) seg_graph = false;
+    for (auto& sg : ctx->seg) sg.valid = false;
+    // (eager, a schedule with a level-1 visit inside it, e.g. the F-cycle,
+    // stops at that visit's stop test as the whole cycle would)
+    auto segment = [&](bool& done, bool& brk) -> int {
+        for (int p = 2; p <= P.nsteps && !brk; ++p) TC_TRY(step(p, done, brk));
+        return 0;
+    };
+    auto run_segment = [&](bool& done, bool& brk) -> int {
+        if (!seg_graph) return segment(done, brk);
+        int par[TC_MAX_LEVELS + 1] = {};
+        for (int l = 2; l <= G; ++l) par[l] = ctx->lv[l].cur32;
+        for (auto& sg : ctx->seg) {
+            if (!sg.valid || std::memcmp(sg.par0, par, sizeof par) != 0) continue;
+            TC_HIP(hipGraphLaunch(sg.exec, ctx->stream));
+            for (int l = 2; l <= G; ++l) ctx->lv[l].cur32 = sg.par1[l];
+            for (int l = 0; l < TC_MAX_LEVELS; ++l) {
+                ctx->stats.sweeps_exec[l] += sg.dsweeps[l];
+                ctx->stats.residuals_exec[l] += sg.dres[l];
+            }
+            ctx->stats.coarse_solves += sg.dcoarse;
+            ctx->seg_replays++;
+            return 0;
+        }
+        // capture this parity's segment (the coarse inverse must be ready:
+        // a wait on the side stream's event cannot be recorded)
+        tc_ctx::SegGraph* sg = !ctx->seg[0].valid ? &ctx->seg[0] : (!ctx->seg[1].valid ? &ctx->seg[1] : nullptr);
+        if (!sg) return segment(done, brk);
+        TC_TRY(wait_coarse(ctx, d));
+        const tc_stats s0 = ctx->stats;
+        TC_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int st = segment(done, brk);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+        if (st) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        TC_HIP(ec);
+        bool updated = false;
+        if (sg->exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res;
+            updated = hipGraphExecUpdate(sg->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                TC_HIP(hipGraphExecDestroy(sg->exec));
+                sg->exec = nullptr;
+            }
+        }
+        if (!updated) {
+            const hipError_t ei = hipGraphInstantiate(&sg->exec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                sg->exec = nullptr;
+                ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+                return TC_ERR_HIP;
+            }
+        }
+        TC_HIP(hipGraphDestroy(graph));
+        TC_HIP(hipGraphLaunch(sg->exec, ctx->stream));
+        std::memcpy(sg->par0, par, sizeof par);
+        for (int l = 2; l <= G; ++l) sg->par1[l] = ctx->lv[l].cur32;
+        for (int l = 0; l < TC_MAX_LEVELS; ++l) {
+            sg->dsweeps[l] = ctx->stats.sweeps_exec[l] - s0.sweeps_exec[l];
+            sg->dres[l] = ctx->stats.residuals_exec[l] - s0.residuals_exec[l];
+        }
+        sg->dcoarse = ctx->stats.coarse_solves - s0.coarse_solves;
+        sg->valid = true;
+        ctx->seg_captures++;
+        return 0;
+    };
+    for (;;) {
+        bool done = false, brk = false;
+        if (first_visit_done && q == 1) {
+            // level 1's first visit ran while judging the guess
+            status = stop_test(ctx, d, q, done);
+            brk = status || done;
+        } else {
+            TC_TRY(step(1, done, brk));
+        }
+        if (!brk) TC_TRY(run_segment(done, brk));
         ctx->stats.mg_cycles++;
         if (status || done) break;
         q++;
@@ -1815,6 +1901,8 @@ int tc_finalize(tc_ctx* ctx)
     if (!ctx) return TC_ERR_ARG;
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->side);
+    for (auto& sg : ctx->seg)
+        if (sg.exec) (void)hipGraphExecDestroy(sg.exec);
     free_ctx(ctx);
     delete ctx;
     return 0;
@@ -1827,6 +1915,21 @@ int tc_set_mode(tc_ctx* ctx, int mode)
     if (!ctx) return TC_ERR_ARG;
     if (mode != TC_MODE_FP64 && mode != TC_MODE_MIXED && mode != TC_MODE_DC64) return TC_ERR_ARG;
     ctx->mode = mode;
+    return 0;
+}
+
+int tc_set_graphs(tc_ctx* ctx, int32_t enable)
+{
+    if (!ctx) return TC_ERR_ARG;
+    ctx->graphs = enable != 0;
+    return 0;
+}
+
+int tc_graph_counts(const tc_ctx* ctx, int64_t* captures, int64_t* replays)
+{
+    if (!ctx || !captures || !replays) return TC_ERR_ARG;
+    *captures = ctx->seg_captures;
+    *replays = ctx->seg_replays;
     return 0;
 }
 

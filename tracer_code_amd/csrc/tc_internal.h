@@ -106,6 +106,17 @@ struct tc_ctx {
     int64_t ntr = 0;
     int mode = TC_MODE_FP64;
     int guess = 0;                // initial guess of the defect-correction modes (TC_GUESS_*)
+    // graph replay of the coarse segment of each FP32 defect-correction cycle
+    // (tc_set_graphs): up to two executable graphs per solve, keyed by the
+    // ping-pong parity of every level's correction at the segment's start
+    bool graphs = false;
+    struct SegGraph {
+        hipGraphExec_t exec = nullptr;
+        bool valid = false;                          // captured in the running solve
+        int par0[TC_MAX_LEVELS + 1] = {}, par1[TC_MAX_LEVELS + 1] = {};
+        int64_t dsweeps[TC_MAX_LEVELS] = {}, dres[TC_MAX_LEVELS] = {}, dcoarse = 0;
+    } seg[2];
+    int64_t seg_replays = 0, seg_captures = 0;
     int dc_sweeps = 0;            // sweeps per visit of the defect-correction cycle (0: p.Nmulti)
     int dc_sweeps_active = 0;     // the count the running defect-correction solve uses
     int dc_level_sweeps[TC_MAX_LEVELS + 1] = {};  // per-level counts of the tuned cycle (0: dc_sweeps)

@@ -96,12 +96,15 @@ def ensemble_step(solvers, steps, threads=0):
         raise RuntimeError(f"tc_ensemble_step: status {st} in case {failed.value}: {msg.decode() if msg else ''}")
 
 
-def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print):
+def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print, graphs=False):
     """Throughput of the cases run concurrently and one after another, from
-    the same initial states (each after one untimed warm-up step)."""
+    the same initial states (each after one untimed warm-up step).  graphs:
+    replay the coarse part of each defect-correction cycle as a HIP graph."""
     out = {}
     for concurrent in (False, True):
         solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune)
+        for g in solvers:
+            g.set_graphs(graphs)
         ensemble_step(solvers, 1, threads)  # warm-up, concurrently in both arms
         for g in solvers:
             g.sync()
@@ -137,6 +140,7 @@ def main(argv=None):
     ap.add_argument("--guess", default="plain", choices=["plain", "extrap", "learned"])
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="tune sweeps and tiles per case first (untimed)")
+    ap.add_argument("--graphs", action="store_true", help="HIP-graph replay of each cycle's coarse part")
     a = ap.parse_args(argv)
     cases = draw_cases(a.cases, a.seed)
     w = None
@@ -144,7 +148,8 @@ def main(argv=None):
         w, _ = learn.load_weights(os.path.join(os.path.dirname(__file__), "weights", "unet_cfg1_L4.npz"))
     tune = autotune_cases(cases, a.mode) if (a.autotune and a.mode != "fp64") else None
     res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune,
-              log=lambda m: print(m, file=sys.stderr, flush=True))
+              log=lambda m: print(m, file=sys.stderr, flush=True), graphs=a.graphs)
+    res["graphs"] = a.graphs
     if tune:
         res["tuned"] = [{"sweeps": sw, "tiles": t} for sw, t in tune]
     res["cases"] = [{k: c[k] for k in ("nx", "nz", "RaT", "visT", "tpc")} for c in cases]

@@ -272,6 +272,16 @@ module tracer_hip
       type(c_ptr), value :: ctx
       type(tc_equalize_info), intent(out) :: info
     end function
+    integer(c_int) function tc_set_graphs(ctx, enable) bind(C, name="tc_set_graphs")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function
+    integer(c_int) function tc_graph_counts(ctx, captures, replays) bind(C, name="tc_graph_counts")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: captures, replays
+    end function
     integer(c_int) function tc_set_dc_level_sweeps(ctx, level, sweeps) bind(C, name="tc_set_dc_level_sweeps")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

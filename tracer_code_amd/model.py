@@ -280,6 +280,17 @@ class Solver:
         """Sweeps per level visit of the defect-correction cycle (0: Nmulti)."""
         self._check(self.lib.tc_set_dc_sweeps(self.h, int(sweeps)), "set_dc_sweeps")
 
+    def set_graphs(self, enable):
+        """Replay the coarse part of each FP32 defect-correction cycle as one
+        captured HIP graph (bit-identical results; for many concurrent cases)."""
+        self._check(self.lib.tc_set_graphs(self.h, int(bool(enable))), "set_graphs")
+
+    def graph_counts(self):
+        """(captures, replays) of the cycle-segment graphs so far."""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.tc_graph_counts(self.h, ctypes.byref(c), ctypes.byref(r)), "graph_counts")
+        return c.value, r.value
+
     def set_dc_level_sweeps(self, level, sweeps):
         """Sweeps per visit of one level of the tuned cycle (0: the common count)."""
         self._check(self.lib.tc_set_dc_level_sweeps(self.h, int(level), int(sweeps)), "set_dc_level_sweeps")
