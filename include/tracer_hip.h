@@ -140,7 +140,7 @@ typedef struct tc_stats {
     int32_t n_eta;           /* smoother_vis passes at level 1 of the last solve */
     int32_t n_temp;          /* smoother_time_T iterations of the last call */
     int32_t n_comp[TC_MAX_TYPES];
-    int32_t guess_used;      /* start of the last solve: 0 plain, 1 extrapolation, 2 extrapolation + network */
+    int32_t guess_used;      /* start of the last solve: 0 plain, 1 stage increment, 2 network */
     int32_t pad;
     double residual_last;    /* residual_mag(1) of the last solve */
     double res_ref_last;     /* res_ref(1) of the last solve */
@@ -156,7 +156,7 @@ typedef struct tc_stats {
     int64_t mg_cycles;       /* multigrid cycles (cumulative) */
     int64_t guesses_used[3]; /* solves started from each TC_GUESS_* candidate (cumulative) */
     /* Last solve: residual_mag(1) after the first level-1 smoothing from the
-     * plain start, the extrapolation and the learned guess (0: not judged). */
+     * plain start, the stage increment and the learned guess (0: not judged). */
     double guess_res[3];
     int64_t dc_retries;      /* defect-correction solves redone with Nmulti sweeps (cumulative) */
 } tc_stats;
@@ -418,9 +418,11 @@ int tc_learn_samples(const tc_ctx* ctx, int32_t* count);
 int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_t seed, double* mean_loss);
 /* Initial guess of the defect-correction modes' solves inside tc_time_step.
  * TC_GUESS_PLAIN: the last solution (the reference's warm start).
- * TC_GUESS_EXTRAP: also the stage's extrapolation in time from its solutions
- * in the last two steps.  TC_GUESS_LEARNED: also the extrapolation plus the
- * attached network's predicted correction of it.  Every candidate is
+ * TC_GUESS_EXTRAP: also the stage increment: the last solution (the previous
+ * RK stage's) plus the change between the same two stages in the last step.
+ * TC_GUESS_LEARNED: also the stage increment, and the stage's extrapolation
+ * in time from its solutions in the last two steps plus the attached
+ * network's predicted correction of that extrapolation.  Every candidate is
  * judged by the residual after the cycle's first level-1 smoothing and the
  * smallest wins; the plain start is always a candidate, and every solve
  * iterates to the same tolerance.  A candidate that lost its last 8

@@ -104,15 +104,15 @@ def test_guess_in_solver_keeps_tolerance(level):
             g.time_step(1)
             st = g.stats()
             assert st.residual_last <= params.tolerance * st.res_ref_last
-            # the learned mode judges the plain start and the learned one
-            assert st.guess_res[1] == 0
+            # the learned mode judges the plain start, the stage increment
+            # and the learned guess; the plain start is always judged
             judged += st.guess_res[2] > 0
-            if st.guess_res[2] > 0:
+            if st.guess_res[1] > 0 or st.guess_res[2] > 0:
                 assert st.guess_res[0] > 0
         starts = [st.guesses_used[i] - st0.guesses_used[i] for i in range(3)]
-        print(f"starts over the last 3 steps (plain, extrapolation, learned): {starts}; last solves judged: {judged}",
+        print(f"starts over the last 3 steps (plain, stage increment, learned): {starts}; last solves judged: {judged}",
               flush=True)
-        assert sum(starts) == 12 and starts[1] == 0 and judged >= 1
+        assert sum(starts) == 12 and judged >= 1
         g.learn_attach(None)
 
 
@@ -120,7 +120,7 @@ def test_trained_network_gain_on_held_out_run():
     """What the trained network gains: the committed weights (768x256 grid,
     network on multigrid level 4, trained on 720 solves of 20 seeded runs
     with Ra log-uniform in [3e5, 3e6]) against the plain warm start and the
-    extrapolation alone, on a held-out seed.  Every start is judged by the
+    stage increment alone, on a held-out seed.  Every start is judged by the
     residual after the first level-1 smoothing, so the learned form can only
     be chosen where it is better there; the test reports the cycles and
     times of the three forms and checks that the learned form needs no more

@@ -628,8 +628,12 @@ int prolong_add(tc_ctx* ctx, const double* e, int m, double* fine, double zsg)
 // Candidate initial guesses of a solve inside the time step, built before
 // it (stage s = solve_stage, with solutions of stage s in the last two
 // steps):
+//   inc_psi   = SF + (psi_1 - psi'_1): the plain start (the previous RK
+//               stage's solution) plus the increment from the previous stage
+//               to this one in the last step (psi'_1: the previous stage's
+//               solution in that step);
 //   ext_psi   = psi_1 + c (psi_1 - psi_2), c = (tau - tau_1) / (tau_1 - tau_2):
-//               the stage's linear extrapolation in time;
+//               the stage's linear extrapolation in time, the network's base;
 //   guess_psi = ext_psi + the network's predicted correction of it.
 // The network works on multigrid level m = nn_level (its H x W are that
 // level's nz x nx): the inputs are restricted there, and its correction is
@@ -653,11 +657,20 @@ int guess_pre_solve(tc_ctx* ctx, bool dc, int* slot, bool* nn_guess)
     *slot = -1;
     *nn_guess = false;
     ctx->have_ext = false;
+    ctx->have_inc = false;
     const int s = ctx->solve_stage;
     if (s < 0 || !ctx->sh1[s] || ctx->scount[s] < 2) return 0;
     const bool want_ext = dc && ctx->guess >= TC_GUESS_EXTRAP;
     if (!want_ext && !ctx->nn) return 0;
     const Grid& g = ctx->lv[1].g;
+    // the stage increment: the composition is rebinned from the tracers before
+    // every stage, so psi is not smooth in time, but the change from one
+    // stage to the next is much alike from step to step
+    const int sp = (s + 3) % 4;
+    if (want_ext && ctx->sh1[sp] && ctx->scount[sp] >= 2) {
+        launch_add_diff(ctx->stream, ctx->SF, ctx->sh1[s], ctx->sh2[sp], ctx->inc_psi, g.elems);
+        ctx->have_inc = true;
+    }
     const double den = ctx->tau1[s] - ctx->tau2[s];
     const double c = (den != 0.0) ? (ctx->solve_tau - ctx->tau1[s]) / den : 1.0;
     launch_extrap(ctx->stream, ctx->sh1[s], ctx->sh2[s], c, ctx->ext_psi, g.elems);
@@ -754,6 +767,7 @@ int guess_alloc(tc_ctx* ctx)
         ctx->scount[s] = 0;
     }
     TC_TRY(dalloc(ctx, &ctx->ext_psi, g.elems));
+    TC_TRY(dalloc(ctx, &ctx->inc_psi, g.elems));
     TC_TRY(dalloc(ctx, &ctx->guess_psi, g.elems));
     TC_TRY(dalloc(ctx, &ctx->guess_lo, g.elems));
     TC_TRY(dalloc(ctx, &ctx->stage_err, 4 * NN_NSCALE));
@@ -886,7 +900,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     // TC_GUESS_EXTRAP judges the extrapolation, TC_GUESS_LEARNED the
     // extrapolation plus the network's correction (the network's base)
     double** cand[3] = {nullptr,
-                        (ctx->have_ext && ctx->guess == TC_GUESS_EXTRAP && guess_due(ctx, 1)) ? &ctx->ext_psi : nullptr,
+                        (ctx->have_inc && guess_due(ctx, 1)) ? &ctx->inc_psi : nullptr,
                         nn_guess ? &ctx->guess_psi : nullptr};
     const bool judge = cand[1] || cand[2];
     if (judge) {
@@ -1685,7 +1699,7 @@ void free_ctx(tc_ctx* ctx)
     double* b[] = {ctx->T, ctx->SF, ctx->u, ctx->w, ctx->strain, ctx->Tvis, ctx->Tbuoy, ctx->conduct, ctx->conduct_r,
                    ctx->conduct_s, ctx->dts, ctx->DERr_u, ctx->DERr_w, ctx->T0, ctx->T1, ctx->T2, ctx->tmp, ctx->tmp2,
                    ctx->zg, ctx->Mc, ctx->RowB, ctx->Ainv, ctx->RowB_next, ctx->Ainv_next, ctx->Bc, ctx->Ur, ctx->rtr, ctx->str, ctx->rtr0, ctx->str0, ctx->rtr1,
-                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->ext_psi, ctx->stage_err,
+                   ctx->str1, ctx->rtr2, ctx->str2, ctx->psi_hi, ctx->psi_lo, ctx->ext_psi, ctx->inc_psi, ctx->stage_err,
                    ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi, ctx->guess_lo, ctx->nn_T,
                    ctx->nn_C, ctx->nn_p1, ctx->nn_p2, ctx->nn_ext, ctx->nn_psi, ctx->nn_inc, ctx->nn_ra, ctx->nn_rb};
     for (double* p : b) F(p);

@@ -410,8 +410,22 @@ __global__ __launch_bounds__(256) void k_extrap(const double* __restrict__ a, co
     if (e < n) out[e] = fma(c, a[e] - b[e], a[e]);
 }
 
+// out = a + (b - c) over whole planes: the stage-increment guess (plain start
+// plus a difference of two solutions); linear, so it keeps the ghost mirrors
+__global__ __launch_bounds__(256) void k_add_diff(const double* __restrict__ a, const double* __restrict__ b,
+                                                  const double* __restrict__ c, double* __restrict__ out, size_t n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = a[e] + (b[e] - c[e]);
+}
+
 // ---- launchers -------------------------------------------------------------
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+void launch_add_diff(hipStream_t st, const double* a, const double* b, const double* c, double* out, size_t n)
+{
+    hipLaunchKernelGGL(k_add_diff, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, c, out, n);
+}
 
 void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n)
 {

@@ -184,8 +184,10 @@ struct tc_ctx {
     int scount[4] = {0, 0, 0, 0};
     int guess_losses[3] = {0, 0, 0};  // consecutive lost judgements of each candidate
     int64_t guess_solves = 0;         // solves inside time steps with candidates enabled
-    double* ext_psi = nullptr;    // the extrapolated guess of the running solve
+    double* ext_psi = nullptr;    // the extrapolated guess of the running solve (the network's base)
     bool have_ext = false;
+    double* inc_psi = nullptr;    // the stage-increment guess of the running solve
+    bool have_inc = false;
     int ecount[4] = {0, 0, 0, 0}; // per stage: extrapolation errors measured with the network attached
     double* stage_err = nullptr;  // 4 x [NN_NSCALE]; [3] = the stage's last max |R psi - R ext|
     // learned correction (tc_nn.hip): attached network and recorded samples

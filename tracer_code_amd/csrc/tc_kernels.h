@@ -85,6 +85,7 @@ void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB,
                          double zsg);
 
 // --- tc_dc.hip ------------------------------------------------------------------
+void launch_add_diff(hipStream_t st, const double* a, const double* b, const double* c, double* out, size_t n);
 void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n);
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial);

@@ -206,8 +206,9 @@ def train(config, nx, nz, seeds, steps, epochs, lr, batch, out, mode="mixed", tp
 def evaluate(config, nx, nz, weights, seed, steps, mode="mixed", tpc=None, warm=3, ra=None, level=4, dc_sweeps=0):
     """Multigrid cycles and time per step from the same seeded state with
     each initial-guess mode: "plain" (the reference's warm start), "extrap"
-    (plain or the RK stage's extrapolation in time) and "learned" (also the
-    extrapolation plus the network's correction).  `warm` steps first build
+    (plain or the stage increment: plain plus the last step's change between
+    the same two RK stages) and "learned" (also the stage's extrapolation in
+    time plus the network's correction of it).  `warm` steps first build
     the per-stage history.  Reports which start won how often and the median
     residual after the first level-1 smoothing of each candidate relative to
     the plain start (over the solves where both were judged)."""
