@@ -343,10 +343,13 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
             double temp[8], cc[8];
             for (int s = 0; s < nk; ++s) cheb_factors(n + s, Niter, dir, temp[s], cc[s]);
             if (f32) {
-                // the first launch of a zero-started visit stages zeros instead of reading them
+                // the first launch of a zero-started visit stages zeros instead of reading them;
+                // the first launch after an ascent adds the prolonged coarse correction
                 const float* in = (zero_start && n == 1) ? nullptr : L.err32[L.cur32];
+                const bool pro = n == 1 && ctx->pend_src && ctx->pend_level == level;
                 launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, in, L.err32[1 - L.cur32], d.F, d.h[level], temp,
-                                  cc, nk, zsg);
+                                  cc, nk, zsg, pro ? ctx->pend_src : nullptr, pro ? &ctx->lv[ctx->pend_from].g : nullptr);
+                if (pro) ctx->pend_src = nullptr;
                 L.cur32 = 1 - L.cur32;
             } else {
                 launch_sweep_tb64(ctx->stream, use_ratio, K, L, view.rhs, L.err[L.cur], L.err[1 - L.cur], d.F,
@@ -950,6 +953,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     const bool first_visit_done = judge;
     int q = 1;
     int status = 0;
+    ctx->pend_src = nullptr;
     // one step p of the cycle schedule: the descent into the level, its
     // visit, and (level 1) the stop test or the ascent out of it
     auto step = [&](int p, bool& done, bool& brk) -> int {
@@ -1003,6 +1007,11 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
                 launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res, f32 ? Lf.res32 : nullptr,
                                    d.F, d.h[1], P.ivis, nullptr);
                 ctx->stats.residuals_exec[0]++;
+            } else if (f32 && tb_zero_start(ctx, d, level - 1, f32) && multi[p + 1] == level - 1) {
+                // folded into the first blocked launch of level-1's next visit
+                ctx->pend_src = L.err32[L.cur32];
+                ctx->pend_level = level - 1;
+                ctx->pend_from = level;
             } else if (f32) {
                 launch_prolong32(ctx->stream, Lf.err32[Lf.cur32], Lf.g, L.err32[L.cur32], L.g, zsg);
             } else {

@@ -329,15 +329,7 @@ __global__ __launch_bounds__(256) void k_prolong32(float* __restrict__ Ef, Grid 
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > gf.nx - 1 || k > gf.nz - 1) return;
     const size_t c = gf.at(i, k);
-    float add;
-    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
-    if (ie && ke) add = Ec[gc.at(i / 2, k / 2)];
-    else if (ie) add = (Ec[gc.at(i / 2, (k - 1) / 2)] + Ec[gc.at(i / 2, (k + 1) / 2)]) * 0.5f;
-    else if (ke) add = (Ec[gc.at((i - 1) / 2, k / 2)] + Ec[gc.at((i + 1) / 2, k / 2)]) * 0.5f;
-    else
-        add = (Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
-               Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) * 0.25f;
-    float v = Ef[c] + add;
+    const float v = __fadd_rn(Ef[c], prolong_add32(Ec, gc, i, k));
     Ef[c] = v;
     stream_ghosts(Ef, gf, i, k, v, zsg);
 }

@@ -109,6 +109,10 @@ struct tc_ctx {
     // graph replay of the coarse segment of each FP32 defect-correction cycle
     // (tc_set_graphs): up to two executable graphs per solve, keyed by the
     // ping-pong parity of every level's correction at the segment's start
+    // a coarse correction whose prolongation the next visit's first blocked
+    // launch folds into its staging (level it goes to, source, source level)
+    int pend_level = 0, pend_from = 0;
+    const float* pend_src = nullptr;
     bool graphs = false;
     struct SegGraph {
         hipGraphExec_t exec = nullptr;

@@ -47,6 +47,21 @@ __device__ __forceinline__ double block_reduce(double a)
     }
     return a;
 }
+// prolong (iterative_solver.f90:175-233) of a coarse FP32 correction at the
+// fine interior point (i,k): the value added there.  Explicitly rounded, in
+// the reference's order, so that every kernel adding it gets the same bits.
+__device__ __forceinline__ float prolong_add32(const float* __restrict__ Ec, const Grid& gc, int i, int k)
+{
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) return Ec[gc.at(i / 2, k / 2)];
+    if (ie) return __fmul_rn(__fadd_rn(Ec[gc.at(i / 2, (k - 1) / 2)], Ec[gc.at(i / 2, (k + 1) / 2)]), 0.5f);
+    if (ke) return __fmul_rn(__fadd_rn(Ec[gc.at((i - 1) / 2, k / 2)], Ec[gc.at((i + 1) / 2, k / 2)]), 0.5f);
+    const float s = __fadd_rn(__fadd_rn(__fadd_rn(Ec[gc.at((i - 1) / 2, (k - 1) / 2)], Ec[gc.at((i + 1) / 2, (k - 1) / 2)]),
+                                        Ec[gc.at((i + 1) / 2, (k + 1) / 2)]),
+                              Ec[gc.at((i - 1) / 2, (k + 1) / 2)]);
+    return __fmul_rn(s, 0.25f);
+}
+
 __device__ __forceinline__ double block_sum(double a) { return block_reduce<0>(a); }
 __device__ __forceinline__ double block_min(double a) { return block_reduce<1>(a); }
 __device__ __forceinline__ double block_max(double a) { return block_reduce<2>(a); }
@@ -114,9 +129,11 @@ void launch_tb_weights32(hipStream_t st, Level& L, const FD& F, const Spacing& h
 void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
                        double* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
                        double zsg);
+// Pc (optional): a coarse correction on grid *gc to prolong into S0 while
+// staging (the prolongation folded into the visit's first launch)
 void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
                        float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
-                       double zsg);
+                       double zsg, const float* Pc = nullptr, const Grid* gc = nullptr);
 
 // --- tc_fields.hip -------------------------------------------------------------
 void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,

@@ -33,6 +33,13 @@ struct TBGeom {
     static constexpr int NPT = (CELLS + TB_THREADS - 1) / TB_THREADS;  // staged cells per thread
 };
 
+// a coarse correction to prolong into the level while staging (FP32 only)
+template <typename T>
+struct Prolong {
+    const T* Ec = nullptr;
+    Grid gc;
+};
+
 template <typename T>
 struct SweepFactors {
     T temp[TB_KMAX];
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(256) void k_tb_weights32(Coefs<float> C, Tables<flo
 template <typename T, bool RATIO, int K, int TT, bool PRE = false>
 __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
                                                         Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg,
-                                                        const T* __restrict__ W)
+                                                        const T* __restrict__ W, Prolong<T> pr)
 {
     using G = TBGeom<TT, K>;
     constexpr int R = G::R, H = G::H, NPT = G::NPT;
@@ -162,7 +169,22 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
         const int i = gi0 + li, k = gk0 + lk;
         T v = T(0);
         // Ein == nullptr: a visit that starts from a zero correction
-        if (Ein && i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) v = Ein[g.at(i, k)];
+        if (Ein && i >= -SPAN && i <= g.nx + SPAN && k >= -SPAN && k <= g.nz + SPAN) {
+            v = Ein[g.at(i, k)];
+            if constexpr (sizeof(T) == 4) {
+                // the coarse-grid correction prolonged into this level (what
+                // k_prolong32 would have added): at interior points, and at
+                // the ghosts the mirror of their interior point's increment
+                if (pr.Ec) {
+                    int mi = i, mk = k;
+                    T sg = T(1);
+                    if (i < 0) { mi = -i; sg = -sg; } else if (i > g.nx) { mi = 2 * g.nx - i; sg = -sg; }
+                    if (k < 0) { mk = -k; sg = sg * zsg; } else if (k > g.nz) { mk = 2 * g.nz - k; sg = sg * zsg; }
+                    if (mi >= 1 && mi <= g.nx - 1 && mk >= 1 && mk <= g.nz - 1)
+                        v = __fadd_rn(v, sg * prolong_add32(pr.Ec, pr.gc, mi, mk));
+                }
+            }
+        }
         buf[0][l] = v;
         buf[1][l] = v;
         // sweep s updates the cells at least 2(s+1) from the region edge
@@ -262,49 +284,49 @@ static int tb_tile(const Grid& g)
 
 template <typename T, int TT>
 static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
-                        const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr)
+                        const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W, const Prolong<T>& pr)
 {
     const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
     if constexpr (sizeof(T) == 4) {
         if (W && K == 4) {
             if (use_ratio)
-                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
             else
-                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
             return;
         }
         if constexpr (TT == 16) {
             if (W && K == 8) {
                 if (use_ratio)
-                    hipLaunchKernelGGL((k_sweep_tb<T, true, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+                    hipLaunchKernelGGL((k_sweep_tb<T, true, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
                 else
                     hipLaunchKernelGGL((k_sweep_tb<T, false, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg,
-                                       W);
+                                       W, pr);
                 return;
             }
         }
     }
     if (K == 8) {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
     } else {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
     }
 }
 
 template <typename T>
 static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, const T* S0, T* E, const Coefs<T>& C,
                       const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W = nullptr,
-                      int tile = 0)
+                      int tile = 0, const Prolong<T>& pr = Prolong<T>{})
 {
     if constexpr (sizeof(T) == 4) {
         if ((tile ? tile : tb_tile<T>(g)) == 32 && K == 4) {
-            tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
+            tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W, pr);
             return;
         }
     }
-    tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W);
+    tb_launch_t<T, 16>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W, pr);
 }
 
 // precomputed weights on every FP32 level: they save each launch the weight
@@ -339,13 +361,18 @@ void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, co
 
 void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
                        float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
-                       double zsg)
+                       double zsg, const float* Pc, const Grid* gc)
 {
     Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, rhs, L.dts32, nullptr, L.lb32};
     SweepFactors<float> sf{};
     for (int s = 0; s < nk; ++s) { sf.temp[s] = (float)temp[s]; sf.cconst[s] = (float)cconst[s]; }
+    Prolong<float> pr;
+    if (Pc && gc) {
+        pr.Ec = Pc;
+        pr.gc = *gc;
+    }
     tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg,
-                     tb_uses_w13(L) ? L.w13 : nullptr, L.tb_tile);
+                     tb_uses_w13(L) ? L.w13 : nullptr, L.tb_tile, pr);
 }
 
 }  // namespace tc
