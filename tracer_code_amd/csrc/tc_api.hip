@@ -75,7 +75,7 @@ struct tc_dev {
     bool tb = true;            // temporally blocked sweeps on levels >= 2
     bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
     int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
-    unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (two slot sets)
+    unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (three slot sets)
     int32_t* h_ctl = nullptr;  // pinned mirror
     std::vector<double> hbuf;  // host staging plane
 };
@@ -1165,17 +1165,18 @@ int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double 
 {
     const tc_params& P = ctx->p;
     // ctl: [0] stop flag, [1] stop index; fslots: the iterations' maxima
+    // (three rotating sets; the first iterate's measure goes into set 0)
     launch_zero_i32(ctx->stream, d.ctl, 2);
     launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(d.fslots), 2 * filter_slot_words());
     unsigned long long* slots = d.fslots;
+    launch_smoothness_max(ctx->stream, A, ctx->g, slots, d.ctl);
     int m = 0, batch = 8;
     for (;;) {
         for (int b = 0; b < batch; ++b, ++m) {
             double* cur = (m % 2 == 0) ? A : B;
             double* nxt = (m % 2 == 0) ? B : A;
-            launch_smoothness_max(ctx->stream, cur, ctx->g, slots + (size_t)(m & 1) * (filter_slot_words() / 2), d.ctl);
-            launch_filter_step_bcs(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2,
-                                   ctx->lv[1].ds2, cfac(m), courant, P.Tbot, slots, m, delta, d.ctl);
+            launch_filter_iter(ctx->stream, comp, cur, nxt, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2,
+                               cfac(m), courant, P.Tbot, slots, m, delta, d.ctl);
         }
         // a launch that failed would leave the stop flag unset for ever
         TC_HIP(hipGetLastError());

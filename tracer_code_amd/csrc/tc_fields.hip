@@ -72,124 +72,49 @@ __global__ void k_sym_bcs(double* __restrict__ f, Grid g)
         for (int k = 1; k <= SPAN; ++k) { f[g.at(i, -k)] = f[g.at(i, k)]; f[g.at(i, nz + k)] = f[g.at(i, nz - k)]; }
 }
 
-// smoothness (smoother.f90:75-95): per-block maxima of the largest jump to
-// the five forward neighbours, i=0..nx-1, k=1..nz-1.
-__global__ __launch_bounds__(256) void k_smoothness(const double* __restrict__ T, Grid g, double* __restrict__ partial,
-                                                    const int32_t* __restrict__ stop)
-{
-    if (stop && stop[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    double a = 0.0;
-    if (i <= g.nx - 1 && k <= g.nz - 1) {
-        const size_t c = g.at(i, k);
-        const double t0 = T[c];
-        double a1 = fabs(T[g.at(i, k - 1)] - t0);
-        double a2 = fabs(T[g.at(i + 1, k - 1)] - t0);
-        double a3 = fabs(T[g.at(i + 1, k)] - t0);
-        double a4 = fabs(T[g.at(i, k + 1)] - t0);
-        double a5 = fabs(T[g.at(i + 1, k + 1)] - t0);
-        a = a1;
-        if (a2 > a) a = a2;
-        if (a3 > a) a = a3;
-        if (a4 > a) a = a4;
-        if (a5 > a) a = a5;
-    }
-    a = block_max(a);
-    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = a;
-}
-
-// One smoother_time_T sweep (smoother.f90:363-371) on 0..nx x 0..nz.
-__global__ __launch_bounds__(256) void k_smooth_T(const double* __restrict__ T0, double* __restrict__ T,
-                                                  const double* __restrict__ dts, Grid g, FD F, double dr2, double ds2,
-                                                  double courantT)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > g.nx || k > g.nz) return;
-    const size_t c = g.at(i, k);
-    double Trr = dot3(F.D2, T0, c, 1) / dr2;
-    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
-    double temp = Trr + Tss;
-    double d = dts[c] * courantT;
-    T[c] = T0[c] + d * temp;
-}
-
-// One smoother_time sweep (smoother.f90:319-329) with the clamp to [0,1];
-// cfac = (Chebyshev factor c, courantC) packed as c*dts*temp.
-__global__ __launch_bounds__(256) void k_smooth_C(const double* __restrict__ T0, double* __restrict__ C,
-                                                  const double* __restrict__ dts, Grid g, FD F, double dr2, double ds2,
-                                                  double cfac, double courantC)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i > g.nx || k > g.nz) return;
-    const size_t c = g.at(i, k);
-    double Trr = dot3(F.D2, T0, c, 1) / dr2;
-    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
-    double temp = Trr + Tss;
-    double d = dts[c] * courantC;
-    double v = T0[c] + cfac * d * temp;
-    if (v < 0.0) v = 0.0;
-    if (v > 1.0) v = 1.0;
-    C[c] = v;
-}
-
-// enforceBCs (energy.f90:455-475), one workgroup.
-// One filter iteration from T0 into T for the device-controlled loops of
-// smoother_time_T / smoother_time: the smoothing update of k_smooth_T /
-// k_smooth_C on i=0..nx, k=0..nz, and the ghost cells copied from T0 (the
-// reference smooths from a full copy of the field, so the ghosts the
-// boundary routine then reads are the previous iterate's).  Skipped once
-// the loop's stop flag is set.
-template <bool COMP>
-__global__ __launch_bounds__(256) void k_filter_step(const double* __restrict__ T0, double* __restrict__ T,
-                                                     const double* __restrict__ dts, Grid g, FD F, double dr2,
-                                                     double ds2, double cfac, double courant,
-                                                     const int32_t* __restrict__ stop)
-{
-    if (stop[0]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x - SPAN;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
-    if (i > g.nx + SPAN || k > g.nz + SPAN) return;
-    const size_t c = g.at(i, k);
-    if (i < 0 || i > g.nx || k < 0 || k > g.nz) {
-        T[c] = T0[c];
-        return;
-    }
-    double Trr = dot3(F.D2, T0, c, 1) / dr2;
-    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
-    double temp = Trr + Tss;
-    if constexpr (COMP) {
-        double d = dts[c] * courant;
-        double v = T0[c] + cfac * d * temp;
-        if (v < 0.0) v = 0.0;
-        if (v > 1.0) v = 1.0;
-        T[c] = v;
-    } else {
-        double d = dts[c] * courant;
-        T[c] = T0[c] + d * temp;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Fused filter iteration: two launches instead of four.
-//   k_smoothness_max: the smoothness measure of the current field, as
-//     k_smoothness, reduced by atomicMax on the bit patterns (the measure
-//     is >= 0, whose IEEE order is the integer order) into FILT_SLOTS
-//     maxima on separate cache lines, block b into slot b % FILT_SLOTS, so
-//     that the atomics of different blocks rarely meet on one address.
-//   k_filter_step_bcs: the stop test on that maximum (the whole grid reads
-//     the same value, so every block takes the same branch; block (0,0)
-//     records it), then the filter step with the boundary conditions folded
-//     in: each cell computes its final value after the step and the
-//     reference's BC routine, evaluating the step at the interior cell a
-//     ghost mirrors.
-// Iteration m uses the slot set m%2; the step kernel clears the other set
-// for the next iteration's reduction.
+// Filter iterations of smoother_time_T / smoother_time, one launch each.
+// The smoothness measure (smoother.f90:75-95) is the largest jump from a
+// node to its five forward neighbours over i=0..nx-1, k=1..nz-1.  Maxima are
+// combined by atomicMax on the bit patterns (the measure is >= 0, whose IEEE
+// order is the integer order) into FILT_SLOTS slots on separate cache
+// lines, block b into slot b % FILT_SLOTS, so that the atomics of different
+// blocks rarely meet on one address.  Three slot sets rotate:
+//   k_smoothness_max writes the measure of the loop's first iterate into
+//     set 0;
+//   k_filter_iter (iteration m) reads set m%3 and runs the stop test on it
+//     (the whole grid reads the same value, so every block takes the same
+//     branch; block (0,0) records it).  Otherwise it does the filter step
+//     with the boundary conditions folded in: each cell gets its final value
+//     after the step and the reference's BC routine, with the step
+//     evaluated at the interior cell that a ghost mirrors.  It also writes
+//     the measure of the iterate it produces into set (m+1)%3, from an LDS
+//     tile of that iterate with a one-cell halo that the block recomputes,
+//     and clears set (m+2)%3 for the iteration after.
 // ---------------------------------------------------------------------------
 constexpr int FILT_SLOTS = 64, FILT_STRIDE = 16;  // 16 x 8 B = one 128-byte line per slot
-constexpr int FILT_ROWS = 4;                      // rows per thread of k_smoothness_max
+constexpr int FILT_SET = FILT_SLOTS * FILT_STRIDE;  // u64 words per slot set
+constexpr int FILT_ROWS = 4;                      // rows per thread of both kernels
+
+// the largest jump at one node from its own value t0 and the five forward
+// neighbours, merged into the running maximum a (a NaN jump is kept)
+__device__ __forceinline__ double smooth_merge(double a, double t0, double n1, double n2, double n3, double n4,
+                                               double n5)
+{
+    double b = fabs(n1 - t0);
+    const double a2 = fabs(n2 - t0), a3 = fabs(n3 - t0), a4 = fabs(n4 - t0), a5 = fabs(n5 - t0);
+    if (a2 > b) b = a2;
+    if (a3 > b) b = a3;
+    if (a4 > b) b = a4;
+    if (a5 > b) b = a5;
+    return (b > a || b != b) ? b : a;
+}
+
+__device__ __forceinline__ void slot_max(unsigned long long* __restrict__ set, double a)
+{
+    const int s = (int)((blockIdx.y * gridDim.x + blockIdx.x) % FILT_SLOTS);
+    atomicMax(set + s * FILT_STRIDE, (unsigned long long)__double_as_longlong(a));
+}
 
 __global__ __launch_bounds__(256) void k_smoothness_max(const double* __restrict__ T, Grid g,
                                                         unsigned long long* __restrict__ slots,
@@ -200,30 +125,16 @@ __global__ __launch_bounds__(256) void k_smoothness_max(const double* __restrict
     double a = 0.0;
     for (int r = 0; r < FILT_ROWS; ++r) {
         const int k = (blockIdx.y * blockDim.y + threadIdx.y) * FILT_ROWS + r + 1;
-        if (i <= g.nx - 1 && k <= g.nz - 1) {
-            const size_t c = g.at(i, k);
-            const double t0 = T[c];
-            double a1 = fabs(T[g.at(i, k - 1)] - t0);
-            double a2 = fabs(T[g.at(i + 1, k - 1)] - t0);
-            double a3 = fabs(T[g.at(i + 1, k)] - t0);
-            double a4 = fabs(T[g.at(i, k + 1)] - t0);
-            double a5 = fabs(T[g.at(i + 1, k + 1)] - t0);
-            double b = a1;
-            if (a2 > b) b = a2;
-            if (a3 > b) b = a3;
-            if (a4 > b) b = a4;
-            if (a5 > b) b = a5;
-            if (b > a || b != b) a = b;
-        }
+        if (i <= g.nx - 1 && k <= g.nz - 1)
+            a = smooth_merge(a, T[g.at(i, k)], T[g.at(i, k - 1)], T[g.at(i + 1, k - 1)], T[g.at(i + 1, k)],
+                             T[g.at(i, k + 1)], T[g.at(i + 1, k + 1)]);
     }
     a = block_max(a);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const int s = (int)((blockIdx.y * gridDim.x + blockIdx.x) % FILT_SLOTS);
-        atomicMax(slots + s * FILT_STRIDE, (unsigned long long)__double_as_longlong(a));
-    }
+    if (threadIdx.x == 0 && threadIdx.y == 0) slot_max(slots, a);
 }
 
-// the step at a node 0..nx x 0..nz (k_filter_step's formula)
+// the step at a node 0..nx x 0..nz (smoother.f90:319-329 with the clamp to
+// [0,1] for C, :363-371 for T)
 template <bool COMP>
 __device__ __forceinline__ double filter_new(const double* __restrict__ T0, const double* __restrict__ dts,
                                              const Grid& g, const FD& F, int i, int k, double dr2, double ds2,
@@ -244,20 +155,60 @@ __device__ __forceinline__ double filter_new(const double* __restrict__ T0, cons
     }
 }
 
+// the value of cell (i,k), i in -SPAN..nx+SPAN, k in -SPAN..nz+SPAN, after
+// one filter step and the boundary routine
 template <bool COMP>
-__global__ __launch_bounds__(256) void k_filter_step_bcs(const double* __restrict__ T0, double* __restrict__ T,
-                                                         const double* __restrict__ dts, Grid g, FD F, double dr2,
-                                                         double ds2, double cfac, double courant, int Tbot,
-                                                         unsigned long long* __restrict__ slots, int m, double delta,
-                                                         int32_t* __restrict__ ctl)
+__device__ __forceinline__ double filter_cell(const double* __restrict__ T0, const double* __restrict__ dts,
+                                              const Grid& g, const FD& F, int i, int k, double dr2, double ds2,
+                                              double cfac, double courant, int Tbot)
+{
+    const int nx = g.nx, nz = g.nz;
+    // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
+    const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant); };
+    if constexpr (COMP) {
+        // enforceBCs_comp (composition.f90:792-803) in its interleaved
+        // order: column +-1, row +-1, column +-2, row +-2; the first row
+        // mirror still reads the old ghost columns +-2 (T0, which the step
+        // copies there)
+        if (k >= 0 && k <= nz) return W(im, k);
+        const int ks = (k < 0) ? -k : 2 * nz - k;  // mirrored row
+        const bool row1 = (k == -1 || k == nz + 1);
+        if (row1 && (i == -2 || i == nx + 2)) return T0[g.at(i, ks)];
+        return W(im, ks);
+    } else {
+        // enforceBCs (T): side mirrors on rows 0..nz, then the bottom/top
+        // ghost rows from the mirrored rows, then the wall values
+        if (k == nz) return 0.0;
+        if (k == 0 && Tbot == 0) return 1.0;
+        if (k >= 0 && k < nz) return W(im, k);
+        if (k < 0) {
+            const double x = W(im, -k);
+            return (Tbot == 0) ? -(x - 1.0) + 1.0 : x;
+        }
+        return -W(im, 2 * nz - k);
+    }
+}
+
+// A block covers 64 columns x FI_TILE rows; the LDS tile holds the new
+// iterate on those rows plus the row above and below, and one more column.
+constexpr int FI_BX = 64, FI_BY = 4, FI_TILE = FI_BY * FILT_ROWS;
+constexpr int FI_HALO = (FI_BX + 1) * 2 + FI_TILE;  // cells the block evaluates beyond its own
+
+template <bool COMP>
+__global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ T0, double* __restrict__ T,
+                                                     const double* __restrict__ dts, Grid g, FD F, double dr2,
+                                                     double ds2, double cfac, double courant, int Tbot,
+                                                     unsigned long long* __restrict__ slots, int m, double delta,
+                                                     int32_t* __restrict__ ctl)
 {
     if (ctl[0]) return;
-    // the iteration's maximum over the slot set (first wave, one slot per lane)
     __shared__ double s_max;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    const unsigned long long* set = slots + (size_t)(m & 1) * FILT_SLOTS * FILT_STRIDE;
+    __shared__ double tile[FI_TILE + 2][FI_BX + 1];
+    const int tid = threadIdx.y * FI_BX + threadIdx.x;
+    // the iteration's maximum over its slot set (first wave, one slot per lane)
     if (tid < 64) {
-        unsigned long long v = set[tid * FILT_STRIDE];
+        unsigned long long v = slots[(size_t)(m % 3) * FILT_SET + tid * FILT_STRIDE];
         for (int o = 32; o > 0; o >>= 1) {
             const unsigned long long w = __shfl_xor(v, o, 64);
             v = (w > v) ? w : v;
@@ -265,72 +216,52 @@ __global__ __launch_bounds__(256) void k_filter_step_bcs(const double* __restric
         if (tid == 0) s_max = __longlong_as_double((long long)v);
     }
     __syncthreads();
-    const double smax = s_max;
     const bool first = blockIdx.x == 0 && blockIdx.y == 0;
-    if (smax < delta) {
+    if (s_max < delta) {
         if (first && tid == 0) {
             ctl[0] = 1;
             ctl[1] = m;
         }
         return;
     }
-    if (first && tid < FILT_SLOTS) slots[((size_t)((m + 1) & 1) * FILT_SLOTS + tid) * FILT_STRIDE] = 0ull;
-    // x from i = -ORG_PAD, so that each row segment starts a cache line
-    const int i = blockIdx.x * blockDim.x + threadIdx.x - ORG_PAD;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y - SPAN;
-    if (i < -SPAN || i > g.nx + SPAN || k > g.nz + SPAN) return;
+    if (first && tid < FILT_SLOTS) slots[(size_t)((m + 2) % 3) * FILT_SET + tid * FILT_STRIDE] = 0ull;
     const int nx = g.nx, nz = g.nz;
-    // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
-    const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
-    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant); };
-    double v;
-    if constexpr (COMP) {
-        // enforceBCs_comp (composition.f90:792-803) in its interleaved
-        // order: column +-1, row +-1, column +-2, row +-2; the first row
-        // mirror still reads the old ghost columns +-2 (T0, which the step
-        // copies there)
-        if (k >= 0 && k <= nz) {
-            v = W(im, k);
-        } else {
-            const int ks = (k < 0) ? -k : 2 * nz - k;  // mirrored row
-            const bool row1 = (k == -1 || k == nz + 1);
-            if (row1 && (i == -2 || i == nx + 2)) v = T0[g.at(i, ks)];
-            else v = W(im, ks);
-        }
-    } else {
-        // enforceBCs (T): side mirrors on rows 0..nz, then the bottom/top
-        // ghost rows from the mirrored rows, then the wall values
-        if (k == nz) {
-            v = 0.0;
-        } else if (k == 0 && Tbot == 0) {
-            v = 1.0;
-        } else if (k >= 0 && k < nz) {
-            v = W(im, k);
-        } else if (k < 0) {
-            const double x = W(im, -k);
-            v = (Tbot == 0) ? -(x - 1.0) + 1.0 : x;
-        } else {
-            v = -W(im, 2 * nz - k);
+    // x from i = -ORG_PAD, so that each row segment starts a cache line
+    const int i0 = blockIdx.x * FI_BX - ORG_PAD, k0 = blockIdx.y * FI_TILE - SPAN;
+    const int i = i0 + threadIdx.x;
+    auto V = [&](int ii, int kk) { return filter_cell<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant, Tbot); };
+    for (int r = 0; r < FILT_ROWS; ++r) {
+        const int tk = threadIdx.y + FI_BY * r, k = k0 + tk;
+        if (i >= -SPAN && i <= nx + SPAN && k <= nz + SPAN) {
+            const double v = V(i, k);
+            T[g.at(i, k)] = v;
+            tile[tk + 1][threadIdx.x] = v;
         }
     }
-    T[g.at(i, k)] = v;
-}
-
-// Stop test of the filter loops: max of the block maxima of k_smoothness;
-// when it is below delta the flag is set and the iteration index recorded.
-__global__ __launch_bounds__(256) void k_stop_test(const double* __restrict__ partial, int n, double delta, int m,
-                                                   int32_t* __restrict__ ctl)
-{
-    if (ctl[0]) return;
-    double a = -INFINITY;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) a = fmax(a, partial[t]);
+    // the halo the measure reads: the rows above and below (65 columns) and
+    // the column to the right; only nodes 0..nx x 0..nz are read
+    for (int e = tid; e < FI_HALO; e += FI_BX * FI_BY) {
+        int ti, tk;
+        if (e < FI_BX + 1) { ti = e; tk = 0; }
+        else if (e < 2 * (FI_BX + 1)) { ti = e - (FI_BX + 1); tk = FI_TILE + 1; }
+        else { ti = FI_BX; tk = e - 2 * (FI_BX + 1) + 1; }
+        const int ii = i0 + ti, kk = k0 + tk - 1;
+        if (ii >= 0 && ii <= nx && kk >= 0 && kk <= nz) tile[tk][ti] = V(ii, kk);
+    }
+    __syncthreads();
+    double a = 0.0;
+    const int tx = threadIdx.x;
+    for (int r = 0; r < FILT_ROWS; ++r) {
+        const int tk = threadIdx.y + FI_BY * r + 1, k = k0 + tk - 1;
+        if (i >= 0 && i <= nx - 1 && k >= 1 && k <= nz - 1)
+            a = smooth_merge(a, tile[tk][tx], tile[tk - 1][tx], tile[tk - 1][tx + 1], tile[tk][tx + 1],
+                             tile[tk + 1][tx], tile[tk + 1][tx + 1]);
+    }
     a = block_max(a);
-    if (threadIdx.x == 0 && a < delta) {
-        ctl[0] = 1;
-        ctl[1] = m;
-    }
+    if (tid == 0) slot_max(slots + (size_t)((m + 1) % 3) * FILT_SET, a);
 }
 
+// enforceBCs (energy.f90:455-475), one workgroup.
 __global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot, const int32_t* __restrict__ stop)
 {
     if (stop && stop[0]) return;
@@ -832,28 +763,6 @@ void launch_stats_rest(hipStream_t st, const double* u, const double* w, const d
                        ds, out);
 }
 
-int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial)
-{
-    dim3 b(64, 4);
-    dim3 gr = grid2(g.nx, g.nz - 1, b);
-    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial, nullptr);
-    return (int)(gr.x * gr.y);
-}
-
-void launch_smooth_T(hipStream_t st, const double* T0, double* T, const double* dts, const Grid& g, const FD& F,
-                     double dr2, double ds2, double courantT)
-{
-    dim3 b(64, 4);
-    hipLaunchKernelGGL(k_smooth_T, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, T0, T, dts, g, F, dr2, ds2, courantT);
-}
-
-void launch_smooth_C(hipStream_t st, const double* T0, double* C, const double* dts, const Grid& g, const FD& F,
-                     double dr2, double ds2, double cfac, double courantC)
-{
-    dim3 b(64, 4);
-    hipLaunchKernelGGL(k_smooth_C, grid2(g.nx + 1, g.nz + 1, b), b, 0, st, T0, C, dts, g, F, dr2, ds2, cfac, courantC);
-}
-
 void launch_sym_bcs(hipStream_t st, double* f, const Grid& g)
 {
     hipLaunchKernelGGL(k_sym_bcs, dim3(1), dim3(256), 0, st, f, g);
@@ -869,30 +778,6 @@ void launch_bcs_comp(hipStream_t st, double* C, const Grid& g, const int32_t* st
     hipLaunchKernelGGL(k_bcs_comp, dim3(1), dim3(1024), 0, st, C, g, stop);
 }
 
-int launch_smoothness_stop(hipStream_t st, const double* T0, const Grid& g, double* partial, const int32_t* stop)
-{
-    dim3 b(64, 4);
-    dim3 gr = grid2(g.nx, g.nz - 1, b);
-    hipLaunchKernelGGL(k_smoothness, gr, b, 0, st, T0, g, partial, stop);
-    return (int)(gr.x * gr.y);
-}
-
-void launch_stop_test(hipStream_t st, const double* partial, int n, double delta, int m, int32_t* ctl)
-{
-    hipLaunchKernelGGL(k_stop_test, dim3(1), dim3(256), 0, st, partial, n, delta, m, ctl);
-}
-
-void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
-                        const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop)
-{
-    dim3 b(64, 4);
-    dim3 gr = grid2(g.nx + 1 + 2 * SPAN, g.nz + 1 + 2 * SPAN, b);
-    if (comp)
-        hipLaunchKernelGGL((k_filter_step<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
-    else
-        hipLaunchKernelGGL((k_filter_step<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, stop);
-}
-
 void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
                            const int32_t* stop)
 {
@@ -900,19 +785,20 @@ void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsi
     hipLaunchKernelGGL(k_smoothness_max, grid2(g.nx, (g.nz - 1 + FILT_ROWS - 1) / FILT_ROWS, b), b, 0, st, T0, g,
                        slots, stop);
 }
-size_t filter_slot_words() { return 2 * (size_t)FILT_SLOTS * FILT_STRIDE; }
+size_t filter_slot_words() { return 3 * (size_t)FILT_SET; }
 
-void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
-                            const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
-                            unsigned long long* slots, int m, double delta, int32_t* ctl)
+void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                        const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
+                        unsigned long long* slots, int m, double delta, int32_t* ctl)
 {
-    dim3 b(64, 4);
-    dim3 gr = grid2(g.nx + 1 + SPAN + ORG_PAD, g.nz + 1 + 2 * SPAN, b);
+    dim3 b(FI_BX, FI_BY);
+    dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FI_BX - 1) / FI_BX),
+            (unsigned)((g.nz + 1 + 2 * SPAN + FI_TILE - 1) / FI_TILE));
     if (comp)
-        hipLaunchKernelGGL((k_filter_step_bcs<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
+        hipLaunchKernelGGL((k_filter_iter<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
                            slots, m, delta, ctl);
     else
-        hipLaunchKernelGGL((k_filter_step_bcs<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
+        hipLaunchKernelGGL((k_filter_iter<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
                            slots, m, delta, ctl);
 }
 

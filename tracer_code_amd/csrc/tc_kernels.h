@@ -140,17 +140,8 @@ void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, c
                        double ds, int Vbc);
 void launch_strain(hipStream_t st, const double* u, const double* w, double* strain, const Grid& g, const FD& F,
                    double dr, double ds);
-int launch_smoothness(hipStream_t st, const double* T0, const Grid& g, double* partial);
-void launch_smooth_T(hipStream_t st, const double* T0, double* T, const double* dts, const Grid& g, const FD& F,
-                     double dr2, double ds2, double courantT);
-void launch_smooth_C(hipStream_t st, const double* T0, double* C, const double* dts, const Grid& g, const FD& F,
-                     double dr2, double ds2, double cfac, double courantC);
 void launch_bcs_T(hipStream_t st, double* T, const Grid& g, int Tbot, const int32_t* stop = nullptr);
 void launch_bcs_comp(hipStream_t st, double* C, const Grid& g, const int32_t* stop = nullptr);
-int launch_smoothness_stop(hipStream_t st, const double* T0, const Grid& g, double* partial, const int32_t* stop);
-void launch_stop_test(hipStream_t st, const double* partial, int n, double delta, int m, int32_t* ctl);
-void launch_filter_step(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
-                        const FD& F, double dr2, double ds2, double cfac, double courant, const int32_t* stop);
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n);
 int diag_blocks(const Grid& g);
 int stats_np();  // partial sums per block of launch_stats_rest
@@ -163,10 +154,10 @@ void launch_diagnostics(hipStream_t st, const double* u, const double* w, const 
                         double* out);
 void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
                            const int32_t* stop);
-size_t filter_slot_words();  // u64 words of the two slot sets of the fused filter loop
-void launch_filter_step_bcs(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
-                            const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
-                            unsigned long long* slots, int m, double delta, int32_t* ctl);
+size_t filter_slot_words();  // u64 words of the three slot sets of the filter loop
+void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
+                        const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
+                        unsigned long long* slots, int m, double delta, int32_t* ctl);
 // --- tc_sort.hip ----------------------------------------------------------------
 void launch_tracer_keys(hipStream_t st, const double* r, const double* s, int64_t n, int nx, double dr, double ds,
                         double aspect, uint32_t bad, uint32_t* key, uint32_t* val);
