@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
                     help="initial-guess candidates of the defect-correction solves (learned: with the committed "
                          "network of this workload, if there is one)")
+    ap.add_argument("--gcr", type=int, default=8,
+                    help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,8 +144,10 @@ def main():
             solver.learn_attach(net)
         else:
             guess = "extrap"
+    gcr = args.gcr if args.mode != "fp64" else 0
     if args.mode != "fp64":
         solver.guess_enable(guess)
+        solver.set_gcr(gcr)
     tune = None
     tiles = None
     dc_sweeps = params.Nmulti
@@ -169,6 +173,7 @@ def main():
         print(f"[bench] warmup step {w}: {time.perf_counter() - t0:.2f}s q_last={st.q_last} "
               f"dt={st.dt_last:.3e}", file=sys.stderr, flush=True)
     before = solver.stats()
+    gcr0 = solver.gcr_counts()[0]
     sweeps0 = list(before.sweeps_exec)
     cycles0 = before.mg_cycles
     solver.set_profiling(True)
@@ -182,6 +187,7 @@ def main():
     avg_us, nlaunch = solver.kernel_timing(_abi.TC_K_SWEEP_L1)
     solver.set_profiling(False)
     after = solver.stats()
+    gcr_updates = solver.gcr_counts()[0] - gcr0
     sweeps_per_step = [(a - b) / args.steps for a, b in zip(after.sweeps_exec, sweeps0)][: params.grids]
     cycles_per_step = (after.mg_cycles - cycles0) / args.steps
     steps_per_s = args.steps / elapsed
@@ -227,6 +233,8 @@ def main():
             "autotune_ms_per_step": tune,
             "tb_tiles_by_level": tiles,
             "dc_retries": after.dc_retries,
+            "gcr_directions": gcr,
+            "gcr_updates": gcr_updates,
             "guess": guess,
             "starts": {k: after.guesses_used[i] - before.guesses_used[i]
                        for i, k in enumerate(("plain", "extrap", "learned"))},

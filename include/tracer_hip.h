@@ -188,6 +188,15 @@ int tc_set_dc_level_sweeps(tc_ctx* ctx, int32_t level, int32_t sweeps);
 int tc_set_graphs(tc_ctx* ctx, int32_t enable);
 /* Segment graphs captured and replayed so far. */
 int tc_graph_counts(const tc_ctx* ctx, int64_t* captures, int64_t* replays);
+/* GCR acceleration of the defect-correction cycles (no counterpart in the
+ * reference).  Each cycle is used as a preconditioner: its change of psi is
+ * a search direction, orthogonalised against up to `directions` (0..8) kept
+ * ones, and psi moves to the point of least residual (2-norm) over them.
+ * The residual is then recomputed exactly, so every solve stops on the same
+ * test and tolerance as without it.  0 (the default) runs the plain cycles. */
+int tc_set_gcr(tc_ctx* ctx, int32_t directions);
+/* GCR updates made and restarts forced by a dependent direction so far. */
+int tc_gcr_counts(const tc_ctx* ctx, int64_t* updates, int64_t* restarts);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

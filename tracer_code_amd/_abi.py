@@ -109,6 +109,8 @@ SIGNATURES = {
     "tc_set_dc_level_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32]),
     "tc_set_graphs": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_graph_counts": (ctypes.c_int, [_VP, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
+    "tc_set_gcr": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_gcr_counts": (ctypes.c_int, [_VP, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     "tc_sort_tracers": (ctypes.c_int, [_VP]),
     "tc_diagnostics": (ctypes.c_int, [_VP, _VP]),
     "tc_restart_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

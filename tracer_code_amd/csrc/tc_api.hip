@@ -847,6 +847,98 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
     return 0;
 }
 
+// GCR over the cycles.  A cycle maps psi_s (residual r_s) to psi'
+// (residual r'): it is a preconditioner, and z = psi' - psi_s its
+// direction, with A z = r_s - r' from the two residuals.  The new direction
+// is orthogonalised (in the operator's image) against the kept ones, and
+// psi = psi_s + alpha z minimises |r|_2 over them; when the set is full it
+// starts again empty.  The residual of the new psi is computed exactly (in
+// double-double, as after every change of psi), so the stop test and the
+// tolerance are the cycles'.  With no kept direction and alpha = 1 this is
+// the plain cycle.
+static_assert(GCR_MAX == 8, "tc_ctx keeps 8 GCR slots");
+
+int gcr_alloc(tc_ctx* ctx)
+{
+    const Grid& g = ctx->lv[1].g;
+    if (!ctx->gcr_hs) {
+        TC_TRY(dalloc(ctx, &ctx->gcr_hs, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->gcr_ls, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->gcr_rs, g.elems));
+        TC_TRY(dalloc(ctx, &ctx->gcr_part, (size_t)gcr_blocks(g) * GCR_NDOT));
+        TC_TRY(dalloc(ctx, &ctx->gcr_out, GCR_NDOT));
+        TC_HIP(hipHostMalloc((void**)&ctx->gcr_h, GCR_NDOT * sizeof(double), hipHostMallocDefault));
+    }
+    for (int j = 0; j < ctx->gcr_m; ++j) {
+        if (!ctx->gcr_Z[j]) TC_TRY(dalloc(ctx, &ctx->gcr_Z[j], g.elems));
+        if (!ctx->gcr_Q[j]) TC_TRY(dalloc(ctx, &ctx->gcr_Q[j], g.elems));
+    }
+    return 0;
+}
+
+int gcr_step(tc_ctx* ctx, tc_dev& d, bool f32)
+{
+    Level& L1 = ctx->lv[1];
+    const Grid& g = L1.g;
+    const size_t bytes = plane_bytes(g);
+    // psi_s, r_s = psi, r: the state the next cycle starts from, nothing kept
+    auto restart = [&]() -> int {
+        TC_HIP(hipMemcpyAsync(ctx->gcr_hs, ctx->psi_hi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        TC_HIP(hipMemcpyAsync(ctx->gcr_ls, ctx->psi_lo, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        TC_HIP(hipMemcpyAsync(ctx->gcr_rs, L1.res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->gcr_have = true;
+        ctx->gcr_nb = 0;
+        return 0;
+    };
+    if (!ctx->gcr_have) return restart();
+    int nb = ctx->gcr_nb < ctx->gcr_m ? ctx->gcr_nb : 0;  // a full set starts again
+    GcrSet Qk{};
+    for (int j = 0; j < nb; ++j) Qk.v[j] = ctx->gcr_Q[j];
+    launch_gcr_dots(ctx->stream, nb, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs, L1.res,
+                    ctx->gcr_Z[nb], ctx->gcr_Q[nb], Qk, g, ctx->gcr_part, ctx->gcr_out);
+    TC_HIP(hipMemcpyAsync(ctx->gcr_h, ctx->gcr_out, (2 * nb + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    const double* h = ctx->gcr_h;
+    const double qq0 = h[2 * nb], rq0 = h[2 * nb + 1];
+    GcrCoef cf{};
+    double qq = qq0, rq = rq0;
+    for (int j = 0; j < nb; ++j) {
+        cf.beta[j] = h[2 * j] / ctx->gcr_qq[j];
+        qq -= cf.beta[j] * h[2 * j];
+        rq -= cf.beta[j] * h[2 * j + 1];
+    }
+    if (nb > 0 && !(qq > 1e-6 * qq0)) {
+        // the new direction is (nearly) in the span of the kept ones: keep
+        // it alone
+        std::swap(ctx->gcr_Z[0], ctx->gcr_Z[nb]);
+        std::swap(ctx->gcr_Q[0], ctx->gcr_Q[nb]);
+        nb = 0;
+        qq = qq0;
+        rq = rq0;
+        ctx->gcr_restarts++;
+    }
+    // no usable direction (the cycle left psi unchanged): its result is the
+    // new state
+    if (!(qq > 0.0) || !std::isfinite(rq / qq)) return restart();
+    cf.alpha = rq / qq;
+    GcrSetW Z{}, Q{};
+    for (int j = 0; j <= nb; ++j) {
+        Z.v[j] = ctx->gcr_Z[j];
+        Q.v[j] = ctx->gcr_Q[j];
+    }
+    const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
+    launch_gcr_update(ctx->stream, nb, Z, Q, cf, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, g, zsg);
+    launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr, d.F,
+                       d.h[1], ctx->p.ivis, nullptr);
+    ctx->stats.residuals_exec[0]++;
+    TC_HIP(hipMemcpyAsync(ctx->gcr_rs, L1.res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->gcr_qq[nb] = qq;
+    ctx->gcr_nb = nb + 1;
+    ctx->gcr_updates++;
+    return 0;
+}
+
 // The reference's cycle schedule applied to corrections: psi is a
 // double-double; level 1 smooths a zero-started correction e(1) against the
 // exact defect of psi and folds it into psi after each visit; the level-2
@@ -954,6 +1046,11 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     int q = 1;
     int status = 0;
     ctx->pend_src = nullptr;
+    ctx->gcr_have = false;
+    ctx->gcr_nb = 0;
+    // the Nmulti retry of a tuned solve runs the plain cycles
+    const bool gcr = ctx->gcr_m > 0 && !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
+    if (gcr) TC_TRY(gcr_alloc(ctx));
     // one step p of the cycle schedule: the descent into the level, its
     // visit, and (level 1) the stop test or the ascent out of it
     auto step = [&](int p, bool& done, bool& brk) -> int {
@@ -1106,6 +1203,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             TC_TRY(step(1, done, brk));
         }
         if (!brk) TC_TRY(run_segment(done, brk));
+        if (!brk && gcr) TC_TRY(gcr_step(ctx, d, f32));
         ctx->stats.mg_cycles++;
         if (status || done) break;
         q++;
@@ -1722,6 +1820,9 @@ void free_ctx(tc_ctx* ctx)
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     F(ctx->eq_ne); F(ctx->eq_start); F(ctx->eq_end); F(ctx->eq_rows); F(ctx->eq_cols); F(ctx->eq_bcnt);
     F(ctx->eq_tmp); F(ctx->eq_red); F(ctx->eq_part); F(ctx->coarse_b);
+    F(ctx->gcr_hs); F(ctx->gcr_ls); F(ctx->gcr_rs); F(ctx->gcr_part); F(ctx->gcr_out);
+    for (int j = 0; j < GCR_MAX; ++j) { F(ctx->gcr_Z[j]); F(ctx->gcr_Q[j]); }
+    if (ctx->gcr_h) (void)hipHostFree(ctx->gcr_h);
     if (ctx->eq_hord) (void)hipHostFree(ctx->eq_hord);
     for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
     if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);
@@ -1946,6 +2047,25 @@ int tc_set_graphs(tc_ctx* ctx, int32_t enable)
 {
     if (!ctx) return TC_ERR_ARG;
     ctx->graphs = enable != 0;
+    return 0;
+}
+
+int tc_set_gcr(tc_ctx* ctx, int32_t directions)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (directions < 0 || directions > GCR_MAX) {
+        ctx->err = "tc_set_gcr: directions must be 0.." + std::to_string(GCR_MAX);
+        return TC_ERR_ARG;
+    }
+    ctx->gcr_m = directions;
+    return 0;
+}
+
+int tc_gcr_counts(const tc_ctx* ctx, int64_t* updates, int64_t* restarts)
+{
+    if (!ctx || !updates || !restarts) return TC_ERR_ARG;
+    *updates = ctx->gcr_updates;
+    *restarts = ctx->gcr_restarts;
     return 0;
 }
 

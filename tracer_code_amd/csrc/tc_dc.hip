@@ -411,8 +411,136 @@ __global__ __launch_bounds__(256) void k_add_diff(const double* __restrict__ a, 
     if (e < n) out[e] = a[e] + (b[e] - c[e]);
 }
 
+// ---- GCR over the defect-correction cycles ----------------------------------
+// One cycle maps psi_s (with residual r_s) to psi' (residual r'); its
+// direction is z = psi' - psi_s and the operator applied to it is
+// q = r_s - r', from the two residuals the cycles compute anyway.  Both are
+// kept in FP64: the fourth-order operator would turn an FP32 rounding of z
+// into a high-frequency defect far above the tolerance, and q enters the
+// orthogonalisation, whose cancellation would amplify an FP32 rounding of it.
+// Interior points only: the update writes psi's ghosts by the mirrors of the
+// boundary routine.
+//
+// k_gcr_dots forms z and q into the new slot and the per-block partial sums
+// over the interior of (q, Q_j) and (r_s, Q_j) for the NB kept directions,
+// then (q, q) and (r_s, q); block b takes rows 1+b, 1+b+gridDim.x, ...
+template <int NB>
+__global__ __launch_bounds__(256) void k_gcr_dots(const double* __restrict__ hi, const double* __restrict__ lo,
+                                                  const double* __restrict__ hs, const double* __restrict__ ls,
+                                                  const double* __restrict__ rs, const double* __restrict__ r,
+                                                  double* __restrict__ zn, double* __restrict__ qn, GcrSet Q, Grid g,
+                                                  double* __restrict__ part)
+{
+    double acc[2 * NB + 2];
+#pragma unroll
+    for (int j = 0; j < 2 * NB + 2; ++j) acc[j] = 0.0;
+    for (int k = 1 + blockIdx.x; k <= g.nz - 1; k += gridDim.x)
+        for (int i = 1 + threadIdx.x; i <= g.nx - 1; i += blockDim.x) {
+            const size_t c = g.at(i, k);
+            const double rv = rs[c], qv = rv - r[c];
+            zn[c] = (hi[c] - hs[c]) + (lo[c] - ls[c]);
+            qn[c] = qv;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const double b = Q.v[j][c];
+                acc[2 * j] += qv * b;
+                acc[2 * j + 1] += rv * b;
+            }
+            acc[2 * NB] += qv * qv;
+            acc[2 * NB + 1] += rv * qv;
+        }
+#pragma unroll
+    for (int j = 0; j < 2 * NB + 2; ++j) {
+        const double t = block_sum(acc[j]);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * GCR_NDOT + j] = t;
+    }
+}
+
+// value j of the dots: the sum of the blocks' partials (one workgroup per
+// value, a fixed order)
+__global__ __launch_bounds__(256) void k_gcr_final(const double* __restrict__ part, int nblk, double* __restrict__ out)
+{
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) a += part[(size_t)b * GCR_NDOT + blockIdx.x];
+    a = block_sum(a);
+    if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
+// the new direction (position NB of Z and Q) orthogonalised against the kept
+// ones and stored back; psi = psi_s + alpha z in double-double, written to
+// psi (with its ghosts) and to psi_s
+template <int NB>
+__global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, GcrCoef cf, double* __restrict__ hi,
+                                                    double* __restrict__ lo, double* __restrict__ hs,
+                                                    double* __restrict__ ls, Grid g, double zsg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    if (i > g.nx - 1 || k > g.nz - 1) return;
+    const size_t c = g.at(i, k);
+    double z = Z.v[NB][c], q = Q.v[NB][c];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        z -= cf.beta[j] * Z.v[j][c];
+        q -= cf.beta[j] * Q.v[j][c];
+    }
+    Z.v[NB][c] = z;
+    Q.v[NB][c] = q;
+    const dd s = dd_add({hs[c], ls[c]}, {cf.alpha * z, 0.0});
+    hi[c] = s.hi;
+    lo[c] = s.lo;
+    hs[c] = s.hi;
+    ls[c] = s.lo;
+    stream_ghosts(hi, g, i, k, s.hi, zsg);
+    stream_ghosts(lo, g, i, k, s.lo, zsg);
+}
+
 // ---- launchers -------------------------------------------------------------
 static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+int gcr_blocks(const Grid& g) { return std::min(g.nz - 1, 1024); }
+
+void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo, const double* hs, const double* ls,
+                     const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
+                     double* part, double* out)
+{
+    const int nblk = gcr_blocks(g);
+    const dim3 gr(nblk), b(256);
+#define TC_GCR_DOTS(N)                                                                                             \
+    hipLaunchKernelGGL((k_gcr_dots<N>), gr, b, 0, st, hi, lo, hs, ls, rs, r, zn, qn, Q, g, part)
+    switch (nb) {
+    case 0: TC_GCR_DOTS(0); break;
+    case 1: TC_GCR_DOTS(1); break;
+    case 2: TC_GCR_DOTS(2); break;
+    case 3: TC_GCR_DOTS(3); break;
+    case 4: TC_GCR_DOTS(4); break;
+    case 5: TC_GCR_DOTS(5); break;
+    case 6: TC_GCR_DOTS(6); break;
+    default: TC_GCR_DOTS(7); break;
+    }
+#undef TC_GCR_DOTS
+    hipLaunchKernelGGL(k_gcr_final, dim3(2 * nb + 2), dim3(256), 0, st, part, nblk, out);
+}
+
+void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
+                       double* lo, double* hs, double* ls, const Grid& g, double zsg)
+{
+    dim3 b(64, 4);
+    const dim3 gr = grid2(g.nx - 1, g.nz - 1, b);
+#define TC_GCR_UPD(N)                                                                                              \
+    hipLaunchKernelGGL((k_gcr_update<N>), gr, b, 0, st, Z, Q, cf, hi, lo, hs, ls, g, zsg)
+    switch (nb) {
+    case 0: TC_GCR_UPD(0); break;
+    case 1: TC_GCR_UPD(1); break;
+    case 2: TC_GCR_UPD(2); break;
+    case 3: TC_GCR_UPD(3); break;
+    case 4: TC_GCR_UPD(4); break;
+    case 5: TC_GCR_UPD(5); break;
+    case 6: TC_GCR_UPD(6); break;
+    default: TC_GCR_UPD(7); break;
+    }
+#undef TC_GCR_UPD
+}
 
 void launch_add_diff(hipStream_t st, const double* a, const double* b, const double* c, double* out, size_t n)
 {

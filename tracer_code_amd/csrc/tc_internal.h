@@ -106,13 +106,25 @@ struct tc_ctx {
     int64_t ntr = 0;
     int mode = TC_MODE_FP64;
     int guess = 0;                // initial guess of the defect-correction modes (TC_GUESS_*)
-    // graph replay of the coarse segment of each FP32 defect-correction cycle
-    // (tc_set_graphs): up to two executable graphs per solve, keyed by the
-    // ping-pong parity of every level's correction at the segment's start
     // a coarse correction whose prolongation the next visit's first blocked
     // launch folds into its staging (level it goes to, source, source level)
     int pend_level = 0, pend_from = 0;
     const float* pend_src = nullptr;
+    // GCR over the defect-correction cycles (tc_set_gcr): directions kept
+    // (0: off), directions held now (slots 0..nb-1), and whether psi_s / r_s
+    // hold the state after the last cycle; Z, Q per slot: a direction and
+    // the operator applied to it, qq its squared norm
+    int gcr_m = 0, gcr_nb = 0;
+    bool gcr_have = false;
+    double *gcr_hs = nullptr, *gcr_ls = nullptr, *gcr_rs = nullptr;
+    double* gcr_Z[8] = {};
+    double* gcr_Q[8] = {};
+    double gcr_qq[8] = {};
+    double *gcr_part = nullptr, *gcr_out = nullptr, *gcr_h = nullptr;
+    int64_t gcr_updates = 0, gcr_restarts = 0;
+    // graph replay of the coarse segment of each FP32 defect-correction cycle
+    // (tc_set_graphs): up to two executable graphs per solve, keyed by the
+    // ping-pong parity of every level's correction at the segment's start
     bool graphs = false;
     struct SegGraph {
         hipGraphExec_t exec = nullptr;

@@ -101,6 +101,22 @@ void launch_coarse_apply(hipStream_t st, const double* Ainv, const double* RowB,
 
 // --- tc_dc.hip ------------------------------------------------------------------
 void launch_add_diff(hipStream_t st, const double* a, const double* b, const double* c, double* out, size_t n);
+// GCR over the defect-correction cycles: at most GCR_MAX directions held
+// (FP64 planes; the new one at position nb <= GCR_MAX - 1), GCR_NDOT dot
+// values per block
+constexpr int GCR_MAX = 8, GCR_NDOT = 2 * GCR_MAX + 2;
+struct GcrSet { const double* v[GCR_MAX]; };
+struct GcrSetW { double* v[GCR_MAX]; };
+struct GcrCoef { double beta[GCR_MAX]; double alpha; };
+int gcr_blocks(const Grid& g);
+// the new direction z = (hi,lo) - (hs,ls) and q = rs - r into zn, qn, and
+// out[2j] = (q, Q_j), out[2j+1] = (rs, Q_j) for j < nb; out[2nb] = (q, q),
+// out[2nb+1] = (rs, q)
+void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo, const double* hs, const double* ls,
+                     const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
+                     double* part, double* out);
+void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
+                       double* lo, double* hs, double* ls, const Grid& g, double zsg);
 void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n);
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial);

@@ -282,6 +282,16 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int64_t), intent(out) :: captures, replays
     end function
+    integer(c_int) function tc_set_gcr(ctx, directions) bind(C, name="tc_set_gcr")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: directions
+    end function
+    integer(c_int) function tc_gcr_counts(ctx, updates, restarts) bind(C, name="tc_gcr_counts")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: updates, restarts
+    end function
     integer(c_int) function tc_set_dc_level_sweeps(ctx, level, sweeps) bind(C, name="tc_set_dc_level_sweeps")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

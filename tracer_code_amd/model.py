@@ -291,6 +291,18 @@ class Solver:
         self._check(self.lib.tc_graph_counts(self.h, ctypes.byref(c), ctypes.byref(r)), "graph_counts")
         return c.value, r.value
 
+    def set_gcr(self, directions):
+        """GCR over the defect-correction cycles with up to `directions` kept
+        directions (0..8; 0 runs the plain cycles).  The stop test and the
+        tolerance are unchanged."""
+        self._check(self.lib.tc_set_gcr(self.h, int(directions)), "set_gcr")
+
+    def gcr_counts(self):
+        """(updates, restarts) of the GCR acceleration so far."""
+        u, r = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.tc_gcr_counts(self.h, ctypes.byref(u), ctypes.byref(r)), "gcr_counts")
+        return u.value, r.value
+
     def set_dc_level_sweeps(self, level, sweeps):
         """Sweeps per visit of one level of the tuned cycle (0: the common count)."""
         self._check(self.lib.tc_set_dc_level_sweeps(self.h, int(level), int(sweeps)), "set_dc_level_sweeps")
