@@ -106,7 +106,7 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
                     help="initial-guess candidates of the defect-correction solves (learned: with the committed "
                          "network of this workload, if there is one)")
-    ap.add_argument("--gcr", type=int, default=8,
+    ap.add_argument("--gcr", type=int, default=16,
                     help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

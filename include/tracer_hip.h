@@ -190,7 +190,7 @@ int tc_set_graphs(tc_ctx* ctx, int32_t enable);
 int tc_graph_counts(const tc_ctx* ctx, int64_t* captures, int64_t* replays);
 /* GCR acceleration of the defect-correction cycles (no counterpart in the
  * reference).  Each cycle is used as a preconditioner: its change of psi is
- * a search direction, orthogonalised against up to `directions` (0..8) kept
+ * a search direction, orthogonalised against up to `directions` (0..16) kept
  * ones, and psi moves to the point of least residual (2-norm) over them.
  * The residual is then recomputed exactly, so every solve stops on the same
  * test and tolerance as without it.  0 (the default) runs the plain cycles. */

@@ -856,7 +856,7 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
 // double-double, as after every change of psi), so the stop test and the
 // tolerance are the cycles'.  With no kept direction and alpha = 1 this is
 // the plain cycle.
-static_assert(GCR_MAX == 8, "tc_ctx keeps 8 GCR slots");
+static_assert(GCR_MAX == 16, "tc_ctx keeps 16 GCR slots");
 
 int gcr_alloc(tc_ctx* ctx)
 {

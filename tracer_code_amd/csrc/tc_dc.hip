@@ -516,7 +516,15 @@ void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo,
     case 4: TC_GCR_DOTS(4); break;
     case 5: TC_GCR_DOTS(5); break;
     case 6: TC_GCR_DOTS(6); break;
-    default: TC_GCR_DOTS(7); break;
+    case 7: TC_GCR_DOTS(7); break;
+    case 8: TC_GCR_DOTS(8); break;
+    case 9: TC_GCR_DOTS(9); break;
+    case 10: TC_GCR_DOTS(10); break;
+    case 11: TC_GCR_DOTS(11); break;
+    case 12: TC_GCR_DOTS(12); break;
+    case 13: TC_GCR_DOTS(13); break;
+    case 14: TC_GCR_DOTS(14); break;
+    default: TC_GCR_DOTS(15); break;
     }
 #undef TC_GCR_DOTS
     hipLaunchKernelGGL(k_gcr_final, dim3(2 * nb + 2), dim3(256), 0, st, part, nblk, out);
@@ -537,7 +545,15 @@ void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& 
     case 4: TC_GCR_UPD(4); break;
     case 5: TC_GCR_UPD(5); break;
     case 6: TC_GCR_UPD(6); break;
-    default: TC_GCR_UPD(7); break;
+    case 7: TC_GCR_UPD(7); break;
+    case 8: TC_GCR_UPD(8); break;
+    case 9: TC_GCR_UPD(9); break;
+    case 10: TC_GCR_UPD(10); break;
+    case 11: TC_GCR_UPD(11); break;
+    case 12: TC_GCR_UPD(12); break;
+    case 13: TC_GCR_UPD(13); break;
+    case 14: TC_GCR_UPD(14); break;
+    default: TC_GCR_UPD(15); break;
     }
 #undef TC_GCR_UPD
 }

@@ -117,9 +117,9 @@ struct tc_ctx {
     int gcr_m = 0, gcr_nb = 0;
     bool gcr_have = false;
     double *gcr_hs = nullptr, *gcr_ls = nullptr, *gcr_rs = nullptr;
-    double* gcr_Z[8] = {};
-    double* gcr_Q[8] = {};
-    double gcr_qq[8] = {};
+    double* gcr_Z[16] = {};
+    double* gcr_Q[16] = {};
+    double gcr_qq[16] = {};
     double *gcr_part = nullptr, *gcr_out = nullptr, *gcr_h = nullptr;
     int64_t gcr_updates = 0, gcr_restarts = 0;
     // graph replay of the coarse segment of each FP32 defect-correction cycle

@@ -104,7 +104,7 @@ void launch_add_diff(hipStream_t st, const double* a, const double* b, const dou
 // GCR over the defect-correction cycles: at most GCR_MAX directions held
 // (FP64 planes; the new one at position nb <= GCR_MAX - 1), GCR_NDOT dot
 // values per block
-constexpr int GCR_MAX = 8, GCR_NDOT = 2 * GCR_MAX + 2;
+constexpr int GCR_MAX = 16, GCR_NDOT = 2 * GCR_MAX + 2;
 struct GcrSet { const double* v[GCR_MAX]; };
 struct GcrSetW { double* v[GCR_MAX]; };
 struct GcrCoef { double beta[GCR_MAX]; double alpha; };

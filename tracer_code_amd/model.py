@@ -293,7 +293,7 @@ class Solver:
 
     def set_gcr(self, directions):
         """GCR over the defect-correction cycles with up to `directions` kept
-        directions (0..8; 0 runs the plain cycles).  The stop test and the
+        directions (0..16; 0 runs the plain cycles).  The stop test and the
         tolerance are unchanged."""
         self._check(self.lib.tc_set_gcr(self.h, int(directions)), "set_gcr")
 
