@@ -42,6 +42,8 @@ SWEEP_BYTES_PER_POINT = {"fp64": 88.0, "dc64": 88.0, "mixed": 44.0}      # algor
 KERNEL_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}     # what the kernels move
 TRACER_BYTES_PER_STEP = 304.0 + 5 * 20.0  # RK4 stages + five binnings, per tracer
 DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
+# sweeps per visit on levels 3 and below (0: the common count), tuned after it
+COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
 
 
@@ -103,6 +105,8 @@ def main():
     ap.add_argument("--nmulti", type=int, default=None, help="smoother sweeps per level visit (default: the preset's)")
     ap.add_argument("--dc-sweeps", type=int, default=None,
                     help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
+    ap.add_argument("--coarse-sweeps", type=int, default=None,
+                    help="sweeps per visit on levels 3 and below with --dc-sweeps (default: that count)")
     ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
                     help="initial-guess candidates of the defect-correction solves (learned: with the committed "
                          "network of this workload, if there is one)")
@@ -150,6 +154,7 @@ def main():
         solver.set_gcr(gcr)
     tune = None
     tiles = None
+    coarse_sweeps, coarse_tune = 0, None
     dc_sweeps = params.Nmulti
     if args.mode != "fp64":
         if args.dc_sweeps is None:
@@ -158,6 +163,17 @@ def main():
             dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
             print(f"[bench] autotuned sweeps per visit: {dc_sweeps} ({time.perf_counter() - t0:.1f}s; "
                   f"ms/step {tune})", file=sys.stderr, flush=True)
+            if params.grids > 3:
+                # levels 3 and below: one count of their own
+                t0 = time.perf_counter()
+                coarse_sweeps, coarse_tune = solver.autotune_dc_coarse_sweeps(3, COARSE_SWEEP_CANDIDATES, 1)
+                print(f"[bench] autotuned sweeps per visit on levels >= 3: {coarse_sweeps or dc_sweeps} "
+                      f"({time.perf_counter() - t0:.1f}s; ms/step {coarse_tune})", file=sys.stderr, flush=True)
+                # the common count again, now that the coarse levels have their own
+                t0 = time.perf_counter()
+                dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+                print(f"[bench] sweeps per visit on levels 1-2, tuned again: {dc_sweeps} "
+                      f"({time.perf_counter() - t0:.1f}s; ms/step {tune})", file=sys.stderr, flush=True)
             if args.mode == "mixed":
                 t0 = time.perf_counter()
                 tiles = solver.autotune_tb_tiles(1)
@@ -166,6 +182,10 @@ def main():
         else:
             dc_sweeps = args.dc_sweeps
             solver.set_dc_sweeps(dc_sweeps)
+            if args.coarse_sweeps:
+                coarse_sweeps = args.coarse_sweeps
+                for L in range(3, params.grids):
+                    solver.set_dc_level_sweeps(L, coarse_sweeps)
     for w in range(args.warmup):
         t0 = time.perf_counter()
         solver.time_step(1)
@@ -229,6 +249,8 @@ def main():
             "mode": args.mode,
             "cycle": args.cycle,
             "sweeps_per_visit": dc_sweeps,
+            "sweeps_per_visit_levels_3_up": coarse_sweeps or dc_sweeps,
+            "coarse_autotune_ms_per_step": coarse_tune,
             "reference_sweeps_per_visit": params.Nmulti,
             "autotune_ms_per_step": tune,
             "tb_tiles_by_level": tiles,

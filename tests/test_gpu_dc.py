@@ -16,6 +16,7 @@ import pytest
 from conftest import rel_err
 from resid_ld import residual_ld
 from tracer_code_amd import Solver, config, ic
+from tracer_code_amd.model import TcError
 
 pytestmark = pytest.mark.gpu
 
@@ -91,6 +92,16 @@ def test_dc_sweeps_retry_and_autotune():
         g.time_step(1)
         check_dc_solution(params, g, f"{best} sweeps per visit")
         assert g.stats().dc_retries == r0
+        # one count of their own for the levels from 3 down (0: the common count)
+        assert params.grids > 3
+        cbest, cms = g.autotune_dc_coarse_sweeps(3, [8, 16, 0], 1)
+        print(f"coarse autotune: best {cbest}, ms/step {cms}", flush=True)
+        assert cbest in (8, 16, 0) and set(cms) == {8, 16, 0}
+        g.time_step(1)
+        check_dc_solution(params, g, f"levels >= 3 at {cbest or best} sweeps per visit")
+        for bad in (1, params.grids):
+            with pytest.raises(TcError):
+                g.autotune_dc_coarse_sweeps(bad, [8], 1)
 
 
 def test_tile_choice_and_autotune():

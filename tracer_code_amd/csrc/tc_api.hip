@@ -2155,6 +2155,42 @@ int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int
     return 0;
 }
 
+int tc_autotune_dc_coarse_sweeps(tc_ctx* ctx, int32_t from_level, const int32_t* candidates, int32_t n,
+                                 int32_t steps_each, int32_t* best, double* ms_per_step)
+{
+    if (!ctx || !candidates || n < 1 || steps_each < 1 || !best || from_level < 2 || from_level >= ctx->p.grids)
+        return TC_ERR_ARG;
+    for (int j = 0; j < n; ++j)
+        if (candidates[j] < 0 || candidates[j] > 10000) return TC_ERR_ARG;
+    auto apply = [&](int c) {
+        for (int L = from_level; L < ctx->p.grids; ++L) ctx->dc_level_sweeps[L] = c;
+    };
+    std::vector<double> ms(n);
+    std::vector<bool> ok(n);
+    for (int j = 0; j < n; ++j) {
+        apply(candidates[j]);
+        const int64_t r0 = ctx->stats.dc_retries;
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        TC_TRY(tc_time_step(ctx, steps_each));
+        ms[j] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps_each;
+        ok[j] = ctx->stats.dc_retries == r0;
+        if (ms_per_step) ms_per_step[j] = ms[j];
+    }
+    // as for the common count: the largest count within 3% of the fastest
+    // (0 stands for the common count)
+    auto eff = [&](int j) { return candidates[j] > 0 ? candidates[j] : ctx->dc_sweeps; };
+    double fastest = 0.0;
+    for (int j = 0; j < n; ++j)
+        if (ok[j] && (fastest == 0.0 || ms[j] < fastest)) fastest = ms[j];
+    int pick = -1;
+    for (int j = 0; j < n; ++j)
+        if (ok[j] && ms[j] <= 1.03 * fastest && (pick < 0 || eff(j) > eff(pick))) pick = j;
+    *best = pick >= 0 ? candidates[pick] : 0;
+    apply(*best);
+    return 0;
+}
+
 int tc_get_stats(const tc_ctx* ctx, tc_stats* out)
 {
     if (!ctx || !out) return TC_ERR_ARG;

@@ -331,6 +331,16 @@ module tracer_hip
       integer(c_int32_t), intent(out) :: best
       real(c_double), intent(out) :: ms_per_step(*)
     end function
+    integer(c_int) function tc_autotune_dc_coarse_sweeps(ctx, from_level, candidates, n, steps_each, best, &
+        ms_per_step) bind(C, name="tc_autotune_dc_coarse_sweeps")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: from_level
+      integer(c_int32_t), intent(in) :: candidates(*)
+      integer(c_int32_t), value :: n, steps_each
+      integer(c_int32_t), intent(out) :: best
+      real(c_double), intent(out) :: ms_per_step(*)
+    end function
     ! learned initial guess: the network handle (tc_nn*) is a c_ptr
     integer(c_int) function tc_nn_create(H, W, trainable, nn) bind(C, name="tc_nn_create")
       import :: c_int, c_int32_t, c_ptr
