@@ -42,8 +42,9 @@ SWEEP_BYTES_PER_POINT = {"fp64": 88.0, "dc64": 88.0, "mixed": 44.0}      # algor
 KERNEL_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}     # what the kernels move
 TRACER_BYTES_PER_STEP = 304.0 + 5 * 20.0  # RK4 stages + five binnings, per tracer
 DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
-# sweeps per visit on levels 3 and below (0: the common count), tuned after it
+# sweeps per visit on levels 3 and below, then on level 2 (0: the common count), tuned after it
 COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
+LEVEL2_SWEEP_CANDIDATES = (8, 12, 16, 24, 0)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
 
 
@@ -107,6 +108,8 @@ def main():
                     help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
     ap.add_argument("--coarse-sweeps", type=int, default=None,
                     help="sweeps per visit on levels 3 and below with --dc-sweeps (default: that count)")
+    ap.add_argument("--level2-sweeps", type=int, default=None,
+                    help="sweeps per visit on level 2 with --dc-sweeps (default: that count)")
     ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
                     help="initial-guess candidates of the defect-correction solves (learned: with the committed "
                          "network of this workload, if there is one)")
@@ -155,6 +158,7 @@ def main():
     tune = None
     tiles = None
     coarse_sweeps, coarse_tune = 0, None
+    level2_sweeps, level2_tune = 0, None
     dc_sweeps = params.Nmulti
     if args.mode != "fp64":
         if args.dc_sweeps is None:
@@ -164,11 +168,14 @@ def main():
             print(f"[bench] autotuned sweeps per visit: {dc_sweeps} ({time.perf_counter() - t0:.1f}s; "
                   f"ms/step {tune})", file=sys.stderr, flush=True)
             if params.grids > 3:
-                # levels 3 and below: one count of their own
+                # levels 3 and below, then level 2: counts of their own
                 t0 = time.perf_counter()
-                coarse_sweeps, coarse_tune = solver.autotune_dc_coarse_sweeps(3, COARSE_SWEEP_CANDIDATES, 1)
-                print(f"[bench] autotuned sweeps per visit on levels >= 3: {coarse_sweeps or dc_sweeps} "
-                      f"({time.perf_counter() - t0:.1f}s; ms/step {coarse_tune})", file=sys.stderr, flush=True)
+                coarse_sweeps, coarse_tune = solver.autotune_dc_level_sweeps(3, params.grids - 1,
+                                                                             COARSE_SWEEP_CANDIDATES, 1)
+                level2_sweeps, level2_tune = solver.autotune_dc_level_sweeps(2, 2, LEVEL2_SWEEP_CANDIDATES, 1)
+                print(f"[bench] autotuned sweeps per visit on levels >= 3: {coarse_sweeps or dc_sweeps}, on level 2: "
+                      f"{level2_sweeps or dc_sweeps} ({time.perf_counter() - t0:.1f}s; ms/step {coarse_tune}, "
+                      f"{level2_tune})", file=sys.stderr, flush=True)
                 # the common count again, now that the coarse levels have their own
                 t0 = time.perf_counter()
                 dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
@@ -186,6 +193,9 @@ def main():
                 coarse_sweeps = args.coarse_sweeps
                 for L in range(3, params.grids):
                     solver.set_dc_level_sweeps(L, coarse_sweeps)
+            if args.level2_sweeps:
+                level2_sweeps = args.level2_sweeps
+                solver.set_dc_level_sweeps(2, level2_sweeps)
     for w in range(args.warmup):
         t0 = time.perf_counter()
         solver.time_step(1)
@@ -249,8 +259,10 @@ def main():
             "mode": args.mode,
             "cycle": args.cycle,
             "sweeps_per_visit": dc_sweeps,
+            "sweeps_per_visit_level_2": level2_sweeps or dc_sweeps,
             "sweeps_per_visit_levels_3_up": coarse_sweeps or dc_sweeps,
             "coarse_autotune_ms_per_step": coarse_tune,
+            "level2_autotune_ms_per_step": level2_tune,
             "reference_sweeps_per_visit": params.Nmulti,
             "autotune_ms_per_step": tune,
             "tb_tiles_by_level": tiles,

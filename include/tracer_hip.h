@@ -203,15 +203,16 @@ int tc_gcr_counts(const tc_ctx* ctx, int64_t* updates, int64_t* restarts);
  * fastest among those that needed no retry (0, i.e. Nmulti, if all did).
  * ms_per_step[n] (may be NULL) receives the measured times, *best the
  * choice, which is also set as with tc_set_dc_sweeps. */
-/* Autotuner of the sweep count of the coarse levels from_level..grids-1
- * (tc_set_dc_level_sweeps on each), after tc_autotune_dc_sweeps: runs
- * steps_each time steps per candidate (0 stands for the common count) and
- * keeps the largest count within 3% of the fastest among those that needed
- * no retry, which it leaves set; *best receives it.  With the cycles
- * accelerated by tc_set_gcr the coarse levels need fewer sweeps than the
- * common count (measured at 2048x1024). */
-int tc_autotune_dc_coarse_sweeps(tc_ctx* ctx, int32_t from_level, const int32_t* candidates, int32_t n,
-                                 int32_t steps_each, int32_t* best, double* ms_per_step);
+/* Autotuner of one sweep count for the levels first_level..last_level
+ * (tc_set_dc_level_sweeps on each; last_level < grids, the coarsest grid is
+ * solved directly), after tc_autotune_dc_sweeps: runs steps_each time steps
+ * per candidate (0 stands for the common count) and keeps the largest count
+ * within 3% of the fastest among those that needed no retry, which it leaves
+ * set; *best receives it.  With the cycles accelerated by tc_set_gcr the
+ * coarse levels need fewer sweeps than the common count (measured at
+ * 2048x1024). */
+int tc_autotune_dc_level_sweeps(tc_ctx* ctx, int32_t first_level, int32_t last_level, const int32_t* candidates,
+                                int32_t n, int32_t steps_each, int32_t* best, double* ms_per_step);
 int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int32_t steps_each, int32_t* best,
                           double* ms_per_step);
 /* Tile of the mixed mode's temporally blocked sweeps on `level` (2 ..

@@ -94,14 +94,14 @@ def test_dc_sweeps_retry_and_autotune():
         assert g.stats().dc_retries == r0
         # one count of their own for the levels from 3 down (0: the common count)
         assert params.grids > 3
-        cbest, cms = g.autotune_dc_coarse_sweeps(3, [8, 16, 0], 1)
+        cbest, cms = g.autotune_dc_level_sweeps(3, params.grids - 1, [8, 16, 0], 1)
         print(f"coarse autotune: best {cbest}, ms/step {cms}", flush=True)
         assert cbest in (8, 16, 0) and set(cms) == {8, 16, 0}
         g.time_step(1)
         check_dc_solution(params, g, f"levels >= 3 at {cbest or best} sweeps per visit")
-        for bad in (1, params.grids):
+        for first, last in ((0, 2), (3, params.grids), (3, 2)):
             with pytest.raises(TcError):
-                g.autotune_dc_coarse_sweeps(bad, [8], 1)
+                g.autotune_dc_level_sweeps(first, last, [8], 1)
 
 
 def test_tile_choice_and_autotune():

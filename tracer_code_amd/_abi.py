@@ -127,8 +127,8 @@ SIGNATURES = {
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
-    "tc_autotune_dc_coarse_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, _VP, ctypes.c_int32, ctypes.c_int32, _VP,
-                                                    _VP]),
+    "tc_autotune_dc_level_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _VP, ctypes.c_int32,
+                                                   ctypes.c_int32, _VP, _VP]),
     "tc_set_tb_tile": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32]),
     "tc_autotune_tb_tiles": (ctypes.c_int, [_VP, ctypes.c_int32, _VP]),
     "tc_get_stats": (ctypes.c_int, [_VP, _P(TcStats)]),

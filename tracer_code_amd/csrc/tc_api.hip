@@ -2155,15 +2155,16 @@ int tc_autotune_dc_sweeps(tc_ctx* ctx, const int32_t* candidates, int32_t n, int
     return 0;
 }
 
-int tc_autotune_dc_coarse_sweeps(tc_ctx* ctx, int32_t from_level, const int32_t* candidates, int32_t n,
-                                 int32_t steps_each, int32_t* best, double* ms_per_step)
+int tc_autotune_dc_level_sweeps(tc_ctx* ctx, int32_t first_level, int32_t last_level, const int32_t* candidates,
+                                int32_t n, int32_t steps_each, int32_t* best, double* ms_per_step)
 {
-    if (!ctx || !candidates || n < 1 || steps_each < 1 || !best || from_level < 2 || from_level >= ctx->p.grids)
+    if (!ctx || !candidates || n < 1 || steps_each < 1 || !best || first_level < 1 || last_level < first_level ||
+        last_level >= ctx->p.grids)
         return TC_ERR_ARG;
     for (int j = 0; j < n; ++j)
         if (candidates[j] < 0 || candidates[j] > 10000) return TC_ERR_ARG;
     auto apply = [&](int c) {
-        for (int L = from_level; L < ctx->p.grids; ++L) ctx->dc_level_sweeps[L] = c;
+        for (int L = first_level; L <= last_level; ++L) ctx->dc_level_sweeps[L] = c;
     };
     std::vector<double> ms(n);
     std::vector<bool> ok(n);

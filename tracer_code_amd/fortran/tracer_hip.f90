@@ -331,11 +331,11 @@ module tracer_hip
       integer(c_int32_t), intent(out) :: best
       real(c_double), intent(out) :: ms_per_step(*)
     end function
-    integer(c_int) function tc_autotune_dc_coarse_sweeps(ctx, from_level, candidates, n, steps_each, best, &
-        ms_per_step) bind(C, name="tc_autotune_dc_coarse_sweeps")
+    integer(c_int) function tc_autotune_dc_level_sweeps(ctx, first_level, last_level, candidates, n, steps_each, &
+        best, ms_per_step) bind(C, name="tc_autotune_dc_level_sweeps")
       import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int32_t), value :: from_level
+      integer(c_int32_t), value :: first_level, last_level
       integer(c_int32_t), intent(in) :: candidates(*)
       integer(c_int32_t), value :: n, steps_each
       integer(c_int32_t), intent(out) :: best

@@ -317,16 +317,16 @@ class Solver:
                                                    ms.ctypes.data), "autotune_dc_sweeps")
         return best.value, {int(k): float(v) for k, v in zip(c, ms)}
 
-    def autotune_dc_coarse_sweeps(self, from_level, candidates, steps_each=1):
-        """After autotune_dc_sweeps: one sweep count for the levels from
-        from_level down (0: the common count), tuned the same way; returns
-        (best, {candidate: ms per step})."""
+    def autotune_dc_level_sweeps(self, first_level, last_level, candidates, steps_each=1):
+        """After autotune_dc_sweeps: one sweep count for the levels
+        first_level..last_level (0: the common count), tuned the same way;
+        returns (best, {candidate: ms per step})."""
         c = np.ascontiguousarray(candidates, dtype=np.int32)
         ms = np.zeros(len(c))
         best = ctypes.c_int32()
-        self._check(self.lib.tc_autotune_dc_coarse_sweeps(self.h, int(from_level), c.ctypes.data, len(c),
-                                                          int(steps_each), ctypes.byref(best), ms.ctypes.data),
-                    "autotune_dc_coarse_sweeps")
+        self._check(self.lib.tc_autotune_dc_level_sweeps(self.h, int(first_level), int(last_level), c.ctypes.data,
+                                                         len(c), int(steps_each), ctypes.byref(best),
+                                                         ms.ctypes.data), "autotune_dc_level_sweeps")
         return best.value, {int(k): float(v) for k, v in zip(c, ms)}
 
     GUESS_MODES = {"plain": _abi.TC_GUESS_PLAIN, "extrap": _abi.TC_GUESS_EXTRAP, "learned": _abi.TC_GUESS_LEARNED}
