@@ -852,10 +852,13 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
 // direction, with A z = r_s - r' from the two residuals.  The new direction
 // is orthogonalised (in the operator's image) against the kept ones, and
 // psi = psi_s + alpha z minimises |r|_2 over them; when the set is full it
-// starts again empty.  The residual of the new psi is computed exactly (in
-// double-double, as after every change of psi), so the stop test and the
-// tolerance are the cycles'.  With no kept direction and alpha = 1 this is
-// the plain cycle.
+// starts again empty.  The residual of the new psi is r_s - alpha q: the
+// directions and their images are FP64 differences of double-double psi and
+// of exact residuals, so it is exact to FP64 rounding of q (an FP32 copy of
+// z would carry its rounding times the fourth-order operator's spread of
+// eigenvalues).  The next level-1 visit smooths against it; its absorb
+// computes the exact residual, so the stop test and the tolerance are the
+// cycles'.  With no kept direction and alpha = 1 this is the plain cycle.
 static_assert(GCR_MAX == 16, "tc_ctx keeps 16 GCR slots");
 
 int gcr_alloc(tc_ctx* ctx)
@@ -876,7 +879,7 @@ int gcr_alloc(tc_ctx* ctx)
     return 0;
 }
 
-int gcr_step(tc_ctx* ctx, tc_dev& d, bool f32)
+int gcr_step(tc_ctx* ctx, bool f32)
 {
     Level& L1 = ctx->lv[1];
     const Grid& g = L1.g;
@@ -928,11 +931,8 @@ int gcr_step(tc_ctx* ctx, tc_dev& d, bool f32)
         Q.v[j] = ctx->gcr_Q[j];
     }
     const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
-    launch_gcr_update(ctx->stream, nb, Z, Q, cf, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, g, zsg);
-    launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr, d.F,
-                       d.h[1], ctx->p.ivis, nullptr);
-    ctx->stats.residuals_exec[0]++;
-    TC_HIP(hipMemcpyAsync(ctx->gcr_rs, L1.res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    launch_gcr_update(ctx->stream, nb, Z, Q, cf, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs,
+                      L1.res, f32 ? L1.res32 : nullptr, g, zsg);
     ctx->gcr_qq[nb] = qq;
     ctx->gcr_nb = nb + 1;
     ctx->gcr_updates++;
@@ -1203,7 +1203,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             TC_TRY(step(1, done, brk));
         }
         if (!brk) TC_TRY(run_segment(done, brk));
-        if (!brk && gcr) TC_TRY(gcr_step(ctx, d, f32));
+        if (!brk && gcr) TC_TRY(gcr_step(ctx, f32));
         ctx->stats.mg_cycles++;
         if (status || done) break;
         q++;

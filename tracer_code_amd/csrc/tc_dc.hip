@@ -468,11 +468,14 @@ __global__ __launch_bounds__(256) void k_gcr_final(const double* __restrict__ pa
 
 // the new direction (position NB of Z and Q) orthogonalised against the kept
 // ones and stored back; psi = psi_s + alpha z in double-double, written to
-// psi (with its ghosts) and to psi_s
+// psi (with its ghosts) and to psi_s; its residual r_s - alpha q written to
+// r (FP64 and, for the FP32 cycle, FP32) and to r_s
 template <int NB>
 __global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, GcrCoef cf, double* __restrict__ hi,
                                                     double* __restrict__ lo, double* __restrict__ hs,
-                                                    double* __restrict__ ls, Grid g, double zsg)
+                                                    double* __restrict__ ls, double* __restrict__ rs,
+                                                    double* __restrict__ r, float* __restrict__ r32, Grid g,
+                                                    double zsg)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -493,6 +496,10 @@ __global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, GcrCoe
     ls[c] = s.lo;
     stream_ghosts(hi, g, i, k, s.hi, zsg);
     stream_ghosts(lo, g, i, k, s.lo, zsg);
+    const double rn = rs[c] - cf.alpha * q;
+    rs[c] = rn;
+    r[c] = rn;
+    if (r32) r32[c] = (float)rn;
 }
 
 // ---- launchers -------------------------------------------------------------
@@ -531,12 +538,13 @@ void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo,
 }
 
 void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
-                       double* lo, double* hs, double* ls, const Grid& g, double zsg)
+                       double* lo, double* hs, double* ls, double* rs, double* r, float* r32, const Grid& g,
+                       double zsg)
 {
     dim3 b(64, 4);
     const dim3 gr = grid2(g.nx - 1, g.nz - 1, b);
 #define TC_GCR_UPD(N)                                                                                              \
-    hipLaunchKernelGGL((k_gcr_update<N>), gr, b, 0, st, Z, Q, cf, hi, lo, hs, ls, g, zsg)
+    hipLaunchKernelGGL((k_gcr_update<N>), gr, b, 0, st, Z, Q, cf, hi, lo, hs, ls, rs, r, r32, g, zsg)
     switch (nb) {
     case 0: TC_GCR_UPD(0); break;
     case 1: TC_GCR_UPD(1); break;

@@ -116,7 +116,8 @@ void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo,
                      const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
                      double* part, double* out);
 void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
-                       double* lo, double* hs, double* ls, const Grid& g, double zsg);
+                       double* lo, double* hs, double* ls, double* rs, double* r, float* r32, const Grid& g,
+                       double zsg);
 void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n);
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial);
