@@ -44,9 +44,10 @@ def draw_cases(n, seed=0):
     return cases
 
 
-def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None):
+def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None, gcr=0):
     """(solvers, networks) initialised from each case's seeded inputs; `tune`
-    (list of (sweeps, tiles) per case) applies autotuned settings."""
+    (per case: dict of sweeps, tiles, level2, coarse) applies autotuned
+    settings; gcr: GCR directions over the defect-correction cycles."""
     solvers, nets = [], []
     for c in cases:
         params = config.make_params(**c)
@@ -62,25 +63,37 @@ def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune
                 g.learn_attach(net)
                 nets.append(net)
             g.guess_enable(guess)
+            g.set_gcr(gcr)
             if tune:
-                sweeps, tiles = tune[len(solvers)]
-                g.set_dc_sweeps(sweeps)
-                for L, t in enumerate(tiles, start=1):
-                    if 2 <= L < params.grids and t:
-                        g.set_tb_tile(L, t)
+                t = tune[len(solvers)]
+                g.set_dc_sweeps(t["sweeps"])
+                for L in range(2, params.grids):
+                    n = t["level2"] if L == 2 else t["coarse"]
+                    if n:
+                        g.set_dc_level_sweeps(L, n)
+                for L, tile in enumerate(t["tiles"], start=1):
+                    if 2 <= L < params.grids and tile:
+                        g.set_tb_tile(L, tile)
         solvers.append(g)
     return solvers, nets
 
 
-def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70)):
-    """Per case: the sweep count and the tiles the autotuners pick (on a
-    scratch copy of the case, advanced by the tuning steps)."""
+def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70), gcr=0):
+    """Per case: the sweep counts and the tiles the autotuners pick (on a
+    scratch copy of the case, advanced by the tuning steps).  With GCR the
+    levels from 3 down and level 2 get counts of their own, and the common
+    count is tuned again after them."""
     out = []
     for c in cases:
-        (g,), _ = make_solvers([c], mode)
+        (g,), _ = make_solvers([c], mode, gcr=gcr)
         sweeps, _ = g.autotune_dc_sweeps(candidates, 1)
+        level2 = coarse = 0
+        if gcr and g.params.grids > 3:
+            coarse, _ = g.autotune_dc_level_sweeps(3, g.params.grids - 1, (4, 8, 12, 16, 24, 0), 1)
+            level2, _ = g.autotune_dc_level_sweeps(2, 2, (8, 12, 16, 24, 0), 1)
+            sweeps, _ = g.autotune_dc_sweeps(candidates, 1)
         tiles = g.autotune_tb_tiles(1) if mode == "mixed" else [0] * 6
-        out.append((sweeps, tiles))
+        out.append(dict(sweeps=sweeps, tiles=tiles, level2=level2, coarse=coarse))
         g.close()
     return out
 
@@ -96,13 +109,14 @@ def ensemble_step(solvers, steps, threads=0):
         raise RuntimeError(f"tc_ensemble_step: status {st} in case {failed.value}: {msg.decode() if msg else ''}")
 
 
-def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print, graphs=False):
+def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print, graphs=False,
+        gcr=0):
     """Throughput of the cases run concurrently and one after another, from
     the same initial states (each after one untimed warm-up step).  graphs:
     replay the coarse part of each defect-correction cycle as a HIP graph."""
     out = {}
     for concurrent in (False, True):
-        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune)
+        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune, gcr=gcr)
         for g in solvers:
             g.set_graphs(graphs)
         ensemble_step(solvers, 1, threads)  # warm-up, concurrently in both arms
@@ -141,17 +155,20 @@ def main(argv=None):
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--autotune", action="store_true", help="tune sweeps and tiles per case first (untimed)")
     ap.add_argument("--graphs", action="store_true", help="HIP-graph replay of each cycle's coarse part")
+    ap.add_argument("--gcr", type=int, default=0, help="GCR directions over the defect-correction cycles")
     a = ap.parse_args(argv)
     cases = draw_cases(a.cases, a.seed)
     w = None
     if a.guess == "learned":
         w, _ = learn.load_weights(os.path.join(os.path.dirname(__file__), "weights", "unet_cfg1_L4.npz"))
-    tune = autotune_cases(cases, a.mode) if (a.autotune and a.mode != "fp64") else None
+    gcr = a.gcr if a.mode != "fp64" else 0
+    tune = autotune_cases(cases, a.mode, gcr=gcr) if (a.autotune and a.mode != "fp64") else None
     res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune,
-              log=lambda m: print(m, file=sys.stderr, flush=True), graphs=a.graphs)
+              log=lambda m: print(m, file=sys.stderr, flush=True), graphs=a.graphs, gcr=gcr)
     res["graphs"] = a.graphs
+    res["gcr"] = gcr
     if tune:
-        res["tuned"] = [{"sweeps": sw, "tiles": t} for sw, t in tune]
+        res["tuned"] = tune
     res["cases"] = [{k: c[k] for k in ("nx", "nz", "RaT", "visT", "tpc")} for c in cases]
     print(json.dumps(res))
     return 0
