@@ -12,7 +12,11 @@ filter, the T filter and the T-field RK stages.  Inputs are synthetic
 Modes: `mixed` (default) and `dc64` solve by defect correction with a
 double-double streamfunction and FP32 / FP64 W-cycle corrections; `fp64` is
 the reference's own cycle, which FP64 rounding keeps above a 1e-8 residual at
-2048x1024 (it stops with "Too Many Multigrid Cycles").
+2048x1024 (it stops with "Too Many Multigrid Cycles").  The defect-correction
+modes run GCR over the cycles (`--gcr 16`) and, before the warm-up, tune their
+sweep counts with real time steps: the common count, one for levels 3 and
+below, one for level 2, then the common count again (level 1's);
+`--dc-sweeps` [`--level2-sweeps`, `--coarse-sweeps`] fix them instead.
 
 Prints one JSON line.  `roofline` reports the level-1 Jacobi-Richardson sweep
 (the dominant kernel): algorithmic bytes per interior point are the
