@@ -870,7 +870,11 @@ int gcr_alloc(tc_ctx* ctx)
         TC_TRY(dalloc(ctx, &ctx->gcr_rs, g.elems));
         TC_TRY(dalloc(ctx, &ctx->gcr_part, (size_t)gcr_blocks(g) * GCR_NDOT));
         TC_TRY(dalloc(ctx, &ctx->gcr_out, GCR_NDOT));
-        TC_HIP(hipHostMalloc((void**)&ctx->gcr_h, GCR_NDOT * sizeof(double), hipHostMallocDefault));
+        TC_TRY(dalloc(ctx, &ctx->gcr_qq, GCR_MAX));
+        TC_TRY(dalloc(ctx, &ctx->gcr_coef, (sizeof(GcrCoef) + sizeof(double) - 1) / sizeof(double)));
+        TC_TRY(dalloc_i(ctx, &ctx->gcr_flag, 1));
+        TC_HIP(hipHostMalloc((void**)&ctx->gcr_hflag, sizeof(int32_t), hipHostMallocDefault));
+        TC_HIP(hipEventCreateWithFlags(&ctx->gcr_ev, hipEventDisableTiming));
     }
     for (int j = 0; j < ctx->gcr_m; ++j) {
         if (!ctx->gcr_Z[j]) TC_TRY(dalloc(ctx, &ctx->gcr_Z[j], g.elems));
@@ -879,63 +883,78 @@ int gcr_alloc(tc_ctx* ctx)
     return 0;
 }
 
+// The outcome of the last step (its flag, copied after it): the direction
+// kept, kept alone, or no direction kept (the cycle's result became the
+// state).  The stop test after that step has synchronised the stream, so
+// the event wait normally returns at once.
+int gcr_settle(tc_ctx* ctx)
+{
+    if (!ctx->gcr_pending) return 0;
+    TC_HIP(hipEventSynchronize(ctx->gcr_ev));
+    ctx->gcr_pending = false;
+    const int f = *ctx->gcr_hflag;
+    if (f == 0) {
+        ctx->gcr_order[ctx->gcr_nb++] = ctx->gcr_new;
+        ctx->gcr_updates++;
+    } else if (f == 1) {
+        ctx->gcr_order[0] = ctx->gcr_new;
+        ctx->gcr_nb = 1;
+        ctx->gcr_updates++;
+        ctx->gcr_restarts++;
+    } else {
+        ctx->gcr_nb = 0;
+    }
+    return 0;
+}
+
+// One step after a cycle: the dots, the coefficients and the update are
+// queued with no host synchronisation.
 int gcr_step(tc_ctx* ctx, bool f32)
 {
     Level& L1 = ctx->lv[1];
     const Grid& g = L1.g;
-    const size_t bytes = plane_bytes(g);
-    // psi_s, r_s = psi, r: the state the next cycle starts from, nothing kept
-    auto restart = [&]() -> int {
+    TC_TRY(gcr_settle(ctx));
+    if (!ctx->gcr_have) {
+        // psi_s, r_s = psi, r: the state the next cycle starts from
+        const size_t bytes = plane_bytes(g);
         TC_HIP(hipMemcpyAsync(ctx->gcr_hs, ctx->psi_hi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
         TC_HIP(hipMemcpyAsync(ctx->gcr_ls, ctx->psi_lo, bytes, hipMemcpyDeviceToDevice, ctx->stream));
         TC_HIP(hipMemcpyAsync(ctx->gcr_rs, L1.res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
         ctx->gcr_have = true;
         ctx->gcr_nb = 0;
         return 0;
-    };
-    if (!ctx->gcr_have) return restart();
-    int nb = ctx->gcr_nb < ctx->gcr_m ? ctx->gcr_nb : 0;  // a full set starts again
+    }
+    if (ctx->gcr_nb >= ctx->gcr_m) ctx->gcr_nb = 0;  // a full set starts again
+    const int nb = ctx->gcr_nb;
+    int sn = 0;  // a slot no kept direction uses (nb < m of them)
+    for (;; ++sn) {
+        bool used = false;
+        for (int j = 0; j < nb; ++j) used = used || ctx->gcr_order[j] == sn;
+        if (!used) break;
+    }
+    GcrIdx idx{};
     GcrSet Qk{};
-    for (int j = 0; j < nb; ++j) Qk.v[j] = ctx->gcr_Q[j];
-    launch_gcr_dots(ctx->stream, nb, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs, L1.res,
-                    ctx->gcr_Z[nb], ctx->gcr_Q[nb], Qk, g, ctx->gcr_part, ctx->gcr_out);
-    TC_HIP(hipMemcpyAsync(ctx->gcr_h, ctx->gcr_out, (2 * nb + 2) * sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    TC_HIP(hipStreamSynchronize(ctx->stream));
-    const double* h = ctx->gcr_h;
-    const double qq0 = h[2 * nb], rq0 = h[2 * nb + 1];
-    GcrCoef cf{};
-    double qq = qq0, rq = rq0;
-    for (int j = 0; j < nb; ++j) {
-        cf.beta[j] = h[2 * j] / ctx->gcr_qq[j];
-        qq -= cf.beta[j] * h[2 * j];
-        rq -= cf.beta[j] * h[2 * j + 1];
-    }
-    if (nb > 0 && !(qq > 1e-6 * qq0)) {
-        // the new direction is (nearly) in the span of the kept ones: keep
-        // it alone
-        std::swap(ctx->gcr_Z[0], ctx->gcr_Z[nb]);
-        std::swap(ctx->gcr_Q[0], ctx->gcr_Q[nb]);
-        nb = 0;
-        qq = qq0;
-        rq = rq0;
-        ctx->gcr_restarts++;
-    }
-    // no usable direction (the cycle left psi unchanged): its result is the
-    // new state
-    if (!(qq > 0.0) || !std::isfinite(rq / qq)) return restart();
-    cf.alpha = rq / qq;
     GcrSetW Z{}, Q{};
-    for (int j = 0; j <= nb; ++j) {
-        Z.v[j] = ctx->gcr_Z[j];
-        Q.v[j] = ctx->gcr_Q[j];
+    for (int j = 0; j < nb; ++j) {
+        idx.v[j] = ctx->gcr_order[j];
+        Qk.v[j] = ctx->gcr_Q[idx.v[j]];
+        Z.v[j] = ctx->gcr_Z[idx.v[j]];
+        Q.v[j] = ctx->gcr_Q[idx.v[j]];
     }
+    idx.v[nb] = sn;
+    Z.v[nb] = ctx->gcr_Z[sn];
+    Q.v[nb] = ctx->gcr_Q[sn];
+    launch_gcr_dots(ctx->stream, nb, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs, L1.res,
+                    Z.v[nb], Q.v[nb], Qk, g, ctx->gcr_part, ctx->gcr_out);
+    GcrCoef* cf = reinterpret_cast<GcrCoef*>(ctx->gcr_coef);
+    launch_gcr_coef(ctx->stream, nb, ctx->gcr_out, idx, ctx->gcr_qq, cf, ctx->gcr_flag);
     const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
     launch_gcr_update(ctx->stream, nb, Z, Q, cf, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs,
                       L1.res, f32 ? L1.res32 : nullptr, g, zsg);
-    ctx->gcr_qq[nb] = qq;
-    ctx->gcr_nb = nb + 1;
-    ctx->gcr_updates++;
+    TC_HIP(hipMemcpyAsync(ctx->gcr_hflag, ctx->gcr_flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TC_HIP(hipEventRecord(ctx->gcr_ev, ctx->stream));
+    ctx->gcr_new = sn;
+    ctx->gcr_pending = true;
     return 0;
 }
 
@@ -1046,6 +1065,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     int q = 1;
     int status = 0;
     ctx->pend_src = nullptr;
+    TC_TRY(gcr_settle(ctx));
     ctx->gcr_have = false;
     ctx->gcr_nb = 0;
     // the Nmulti retry of a tuned solve runs the plain cycles
@@ -1213,6 +1233,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             break;
         }
     }
+    TC_TRY(gcr_settle(ctx));
     return finish_solve(ctx, status, q, ctx->psi_hi, q_out);
 }
 
@@ -1820,9 +1841,11 @@ void free_ctx(tc_ctx* ctx)
     F(ctx->rec_x); F(ctx->rec_t); F(ctx->rec_mask);
     F(ctx->eq_ne); F(ctx->eq_start); F(ctx->eq_end); F(ctx->eq_rows); F(ctx->eq_cols); F(ctx->eq_bcnt);
     F(ctx->eq_tmp); F(ctx->eq_red); F(ctx->eq_part); F(ctx->coarse_b);
-    F(ctx->gcr_hs); F(ctx->gcr_ls); F(ctx->gcr_rs); F(ctx->gcr_part); F(ctx->gcr_out);
+    F(ctx->gcr_hs); F(ctx->gcr_ls); F(ctx->gcr_rs); F(ctx->gcr_part); F(ctx->gcr_out); F(ctx->gcr_qq);
+    F(ctx->gcr_coef); F(ctx->gcr_flag);
     for (int j = 0; j < GCR_MAX; ++j) { F(ctx->gcr_Z[j]); F(ctx->gcr_Q[j]); }
-    if (ctx->gcr_h) (void)hipHostFree(ctx->gcr_h);
+    if (ctx->gcr_hflag) (void)hipHostFree(ctx->gcr_hflag);
+    if (ctx->gcr_ev) (void)hipEventDestroy(ctx->gcr_ev);
     if (ctx->eq_hord) (void)hipHostFree(ctx->eq_hord);
     for (int b = 0; b < 2; ++b) { F(ctx->eq_k[b]); F(ctx->eq_v[b]); }
     if (ctx->eq_hred) (void)hipHostFree(ctx->eq_hred);

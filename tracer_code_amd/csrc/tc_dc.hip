@@ -466,12 +466,50 @@ __global__ __launch_bounds__(256) void k_gcr_final(const double* __restrict__ pa
     if (threadIdx.x == 0) out[blockIdx.x] = a;
 }
 
+// The step's coefficients from the dots (one thread): beta_j against the
+// kept directions (squared norms qq per slot), alpha = (r, q^)/(q^, q^).
+// flag 1: the new direction is (nearly) in the span of the kept ones
+// (|q^|^2 <= 1e-6 |q|^2), so it is used alone; flag 2: no usable direction
+// (the cycle left the residual unchanged), so alpha = 1, beta = 0 keep the
+// cycle's result.  The new slot's squared norm is stored for later steps.
+__global__ void k_gcr_coef(const double* __restrict__ out, int nb, GcrIdx idx, double* __restrict__ qq,
+                           GcrCoef* __restrict__ cf, int32_t* __restrict__ flag)
+{
+    if (threadIdx.x != 0) return;
+    const double qq0 = out[2 * nb], rq0 = out[2 * nb + 1];
+    double q2 = qq0, rq = rq0;
+    double beta[GCR_MAX];
+    for (int j = 0; j < nb; ++j) {
+        beta[j] = out[2 * j] / qq[idx.v[j]];
+        q2 -= beta[j] * out[2 * j];
+        rq -= beta[j] * out[2 * j + 1];
+    }
+    int f = 0;
+    if (nb > 0 && !(q2 > 1e-6 * qq0)) {
+        for (int j = 0; j < nb; ++j) beta[j] = 0.0;
+        q2 = qq0;
+        rq = rq0;
+        f = 1;
+    }
+    double alpha = rq / q2;
+    if (!(q2 > 0.0) || !isfinite(alpha)) {
+        for (int j = 0; j < nb; ++j) beta[j] = 0.0;
+        alpha = 1.0;
+        f = 2;
+    }
+    for (int j = 0; j < nb; ++j) cf->beta[j] = beta[j];
+    cf->alpha = alpha;
+    qq[idx.v[nb]] = q2;
+    *flag = f;
+}
+
 // the new direction (position NB of Z and Q) orthogonalised against the kept
 // ones and stored back; psi = psi_s + alpha z in double-double, written to
 // psi (with its ghosts) and to psi_s; its residual r_s - alpha q written to
 // r (FP64 and, for the FP32 cycle, FP32) and to r_s
 template <int NB>
-__global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, GcrCoef cf, double* __restrict__ hi,
+__global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, const GcrCoef* __restrict__ cfp,
+                                                    double* __restrict__ hi,
                                                     double* __restrict__ lo, double* __restrict__ hs,
                                                     double* __restrict__ ls, double* __restrict__ rs,
                                                     double* __restrict__ r, float* __restrict__ r32, Grid g,
@@ -481,6 +519,7 @@ __global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, GcrCoe
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const size_t c = g.at(i, k);
+    const GcrCoef& cf = *cfp;
     double z = Z.v[NB][c], q = Q.v[NB][c];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -537,7 +576,13 @@ void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo,
     hipLaunchKernelGGL(k_gcr_final, dim3(2 * nb + 2), dim3(256), 0, st, part, nblk, out);
 }
 
-void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
+void launch_gcr_coef(hipStream_t st, int nb, const double* out, const GcrIdx& idx, double* qq, GcrCoef* cf,
+                     int32_t* flag)
+{
+    hipLaunchKernelGGL(k_gcr_coef, dim3(1), dim3(64), 0, st, out, nb, idx, qq, cf, flag);
+}
+
+void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef* cf, double* hi,
                        double* lo, double* hs, double* ls, double* rs, double* r, float* r32, const Grid& g,
                        double zsg)
 {

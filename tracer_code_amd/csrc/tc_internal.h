@@ -111,16 +111,22 @@ struct tc_ctx {
     int pend_level = 0, pend_from = 0;
     const float* pend_src = nullptr;
     // GCR over the defect-correction cycles (tc_set_gcr): directions kept
-    // (0: off), directions held now (slots 0..nb-1), and whether psi_s / r_s
-    // hold the state after the last cycle; Z, Q per slot: a direction and
-    // the operator applied to it, qq its squared norm
+    // (0: off), directions held now (their slots in gcr_order, oldest
+    // first), and whether psi_s / r_s hold the state after the last cycle;
+    // Z, Q per slot: a direction and the operator applied to it.  On the
+    // device: the dots, the step's coefficients, the squared norm per slot
+    // and the step's flag, whose pinned copy is read at the next step
+    // (gcr_pending, after the stop test has synchronised the stream)
     int gcr_m = 0, gcr_nb = 0;
-    bool gcr_have = false;
+    int gcr_order[16] = {};
+    bool gcr_have = false, gcr_pending = false;
     double *gcr_hs = nullptr, *gcr_ls = nullptr, *gcr_rs = nullptr;
     double* gcr_Z[16] = {};
     double* gcr_Q[16] = {};
-    double gcr_qq[16] = {};
-    double *gcr_part = nullptr, *gcr_out = nullptr, *gcr_h = nullptr;
+    double *gcr_part = nullptr, *gcr_out = nullptr, *gcr_qq = nullptr, *gcr_coef = nullptr;
+    int32_t *gcr_flag = nullptr, *gcr_hflag = nullptr;
+    int gcr_new = 0;  // slot of the pending step's direction
+    hipEvent_t gcr_ev = nullptr;
     int64_t gcr_updates = 0, gcr_restarts = 0;
     // graph replay of the coarse segment of each FP32 defect-correction cycle
     // (tc_set_graphs): up to two executable graphs per solve, keyed by the

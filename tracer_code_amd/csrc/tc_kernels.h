@@ -108,6 +108,7 @@ constexpr int GCR_MAX = 16, GCR_NDOT = 2 * GCR_MAX + 2;
 struct GcrSet { const double* v[GCR_MAX]; };
 struct GcrSetW { double* v[GCR_MAX]; };
 struct GcrCoef { double beta[GCR_MAX]; double alpha; };
+struct GcrIdx { int v[GCR_MAX]; };  // slot of each kept direction, then of the new one
 int gcr_blocks(const Grid& g);
 // the new direction z = (hi,lo) - (hs,ls) and q = rs - r into zn, qn, and
 // out[2j] = (q, Q_j), out[2j+1] = (rs, Q_j) for j < nb; out[2nb] = (q, q),
@@ -115,7 +116,11 @@ int gcr_blocks(const Grid& g);
 void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo, const double* hs, const double* ls,
                      const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
                      double* part, double* out);
-void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef& cf, double* hi,
+// the coefficients of the step (device-resident cf; flag: 0, or 1 = the new
+// direction used alone, 2 = the cycle's result kept; qq: squared norm per slot)
+void launch_gcr_coef(hipStream_t st, int nb, const double* out, const GcrIdx& idx, double* qq, GcrCoef* cf,
+                     int32_t* flag);
+void launch_gcr_update(hipStream_t st, int nb, const GcrSetW& Z, const GcrSetW& Q, const GcrCoef* cf, double* hi,
                        double* lo, double* hs, double* ls, double* rs, double* r, float* r32, const Grid& g,
                        double zsg);
 void launch_extrap(hipStream_t st, const double* a, const double* b, double c, double* out, size_t n);
