@@ -19,7 +19,7 @@ from tracer_code_amd.model import library
 pytestmark = pytest.mark.gpu
 
 
-def run(preset, ov, graphs, steps=3, sweeps=32):
+def run(preset, ov, graphs, steps=3, sweeps=32, gcr=0):
     params, cfg = config.preset_params(preset, **ov)
     T, r, s, ty = ic.workload(cfg, seed=1)
     with Solver(params, len(r)) as g:
@@ -27,6 +27,7 @@ def run(preset, ov, graphs, steps=3, sweeps=32):
         g.set_mode("mixed")
         g.set_dc_sweeps(sweeps)
         g.set_graphs(graphs)
+        g.set_gcr(gcr)
         g.time_step(steps)
         out = {"SF": g.get("SF"), "T": g.get("T"), "r": g.get_tracers("x")[0], "cycles": g.stats().mg_cycles,
                "counts": g.graph_counts(), "sweeps": list(g.stats().sweeps_exec)}
@@ -46,6 +47,20 @@ def test_graph_replay_is_bit_identical(preset, ov, sweeps):
     assert captures >= 1 and replays > captures
     assert a["cycles"] == b["cycles"]
     assert a["sweeps"] == b["sweeps"]
+    for k in ("SF", "T", "r"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_graph_replay_with_gcr_is_bit_identical():
+    """GCR's step runs after each replayed segment and changes psi and the
+    residual in place; the replays stay bit-identical to the launches."""
+    ov = dict(nx=64, nz=64, tpc=4)
+    print(f"[graphs + gcr 16 cfg0 {ov}]", flush=True)
+    a = run("cfg0", ov, False, gcr=16)
+    b = run("cfg0", ov, True, gcr=16)
+    print(f"captures/replays {b['counts']} cycles {b['cycles']}", flush=True)
+    assert b["counts"][1] > b["counts"][0] >= 1
+    assert a["cycles"] == b["cycles"]
     for k in ("SF", "T", "r"):
         assert np.array_equal(a[k], b[k]), k
 
