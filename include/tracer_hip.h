@@ -159,6 +159,10 @@ typedef struct tc_stats {
      * plain start, the stage increment and the learned guess (0: not judged). */
     double guess_res[3];
     int64_t dc_retries;      /* defect-correction solves redone with Nmulti sweeps (cumulative) */
+    /* Cycles of each solve of the last tc_time_step, in call order: the solve
+     * before energy_time (main.f90:181), then the RK_update solves of stages
+     * 2, 3 and 4 (energy.f90:240, 331, 397); 0 where a stage had none. */
+    int32_t q_step[4];
 } tc_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */

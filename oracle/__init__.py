@@ -33,6 +33,8 @@ ORS_DT, ORS_TIME, ORS_RESID1, ORS_RESREF1, ORS_EIG_FINE, ORS_DELTA_T = range(6)
 (ORS_Q, ORS_N_ETA, ORS_N_TEMP, ORS_N_COMP1, ORS_STATUS, ORS_KL, ORS_KU, ORS_NGC, ORS_TSTEP,
  ORS_SMOOTH_VIS, ORS_NVIS_STORE, ORS_NN_STORE, ORS_CYCLES_TOTAL, ORS_VIS_ITERS, ORS_C_ITERS,
  ORS_T_ITERS) = range(16, 32)
+ORS_Q_STEP = tuple(range(32, 36))
+ORS_N_COMP_T = tuple(range(36, 40))  # n_comp of types 1..4
 
 _lib = None
 

@@ -70,6 +70,9 @@ struct or_state {
     int tstep, q, n_eta, n_temp, n_comp[TC_MAX_TYPES + 1], status, smooth_vis;
     int Nvis_store, nn_store;
     int64_t cycles_total, vis_iters, c_iters, t_iters;
+    /* cycles of each solve of the running or_time_step (q_step_n = solves
+     * recorded so far; -1 outside a time step) */
+    int q_step[4], q_step_n;
     /* equalize_tracers in the time step */
     int eq_on, eq_tpc;
     double eq_Cinit[TC_MAX_TYPES];   /* type 1..ntype at [0..ntype-1] */
@@ -305,6 +308,7 @@ void or_derivative_eigenvalues(const double D1[5], const double D2[5], const dou
 or_state* or_create(const tc_params* p, int64_t ntr)
 {
     or_state* s = (or_state*)calloc(1, sizeof(or_state));
+    s->q_step_n = -1;
     s->p = *p;
     s->nx = p->nx; s->nz = p->nz; s->grids = p->grids; s->ntype = p->ntype;
     s->ld = p->nx + 2 * SPAN + 1;
@@ -483,6 +487,9 @@ double or_get_scalar(const or_state* s, int which)
     case ORS_NN_STORE: return s->nn_store; case ORS_CYCLES_TOTAL: return (double)s->cycles_total;
     case ORS_VIS_ITERS: return (double)s->vis_iters; case ORS_C_ITERS: return (double)s->c_iters;
     case ORS_T_ITERS: return (double)s->t_iters;
+    case ORS_Q_STEP0: case ORS_Q_STEP1: case ORS_Q_STEP2: case ORS_Q_STEP3: return s->q_step[which - ORS_Q_STEP0];
+    case ORS_N_COMP_T1: case ORS_N_COMP_T2: case ORS_N_COMP_T3: case ORS_N_COMP_T4:
+        return s->n_comp[1 + which - ORS_N_COMP_T1];
     default: return NAN;
     }
 }
@@ -1476,6 +1483,7 @@ int or_multigrid(or_state* s)
     }
     memcpy(s->SF, s->error[1], s->plane * sizeof(double));
     s->q = q;
+    if (s->q_step_n >= 0 && s->q_step_n < 4) s->q_step[s->q_step_n++] = q;
     return 0;
 }
 
@@ -2081,10 +2089,16 @@ int or_energy_time(or_state* s)
 int or_time_step(or_state* s)
 {
     int st;
-    if ((st = or_multigrid(s))) return st;
-    or_compute_velocities(s);
-    or_compute_strain_rate_invariant(s);
-    if ((st = or_energy_time(s))) return st;
+    memset(s->q_step, 0, sizeof(s->q_step));
+    s->q_step_n = 0;
+    st = or_multigrid(s);
+    if (!st) {
+        or_compute_velocities(s);
+        or_compute_strain_rate_invariant(s);
+        st = or_energy_time(s);
+    }
+    s->q_step_n = -1;
+    if (st) return st;
     s->time += s->dt;
     s->tstep++;
     return 0;

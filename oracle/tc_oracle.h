@@ -47,7 +47,11 @@ enum {
     ORS_DT = 0, ORS_TIME, ORS_RESID1, ORS_RESREF1, ORS_EIG_FINE, ORS_DELTA_T,
     ORS_Q = 16, ORS_N_ETA, ORS_N_TEMP, ORS_N_COMP1, ORS_STATUS, ORS_KL, ORS_KU,
     ORS_NGC, ORS_TSTEP, ORS_SMOOTH_VIS, ORS_NVIS_STORE, ORS_NN_STORE,
-    ORS_CYCLES_TOTAL, ORS_VIS_ITERS, ORS_C_ITERS, ORS_T_ITERS
+    ORS_CYCLES_TOTAL, ORS_VIS_ITERS, ORS_C_ITERS, ORS_T_ITERS,
+    /* cycles of each solve of the last or_time_step, in call order */
+    ORS_Q_STEP0 = 32, ORS_Q_STEP1, ORS_Q_STEP2, ORS_Q_STEP3,
+    /* n_comp of types 1..4 after the last smoother_time */
+    ORS_N_COMP_T1 = 36, ORS_N_COMP_T2, ORS_N_COMP_T3, ORS_N_COMP_T4
 };
 
 or_state* or_create(const tc_params* p, int64_t ntr);

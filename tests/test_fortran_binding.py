@@ -44,7 +44,8 @@ def test_fortran_type_layout_matches_c():
     want = {"sizeof_params": ctypes.sizeof(P), "off_multi": P.multi.offset, "off_fixes": P.fixes.offset,
             "off_aspect": P.aspect.offset, "off_D1": P.D1.offset, "off_Dint": P.Dint.offset,
             "off_Trrss_base": P.Trrss_base.offset, "sizeof_stats": ctypes.sizeof(S),
-            "off_residual_last": S.residual_last.offset, "off_mg_cycles": S.mg_cycles.offset}
+            "off_residual_last": S.residual_last.offset, "off_mg_cycles": S.mg_cycles.offset,
+            "off_q_step": S.q_step.offset}
     E, R = _abi.TcEqualizeInfo, _abi.TcStatsRow
     want.update({"sizeof_equalize": ctypes.sizeof(E), "off_deficit": E.deficit.offset, "off_count": E.count.offset,
                  "sizeof_stats_row": ctypes.sizeof(R), "off_empty_cells": R.empty_cells.offset,

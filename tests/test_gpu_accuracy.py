@@ -20,17 +20,45 @@ pytestmark = pytest.mark.gpu
 STEPS = 3
 
 
-def run(mode, params, T, r, s, ty, dc_sweeps=0, guess="plain"):
+# the measured form of the 2048x1024 benchmark: GCR over the cycles with 16
+# directions, 40 sweeps per visit on level 1, 16 on level 2, 8 below, and
+# the learned guess (network of that workload) judged beside the stage
+# increment and the plain start
+BENCH_FORM = dict(gcr=16, level_sweeps={2: 16, "coarse": 8}, guess="learned")
+
+
+def run(mode, params, T, r, s, ty, dc_sweeps=0, guess="plain", gcr=0, level_sweeps=None):
     g = Solver(params, len(r))
     g.initialize(T, r, s, ty)
     g.set_mode(mode)
+    net = None
     if mode != "fp64":
         g.set_dc_sweeps(dc_sweeps)
+        if guess == "learned":
+            import os
+
+            from tracer_code_amd import learn
+            wpath = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tracer_code_amd",
+                                 "weights", "unet_cfg2_L4.npz")
+            w, meta = learn.load_weights(wpath)
+            net = learn.Network(*learn.level_shape(params.nx, params.nz, meta["level"]), trainable=False)
+            net.set_weights(w)
+            g.learn_attach(net)
         g.guess_enable(guess)
+        g.set_gcr(gcr)
+        for L, n in (level_sweeps or {}).items():
+            for lv in (range(3, params.grids) if L == "coarse" else (L,)):
+                g.set_dc_level_sweeps(lv, n)
     g.time_step(STEPS)
     g.compute_velocities()
     out = {k: g.get(k) for k in ("SF", "u", "w", "T", "conduct", "Cnew", "nempty", "count_type")}
     out["r"], out["s"] = g.get_tracers("x")
+    st = g.stats()
+    out["retries"] = st.dc_retries
+    out["starts"] = list(st.guesses_used)
+    if net is not None:
+        g.learn_attach(None)
+        net.close()
     g.close()
     return out
 
@@ -56,16 +84,22 @@ def test_accelerated_forms_against_fp64():
     dref = diagnostics(params, ref)
     # the defect-correction modes with the reference's sweep count, and the
     # mixed mode with a tuned count and the extrapolated starts judged
-    for mode, sweeps, guess in (("dc64", 0, "plain"), ("mixed", 0, "plain"), ("mixed", 32, "extrap")):
-        f = run(mode, params, T, r, s, ty, sweeps, guess)
+    forms = [("dc64", 0, "plain", {}), ("mixed", 0, "plain", {}), ("mixed", 32, "extrap", {}),
+             ("mixed", 40, BENCH_FORM["guess"], BENCH_FORM)]
+    for mode, sweeps, guess, extra in forms:
+        kw = {k: v for k, v in extra.items() if k != "guess"}
+        f = run(mode, params, T, r, s, ty, sweeps, guess, **kw)
         d = diagnostics(params, f)
         err = {k: rel_err(f[k], ref[k]) for k in ("SF", "u", "w", "T")}
         derr = {k: abs(d[k] - dref[k]) / abs(dref[k]) for k in d}
         pos = max(float(np.max(np.abs(f["r"] - ref["r"]))), float(np.max(np.abs(f["s"] - ref["s"]))))
         counts_equal = bool(np.array_equal(f["count_type"], ref["count_type"]))
-        print(f"{mode} sweeps={sweeps or params.Nmulti} guess={guess}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()) + " "
+        print(f"{mode} sweeps={sweeps or params.Nmulti} guess={guess} {kw} starts={f['starts']} "
+              f"retries={f['retries']}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()) + " "
               + " ".join(f"{k}={v:.2e}" for k, v in derr.items())
               + f" tracer pos={pos:.2e} counts equal={counts_equal}", flush=True)
-        assert derr["Nu"] <= 1e-6 and derr["vrms"] <= 1e-6 and derr["Cavg"] <= 1e-6
+        assert derr["Nu"] <= 1e-6 and derr["vrms"] <= 1e-6
+        # composition mass: bit-exact (same tracer counts, same binning)
+        assert derr["Cavg"] == 0.0 and counts_equal
         assert err["SF"] <= 1e-5 and err["u"] <= 1e-5 and err["w"] <= 1e-5
         assert pos <= 1e-7

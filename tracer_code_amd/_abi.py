@@ -73,7 +73,7 @@ class TcStats(ctypes.Structure):
         ("sweeps_exec", ctypes.c_int64 * TC_MAX_LEVELS), ("residuals_exec", ctypes.c_int64 * TC_MAX_LEVELS),
         ("coarse_solves", ctypes.c_int64), ("vis_iters", ctypes.c_int64), ("c_iters", ctypes.c_int64),
         ("t_iters", ctypes.c_int64), ("mg_cycles", ctypes.c_int64), ("guesses_used", ctypes.c_int64 * 3),
-        ("guess_res", ctypes.c_double * 3), ("dc_retries", ctypes.c_int64),
+        ("guess_res", ctypes.c_double * 3), ("dc_retries", ctypes.c_int64), ("q_step", ctypes.c_int32 * 4),
     ]
 
 

@@ -223,6 +223,10 @@ def derivative_eigenvalues(D1, D2, D3, D4, lib=None):
     return out
 
 
+# Keys of a preset that describe the workload (initial conditions, run
+# length) rather than a solver parameter of tc_params.
+WORKLOAD_KEYS = frozenset({"tpc", "dlayer", "initial_comp", "t_variant", "steps", "seed"})
+
 # Benchmark workloads (reference parameter names, main.f90:8-116).
 PRESETS = {
     "cfg0": dict(aspect=1.0, nx=128, nz=128, RaT=1e5, RaC=(0.5e5,), ivis=1, visT=100.0, visC=(1.0,), tpc=20,
@@ -242,8 +246,15 @@ def make_params(nx, nz, aspect=1.0, grids=6, mingrid=4, RaT=1e5, comp=1, ntype=1
                 visT=100.0, visP=1.0, visC=(1.0,), delta_visT=100.0, delta_visC=(2.0,), conduct_factor=(1.0,),
                 H=0.0, Htype=(0.0,), tolerance=1e-8, Nmulti=70, vis_iter=12, max_cycles=1000, eig_ratio=4.0,
                 eig_ratio_min=4.0, courant=0.99, courantT=0.99, courantC=0.99, courant_stream=0.99, torder=4,
-                RK_update=1, Vbc=0, Tbot=0, order=2, fixes=0, cycle="W", lib=None, **_ignored):
-    """tc_params for one run (order=2, iterate_order=1, uniform grid)."""
+                RK_update=1, Vbc=0, Tbot=0, order=2, fixes=0, cycle="W", lib=None, **workload):
+    """tc_params for one run (order=2, iterate_order=1, uniform grid).
+
+    Besides the solver parameters, only the workload keys of a preset
+    (WORKLOAD_KEYS: tracers per node, layer, initial composition, T0 variant,
+    step count) are accepted; any other keyword is a misspelling and raises."""
+    unknown = sorted(set(workload) - WORKLOAD_KEYS)
+    if unknown:
+        raise TypeError(f"make_params: unknown parameter(s) {', '.join(unknown)}")
     nx, nz, grids = optimize_grid(nx, nz, grids, mingrid)
     nsteps, multi = multigrid_setup(grids, cycle)
     D1, D2, D3, D4, Dint = fd_coefficients(order)

@@ -515,6 +515,7 @@ int finish_solve(tc_ctx* ctx, int status, int q, const double* psi, int32_t* q_o
     }
     DEV(ctx).learn_slot = -1;
     ctx->stats.q_last = q;
+    if (ctx->solve_stage >= 0 && ctx->solve_stage < 4) ctx->stats.q_step[ctx->solve_stage] = q;
     ctx->stats.residual_last = ctx->residual_mag[1];
     ctx->stats.res_ref_last = ctx->res_ref[1];
     if (q_out) *q_out = q;
@@ -1069,7 +1070,8 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     ctx->gcr_have = false;
     ctx->gcr_nb = 0;
     // the Nmulti retry of a tuned solve runs the plain cycles
-    const bool gcr = ctx->gcr_m > 0 && !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
+    const bool gcr = ctx->gcr_m > 0 && !ctx->gcr_suspended &&
+                     !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
     if (gcr) TC_TRY(gcr_alloc(ctx));
     // one step p of the cycle schedule: the descent into the level, its
     // visit, and (level 1) the stop test or the ascent out of it
@@ -1239,14 +1241,24 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
 
 // A defect-correction solve with the tuned sweep count, redone with the
 // parameters' Nmulti if it diverges or stalls (SF is only written by a
-// solve that converged, so the retry starts from the same state).
+// solve that converged, so the retry starts from the same state).  With
+// the parameters' own count and GCR on, the retry runs the plain cycles:
+// GCR's recursive residual r_s is never recomputed from psi_s, so its
+// rounding drift could stall a solve that the plain cycles finish.
 int run_multigrid_dc(tc_ctx* ctx, int32_t* q_out)
 {
     const tc_params& P = ctx->p;
     const int fast = ctx->dc_sweeps;
     if (fast <= 0 || fast == P.Nmulti) {
         ctx->dc_sweeps_active = P.Nmulti;
-        return dc_solve(ctx, q_out, P.max_cycles);
+        if (ctx->gcr_m <= 0) return dc_solve(ctx, q_out, P.max_cycles);
+        const int st = dc_solve(ctx, q_out, std::min(P.max_cycles, TC_DC_FAST_MAX_CYCLES));
+        if (st != TC_ERR_RESID_BLOWUP && st != TC_ERR_MAX_CYCLES) return st;
+        ctx->stats.dc_retries++;
+        ctx->gcr_suspended = true;
+        const int st2 = dc_solve(ctx, q_out, P.max_cycles);
+        ctx->gcr_suspended = false;
+        return st2;
     }
     ctx->dc_sweeps_active = fast;
     const int st = dc_solve(ctx, q_out, std::min(P.max_cycles, TC_DC_FAST_MAX_CYCLES));
@@ -1330,27 +1342,31 @@ int smoother_time_T(tc_ctx* ctx, tc_dev& d, const double* Tin, int32_t* n_out)
     return 0;
 }
 
-// smoother_time (smoother.f90:286-340): a Chebyshev pass of 10 iterations
-// (lb=-5, la=-1), then single iterations with factor 1 until smooth; nn
-// counts the passes after the first.
+// smoother_time (smoother.f90:286-340): a Chebyshev pass of Niter
+// iterations (lb=-5, la=-1), then single iterations with factor 1 until
+// smooth; nn counts the passes after the first.  Niter is 10 on entry and is
+// set to 1 (smoother.f90:335) once a type's first pass has run to its end
+// without meeting the stop test; it is not reset for the later types.
 int smoother_time(tc_ctx* ctx, tc_dev& d, int32_t* n_comp)
 {
     const tc_params& P = ctx->p;
-    const int Niter = 10;
+    int Niter = 10;
     const double lb = -5.0, la = -1.0;
     for (int t = 0; t < P.ntype; ++t) {
         TC_TRY(copy_plane(ctx, ctx->Cvis[t], ctx->Cnew[t], ctx->g));
         TC_TRY(copy_plane(ctx, ctx->Cbuoy[t], ctx->Cnew[t], ctx->g));
     }
     for (int t = 0; t < P.ntype; ++t) {
-        auto cfac = [&](int m) {
-            if (m >= Niter) return 1.0;
-            const int ntemp = Niter - (m + 1) + 1;
-            return 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * Niter)));
+        const int N = Niter;
+        auto cfac = [N, lb, la](int m) {
+            if (m >= N) return 1.0;
+            const int ntemp = N - (m + 1) + 1;
+            return 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * N)));
         };
         int iters = 0;
         TC_TRY(filter_loop(ctx, d, ctx->Cvis[t], ctx->tmp2, true, P.delta_C[t], P.courantC, cfac, &iters));
-        const int nn = iters < Niter ? 0 : iters - Niter + 1;
+        const int nn = iters < N ? 0 : iters - N + 1;
+        if (iters >= N) Niter = 1;
         ctx->stats.c_iters += iters;
         ctx->stats.n_comp[t] = nn;
         if (n_comp) n_comp[t] = nn;
@@ -2769,6 +2785,7 @@ int tc_time_step(tc_ctx* ctx, int nsteps)
     tc_dev& d = DEV(ctx);
     for (int s = 0; s < nsteps; ++s) {
         if (ctx->sort_interval > 0 && ctx->tstep % ctx->sort_interval == 0) TC_TRY(sort_tracers(ctx));
+        for (int j = 0; j < 4; ++j) ctx->stats.q_step[j] = 0;
         ctx->solve_stage = 0;
         ctx->solve_tau = ctx->time;
         TC_TRY(run_multigrid(ctx, nullptr));

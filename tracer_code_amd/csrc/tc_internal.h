@@ -120,6 +120,7 @@ struct tc_ctx {
     int gcr_m = 0, gcr_nb = 0;
     int gcr_order[16] = {};
     bool gcr_have = false, gcr_pending = false;
+    bool gcr_suspended = false;  // the plain-cycle retry of a solve that failed under GCR
     double *gcr_hs = nullptr, *gcr_ls = nullptr, *gcr_rs = nullptr;
     double* gcr_Z[16] = {};
     double* gcr_Q[16] = {};

@@ -22,6 +22,7 @@ program tc_layout
   write(*, '(a,i0)') 'sizeof_stats ', c_sizeof(s)
   b1 = transfer(c_loc(s%residual_last), b1); write(*, '(a,i0)') 'off_residual_last ', b1 - b0
   b1 = transfer(c_loc(s%mg_cycles), b1);     write(*, '(a,i0)') 'off_mg_cycles ', b1 - b0
+  b1 = transfer(c_loc(s%q_step), b1);        write(*, '(a,i0)') 'off_q_step ', b1 - b0
   b0 = transfer(c_loc(e), b0)
   write(*, '(a,i0)') 'sizeof_equalize ', c_sizeof(e)
   b1 = transfer(c_loc(e%deficit), b1);       write(*, '(a,i0)') 'off_deficit ', b1 - b0

@@ -48,6 +48,7 @@ module tracer_hip
     integer(c_int64_t) :: coarse_solves, vis_iters, c_iters, t_iters, mg_cycles, guesses_used(3)
     real(c_double) :: guess_res(3)
     integer(c_int64_t) :: dc_retries
+    integer(c_int32_t) :: q_step(4)
   end type tc_stats
 
   ! one row of stats.dat (basics.f90:236-324)
