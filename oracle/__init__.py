@@ -17,7 +17,9 @@ import numpy as np
 from tracer_code_amd._abi import TcEqualizeInfo, TcParams, TcStatsRow
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB selects another build of tc_oracle.c (the mutation test of the
+# pins loads deliberately broken copies this way)
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 # ids (tc_oracle.h)
 OR_T, OR_SF, OR_U, OR_W, OR_STRAIN, OR_TVIS, OR_TBUOY, OR_CONDUCT = range(8)
@@ -35,6 +37,7 @@ ORS_DT, ORS_TIME, ORS_RESID1, ORS_RESREF1, ORS_EIG_FINE, ORS_DELTA_T = range(6)
  ORS_T_ITERS) = range(16, 32)
 ORS_Q_STEP = tuple(range(32, 36))
 ORS_N_COMP_T = tuple(range(36, 40))  # n_comp of types 1..4
+ORS_INTERP_CLAMPS = 40
 
 _lib = None
 
@@ -101,6 +104,7 @@ def lib():
             ("or_compute_velocities", [vp]), ("or_compute_strain_rate_invariant", [vp]),
             ("or_smoother_time", [vp]), ("or_load_conductivity_array", [vp]),
             ("or_compute_derivatives", [vp, vp, vp]), ("or_energy_stage", [vp, i, d]), ("or_stats", [vp, vp]),
+            ("or_visf_plane", [vp, vp]),
         ]:
             getattr(L, n).argtypes = a
             getattr(L, n).restype = None

@@ -70,6 +70,9 @@ struct or_state {
     int tstep, q, n_eta, n_temp, n_comp[TC_MAX_TYPES + 1], status, smooth_vis;
     int Nvis_store, nn_store;
     int64_t cycles_total, vis_iters, c_iters, t_iters;
+    /* interpolations whose cell index needed the clamp (positions outside
+     * the domain; the reference would read outside its arrays there) */
+    int64_t interp_clamps;
     /* cycles of each solve of the running or_time_step (q_step_n = solves
      * recorded so far; -1 outside a time step) */
     int q_step[4], q_step_n;
@@ -490,6 +493,7 @@ double or_get_scalar(const or_state* s, int which)
     case ORS_Q_STEP0: case ORS_Q_STEP1: case ORS_Q_STEP2: case ORS_Q_STEP3: return s->q_step[which - ORS_Q_STEP0];
     case ORS_N_COMP_T1: case ORS_N_COMP_T2: case ORS_N_COMP_T3: case ORS_N_COMP_T4:
         return s->n_comp[1 + which - ORS_N_COMP_T1];
+    case ORS_INTERP_CLAMPS: return (double)s->interp_clamps;
     default: return NAN;
     }
 }
@@ -822,6 +826,15 @@ static double visf(const or_state* s, int i, int k)
         return exp(-dp + (1.0 - zgk) * log(s->p.visP));
     }
     return 1.0;
+}
+
+/* visf over the whole ghosted plane of level 1 (the first loop of
+ * compute_viscosity, iterative_solver.f90:340-345), without the filter that
+ * follows it there. */
+void or_visf_plane(or_state* s, double* out)
+{
+    for (int k = -SPAN; k <= s->nz + SPAN; ++k)
+        for (int i = -SPAN; i <= s->nx + SPAN; ++i) out[IX(s, i, k)] = visf(s, i, k);
 }
 
 /* viscosity_gradients (iterative_solver.f90:290-329). */
@@ -1807,7 +1820,9 @@ double or_interpolate(const or_state* s, double r, double sv, const double* DERr
     int k = (int)(sv / s->ds);
     /* In-domain positions give 0<=i<=nx-1, 0<=k<=nz-1.  The reference reads
      * outside its arrays otherwise; the clamp (same as the HIP kernel) only
-     * keeps invalid input inside the ghost ring. */
+     * keeps invalid input inside the ghost ring, and is counted. */
+    if (i < 0 || i > s->nx - 1 || k < 0 || k > s->nz - 1)
+        __atomic_fetch_add(&((or_state*)s)->interp_clamps, 1, __ATOMIC_RELAXED);
     i = i < 0 ? 0 : (i > s->nx - 1 ? s->nx - 1 : i);
     k = k < 0 ? 0 : (k > s->nz - 1 ? s->nz - 1 : k);
     double hr = r - (double)i * s->dr;

@@ -51,7 +51,9 @@ enum {
     /* cycles of each solve of the last or_time_step, in call order */
     ORS_Q_STEP0 = 32, ORS_Q_STEP1, ORS_Q_STEP2, ORS_Q_STEP3,
     /* n_comp of types 1..4 after the last smoother_time */
-    ORS_N_COMP_T1 = 36, ORS_N_COMP_T2, ORS_N_COMP_T3, ORS_N_COMP_T4
+    ORS_N_COMP_T1 = 36, ORS_N_COMP_T2, ORS_N_COMP_T3, ORS_N_COMP_T4,
+    /* interpolations of out-of-domain positions (cumulative) */
+    ORS_INTERP_CLAMPS = 40
 };
 
 or_state* or_create(const tc_params* p, int64_t ntr);
@@ -86,6 +88,7 @@ void or_compute_RHS(or_state* s, int level);
 void or_compute_reference_residual(or_state* s, int level);
 void or_restrict(or_state* s, int level, const double* in, double* out);
 void or_prolong(or_state* s, int level_coarse);
+void or_visf_plane(or_state* s, double* out);
 void or_viscosity_gradients(or_state* s, int level);
 void or_test_eigenvalues(or_state* s, int level);
 void or_iterate_local_time_steps(or_state* s, int level);

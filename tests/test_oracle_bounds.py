@@ -1,0 +1,98 @@
+"""Bounds of the oracle's tracer interpolation at aspect 3 (CPU only).
+
+interpolate (composition.f90:838-898) indexes its arrays with
+i = int(r/dr), k = int(s/ds) and reads f(i..i+1, k-1..k+2) and DERr; for a
+position inside the domain that is always inside the arrays.  The oracle (and
+the HIP kernel) clamp i and k so that an out-of-domain position cannot read
+outside the ghost ring, and the oracle counts every clamped call.
+
+* The round-1 faulting case (768x256 physics at 96x32, RK4 stages driven
+  back to back from one solve's velocities, as the GPU parity test does):
+  every stage position stays in the domain and the clamp is never reached,
+  under an AddressSanitizer build of the oracle too.
+* A full time step of the same physics from another seed, where the third
+  solve's velocities outgrow the step's dt and tracers cross the bottom
+  wall: the reference stops with "tracer out of domain" at the binning that
+  precedes every interpolation of a stage position, so again no clamp.
+* The counter itself: an out-of-domain call is counted.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+import oracle
+from conftest import load_oracle_state
+from tracer_code_amd import config, ic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+STAGES = """
+import sys
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+import numpy as np, oracle
+from conftest import load_oracle_state
+from tracer_code_amd import config, ic
+params, cfg = config.preset_params("cfg1", nx=96, nz=32, tpc=8)
+T, r, s, ty = ic.workload(cfg, seed=1)
+o = load_oracle_state(params, T, r, s, ty)
+assert o.multigrid() == 0
+o.compute_velocities()
+dt = o.stability_RK4()
+for stage, (a, b) in ((1, (oracle.OR_RTR1, oracle.OR_STR1)), (2, (oracle.OR_RTR2, oracle.OR_STR2)),
+                      (3, (oracle.OR_RTR1, oracle.OR_STR1)), (4, (oracle.OR_RTR, oracle.OR_STR))):
+    assert o.tracer_stage(stage, dt) == 0
+    R, S = o.tracers(a), o.tracers(b)
+    assert R.min() > 0 and R.max() < params.aspect and S.min() > 0 and S.max() < 1, stage
+print("clamps", int(o.scalar(oracle.ORS_INTERP_CLAMPS)))
+"""
+
+
+def test_aspect3_stages_stay_in_bounds():
+    code = STAGES.format(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "clamps 0" in r.stdout
+
+
+def test_aspect3_stages_under_address_sanitizer(tmp_path):
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    if not asan or not os.path.isabs(asan) or not os.path.exists(asan):
+        import pytest
+        pytest.skip("no libasan")
+    lib = str(tmp_path / "liboracle_asan.so")
+    subprocess.run(["gcc", "-O1", "-g", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fsanitize=address",
+                    "-fno-omit-frame-pointer", "-shared", "-I", os.path.join(ROOT, "oracle"), "-o", lib,
+                    os.path.join(ROOT, "oracle", "tc_oracle.c"), "-lm"], check=True, capture_output=True)
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
+               ORACLE_LIB=lib, OMP_NUM_THREADS="5")
+    code = STAGES.format(root=ROOT, tests=os.path.join(ROOT, "tests"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600, env=env)
+    assert "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "clamps 0" in r.stdout
+
+
+def test_domain_stop_precedes_interpolation():
+    params, cfg = config.preset_params("cfg1", nx=96, nz=32, tpc=8)
+    T, r, s, ty = ic.workload(cfg, seed=2)
+    o = load_oracle_state(params, T, r, s, ty)
+    st = o.time_step()
+    assert st == 3  # TC_ERR_TRACER_DOMAIN: "tracer out of domain"
+    assert int(o.scalar(oracle.ORS_INTERP_CLAMPS)) == 0
+    o.close()
+
+
+def test_clamp_counter_counts_out_of_domain_calls():
+    params, cfg = config.preset_params("cfg0", nx=16, nz=16, tpc=1)
+    o = oracle.Oracle(params, 0)
+    f = np.ascontiguousarray(o.plane(oracle.OR_U))
+    d = np.zeros_like(f)
+    L = o.L
+    L.or_interpolate(o.h, 0.5, 0.5, d.ctypes.data, f.ctypes.data)
+    assert int(o.scalar(oracle.ORS_INTERP_CLAMPS)) == 0
+    L.or_interpolate(o.h, -0.3, 0.5, d.ctypes.data, f.ctypes.data)
+    L.or_interpolate(o.h, 0.5, 1.2, d.ctypes.data, f.ctypes.data)
+    assert int(o.scalar(oracle.ORS_INTERP_CLAMPS)) == 2
+    o.close()
