@@ -4,33 +4,41 @@
 
 A step is one pass of the reference's time loop body (main.f90:174-205):
 multigrid, compute_velocities, the strain rate, and energy_time with RK4 and
-RK_update=1 -- four multigrid solves to a 1e-8 relative residual, four RK
-tracer stages over all tracers, five tracer-to-node binnings with the C
-filter, the T filter and the T-field RK stages.  Inputs are synthetic
-(seeded) and resident in HBM when the timed region starts.
+RK_update=1 -- four multigrid solves, four RK tracer stages over all
+tracers, five tracer-to-node binnings with the C filter, the T filter and
+the T-field RK stages.  Inputs are synthetic (seeded) and resident in HBM
+when the timed region starts.
 
-Modes: `mixed` (default) and `dc64` solve by defect correction with a
-double-double streamfunction and FP32 / FP64 W-cycle corrections; `fp64` is
-the reference's own cycle, which FP64 rounding keeps above a 1e-8 residual at
-2048x1024 (it stops with "Too Many Multigrid Cycles").  The defect-correction
-modes run GCR over the cycles (`--gcr 16`) and, before the warm-up, tune their
-sweep counts with real time steps: the common count, one for levels 3 and
-below, one for level 2, then the common count again (level 1's);
-`--dc-sweeps` [`--level2-sweeps`, `--coarse-sweeps`] fix them instead.
+The headline (`value`) is the accelerated form at the metric's 1e-8
+relative residual: `mixed` defect correction (double-double streamfunction,
+FP32 W-cycle corrections) with GCR over the cycles, autotuned sweep counts
+and the stage-increment start judged against the plain start.  In the same
+run (`--legs`, N=1) the other forms of the metric are timed on the same
+inputs:
 
-Prints one JSON line.  `roofline` reports the level-1 Jacobi-Richardson sweep
+* `fp64_reference`: the reference's own cycle in FP64 (Jacobi-Richardson
+  sweeps, Nmulti=70, W-cycle) at the smallest tolerance FP64 reaches at
+  2048x1024 (1e-7; its rounding floor is ~8e-8, so at 1e-8 it stops with
+  "Too Many Multigrid Cycles");
+* `dc64`: defect correction with FP64 corrections at 1e-8;
+* `mixed_learned`: the headline form with the trained network's guess
+  judged as a third start.
+
+`roofline` reports the level-1 Jacobi-Richardson sweep of the headline form
 (the dominant kernel): algorithmic bytes per interior point are the
 reference's arrays read or written once -- S, eta, eta_x, eta_z, eta_xx,
 eta_zz, eta_xz, RHS, dt_stream, ratio read, the new iterate written: 11 x 8 B
 = 88 B in FP64, 44 B in FP32 -- over the HIP-event-timed average launch
-duration on the library's stream.  The kernels themselves move less
-(`kernel_bytes_per_launch`: eta_xx - eta_zz is one plane formed once per
-solve, and the ratio plane is read only on the level where the reference's
-ratio is not the reset value): 80 B / 36 B per point.  `step_roofline`
-sums the same per-point bytes over every sweep of the step plus 304 B per
-tracer for the RK stages and 20 B per tracer per binning.  `cpu_baseline`
-times the CPU oracle's level-1 sweep on this host and converts it to
-steps/s with the sweep counts the GPU executed per step.
+duration on the library's stream.  `cpu_baseline` times the CPU port of the
+reference form on this host (one complete FP64 multigrid solve from the
+fp64 leg's state, one RK tracer stage, one binning with the C filter, the T
+filter and one T stage) and composes a step from the counts the fp64 leg
+executed.
+
+    python bench.py --copy-calibration MIB
+
+runs only a device-to-device copy of MIB mebibytes (three times), the
+known-size kernel that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE.
 """
 import argparse
 import json
@@ -50,104 +58,40 @@ DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
 COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
 LEVEL2_SWEEP_CANDIDATES = (8, 12, 16, 24, 0)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
+# smallest relative residual the FP64 reference form reaches at 2048x1024
+FP64_TOLERANCE = 1e-7
 
 
-def cpu_baseline(params, sweeps_per_step, budget_s=12.0):
-    """Oracle (C restatement, OpenMP) level-1 sweep time on the host cores,
-    scaled to steps/s with the GPU's per-level sweep counts (smaller levels
-    scaled by point count, which favours the CPU)."""
-    import numpy as np
-
-    import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    o = oracle.Oracle(params, 0)
-    nx, nz = params.nx, params.nz
-    for oid in (oracle.OR_VIS,):
-        o.plane(oid, 1)[:] = 1.0
-    rng = np.random.default_rng(0)
-    o.plane(oracle.OR_ERROR, 1)[:] = rng.standard_normal((nz + 5, nx + 5))
-    o.plane(oracle.OR_RHS, 1)[:] = rng.standard_normal((nz + 5, nx + 5))
-    o.plane(oracle.OR_DTSTREAM, 1)[:] = 1e-9
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        o.sweep(1, 1 + (n % params.Nmulti), params.Nmulti, 0)
-        n += 1
-        el = time.perf_counter() - t0
-        if (el > budget_s and n >= 2) or n >= 400:
-            break
-    t_sweep = el / n
-    pts1 = (nx - 1) * (nz - 1)
-    fe = 0.0
-    for L, cnt in enumerate(sweeps_per_step, start=1):
-        ptsL = (nx // 2 ** (L - 1) - 1) * (nz // 2 ** (L - 1) - 1)
-        fe += cnt * ptsL / pts1
-    o.close()
-    step_s = fe * t_sweep
-    return {
-        "value": (1.0 / step_s) if step_s > 0 else None,
-        "unit": "steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"{n} level-1 Jacobi-Richardson sweeps of the CPU oracle at {nx}x{nz} "
-                   f"({t_sweep * 1e3:.1f} ms/sweep, {threads} OpenMP threads), scaled by the "
-                   f"{fe:.0f} fine-grid-equivalent sweeps per step the GPU executed; viscosity "
-                   f"hierarchy, coarse solves, filters and tracers not counted (favours the CPU)"),
-    }
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg2")
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "dc64", "fp64"])
-    ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cycle", default="W", choices=["W", "F", "V"])
-    ap.add_argument("--tolerance", type=float, default=None,
-                    help="relative residual of every solve (default: the preset's 1e-8)")
-    ap.add_argument("--nmulti", type=int, default=None, help="smoother sweeps per level visit (default: the preset's)")
-    ap.add_argument("--dc-sweeps", type=int, default=None,
-                    help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
-    ap.add_argument("--coarse-sweeps", type=int, default=None,
-                    help="sweeps per visit on levels 3 and below with --dc-sweeps (default: that count)")
-    ap.add_argument("--level2-sweeps", type=int, default=None,
-                    help="sweeps per visit on level 2 with --dc-sweeps (default: that count)")
-    ap.add_argument("--guess", default="learned", choices=["plain", "extrap", "learned"],
-                    help="initial-guess candidates of the defect-correction solves (learned: with the committed "
-                         "network of this workload, if there is one)")
-    ap.add_argument("--gcr", type=int, default=16,
-                    help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs the hot path on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
-        return 2
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst[-16:], src[-16:]))
+    print(f"copied {mib} MiB x3 (bytes per copy read={mib << 20} written={mib << 20}) ok={ok}", flush=True)
+    return 0 if ok else 1
 
-    from tracer_code_amd import Solver, _abi, config, ic
 
-    ov = dict(cycle=args.cycle)
-    if args.tolerance:
-        ov["tolerance"] = args.tolerance
-    if args.nmulti:
-        ov["Nmulti"] = args.nmulti
-    params, cfg = config.preset_params(args.config, **ov)
-    t_setup = time.perf_counter()
-    T, r, s, ty = ic.workload(cfg, seed=args.seed)
-    ntr = len(r)
-    print(f"[bench] {args.config}: {params.nx}x{params.nz}, {params.grids} levels, {ntr} tracers, "
-          f"inputs generated in {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
-    solver = Solver(params, ntr)
-    solver.set_mode(args.mode)
+def make_solver(params, inputs, mode, guess, gcr, config_name):
+    """A solver on the seeded inputs; returns (solver, network or None, guess)."""
+    from tracer_code_amd import Solver
+    T, r, s, ty = inputs
+    solver = Solver(params, len(r))
+    solver.set_mode(mode)
     solver.initialize(T, r, s, ty)
-    del T, r, s, ty
     net = None
-    guess = args.guess if args.mode != "fp64" else "plain"
+    if mode == "fp64":
+        return solver, None, "plain"
     if guess == "learned":
         from tracer_code_amd import learn
-        wpath = os.path.join(ROOT, "tracer_code_amd", "weights", f"unet_{args.config}_L4.npz")
+        wpath = os.path.join(ROOT, "tracer_code_amd", "weights", f"unet_{config_name}_L4.npz")
         if os.path.exists(wpath):
             w, meta = learn.load_weights(wpath)
             net = learn.Network(*learn.level_shape(params.nx, params.nz, meta["level"]), trainable=False)
@@ -155,78 +99,284 @@ def main():
             solver.learn_attach(net)
         else:
             guess = "extrap"
-    gcr = args.gcr if args.mode != "fp64" else 0
-    if args.mode != "fp64":
-        solver.guess_enable(guess)
-        solver.set_gcr(gcr)
-    tune = None
-    tiles = None
-    coarse_sweeps, coarse_tune = 0, None
-    level2_sweeps, level2_tune = 0, None
-    dc_sweeps = params.Nmulti
-    if args.mode != "fp64":
-        if args.dc_sweeps is None:
-            # untimed: one real time step per candidate
+    solver.guess_enable(guess)
+    solver.set_gcr(gcr)
+    return solver, net, guess
+
+
+def close_solver(solver, net):
+    if net is not None:
+        solver.learn_attach(None)
+        net.close()
+    solver.close()
+
+
+def tune_counts(solver, params, mode, args):
+    """Sweep counts of the defect-correction cycle: autotuned with real time
+    steps (common count, levels 3 and below, level 2, the common count again)
+    or fixed by the command line.  Returns the counts and the tuning record."""
+    rec = {"tune": None, "coarse_tune": None, "level2_tune": None, "tiles": None}
+    common, coarse, level2 = params.Nmulti, 0, 0
+    if args.dc_sweeps is None:
+        t0 = time.perf_counter()
+        common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+        log(f"{mode}: autotuned sweeps per visit: {common} ({time.perf_counter() - t0:.1f}s; ms/step {rec['tune']})")
+        if params.grids > 3:
             t0 = time.perf_counter()
-            dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
-            print(f"[bench] autotuned sweeps per visit: {dc_sweeps} ({time.perf_counter() - t0:.1f}s; "
-                  f"ms/step {tune})", file=sys.stderr, flush=True)
-            if params.grids > 3:
-                # levels 3 and below, then level 2: counts of their own
-                t0 = time.perf_counter()
-                coarse_sweeps, coarse_tune = solver.autotune_dc_level_sweeps(3, params.grids - 1,
-                                                                             COARSE_SWEEP_CANDIDATES, 1)
-                level2_sweeps, level2_tune = solver.autotune_dc_level_sweeps(2, 2, LEVEL2_SWEEP_CANDIDATES, 1)
-                print(f"[bench] autotuned sweeps per visit on levels >= 3: {coarse_sweeps or dc_sweeps}, on level 2: "
-                      f"{level2_sweeps or dc_sweeps} ({time.perf_counter() - t0:.1f}s; ms/step {coarse_tune}, "
-                      f"{level2_tune})", file=sys.stderr, flush=True)
-                # the common count again, now that the coarse levels have their own
-                t0 = time.perf_counter()
-                dc_sweeps, tune = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
-                print(f"[bench] sweeps per visit on levels 1-2, tuned again: {dc_sweeps} "
-                      f"({time.perf_counter() - t0:.1f}s; ms/step {tune})", file=sys.stderr, flush=True)
-            if args.mode == "mixed":
-                t0 = time.perf_counter()
-                tiles = solver.autotune_tb_tiles(1)
-                print(f"[bench] autotuned tiles by level: {tiles} ({time.perf_counter() - t0:.1f}s)",
-                      file=sys.stderr, flush=True)
-        else:
-            dc_sweeps = args.dc_sweeps
-            solver.set_dc_sweeps(dc_sweeps)
-            if args.coarse_sweeps:
-                coarse_sweeps = args.coarse_sweeps
-                for L in range(3, params.grids):
-                    solver.set_dc_level_sweeps(L, coarse_sweeps)
-            if args.level2_sweeps:
-                level2_sweeps = args.level2_sweeps
-                solver.set_dc_level_sweeps(2, level2_sweeps)
-    for w in range(args.warmup):
+            coarse, rec["coarse_tune"] = solver.autotune_dc_level_sweeps(3, params.grids - 1, COARSE_SWEEP_CANDIDATES, 1)
+            level2, rec["level2_tune"] = solver.autotune_dc_level_sweeps(2, 2, LEVEL2_SWEEP_CANDIDATES, 1)
+            log(f"{mode}: levels >= 3: {coarse or common}, level 2: {level2 or common} "
+                f"({time.perf_counter() - t0:.1f}s; ms/step {rec['coarse_tune']}, {rec['level2_tune']})")
+            t0 = time.perf_counter()
+            common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+            log(f"{mode}: levels 1-2 tuned again: {common} ({time.perf_counter() - t0:.1f}s; ms/step {rec['tune']})")
+        if mode == "mixed":
+            t0 = time.perf_counter()
+            rec["tiles"] = solver.autotune_tb_tiles(1)
+            log(f"{mode}: autotuned tiles by level: {rec['tiles']} ({time.perf_counter() - t0:.1f}s)")
+    else:
+        common = args.dc_sweeps
+        coarse = args.coarse_sweeps or 0
+        level2 = args.level2_sweeps or 0
+    counts = {"common": common, "coarse": coarse, "level2": level2, "tiles": rec["tiles"]}
+    apply_counts(solver, params, counts)
+    return counts, rec
+
+
+def apply_counts(solver, params, counts):
+    solver.set_dc_sweeps(counts["common"])
+    for L in range(3, params.grids):
+        solver.set_dc_level_sweeps(L, counts["coarse"])
+    if params.grids > 2:
+        solver.set_dc_level_sweeps(2, counts["level2"])
+    if counts.get("tiles"):
+        for L, t in enumerate(counts["tiles"], start=1):
+            if t:
+                solver.set_tb_tile(L, t)
+
+
+def run_timed(solver, params, warmup, steps, label, profile=False):
+    """W untimed steps, then K steps timed between two stream syncs."""
+    from tracer_code_amd import _abi
+    for w in range(warmup):
         t0 = time.perf_counter()
         solver.time_step(1)
         st = solver.stats()
-        print(f"[bench] warmup step {w}: {time.perf_counter() - t0:.2f}s q_last={st.q_last} "
-              f"dt={st.dt_last:.3e}", file=sys.stderr, flush=True)
+        log(f"{label}: warmup step {w}: {time.perf_counter() - t0:.2f}s q={list(st.q_step)} dt={st.dt_last:.3e}")
     before = solver.stats()
     gcr0 = solver.gcr_counts()[0]
     sweeps0 = list(before.sweeps_exec)
-    cycles0 = before.mg_cycles
-    solver.set_profiling(True)
+    if profile:
+        solver.set_profiling(True)
     solver.sync()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    qs = []
+    for k in range(steps):
         solver.time_step(1)
-        print(f"[bench] step {k} done at {time.perf_counter() - t0:.2f}s", file=sys.stderr, flush=True)
+        qs.append(list(solver.stats().q_step))
+        log(f"{label}: step {k} done at {time.perf_counter() - t0:.2f}s q={qs[-1]}")
     solver.sync()
     elapsed = time.perf_counter() - t0
-    avg_us, nlaunch = solver.kernel_timing(_abi.TC_K_SWEEP_L1)
-    solver.set_profiling(False)
+    avg_us, nlaunch = (solver.kernel_timing(_abi.TC_K_SWEEP_L1) if profile else (0.0, 0))
+    if profile:
+        solver.set_profiling(False)
     after = solver.stats()
-    gcr_updates = solver.gcr_counts()[0] - gcr0
-    sweeps_per_step = [(a - b) / args.steps for a, b in zip(after.sweeps_exec, sweeps0)][: params.grids]
-    cycles_per_step = (after.mg_cycles - cycles0) / args.steps
-    steps_per_s = args.steps / elapsed
+    return {
+        "steps_per_s": steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "elapsed": elapsed,
+        "mg_cycles_per_step": (after.mg_cycles - before.mg_cycles) / steps,
+        "q_per_solve": qs,
+        "sweeps_per_step": [(a - b) / steps for a, b in zip(after.sweeps_exec, sweeps0)][: params.grids],
+        "c_filter_iters_per_step": (after.c_iters - before.c_iters) / steps,
+        "t_filter_iters_per_step": (after.t_iters - before.t_iters) / steps,
+        "vis_iters_per_step": (after.vis_iters - before.vis_iters) / steps,
+        "starts": {k: after.guesses_used[i] - before.guesses_used[i]
+                   for i, k in enumerate(("plain", "extrap", "learned"))},
+        "dc_retries": after.dc_retries - before.dc_retries,
+        "gcr_updates": solver.gcr_counts()[0] - gcr0,
+        "avg_sweep_us": avg_us,
+        "sweep_launches": nlaunch,
+        "dt_last": after.dt_last,
+    }
+
+
+def _abi_tr_x1():
+    from tracer_code_amd import _abi
+    return _abi.TC_TR_X1
+
+
+def cpu_baseline(params, state, leg, budget_s=30.0):
+    """The CPU port of the reference form (oracle/tc_oracle.c, OpenMP over
+    this host's threads), timed piece by piece on the fp64 leg's final state:
+
+    * the first solve of the next step, warm-started from the leg's SF,
+      stopped after one and after two cycles: the difference is the time of a
+      cycle, the rest the per-solve set-up (RHS, viscosity hierarchy, coarse
+      LU, local steps); the whole solve too when its estimate fits budget_s;
+    * one RK4 tracer stage over all tracers;
+    * one binning without the filter, the C filter, the T filter and one
+      T-field stage.
+
+    A step is composed with the counts the fp64 GPU leg executed per step:
+    its cycles, four solve set-ups, four tracer stages, five binnings, its
+    C and T filter iterations and four T stages."""
+    import numpy as np
+
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    T, SF, r, s, ty = state
+    o = oracle.Oracle(params, len(r))
+    o.plane(oracle.OR_T)[:, :] = T
+    o.plane(oracle.OR_SF)[:, :] = SF
+    o.tracers(oracle.OR_RTR)[:] = r
+    o.tracers(oracle.OR_STR)[:] = s
+    o.tracers(oracle.OR_TTYPE, np.int32)[:] = ty
+    if o.initialize_state() != 0:   # binning, T filter, C filter, conductivity
+        o.close()
+        return None
+    # a failed solve leaves SF as it was, so each run starts from the same state
+    times = {}
+    for cyc in (1, 2):
+        o.set_scalar(oracle.ORS_MAX_CYCLES, cyc)
+        t0 = time.perf_counter()
+        o.multigrid()
+        times[cyc] = time.perf_counter() - t0
+    t_cycle = times[2] - times[1]
+    t_setup = max(times[1] - t_cycle, 0.0)
+    log(f"cpu: solve set-up {t_setup:.2f}s, {t_cycle:.2f}s per cycle")
+    q_gpu = leg["q_per_solve"][-1][0] if leg.get("q_per_solve") else 0
+    full = None
+    if q_gpu and t_setup + q_gpu * t_cycle <= budget_s:
+        o.set_scalar(oracle.ORS_MAX_CYCLES, params.max_cycles)
+        t0 = time.perf_counter()
+        st = o.multigrid()
+        full = (time.perf_counter() - t0, int(o.scalar(oracle.ORS_Q)), st)
+        log(f"cpu: complete solve {full[0]:.2f}s, q={full[1]} (GPU's last first solve: {q_gpu}), status {st}")
+    o.compute_velocities()
+    dt = o.stability_RK4()
+    t0 = time.perf_counter()
+    o.tracer_stage(1, dt)
+    t_stage = time.perf_counter() - t0
+    for t in range(1, params.ntype + 1):
+        o.plane(oracle.OR_C0, t)[:] = o.plane(oracle.OR_CNEW, t)
+    o.plane(oracle.OR_T0)[:] = o.plane(oracle.OR_T)
+    t0 = time.perf_counter()
+    o.energy_stage(1, dt)
+    t_tstage = time.perf_counter() - t0
+    o.set_scalar(oracle.ORS_SMOOTH_VIS, 0)
+    t0 = time.perf_counter()
+    o.tracers_to_corners(_abi_tr_x1())   # binning of the stage positions + conductivity
+    t_bin = time.perf_counter() - t0
+    c0 = o.scalar(oracle.ORS_C_ITERS)
+    t0 = time.perf_counter()
+    o.smoother_time()
+    t_cfilter = time.perf_counter() - t0
+    nC = o.scalar(oracle.ORS_C_ITERS) - c0
+    t0 = time.perf_counter()
+    nT = o.smoother_time_T(o.plane(oracle.OR_T1).ctypes.data)
+    t_tfilter = time.perf_counter() - t0
+    o.close()
+    t_C_iter = t_cfilter / nC if nC > 0 else 0.0
+    t_T_iter = t_tfilter / nT if nT > 0 else 0.0
+    parts = {
+        "solve_setups": 4 * t_setup,
+        "cycles": leg["mg_cycles_per_step"] * t_cycle,
+        "tracer_stages": 4 * t_stage,
+        "binnings": 5 * t_bin,
+        "c_filter": leg["c_filter_iters_per_step"] * t_C_iter,
+        "t_filter": leg["t_filter_iters_per_step"] * t_T_iter,
+        "t_stages": 4 * t_tstage,
+    }
+    step = sum(parts.values())
+    solve_txt = (f"the complete solve took {full[0]:.1f}s in {full[1]} cycles" if full else
+                 f"the complete solve (~{t_setup + q_gpu * t_cycle:.0f}s estimated) was not run")
+    return {
+        "value": 1.0 / step,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"CPU port of the reference form (FP64, Nmulti={params.Nmulti}, W-cycle, tolerance "
+                   f"{params.tolerance:g}, zeroed coarse corrections) at {params.nx}x{params.nz} with {len(r)} "
+                   f"tracers, {threads} OpenMP threads, on the fp64 leg's final state: the next solve stopped "
+                   f"after 1 and 2 cycles ({t_cycle:.2f}s per cycle, {t_setup:.2f}s set-up; {solve_txt}), one "
+                   f"RK stage over all tracers ({t_stage:.2f}s), one binning ({t_bin:.2f}s), the C filter "
+                   f"({t_cfilter:.2f}s, {nC:.0f} iterations), the T filter ({t_tfilter:.2f}s, {nT} iterations), "
+                   f"one T stage ({t_tstage:.3f}s); one step composed with the fp64 leg's counts per step "
+                   f"({leg['mg_cycles_per_step']:.1f} cycles, 4 solves, 4 stages, 5 binnings, "
+                   f"{leg['c_filter_iters_per_step']:.0f} C and {leg['t_filter_iters_per_step']:.0f} T filter "
+                   f"iterations): {step:.1f}s"),
+        "step_parts_s": parts,
+        "complete_solve": ({"seconds": full[0], "cycles": full[1], "status": full[2], "gpu_cycles": q_gpu}
+                           if full else None),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "dc64", "fp64"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cycle", default="W", choices=["W", "F", "V"])
+    ap.add_argument("--tolerance", type=float, default=None,
+                    help="relative residual of every solve (default: the preset's 1e-8)")
+    ap.add_argument("--nmulti", type=int, default=None, help="smoother sweeps per level visit (default: the preset's)")
+    ap.add_argument("--delta-visc", type=float, default=None,
+                    help="delta_visC of the C filter threshold (default: the preset's)")
+    ap.add_argument("--dc-sweeps", type=int, default=None,
+                    help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
+    ap.add_argument("--coarse-sweeps", type=int, default=None,
+                    help="sweeps per visit on levels 3 and below with --dc-sweeps (default: that count)")
+    ap.add_argument("--level2-sweeps", type=int, default=None,
+                    help="sweeps per visit on level 2 with --dc-sweeps (default: that count)")
+    ap.add_argument("--guess", default="extrap", choices=["plain", "extrap", "learned"],
+                    help="initial-guess candidates of the headline's solves")
+    ap.add_argument("--gcr", type=int, default=16,
+                    help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
+    ap.add_argument("--legs", default="fp64,dc64,learned",
+                    help="other forms timed in the same run (comma list of fp64, dc64, learned; empty: none)")
+    ap.add_argument("--leg-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=None, metavar="MIB")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the hot path on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is a design note only)", file=sys.stderr)
+        return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+
+    from tracer_code_amd import _abi, config, ic
+
+    ov = dict(cycle=args.cycle)
+    if args.tolerance:
+        ov["tolerance"] = args.tolerance
+    if args.nmulti:
+        ov["Nmulti"] = args.nmulti
+    if args.delta_visc:
+        ov["delta_visC"] = (args.delta_visc,)
+    params, cfg = config.preset_params(args.config, **ov)
+    t_setup = time.perf_counter()
+    inputs = ic.workload(cfg, seed=args.seed)
+    ntr = len(inputs[1])
+    log(f"{args.config}: {params.nx}x{params.nz}, {params.grids} levels, {ntr} tracers, "
+        f"inputs generated in {time.perf_counter() - t_setup:.1f}s")
+
+    # ---- headline form ----------------------------------------------------
+    gcr = args.gcr if args.mode != "fp64" else 0
+    solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config)
+    counts, rec = ({"common": params.Nmulti, "coarse": 0, "level2": 0, "tiles": None}, {})
+    if args.mode != "fp64":
+        counts, rec = tune_counts(solver, params, args.mode, args)
+    head = run_timed(solver, params, args.warmup, args.steps, args.mode, profile=True)
+    close_solver(solver, net)
+
     pts1 = (params.nx - 1) * (params.nz - 1)
     bytes_per_launch = SWEEP_BYTES_PER_POINT[args.mode] * pts1
+    avg_us = head["avg_sweep_us"]
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else None
     traffic = None
     kname, kfile = SWEEP_KERNEL[args.mode]
@@ -239,47 +389,98 @@ def main():
     # the tracer stages and binnings (residuals, transfers, filters and the
     # viscosity set-up are not counted)
     step_bytes = TRACER_BYTES_PER_STEP * ntr
-    for L, cnt in enumerate(sweeps_per_step, start=1):
+    for L, cnt in enumerate(head["sweeps_per_step"], start=1):
         step_bytes += cnt * (params.nx // 2 ** (L - 1) - 1) * (params.nz // 2 ** (L - 1) - 1) * \
             SWEEP_BYTES_PER_POINT[args.mode]
-    step_gbs = step_bytes / (elapsed / args.steps) / 1e9
+    step_gbs = step_bytes / (head["elapsed"] / args.steps) / 1e9
+
+    # ---- other forms of the metric, same inputs ---------------------------
+    legs = {}
+    want = [x for x in args.legs.split(",") if x] if args.mode == "mixed" else []
+    fp64_state = None
+    for name in want:
+        if name == "learned":
+            s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config)
+            apply_counts(s2, params, counts)
+            res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned")
+            close_solver(s2, n2)
+            res.update({"mode": "mixed", "guess": g2, "tolerance": params.tolerance, "gcr_directions": gcr,
+                        "sweeps_per_visit": [counts["common"], counts["level2"] or counts["common"],
+                                             counts["coarse"] or counts["common"]]})
+            legs["mixed_learned"] = res
+        elif name == "dc64":
+            s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config)
+            c2, _ = tune_counts(s2, params, "dc64", args)
+            res = run_timed(s2, params, 1, args.leg_steps, "dc64")
+            close_solver(s2, n2)
+            res.update({"mode": "dc64", "guess": g2, "tolerance": params.tolerance, "gcr_directions": gcr,
+                        "sweeps_per_visit": [c2["common"], c2["level2"] or c2["common"],
+                                             c2["coarse"] or c2["common"]]})
+            legs["dc64"] = res
+        elif name == "fp64":
+            tol = max(params.tolerance, FP64_TOLERANCE) if params.nx >= 2048 else params.tolerance
+            p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol))
+            s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config)
+            res = run_timed(s2, p64, 1, args.leg_steps, "fp64")
+            T, SF = s2.get("T"), s2.get("SF")
+            r, s = s2.get_tracers("x")
+            fp64_state = (p64, (T, SF, r, s, s2.get_tracer_types()), res)
+            close_solver(s2, n2)
+            res.update({"mode": "fp64", "tolerance": tol, "fixes": int(p64.fixes), "sweeps_per_visit": p64.Nmulti,
+                        "note": ("the reference's cycle in FP64; its rounding floor at this size is ~8e-8, so "
+                                 "1e-8 is out of its reach" if tol != params.tolerance else "")})
+            legs["fp64_reference"] = res
+    for k, v in legs.items():
+        v["headline_speedup"] = head["steps_per_s"] / v["steps_per_s"]
+        v.pop("elapsed", None)
+        v.pop("avg_sweep_us", None)
+        v.pop("sweep_launches", None)
+
     out = {
         "metric": "steps/s @2048x1024+42M tracers to 1e-8 resid: mixed+learned vs plain FP64",
-        "value": steps_per_s,
+        "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
+        "vs_plain_fp64": (head["steps_per_s"] / legs["fp64_reference"]["steps_per_s"]
+                          if "fp64_reference" in legs else None),
         "dtype": {"fp64": "f64", "dc64": "f64 (double-double residual)",
                   "mixed": "f32 cycle, f64/double-double residual"}[args.mode],
         "data": "synthetic (seeded T0 with Fourier modes, jittered tracers)",
         "config": {
-            "workload": f"{params.nx}x{params.nz} aspect {params.aspect:g}, Ra={params.RaT:g}, "
-                        f"{ntr} tracers, multigrid to {params.tolerance:g} relative residual",
+            "workload": f"{params.nx}x{params.nz} aspect {params.aspect:g}, Ra={params.RaT:g}, B={params.RaC[0] / params.RaT:g}, "
+                        f"visT={params.visT:g}, visC={params.visC[0]:g}, {ntr} tracers, multigrid to "
+                        f"{params.tolerance:g} relative residual, fixes={int(params.fixes)} (zeroed coarse "
+                        f"corrections), delta_visC={cfg.get('delta_visC', (2.0,))[0]:g} (C filter threshold "
+                        f"delta_C={params.delta_C[0]:.4f})",
             "mode": args.mode,
             "cycle": args.cycle,
-            "sweeps_per_visit": dc_sweeps,
-            "sweeps_per_visit_level_2": level2_sweeps or dc_sweeps,
-            "sweeps_per_visit_levels_3_up": coarse_sweeps or dc_sweeps,
-            "coarse_autotune_ms_per_step": coarse_tune,
-            "level2_autotune_ms_per_step": level2_tune,
-            "reference_sweeps_per_visit": params.Nmulti,
-            "autotune_ms_per_step": tune,
-            "tb_tiles_by_level": tiles,
-            "dc_retries": after.dc_retries,
-            "gcr_directions": gcr,
-            "gcr_updates": gcr_updates,
             "guess": guess,
-            "starts": {k: after.guesses_used[i] - before.guesses_used[i]
-                       for i, k in enumerate(("plain", "extrap", "learned"))},
+            "gcr_directions": gcr,
+            "sweeps_per_visit": counts["common"],
+            "sweeps_per_visit_level_2": counts["level2"] or counts["common"],
+            "sweeps_per_visit_levels_3_up": counts["coarse"] or counts["common"],
+            "reference_sweeps_per_visit": params.Nmulti,
+            "autotune_ms_per_step": rec.get("tune"),
+            "coarse_autotune_ms_per_step": rec.get("coarse_tune"),
+            "level2_autotune_ms_per_step": rec.get("level2_tune"),
+            "tb_tiles_by_level": counts.get("tiles"),
+            "dc_retries": head["dc_retries"],
+            "gcr_updates": head["gcr_updates"],
+            "starts": head["starts"],
             "levels": params.grids,
-            "mg_cycles_per_step": cycles_per_step,
-            "sweeps_per_step": sweeps_per_step,
+            "mg_cycles_per_step": head["mg_cycles_per_step"],
+            "q_per_solve": head["q_per_solve"],
+            "sweeps_per_step": head["sweeps_per_step"],
+            "c_filter_iters_per_step": head["c_filter_iters_per_step"],
+            "t_filter_iters_per_step": head["t_filter_iters_per_step"],
         },
+        "legs": legs,
         "roofline": {
             "kernel": f"{kname} level 1 (Jacobi-Richardson smoother)",
             "bound": "hbm",
@@ -289,19 +490,17 @@ def main():
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": traffic,
             "avg_launch_us": avg_us,
-            "launches": nlaunch,
+            "launches": head["sweep_launches"],
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "kernel_bytes_per_launch": KERNEL_BYTES_PER_POINT[args.mode] * pts1,
         },
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
     }
-    if net is not None:
-        solver.learn_attach(None)
-        net.close()
-    solver.close()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(params, sweeps_per_step)
+    if not args.no_cpu_baseline and fp64_state is not None:
+        p64, state, leg = fp64_state
+        del inputs
+        out["cpu_baseline"] = cpu_baseline(p64, state, leg)
     print(json.dumps(out), flush=True)
     return 0
 

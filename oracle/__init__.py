@@ -38,6 +38,7 @@ ORS_DT, ORS_TIME, ORS_RESID1, ORS_RESREF1, ORS_EIG_FINE, ORS_DELTA_T = range(6)
 ORS_Q_STEP = tuple(range(32, 36))
 ORS_N_COMP_T = tuple(range(36, 40))  # n_comp of types 1..4
 ORS_INTERP_CLAMPS = 40
+ORS_MAX_CYCLES = 41
 
 _lib = None
 

@@ -494,6 +494,7 @@ double or_get_scalar(const or_state* s, int which)
     case ORS_N_COMP_T1: case ORS_N_COMP_T2: case ORS_N_COMP_T3: case ORS_N_COMP_T4:
         return s->n_comp[1 + which - ORS_N_COMP_T1];
     case ORS_INTERP_CLAMPS: return (double)s->interp_clamps;
+    case ORS_MAX_CYCLES: return s->p.max_cycles;
     default: return NAN;
     }
 }
@@ -505,6 +506,7 @@ void or_set_scalar(or_state* s, int which, double v)
     case ORS_TSTEP: s->tstep = (int)v; break; case ORS_SMOOTH_VIS: s->smooth_vis = (int)v; break;
     case ORS_STATUS: s->status = (int)v; break;
     case ORS_NVIS_STORE: s->Nvis_store = (int)v; break; case ORS_NN_STORE: s->nn_store = (int)v; break;
+    case ORS_MAX_CYCLES: s->p.max_cycles = (int)v; break;
     default: break;
     }
 }

@@ -53,7 +53,9 @@ enum {
     /* n_comp of types 1..4 after the last smoother_time */
     ORS_N_COMP_T1 = 36, ORS_N_COMP_T2, ORS_N_COMP_T3, ORS_N_COMP_T4,
     /* interpolations of out-of-domain positions (cumulative) */
-    ORS_INTERP_CLAMPS = 40
+    ORS_INTERP_CLAMPS = 40,
+    /* the cycle limit of or_multigrid (tc_params.max_cycles) */
+    ORS_MAX_CYCLES = 41
 };
 
 or_state* or_create(const tc_params* p, int64_t ntr);
