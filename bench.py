@@ -155,14 +155,22 @@ def apply_counts(solver, params, counts):
                 solver.set_tb_tile(L, t)
 
 
-def run_timed(solver, params, warmup, steps, label, profile=False):
-    """W untimed steps, then K steps timed between two stream syncs."""
+def run_timed(solver, params, warmup, steps, label, profile=False, until_time=None):
+    """W untimed steps (and more until the model time reaches until_time,
+    so that another form is timed over the same stretch of the run), then K
+    steps timed between two stream syncs."""
     from tracer_code_amd import _abi
-    for w in range(warmup):
+    w = 0
+    while True:
+        st = solver.stats()
+        if w >= warmup and (until_time is None or st.time >= until_time - 0.5 * st.dt_last):
+            break
         t0 = time.perf_counter()
         solver.time_step(1)
         st = solver.stats()
-        log(f"{label}: warmup step {w}: {time.perf_counter() - t0:.2f}s q={list(st.q_step)} dt={st.dt_last:.3e}")
+        log(f"{label}: warmup step {w}: {time.perf_counter() - t0:.2f}s q={list(st.q_step)} dt={st.dt_last:.3e} "
+            f"t={st.time:.4e}")
+        w += 1
     before = solver.stats()
     gcr0 = solver.gcr_counts()[0]
     sweeps0 = list(before.sweeps_exec)
@@ -198,6 +206,8 @@ def run_timed(solver, params, warmup, steps, label, profile=False):
         "avg_sweep_us": avg_us,
         "sweep_launches": nlaunch,
         "dt_last": after.dt_last,
+        "model_time_start": before.time,
+        "warmup_steps": w,
     }
 
 
@@ -372,6 +382,7 @@ def main():
     if args.mode != "fp64":
         counts, rec = tune_counts(solver, params, args.mode, args)
     head = run_timed(solver, params, args.warmup, args.steps, args.mode, profile=True)
+    t_head = head["model_time_start"]   # the other forms are timed from the same model time
     close_solver(solver, net)
 
     pts1 = (params.nx - 1) * (params.nz - 1)
@@ -402,7 +413,7 @@ def main():
         if name == "learned":
             s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config)
             apply_counts(s2, params, counts)
-            res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned")
+            res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned", until_time=t_head)
             close_solver(s2, n2)
             res.update({"mode": "mixed", "guess": g2, "tolerance": params.tolerance, "gcr_directions": gcr,
                         "sweeps_per_visit": [counts["common"], counts["level2"] or counts["common"],
@@ -411,7 +422,7 @@ def main():
         elif name == "dc64":
             s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config)
             c2, _ = tune_counts(s2, params, "dc64", args)
-            res = run_timed(s2, params, 1, args.leg_steps, "dc64")
+            res = run_timed(s2, params, 1, args.leg_steps, "dc64", until_time=t_head)
             close_solver(s2, n2)
             res.update({"mode": "dc64", "guess": g2, "tolerance": params.tolerance, "gcr_directions": gcr,
                         "sweeps_per_visit": [c2["common"], c2["level2"] or c2["common"],
@@ -421,7 +432,7 @@ def main():
             tol = max(params.tolerance, FP64_TOLERANCE) if params.nx >= 2048 else params.tolerance
             p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol))
             s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config)
-            res = run_timed(s2, p64, 1, args.leg_steps, "fp64")
+            res = run_timed(s2, p64, 1, args.leg_steps, "fp64", until_time=t_head)
             T, SF = s2.get("T"), s2.get("SF")
             r, s = s2.get_tracers("x")
             fp64_state = (p64, (T, SF, r, s, s2.get_tracer_types()), res)
