@@ -453,6 +453,27 @@ int tc_learn_train(tc_ctx* ctx, int32_t epochs, float lr, int32_t batch, uint64_
 enum { TC_GUESS_PLAIN = 0, TC_GUESS_EXTRAP = 1, TC_GUESS_LEARNED = 2 };
 int tc_guess_enable(tc_ctx* ctx, int mode);
 
+/* ---- binning without allocation ------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The binning of tc_tracers_to_corners -- nearest-node
+ * counts per type, mirrored ghost counts, nempty, the tracer ratio Cnew
+ * (empty nodes keep their value) and enforceBCs_comp (composition.f90:
+ * 669-803), without the C filter -- on caller-owned DEVICE buffers, with no
+ * context and no allocation.  Planes use the reference layout
+ * f(-2:nx+2,-2:nz+2), i fastest, one plane after another per type:
+ * count_type[t] (t = 0..ntype), Cnew[t-1] (t = 1..ntype).  count_type may be
+ * NULL: the counts are then kept in the library's fixed per-GPU work array
+ * of TC_BIN_NOALLOC_MAX_COUNTS int32 words, which (ntype+1)(nx+5)(nz+5) must
+ * not exceed (else TC_ERR_NOMEM).  Runs on `stream` (a hipStream_t, NULL for
+ * the default stream) and synchronizes it; returns TC_ERR_TRACER_DOMAIN
+ * when a tracer is outside the domain.  Calls that overlap on one GPU share
+ * that array and race. */
+#define TC_BIN_NOALLOC_MAX_COUNTS (8u << 20)
+int tc_bin_tracers_noalloc(int32_t nx, int32_t nz, double aspect, int32_t ntype, int32_t comp, const double* r,
+                           const double* s, const int32_t* ttype, int64_t ntr, int32_t* count_type, int32_t* nempty,
+                           double* Cnew, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

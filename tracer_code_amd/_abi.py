@@ -21,6 +21,8 @@ TC_ERR_HIP = -2
 TC_ERR_UNSUPPORTED = -3
 TC_ERR_NOMEM = -4
 
+TC_BIN_NOALLOC_MAX_COUNTS = 8 << 20  # int32 words of tc_bin_tracers_noalloc's per-GPU work array
+
 TC_MODE_FP64 = 0
 TC_MODE_MIXED = 1
 TC_MODE_DC64 = 2
@@ -151,6 +153,8 @@ SIGNATURES = {
     "tc_energy_stage": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double]),
     "tc_time_step": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_guess_enable": (ctypes.c_int, [_VP, ctypes.c_int]),
+    "tc_bin_tracers_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                              ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int64, _VP, _VP, _VP, _VP]),
     "tc_set_profiling": (ctypes.c_int, [_VP, ctypes.c_int]),
     "tc_kernel_timing": (ctypes.c_int, [_VP, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int64)]),
     "tc_derivative_eigenvalues": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP]),

@@ -206,6 +206,16 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
     end function
+    ! binning on device buffers without allocation (no reference counterpart;
+    ! for callers that cannot allocate); pointers are device addresses
+    integer(c_int) function tc_bin_tracers_noalloc(nx, nz, aspect, ntype, comp, r, s, ttype, ntr, count_type, &
+                                                   nempty, Cnew, stream) bind(C, name="tc_bin_tracers_noalloc")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr
+      integer(c_int32_t), value :: nx, nz, ntype, comp
+      real(c_double), value :: aspect
+      type(c_ptr), value :: r, s, ttype, count_type, nempty, Cnew, stream
+      integer(c_int64_t), value :: ntr
+    end function
     integer(c_int) function tc_ensemble_step(ctxs, n, nsteps, threads, failed) bind(C, name="tc_ensemble_step")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), intent(in) :: ctxs(*)
