@@ -17,9 +17,10 @@ run (`--legs`, N=1) the other forms of the metric are timed on the same
 inputs:
 
 * `fp64_reference`: the reference's own cycle in FP64 (Jacobi-Richardson
-  sweeps, Nmulti=70, W-cycle) at the smallest tolerance FP64 reaches at
-  2048x1024 (1e-7; its rounding floor is ~8e-8, so at 1e-8 it stops with
-  "Too Many Multigrid Cycles");
+  sweeps, Nmulti=70, W-cycle) at the smallest of 1e-7, 3e-7, 1e-6 that it
+  reaches over the timed steps at 2048x1024 (its rounding floor is ~8e-8 at
+  the start and grows with the flow, so at 1e-8 it stops with "Too Many
+  Multigrid Cycles");
 * `dc64`: defect correction with FP64 corrections at 1e-8;
 * `mixed_learned`: the headline form with the trained network's guess
   judged as a third start.
@@ -58,8 +59,10 @@ DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
 COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
 LEVEL2_SWEEP_CANDIDATES = (8, 12, 16, 24, 0)
 SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), "mixed": ("k_sweep32", "sweep32")}
-# smallest relative residual the FP64 reference form reaches at 2048x1024
-FP64_TOLERANCE = 1e-7
+# relative residuals tried, in order, for the FP64 reference form at
+# 2048x1024 (1e-8 is below its rounding floor there), and its cycle limit
+FP64_TOLERANCES = (1e-7, 3e-7, 1e-6)
+FP64_MAX_CYCLES = 200
 
 
 def log(msg):
@@ -429,18 +432,33 @@ def main():
                                              c2["coarse"] or c2["common"]]})
             legs["dc64"] = res
         elif name == "fp64":
-            tol = max(params.tolerance, FP64_TOLERANCE) if params.nx >= 2048 else params.tolerance
-            p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol))
-            s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config)
-            res = run_timed(s2, p64, 1, args.leg_steps, "fp64", until_time=t_head)
-            T, SF = s2.get("T"), s2.get("SF")
-            r, s = s2.get_tracers("x")
-            fp64_state = (p64, (T, SF, r, s, s2.get_tracer_types()), res)
-            close_solver(s2, n2)
-            res.update({"mode": "fp64", "tolerance": tol, "fixes": int(p64.fixes), "sweeps_per_visit": p64.Nmulti,
-                        "note": ("the reference's cycle in FP64; its rounding floor at this size is ~8e-8, so "
-                                 "1e-8 is out of its reach" if tol != params.tolerance else "")})
-            legs["fp64_reference"] = res
+            from tracer_code_amd.model import TcError
+            # the smallest of FP64_TOLERANCES the reference form reaches over
+            # the timed stretch (its rounding floor grows with |psi|); a leg
+            # that stalls is redone with the next one from the start
+            tols = [t for t in FP64_TOLERANCES if t >= params.tolerance] if params.nx >= 2048 else [params.tolerance]
+            failed = []
+            for tol in tols:
+                p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol, max_cycles=FP64_MAX_CYCLES))
+                s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config)
+                try:
+                    res = run_timed(s2, p64, 1, args.leg_steps, "fp64", until_time=t_head)
+                except TcError as e:
+                    log(f"fp64 at {tol:g}: {e}")
+                    failed.append(tol)
+                    close_solver(s2, n2)
+                    continue
+                T, SF = s2.get("T"), s2.get("SF")
+                r, s = s2.get_tracers("x")
+                fp64_state = (p64, (T, SF, r, s, s2.get_tracer_types()), res)
+                close_solver(s2, n2)
+                res.update({"mode": "fp64", "tolerance": tol, "tolerances_not_reached": failed,
+                            "fixes": int(p64.fixes), "sweeps_per_visit": p64.Nmulti,
+                            "note": ("the reference's cycle in FP64; its rounding floor at this size (~8e-8 at "
+                                     "the start, growing with the flow) keeps it above the metric's "
+                                     f"{params.tolerance:g}" if tol != params.tolerance else "")})
+                legs["fp64_reference"] = res
+                break
     for k, v in legs.items():
         v["headline_speedup"] = head["steps_per_s"] / v["steps_per_s"]
         v.pop("elapsed", None)
