@@ -82,7 +82,7 @@ def copy_calibration(mib):
     return 0 if ok else 1
 
 
-def make_solver(params, inputs, mode, guess, gcr, config_name):
+def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1):
     """A solver on the seeded inputs; returns (solver, network or None, guess)."""
     from tracer_code_amd import Solver
     T, r, s, ty = inputs
@@ -104,6 +104,7 @@ def make_solver(params, inputs, mode, guess, gcr, config_name):
             guess = "extrap"
     solver.guess_enable(guess)
     solver.set_gcr(gcr)
+    solver.set_coarse_refresh(coarse_refresh)
     return solver, net, guess
 
 
@@ -349,6 +350,8 @@ def main():
                     help="initial-guess candidates of the headline's solves")
     ap.add_argument("--gcr", type=int, default=16,
                     help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
+    ap.add_argument("--coarse-refresh", type=int, default=1,
+                    help="refactorise the coarsest operator every n-th solve of the defect-correction forms")
     ap.add_argument("--legs", default="fp64,dc64,learned",
                     help="other forms timed in the same run (comma list of fp64, dc64, learned; empty: none)")
     ap.add_argument("--leg-steps", type=int, default=3)
@@ -380,7 +383,7 @@ def main():
 
     # ---- headline form ----------------------------------------------------
     gcr = args.gcr if args.mode != "fp64" else 0
-    solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config)
+    solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config, args.coarse_refresh)
     counts, rec = ({"common": params.Nmulti, "coarse": 0, "level2": 0, "tiles": None}, {})
     if args.mode != "fp64":
         counts, rec = tune_counts(solver, params, args.mode, args)
@@ -414,7 +417,7 @@ def main():
     fp64_state = None
     for name in want:
         if name == "learned":
-            s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config)
+            s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config, args.coarse_refresh)
             apply_counts(s2, params, counts)
             res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned", until_time=t_head)
             close_solver(s2, n2)
@@ -423,7 +426,7 @@ def main():
                                              counts["coarse"] or counts["common"]]})
             legs["mixed_learned"] = res
         elif name == "dc64":
-            s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config)
+            s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config, args.coarse_refresh)
             c2, _ = tune_counts(s2, params, "dc64", args)
             res = run_timed(s2, params, 1, args.leg_steps, "dc64", until_time=t_head)
             close_solver(s2, n2)
@@ -491,6 +494,7 @@ def main():
             "cycle": args.cycle,
             "guess": guess,
             "gcr_directions": gcr,
+            "coarse_refresh_solves": args.coarse_refresh,
             "sweeps_per_visit": counts["common"],
             "sweeps_per_visit_level_2": counts["level2"] or counts["common"],
             "sweeps_per_visit_levels_3_up": counts["coarse"] or counts["common"],

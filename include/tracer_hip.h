@@ -201,6 +201,15 @@ int tc_graph_counts(const tc_ctx* ctx, int64_t* captures, int64_t* replays);
 int tc_set_gcr(tc_ctx* ctx, int32_t directions);
 /* GCR updates made and restarts forced by a dependent direction so far. */
 int tc_gcr_counts(const tc_ctx* ctx, int64_t* updates, int64_t* restarts);
+/* Coarse-grid factorisation interval of the defect-correction modes (no
+ * counterpart in the reference, whose Newtonian multigrid refactorises the
+ * coarsest operator in every call, iterative_solver.f90:95-98): the banded
+ * LU and explicit inverse of the coarsest operator are recomputed (on a side
+ * stream, used from the next refresh on) every `solves`-th solve (>= 1; 1,
+ * the default, is every solve).  The coarse solve is inside the correction
+ * cycle, so the exact outer residual, the stop test and the tolerance are
+ * unchanged. */
+int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

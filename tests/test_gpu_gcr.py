@@ -90,3 +90,26 @@ def test_gcr_direction_count_is_checked():
                 g.set_gcr(bad)
         g.set_gcr(8)
         g.set_gcr(0)
+
+
+def test_coarse_refresh_interval_keeps_the_tolerance():
+    """The coarsest operator refactorised every 4th solve (tc_set_coarse_refresh):
+    3 time steps of the 2048x1024 physics at 256x128 (12 solves), every solve
+    meets the stop test on the device and on the extended-precision host
+    residual; the interval is checked at the boundary."""
+    out = {}
+    for n in (1, 4):
+        params, g, _ = solver_for("cfg2", nx=256, nz=128, tpc=4)
+        with g:
+            g.set_mode("mixed")
+            g.guess_enable("extrap")
+            g.set_gcr(16)
+            g.set_coarse_refresh(n)
+            g.time_step(3)
+            st = g.stats()
+            check_dc_solution(params, g, f"coarse refresh {n}")
+            out[n] = (st.mg_cycles, st.dc_retries)
+            with pytest.raises(TcError):
+                g.set_coarse_refresh(0)
+    print(f"cycles, retries over 3 steps: refresh every solve {out[1]}, every 4th {out[4]}", flush=True)
+    assert out[4][1] == 0

@@ -400,14 +400,17 @@ int iterate_stream(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool& 
 
 // Assembly, DGBEQU/DGBTF2 and the explicit inverse of the coarsest
 // operator on the side stream.  lagged (defect-correction modes): the result
-// goes to the *_next buffers for the next solve, and this solve uses the
-// previous solve's (swapped in here); the first solve waits for its own.
+// goes to the *_next buffers for the next refresh, and this solve uses the
+// previous refresh's (swapped in here); the first solve waits for its own.
+// Between refreshes (every coarse_refresh-th solve) the inverse in use stays.
 int coarse_setup(tc_ctx* ctx, tc_dev& d, bool lagged)
 {
     Level& L = ctx->lv[ctx->grids];
     const tc_params& P = ctx->p;
     bool to_next = false;
     if (lagged && d.coarse_ready) {
+        if (++ctx->coarse_since < ctx->coarse_refresh) return 0;
+        ctx->coarse_since = 0;
         if (ctx->next_pending) {
             std::swap(ctx->Ainv, ctx->Ainv_next);
             std::swap(ctx->RowB, ctx->RowB_next);
@@ -2097,6 +2100,18 @@ int tc_set_gcr(tc_ctx* ctx, int32_t directions)
         return TC_ERR_ARG;
     }
     ctx->gcr_m = directions;
+    return 0;
+}
+
+int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (solves < 1) {
+        ctx->err = "tc_set_coarse_refresh: solves must be >= 1";
+        return TC_ERR_ARG;
+    }
+    ctx->coarse_refresh = solves;
+    ctx->coarse_since = 0;
     return 0;
 }
 

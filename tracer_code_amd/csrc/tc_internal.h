@@ -179,6 +179,9 @@ struct tc_ctx {
     double* coarse_b = nullptr;   // equilibrated right-hand side of the FP32-cycle coarse solve
     int32_t* d_info_next = nullptr;
     bool next_pending = false;
+    // refactorise the coarse operator every coarse_refresh-th solve of the
+    // defect-correction modes (tc_set_coarse_refresh); solves since the last
+    int coarse_refresh = 1, coarse_since = 0;
     // tracers (SoA)
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;

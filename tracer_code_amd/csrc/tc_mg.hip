@@ -316,40 +316,64 @@ __global__ __launch_bounds__(256) void k_vis_sweep(const double* __restrict__ T0
     V[cc] = T0[cc] + c * dts * temp;
 }
 
-// viscosityBCs / viscosityBCs_initialize (iterative_solver.f90:379-453),
-// one workgroup, phases separated by barriers.
-__global__ void k_visc_bcs(double* __restrict__ V, Grid g, const double* __restrict__ Vf, Grid gf, int level,
-                           int initialize, double RaT, double visT2)
+// viscosityBCs / viscosityBCs_initialize (iterative_solver.f90:379-453).
+// The routine runs three dependent phases: (1) on coarser levels, rows 0
+// and nz (and with initialize the columns 0 and nx) from the finer level;
+// (2) the side ghost columns mirrored for k=0..nz; (3) the bottom/top ghost
+// rows from rows 1..SPAN and nz-1..nz-SPAN over all columns.  Each output
+// cell here is computed straight from the state before the call, by
+// following its chain of sources back through the phases (val1 = the value
+// after phase 1), so one thread per boundary cell needs no barrier; the
+// arithmetic is the reference's.
+__device__ __forceinline__ double visc_val1(const double* __restrict__ V, const Grid& g,
+                                            const double* __restrict__ Vf, const Grid& gf, int level,
+                                            int initialize, int i, int k)
+{
+    if (level > 1) {
+        if ((k == 0 || k == g.nz) && i >= 0 && i <= g.nx) return Vf[gf.at(2 * i, k == 0 ? 0 : gf.nz)];
+        if (initialize && (i == 0 || i == g.nx) && k >= 1 && k <= g.nz - 1)
+            return Vf[gf.at(i == 0 ? 0 : gf.nx, 2 * k)];
+    }
+    return V[g.at(i, k)];
+}
+
+__global__ __launch_bounds__(256) void k_visc_bcs(double* __restrict__ V, Grid g, const double* __restrict__ Vf,
+                                                  Grid gf, int level, int initialize, double RaT, double visT2)
 {
     const int nx = g.nx, nz = g.nz;
-    if (level > 1) {
-        if (initialize)
-            for (int k = 1 + (int)threadIdx.x; k <= nz - 1; k += blockDim.x) {
-                V[g.at(0, k)] = Vf[gf.at(0, 2 * k)];
-                V[g.at(nx, k)] = Vf[gf.at(gf.nx, 2 * k)];
-            }
-        for (int i = threadIdx.x; i <= nx; i += blockDim.x) {
-            V[g.at(i, 0)] = Vf[gf.at(2 * i, 0)];
-            V[g.at(i, nz)] = Vf[gf.at(2 * i, gf.nz)];
-        }
-        __syncthreads();
+    const int nA = (level > 1) ? 2 * (nx + 1) + (initialize ? 2 * (nz - 1) : 0) : 0;  // phase 1 cells
+    const int nB = 2 * SPAN * (nz + 1);                                                // side ghosts
+    const int nC = 2 * SPAN * (nx + 2 * SPAN + 1);                                     // row ghosts
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    auto v1 = [&](int i, int k) { return visc_val1(V, g, Vf, gf, level, initialize, i, k); };
+    if (e < nA) {
+        int i, k;
+        if (e < 2 * (nx + 1)) { i = e % (nx + 1); k = (e < nx + 1) ? 0 : nz; }
+        else { const int f = e - 2 * (nx + 1); k = 1 + f % (nz - 1); i = (f < nz - 1) ? 0 : nx; }
+        V[g.at(i, k)] = v1(i, k);
+        return;
     }
-    for (int k = threadIdx.x; k <= nz; k += blockDim.x)
-        for (int i = 1; i <= SPAN; ++i) {
-            V[g.at(-i, k)] = V[g.at(i, k)];
-            V[g.at(nx + i, k)] = V[g.at(nx - i, k)];
-        }
-    __syncthreads();
-    for (int i = -SPAN + (int)threadIdx.x; i <= nx + SPAN; i += blockDim.x)
-        for (int k = 1; k <= SPAN; ++k) {
-            if (RaT != 0.0) {
-                V[g.at(i, -k)] = 1.0 / V[g.at(i, k)] / visT2;
-                V[g.at(i, nz + k)] = 1.0 / V[g.at(i, nz - k)];
-            } else {
-                V[g.at(i, -k)] = V[g.at(i, k)];
-                V[g.at(i, nz + k)] = V[g.at(i, nz - k)];
-            }
-        }
+    e -= nA;
+    if (e < nB) {
+        const int k = e % (nz + 1), f = e / (nz + 1);  // f = 0..2*SPAN-1
+        const int j = 1 + f % SPAN;
+        if (f < SPAN) V[g.at(-j, k)] = v1(j, k);
+        else V[g.at(nx + j, k)] = v1(nx - j, k);
+        return;
+    }
+    e -= nB;
+    if (e < nC) {
+        const int w = nx + 2 * SPAN + 1;
+        const int i = -SPAN + e % w, f = e / w;  // f = 0..2*SPAN-1
+        const int kk = 1 + f % SPAN;
+        // the value after phase 2 at (i, kr): ghost columns are mirrors
+        auto v2 = [&](int kr) {
+            const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+            return v1(im, kr);
+        };
+        if (f < SPAN) V[g.at(i, -kk)] = (RaT != 0.0) ? 1.0 / v2(kk) / visT2 : v2(kk);
+        else V[g.at(i, nz + kk)] = (RaT != 0.0) ? 1.0 / v2(nz - kk) : v2(nz - kk);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -475,7 +499,10 @@ void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, co
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
                      double RaT, double visT2)
 {
-    hipLaunchKernelGGL(k_visc_bcs, dim3(1), dim3(256), 0, st, V, g, Vf, gf, level, init, RaT, visT2);
+    const int n = ((level > 1) ? 2 * (g.nx + 1) + (init ? 2 * (g.nz - 1) : 0) : 0) + 2 * SPAN * (g.nz + 1) +
+                  2 * SPAN * (g.nx + 2 * SPAN + 1);
+    hipLaunchKernelGGL(k_visc_bcs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, V, g, Vf, gf, level, init, RaT,
+                       visT2);
 }
 
 }  // namespace tc

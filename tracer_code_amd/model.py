@@ -297,6 +297,11 @@ class Solver:
         tolerance are unchanged."""
         self._check(self.lib.tc_set_gcr(self.h, int(directions)), "set_gcr")
 
+    def set_coarse_refresh(self, solves):
+        """Refactorise the coarsest operator every `solves`-th solve of the
+        defect-correction modes (1: every solve)."""
+        self._check(self.lib.tc_set_coarse_refresh(self.h, int(solves)), "set_coarse_refresh")
+
     def gcr_counts(self):
         """(updates, restarts) of the GCR acceleration so far."""
         u, r = ctypes.c_int64(), ctypes.c_int64()
