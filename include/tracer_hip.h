@@ -210,6 +210,11 @@ int tc_gcr_counts(const tc_ctx* ctx, int64_t* updates, int64_t* restarts);
  * cycle, so the exact outer residual, the stop test and the tolerance are
  * unchanged. */
 int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves);
+/* smoother_time's C filter loop (smoother.f90:286-340) with several filter
+ * iterations per kernel launch, staged in on-chip memory (enable=1, the
+ * default), or one launch per iteration (enable=0).  Both give the same
+ * iterates, stop test, iteration counts and bits. */
+int tc_set_filter_blocking(tc_ctx* ctx, int32_t enable);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

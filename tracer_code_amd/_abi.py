@@ -113,6 +113,7 @@ SIGNATURES = {
     "tc_graph_counts": (ctypes.c_int, [_VP, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     "tc_set_gcr": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_set_coarse_refresh": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "tc_set_filter_blocking": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_gcr_counts": (ctypes.c_int, [_VP, _P(ctypes.c_int64), _P(ctypes.c_int64)]),
     "tc_sort_tracers": (ctypes.c_int, [_VP]),
     "tc_diagnostics": (ctypes.c_int, [_VP, _VP]),

@@ -75,7 +75,8 @@ struct tc_dev {
     bool tb = true;            // temporally blocked sweeps on levels >= 2
     bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
     int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
-    unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (three slot sets)
+    unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (rotating slot sets)
+    double* fbuf = nullptr;    // third iterate buffer of the blocked C filter loop
     int32_t* h_ctl = nullptr;  // pinned mirror
     std::vector<double> hbuf;  // host staging plane
 };
@@ -1329,6 +1330,50 @@ int filter_loop(tc_ctx* ctx, tc_dev& d, double* A, double* B, bool comp, double 
     return 0;
 }
 
+// The C filter loop with K iterations per launch (launch_filter_multi):
+// the same iterates, stop test and stopping index as filter_loop, with the
+// buffers A (the first iterate and the result), B and C rotating.
+template <typename CFac>
+int filter_loop_multi(tc_ctx* ctx, tc_dev& d, double* A, double* B, double* C, double delta, double courant,
+                      CFac cfac, int* iters)
+{
+    const int K = filter_multi_k();
+    launch_zero_i32(ctx->stream, d.ctl, 3);
+    launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(d.fslots), 2 * filter_slot_words());
+    launch_smoothness_max(ctx->stream, A, ctx->g, d.fslots, d.ctl);
+    double* bufs[3] = {A, B, C};
+    std::vector<double> fac(2 * K);
+    int l = 0, batch = 4;
+    for (;;) {
+        for (int b = 0; b < batch; ++b, ++l) {
+            const int m = l * K;
+            for (int j = 0; j < K; ++j) {
+                fac[j] = (m - K + j >= 0) ? cfac(m - K + j) : 0.0;
+                fac[K + j] = cfac(m + j);
+            }
+            launch_filter_multi(ctx->stream, bufs, l % 3, ctx->dts, ctx->g, d.F, ctx->lv[1].dr2, ctx->lv[1].ds2, courant,
+                                fac.data(), m, delta, d.fslots, d.ctl);
+        }
+        TC_HIP(hipGetLastError());
+        TC_HIP(hipMemcpyAsync(d.h_ctl, d.ctl, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        if (d.h_ctl[0]) break;
+        if (l * K >= FILTER_MAX_ITERS) {
+            ctx->err = "filter loop: no stop after " + std::to_string(l * K) + " iterations";
+            return TC_ERR_HIP;
+        }
+        if (batch < 16) batch *= 2;
+    }
+    const int where = d.h_ctl[2];
+    if (where < 0 || where > 2) {
+        ctx->err = "filter loop: bad result buffer";
+        return TC_ERR_HIP;
+    }
+    if (where != 0) TC_TRY(copy_plane(ctx, A, bufs[where], ctx->g));
+    *iters = d.h_ctl[1];
+    return 0;
+}
+
 // smoother_time_T (smoother.f90:342-376)
 int smoother_time_T(tc_ctx* ctx, tc_dev& d, const double* Tin, int32_t* n_out)
 {
@@ -1367,7 +1412,10 @@ int smoother_time(tc_ctx* ctx, tc_dev& d, int32_t* n_comp)
             return 2.0 / (-lb - la + (lb - la) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * N)));
         };
         int iters = 0;
-        TC_TRY(filter_loop(ctx, d, ctx->Cvis[t], ctx->tmp2, true, P.delta_C[t], P.courantC, cfac, &iters));
+        if (ctx->filter_blocking)
+            TC_TRY(filter_loop_multi(ctx, d, ctx->Cvis[t], ctx->tmp2, d.fbuf, P.delta_C[t], P.courantC, cfac, &iters));
+        else
+            TC_TRY(filter_loop(ctx, d, ctx->Cvis[t], ctx->tmp2, true, P.delta_C[t], P.courantC, cfac, &iters));
         const int nn = iters < N ? 0 : iters - N + 1;
         if (iters >= N) Niter = 1;
         ctx->stats.c_iters += iters;
@@ -1877,7 +1925,7 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
-        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
+        F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->fbuf); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         delete d;
@@ -2014,6 +2062,7 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_TRY(dalloc(ctx, &d->d_scal, 16));
         TC_TRY(dalloc_i(ctx, &d->ctl, 4));
         TC_HIP(hipMalloc((void**)&d->fslots, filter_slot_words() * sizeof(unsigned long long)));
+        TC_TRY(dalloc(ctx, &d->fbuf, g.elems));
         TC_HIP(hipHostMalloc((void**)&d->h_ctl, 4 * sizeof(int32_t), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_scal, 16 * sizeof(double), hipHostMallocDefault));
     }
@@ -2112,6 +2161,13 @@ int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves)
     }
     ctx->coarse_refresh = solves;
     ctx->coarse_since = 0;
+    return 0;
+}
+
+int tc_set_filter_blocking(tc_ctx* ctx, int32_t enable)
+{
+    if (!ctx) return TC_ERR_ARG;
+    ctx->filter_blocking = enable != 0;
     return 0;
 }
 

@@ -261,6 +261,155 @@ __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ 
     if (tid == 0) slot_max(slots + (size_t)((m + 1) % 3) * FILT_SET, a);
 }
 
+// ---------------------------------------------------------------------------
+// Several iterations of smoother_time's C filter per launch (temporal
+// blocking).  A block owns a FM_TX x FM_TY tile of the ghosted plane and
+// stages the tile with a halo of FM_H = K+1 cells (the iterate and dts) in
+// LDS.  Iteration j evaluates, for every cell of the halo region shrunk by j,
+// the value after one filter step and enforceBCs_comp (filter_cell's rules:
+// the step at the cell a ghost mirrors, the row-1 corner read from the
+// previous iterate) from the previous iterate in LDS, so the results are
+// bit-identical to K single launches.  After each iteration the block's own
+// nodes give that iterate's smoothness maximum (one slot set per iterate).
+//
+// The stop test needs the whole grid's maximum, which a launch only sees
+// for the iterates of the launch before it.  Launch l (iterate m = input)
+// therefore first tests iterates m-K+1..m: if iterate j0 of them is smooth,
+// the loop stops there.  j0 = m: nothing is computed (the input is the
+// result).  j0 < m: the previous launch's input (iterate m-K, kept in a
+// third buffer) is advanced j0-(m-K) iterations into the output.  Block
+// (0,0) records the stop, the stopping iterate and the buffer holding it.
+// Slot sets: 3K, set j mod 3K for iterate j; a launch reads K, writes K and
+// clears the K its successor writes.
+// ---------------------------------------------------------------------------
+constexpr int FM_K = 4;                  // iterations per launch
+constexpr int FM_H = FM_K + 1;           // halo: one cell per iteration, one for the last measure
+constexpr int FM_TX = 64, FM_TY = 16;    // tile (256 threads, 64 x 4)
+constexpr int FM_RX = FM_TX + 2 * FM_H, FM_RY = FM_TY + 2 * FM_H;
+constexpr int FM_SETS = 3 * FM_K;
+
+struct FmFac { double v[2 * FM_K]; };    // factors of the previous launch's iterations, then this one's
+struct FmBufs { double* v[3]; };
+
+__global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, const double* __restrict__ dts, Grid g,
+                                                      FD F, double dr2, double ds2, double courant, FmFac fac, int m,
+                                                      double delta, unsigned long long* __restrict__ slots,
+                                                      int32_t* __restrict__ ctl)
+{
+    if (ctl[0]) return;
+    __shared__ double sbuf[2][FM_RY][FM_RX];
+    __shared__ double sdts[FM_RY][FM_RX];
+    __shared__ int s_j0;
+    const int tid = threadIdx.y * FM_TX + threadIdx.x, nth = FM_TX * 4;
+    // the stop test on the iterates the previous launch produced (or the input)
+    if (tid == 0) {
+        int j0 = -1;
+        for (int j = (m - FM_K + 1 > 0 ? m - FM_K + 1 : 0); j <= m && j0 < 0; ++j) {
+            const unsigned long long* set = slots + (size_t)(j % FM_SETS) * FILT_SET;
+            unsigned long long v = 0ull;
+            for (int s = 0; s < FILT_SLOTS; ++s) {
+                const unsigned long long w = set[s * FILT_STRIDE];
+                v = (w > v) ? w : v;
+            }
+            if (__longlong_as_double((long long)v) < delta) j0 = j;
+        }
+        s_j0 = j0;
+    }
+    __syncthreads();
+    const int j0 = s_j0;
+    const bool first = blockIdx.x == 0 && blockIdx.y == 0;
+    const int b_prev = (b_in + 2) % 3, b_out = (b_in + 1) % 3;
+    if (j0 == m) {
+        if (first && tid == 0) { ctl[0] = 1; ctl[1] = m; ctl[2] = b_in; }
+        return;
+    }
+    const bool replay = j0 >= 0;
+    const int n = replay ? j0 - (m - FM_K) : FM_K;
+    const double* __restrict__ src = replay ? bufs.v[b_prev] : bufs.v[b_in];
+    double* __restrict__ dst = bufs.v[b_out];
+    const int fo = replay ? 0 : FM_K;  // factor offset
+    if (!replay && first && tid < FILT_SLOTS)
+        for (int j = m + FM_K + 1; j <= m + 2 * FM_K; ++j)
+            slots[(size_t)(j % FM_SETS) * FILT_SET + tid * FILT_STRIDE] = 0ull;
+    const int nx = g.nx, nz = g.nz;
+    // tile origin: rows from k = -SPAN, columns from i = -ORG_PAD (row segments start a cache line)
+    const int i0 = blockIdx.x * FM_TX - ORG_PAD, k0 = blockIdx.y * FM_TY - SPAN;
+    const int ri0 = i0 - FM_H, rk0 = k0 - FM_H;  // region origin
+    auto inplane = [&](int i, int k) { return i >= -SPAN && i <= nx + SPAN && k >= -SPAN && k <= nz + SPAN; };
+    for (int e = tid; e < FM_RX * FM_RY; e += nth) {
+        const int vi = e % FM_RX, vk = e / FM_RX, i = ri0 + vi, k = rk0 + vk;
+        double v = 0.0, d = 0.0;
+        if (inplane(i, k)) {
+            const size_t c = g.at(i, k);
+            v = src[c];
+            if (i >= 0 && i <= nx && k >= 0 && k <= nz) d = dts[c];
+        }
+        sbuf[0][vk][vi] = v;
+        sdts[vk][vi] = d;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int it = 1; it <= n; ++it) {
+        const double cfac = fac.v[fo + it - 1];
+        const double (*P)[FM_RX] = sbuf[cur];
+        double (*Q)[FM_RX] = sbuf[cur ^ 1];
+        // the step at interior node (ii, kk) from the previous iterate P
+        auto W = [&](int ii, int kk) {
+            const int vi = ii - ri0, vk = kk - rk0;
+            double acc = F.D2[1] * P[vk][vi - 1];
+            acc += F.D2[2] * P[vk][vi];
+            acc += F.D2[3] * P[vk][vi + 1];
+            const double Trr = acc / dr2;
+            acc = F.D2[1] * P[vk - 1][vi];
+            acc += F.D2[2] * P[vk][vi];
+            acc += F.D2[3] * P[vk + 1][vi];
+            const double Tss = acc / ds2;
+            const double temp = Trr + Tss;
+            const double d = sdts[vk][vi] * courant;
+            double v = P[vk][vi] + cfac * d * temp;
+            if (v < 0.0) v = 0.0;
+            if (v > 1.0) v = 1.0;
+            return v;
+        };
+        const int w = FM_RX - 2 * it, h = FM_RY - 2 * it;
+        for (int e = tid; e < w * h; e += nth) {
+            const int vi = it + e % w, vk = it + e / w, i = ri0 + vi, k = rk0 + vk;
+            if (!inplane(i, k)) continue;
+            const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+            double v;
+            if (k >= 0 && k <= nz) {
+                v = W(im, k);
+            } else {
+                const int ks = (k < 0) ? -k : 2 * nz - k;
+                const bool row1 = (k == -1 || k == nz + 1);
+                v = (row1 && (i == -2 || i == nx + 2)) ? P[ks - rk0][vi] : W(im, ks);
+            }
+            Q[vk][vi] = v;
+        }
+        __syncthreads();
+        cur ^= 1;
+        if (!replay) {
+            // the new iterate's smoothness over the block's own nodes
+            double a = 0.0;
+            const int tx = threadIdx.x, i = i0 + tx, vi = i - ri0;
+            for (int r = 0; r < FM_TY / 4; ++r) {
+                const int k = k0 + threadIdx.y + 4 * r, vk = k - rk0;
+                if (i >= 0 && i <= nx - 1 && k >= 1 && k <= nz - 1)
+                    a = smooth_merge(a, Q[vk][vi], Q[vk - 1][vi], Q[vk - 1][vi + 1], Q[vk][vi + 1], Q[vk + 1][vi],
+                                     Q[vk + 1][vi + 1]);
+            }
+            a = block_max(a);
+            if (tid == 0) slot_max(slots + (size_t)((m + it) % FM_SETS) * FILT_SET, a);
+        }
+    }
+    // the block's own cells of the last iterate
+    for (int e = tid; e < FM_TX * FM_TY; e += nth) {
+        const int tx = e % FM_TX, ty = e / FM_TX, i = i0 + tx, k = k0 + ty;
+        if (inplane(i, k)) dst[g.at(i, k)] = sbuf[cur][k - rk0][i - ri0];
+    }
+    if (replay && first && tid == 0) { ctl[0] = 1; ctl[1] = j0; ctl[2] = b_out; }
+}
+
 // enforceBCs (energy.f90:455-475), one workgroup.
 __global__ void k_bcs_T(double* __restrict__ T, Grid g, int Tbot, const int32_t* __restrict__ stop)
 {
@@ -785,7 +934,21 @@ void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsi
     hipLaunchKernelGGL(k_smoothness_max, grid2(g.nx, (g.nz - 1 + FILT_ROWS - 1) / FILT_ROWS, b), b, 0, st, T0, g,
                        slots, stop);
 }
-size_t filter_slot_words() { return 3 * (size_t)FILT_SET; }
+size_t filter_slot_words() { return (size_t)FM_SETS * FILT_SET; }
+int filter_multi_k() { return FM_K; }
+
+void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const double* dts, const Grid& g,
+                         const FD& F, double dr2, double ds2, double courant, const double* fac, int m, double delta,
+                         unsigned long long* slots, int32_t* ctl)
+{
+    FmBufs b{{bufs[0], bufs[1], bufs[2]}};
+    FmFac f{};
+    for (int j = 0; j < 2 * FM_K; ++j) f.v[j] = fac[j];
+    dim3 blk(FM_TX, 4);
+    dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FM_TX - 1) / FM_TX),
+            (unsigned)((g.nz + 1 + 2 * SPAN + FM_TY - 1) / FM_TY));
+    hipLaunchKernelGGL(k_filter_multi, gr, blk, 0, st, b, b_in, dts, g, F, dr2, ds2, courant, f, m, delta, slots, ctl);
+}
 
 void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
                         const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,

@@ -182,6 +182,9 @@ struct tc_ctx {
     // refactorise the coarse operator every coarse_refresh-th solve of the
     // defect-correction modes (tc_set_coarse_refresh); solves since the last
     int coarse_refresh = 1, coarse_since = 0;
+    // the C filter loop of smoother_time with several iterations per launch
+    // (tc_set_filter_blocking); off: one launch per iteration
+    bool filter_blocking = true;
     // tracers (SoA)
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;

@@ -176,7 +176,15 @@ void launch_diagnostics(hipStream_t st, const double* u, const double* w, const 
                         double* out);
 void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsigned long long* slots,
                            const int32_t* stop);
-size_t filter_slot_words();  // u64 words of the three slot sets of the filter loop
+size_t filter_slot_words();  // u64 words of the slot sets of the filter loops
+int filter_multi_k();        // C-filter iterations per launch of launch_filter_multi
+// K iterations of the C filter per launch from bufs[b_in] (iterate m) into
+// bufs[(b_in+1)%3], with the stop test of the previous launch's iterates and
+// the replay from bufs[(b_in+2)%3]; fac: the factors of iterations m-K..m-1
+// then m..m+K-1.  ctl: stop flag, stopping iterate, buffer holding it.
+void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const double* dts, const Grid& g,
+                         const FD& F, double dr2, double ds2, double courant, const double* fac, int m, double delta,
+                         unsigned long long* slots, int32_t* ctl);
 void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
                         const FD& F, double dr2, double ds2, double cfac, double courant, int Tbot,
                         unsigned long long* slots, int m, double delta, int32_t* ctl);

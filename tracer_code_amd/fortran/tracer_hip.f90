@@ -303,6 +303,11 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: solves
     end function
+    integer(c_int) function tc_set_filter_blocking(ctx, enable) bind(C, name="tc_set_filter_blocking")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function
     integer(c_int) function tc_gcr_counts(ctx, updates, restarts) bind(C, name="tc_gcr_counts")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: ctx

@@ -302,6 +302,10 @@ class Solver:
         defect-correction modes (1: every solve)."""
         self._check(self.lib.tc_set_coarse_refresh(self.h, int(solves)), "set_coarse_refresh")
 
+    def set_filter_blocking(self, enable):
+        """Several C-filter iterations per launch (True, the default) or one."""
+        self._check(self.lib.tc_set_filter_blocking(self.h, int(bool(enable))), "set_filter_blocking")
+
     def gcr_counts(self):
         """(updates, restarts) of the GCR acceleration so far."""
         u, r = ctypes.c_int64(), ctypes.c_int64()
