@@ -27,6 +27,8 @@ def _run(params, inputs, blocking, steps):
 
 
 @pytest.mark.parametrize("preset,ov", [("cfg2", dict(nx=256, nz=128, tpc=4)), ("cfg1", dict(nx=192, nz=64, tpc=6)),
+                                       # the last column tile holds only ghost columns (shifted LDS region)
+                                       ("cfg1", dict(nx=96, nz=32, tpc=8)),
                                        ("cfg0", dict(nx=64, nz=32, aspect=2.0, tpc=6, ntype=2, ivis=2,
                                                      RaC=(0.5e5, 0.2e5), visC=(1.0 / 30.0, 0.1),
                                                      delta_visC=(2.0, 2.0)))])

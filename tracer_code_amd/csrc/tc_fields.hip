@@ -334,7 +334,13 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
     const int nx = g.nx, nz = g.nz;
     // tile origin: rows from k = -SPAN, columns from i = -ORG_PAD (row segments start a cache line)
     const int i0 = blockIdx.x * FM_TX - ORG_PAD, k0 = blockIdx.y * FM_TY - SPAN;
-    const int ri0 = i0 - FM_H, rk0 = k0 - FM_H;  // region origin
+    // region origin.  A ghost cell nx+d takes the step at nx-d, whose
+    // dependencies over the launch reach nx-d-(K-1)-1 = nx-2-K for d=2; a
+    // tile starting right of nx-1 (ghost columns only, or one interior
+    // column) has its region shifted left by the shortfall (its own cells
+    // end at nx+2, well inside the region's right part).  Rows alike.
+    const int sx = i0 - nx + 1 > 0 ? i0 - nx + 1 : 0, sk = k0 - nz + 1 > 0 ? k0 - nz + 1 : 0;
+    const int ri0 = i0 - FM_H - sx, rk0 = k0 - FM_H - sk;
     auto inplane = [&](int i, int k) { return i >= -SPAN && i <= nx + SPAN && k >= -SPAN && k <= nz + SPAN; };
     for (int e = tid; e < FM_RX * FM_RY; e += nth) {
         const int vi = e % FM_RX, vk = e / FM_RX, i = ri0 + vi, k = rk0 + vk;
