@@ -63,6 +63,7 @@ SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), 
 # 2048x1024 (1e-8 is below its rounding floor there), and its cycle limit
 FP64_TOLERANCES = (1e-7, 3e-7, 1e-6)
 FP64_MAX_CYCLES = 200
+FILTER_BLOCKING = [True]  # several C-filter iterations per launch (--filter-blocking)
 
 
 def log(msg):
@@ -90,6 +91,7 @@ def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1)
     solver.set_mode(mode)
     solver.initialize(T, r, s, ty)
     net = None
+    solver.set_filter_blocking(FILTER_BLOCKING[0])
     if mode == "fp64":
         return solver, None, "plain"
     if guess == "learned":
@@ -352,6 +354,8 @@ def main():
                     help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
     ap.add_argument("--coarse-refresh", type=int, default=1,
                     help="refactorise the coarsest operator every n-th solve of the defect-correction forms")
+    ap.add_argument("--filter-blocking", type=int, default=1,
+                    help="several C-filter iterations per launch (1) or one launch per iteration (0)")
     ap.add_argument("--legs", default="fp64,dc64,learned",
                     help="other forms timed in the same run (comma list of fp64, dc64, learned; empty: none)")
     ap.add_argument("--leg-steps", type=int, default=3)
@@ -364,6 +368,7 @@ def main():
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    FILTER_BLOCKING[0] = bool(args.filter_blocking)
 
     from tracer_code_amd import _abi, config, ic
 

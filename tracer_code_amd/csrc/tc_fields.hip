@@ -133,16 +133,26 @@ __global__ __launch_bounds__(256) void k_smoothness_max(const double* __restrict
     if (threadIdx.x == 0 && threadIdx.y == 0) slot_max(slots, a);
 }
 
+// dr^2 and ds^2 of the filter step, with their reciprocals: when both are
+// powers of two (pow2) the quotient a/dr2 equals a*idr2 exactly, and the
+// multiplication replaces the FP64 division sequence
+struct FScale {
+    double dr2, ds2, idr2, ids2;
+    int pow2;
+};
+__device__ __forceinline__ double fdiv_r(double a, const FScale& sc) { return sc.pow2 ? a * sc.idr2 : a / sc.dr2; }
+__device__ __forceinline__ double fdiv_s(double a, const FScale& sc) { return sc.pow2 ? a * sc.ids2 : a / sc.ds2; }
+
 // the step at a node 0..nx x 0..nz (smoother.f90:319-329 with the clamp to
 // [0,1] for C, :363-371 for T)
 template <bool COMP>
 __device__ __forceinline__ double filter_new(const double* __restrict__ T0, const double* __restrict__ dts,
-                                             const Grid& g, const FD& F, int i, int k, double dr2, double ds2,
+                                             const Grid& g, const FD& F, int i, int k, const FScale& sc,
                                              double cfac, double courant)
 {
     const size_t c = g.at(i, k);
-    double Trr = dot3(F.D2, T0, c, 1) / dr2;
-    double Tss = dot3(F.D2, T0, c, g.ld) / ds2;
+    double Trr = fdiv_r(dot3(F.D2, T0, c, 1), sc);
+    double Tss = fdiv_s(dot3(F.D2, T0, c, g.ld), sc);
     double temp = Trr + Tss;
     double d = dts[c] * courant;
     if constexpr (COMP) {
@@ -159,13 +169,13 @@ __device__ __forceinline__ double filter_new(const double* __restrict__ T0, cons
 // one filter step and the boundary routine
 template <bool COMP>
 __device__ __forceinline__ double filter_cell(const double* __restrict__ T0, const double* __restrict__ dts,
-                                              const Grid& g, const FD& F, int i, int k, double dr2, double ds2,
+                                              const Grid& g, const FD& F, int i, int k, const FScale& sc,
                                               double cfac, double courant, int Tbot)
 {
     const int nx = g.nx, nz = g.nz;
     // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
     const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
-    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant); };
+    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, sc, cfac, courant); };
     if constexpr (COMP) {
         // enforceBCs_comp (composition.f90:792-803) in its interleaved
         // order: column +-1, row +-1, column +-2, row +-2; the first row
@@ -197,8 +207,8 @@ constexpr int FI_HALO = (FI_BX + 1) * 2 + FI_TILE;  // cells the block evaluates
 
 template <bool COMP>
 __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ T0, double* __restrict__ T,
-                                                     const double* __restrict__ dts, Grid g, FD F, double dr2,
-                                                     double ds2, double cfac, double courant, int Tbot,
+                                                     const double* __restrict__ dts, Grid g, FD F, FScale sc,
+                                                     double cfac, double courant, int Tbot,
                                                      unsigned long long* __restrict__ slots, int m, double delta,
                                                      int32_t* __restrict__ ctl)
 {
@@ -229,7 +239,7 @@ __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ 
     // x from i = -ORG_PAD, so that each row segment starts a cache line
     const int i0 = blockIdx.x * FI_BX - ORG_PAD, k0 = blockIdx.y * FI_TILE - SPAN;
     const int i = i0 + threadIdx.x;
-    auto V = [&](int ii, int kk) { return filter_cell<COMP>(T0, dts, g, F, ii, kk, dr2, ds2, cfac, courant, Tbot); };
+    auto V = [&](int ii, int kk) { return filter_cell<COMP>(T0, dts, g, F, ii, kk, sc, cfac, courant, Tbot); };
     for (int r = 0; r < FILT_ROWS; ++r) {
         const int tk = threadIdx.y + FI_BY * r, k = k0 + tk;
         if (i >= -SPAN && i <= nx + SPAN && k <= nz + SPAN) {
@@ -292,7 +302,7 @@ struct FmFac { double v[2 * FM_K]; };    // factors of the previous launch's ite
 struct FmBufs { double* v[3]; };
 
 __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, const double* __restrict__ dts, Grid g,
-                                                      FD F, double dr2, double ds2, double courant, FmFac fac, int m,
+                                                      FD F, FScale sc, double courant, FmFac fac, int m,
                                                       double delta, unsigned long long* __restrict__ slots,
                                                       int32_t* __restrict__ ctl)
 {
@@ -300,7 +310,7 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
     __shared__ double sbuf[2][FM_RY][FM_RX];
     __shared__ double sdts[FM_RY][FM_RX];
     __shared__ int s_j0;
-    const int tid = threadIdx.y * FM_TX + threadIdx.x, nth = FM_TX * 4;
+    const int tid = threadIdx.y * FM_TX + threadIdx.x;
     // the stop test on the iterates the previous launch produced (or the input)
     if (tid == 0) {
         int j0 = -1;
@@ -342,17 +352,19 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
     const int sx = i0 - nx + 1 > 0 ? i0 - nx + 1 : 0, sk = k0 - nz + 1 > 0 ? k0 - nz + 1 : 0;
     const int ri0 = i0 - FM_H - sx, rk0 = k0 - FM_H - sk;
     auto inplane = [&](int i, int k) { return i >= -SPAN && i <= nx + SPAN && k >= -SPAN && k <= nz + SPAN; };
-    for (int e = tid; e < FM_RX * FM_RY; e += nth) {
-        const int vi = e % FM_RX, vk = e / FM_RX, i = ri0 + vi, k = rk0 + vk;
-        double v = 0.0, d = 0.0;
-        if (inplane(i, k)) {
-            const size_t c = g.at(i, k);
-            v = src[c];
-            if (i >= 0 && i <= nx && k >= 0 && k <= nz) d = dts[c];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int vk = ty; vk < FM_RY; vk += 4)
+        for (int vi = tx; vi < FM_RX; vi += FM_TX) {
+            const int i = ri0 + vi, k = rk0 + vk;
+            double v = 0.0, d = 0.0;
+            if (inplane(i, k)) {
+                const size_t c = g.at(i, k);
+                v = src[c];
+                if (i >= 0 && i <= nx && k >= 0 && k <= nz) d = dts[c];
+            }
+            sbuf[0][vk][vi] = v;
+            sdts[vk][vi] = d;
         }
-        sbuf[0][vk][vi] = v;
-        sdts[vk][vi] = d;
-    }
     __syncthreads();
     int cur = 0;
     for (int it = 1; it <= n; ++it) {
@@ -365,11 +377,11 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
             double acc = F.D2[1] * P[vk][vi - 1];
             acc += F.D2[2] * P[vk][vi];
             acc += F.D2[3] * P[vk][vi + 1];
-            const double Trr = acc / dr2;
+            const double Trr = fdiv_r(acc, sc);
             acc = F.D2[1] * P[vk - 1][vi];
             acc += F.D2[2] * P[vk][vi];
             acc += F.D2[3] * P[vk + 1][vi];
-            const double Tss = acc / ds2;
+            const double Tss = fdiv_s(acc, sc);
             const double temp = Trr + Tss;
             const double d = sdts[vk][vi] * courant;
             double v = P[vk][vi] + cfac * d * temp;
@@ -377,29 +389,29 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
             if (v > 1.0) v = 1.0;
             return v;
         };
-        const int w = FM_RX - 2 * it, h = FM_RY - 2 * it;
-        for (int e = tid; e < w * h; e += nth) {
-            const int vi = it + e % w, vk = it + e / w, i = ri0 + vi, k = rk0 + vk;
-            if (!inplane(i, k)) continue;
-            const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
-            double v;
-            if (k >= 0 && k <= nz) {
-                v = W(im, k);
-            } else {
-                const int ks = (k < 0) ? -k : 2 * nz - k;
-                const bool row1 = (k == -1 || k == nz + 1);
-                v = (row1 && (i == -2 || i == nx + 2)) ? P[ks - rk0][vi] : W(im, ks);
+        for (int vk = it + ty; vk < FM_RY - it; vk += 4)
+            for (int vi = it + tx; vi < FM_RX - it; vi += FM_TX) {
+                const int i = ri0 + vi, k = rk0 + vk;
+                if (!inplane(i, k)) continue;
+                const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+                double v;
+                if (k >= 0 && k <= nz) {
+                    v = W(im, k);
+                } else {
+                    const int ks = (k < 0) ? -k : 2 * nz - k;
+                    const bool row1 = (k == -1 || k == nz + 1);
+                    v = (row1 && (i == -2 || i == nx + 2)) ? P[ks - rk0][vi] : W(im, ks);
+                }
+                Q[vk][vi] = v;
             }
-            Q[vk][vi] = v;
-        }
         __syncthreads();
         cur ^= 1;
         if (!replay) {
             // the new iterate's smoothness over the block's own nodes
             double a = 0.0;
-            const int tx = threadIdx.x, i = i0 + tx, vi = i - ri0;
+            const int i = i0 + tx, vi = i - ri0;
             for (int r = 0; r < FM_TY / 4; ++r) {
-                const int k = k0 + threadIdx.y + 4 * r, vk = k - rk0;
+                const int k = k0 + ty + 4 * r, vk = k - rk0;
                 if (i >= 0 && i <= nx - 1 && k >= 1 && k <= nz - 1)
                     a = smooth_merge(a, Q[vk][vi], Q[vk - 1][vi], Q[vk - 1][vi + 1], Q[vk][vi + 1], Q[vk + 1][vi],
                                      Q[vk + 1][vi + 1]);
@@ -409,8 +421,8 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
         }
     }
     // the block's own cells of the last iterate
-    for (int e = tid; e < FM_TX * FM_TY; e += nth) {
-        const int tx = e % FM_TX, ty = e / FM_TX, i = i0 + tx, k = k0 + ty;
+    for (int r = 0; r < FM_TY / 4; ++r) {
+        const int i = i0 + tx, k = k0 + ty + 4 * r;
         if (inplane(i, k)) dst[g.at(i, k)] = sbuf[cur][k - rk0][i - ri0];
     }
     if (replay && first && tid == 0) { ctl[0] = 1; ctl[1] = j0; ctl[2] = b_out; }
@@ -941,6 +953,15 @@ void launch_smoothness_max(hipStream_t st, const double* T0, const Grid& g, unsi
                        slots, stop);
 }
 size_t filter_slot_words() { return (size_t)FM_SETS * FILT_SET; }
+
+// the filter step's divisors; reciprocals used when both are powers of two
+static FScale filter_scale(double dr2, double ds2)
+{
+    int e;
+    FScale sc{dr2, ds2, 1.0 / dr2, 1.0 / ds2, 0};
+    sc.pow2 = std::frexp(dr2, &e) == 0.5 && std::frexp(ds2, &e) == 0.5;
+    return sc;
+}
 int filter_multi_k() { return FM_K; }
 
 void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const double* dts, const Grid& g,
@@ -953,7 +974,8 @@ void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const 
     dim3 blk(FM_TX, 4);
     dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FM_TX - 1) / FM_TX),
             (unsigned)((g.nz + 1 + 2 * SPAN + FM_TY - 1) / FM_TY));
-    hipLaunchKernelGGL(k_filter_multi, gr, blk, 0, st, b, b_in, dts, g, F, dr2, ds2, courant, f, m, delta, slots, ctl);
+    hipLaunchKernelGGL(k_filter_multi, gr, blk, 0, st, b, b_in, dts, g, F, filter_scale(dr2, ds2), courant, f, m, delta,
+                       slots, ctl);
 }
 
 void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
@@ -963,12 +985,13 @@ void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, 
     dim3 b(FI_BX, FI_BY);
     dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FI_BX - 1) / FI_BX),
             (unsigned)((g.nz + 1 + 2 * SPAN + FI_TILE - 1) / FI_TILE));
+    const FScale sc = filter_scale(dr2, ds2);
     if (comp)
-        hipLaunchKernelGGL((k_filter_iter<true>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
-                           slots, m, delta, ctl);
+        hipLaunchKernelGGL((k_filter_iter<true>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots, m,
+                           delta, ctl);
     else
-        hipLaunchKernelGGL((k_filter_iter<false>), gr, b, 0, st, T0, T, dts, g, F, dr2, ds2, cfac, courant, Tbot,
-                           slots, m, delta, ctl);
+        hipLaunchKernelGGL((k_filter_iter<false>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots, m,
+                           delta, ctl);
 }
 
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n)
