@@ -134,25 +134,33 @@ __global__ __launch_bounds__(256) void k_smoothness_max(const double* __restrict
 }
 
 // dr^2 and ds^2 of the filter step, with their reciprocals: when both are
-// powers of two (pow2) the quotient a/dr2 equals a*idr2 exactly, and the
-// multiplication replaces the FP64 division sequence
+// powers of two (the kernels' POW2 instance) the quotient a/dr2 equals
+// a*idr2 exactly, and the multiplication replaces the FP64 division sequence
 struct FScale {
     double dr2, ds2, idr2, ids2;
     int pow2;
 };
-__device__ __forceinline__ double fdiv_r(double a, const FScale& sc) { return sc.pow2 ? a * sc.idr2 : a / sc.dr2; }
-__device__ __forceinline__ double fdiv_s(double a, const FScale& sc) { return sc.pow2 ? a * sc.ids2 : a / sc.ds2; }
+template <bool POW2>
+__device__ __forceinline__ double fdiv_r(double a, const FScale& sc)
+{
+    if constexpr (POW2) return a * sc.idr2; else return a / sc.dr2;
+}
+template <bool POW2>
+__device__ __forceinline__ double fdiv_s(double a, const FScale& sc)
+{
+    if constexpr (POW2) return a * sc.ids2; else return a / sc.ds2;
+}
 
 // the step at a node 0..nx x 0..nz (smoother.f90:319-329 with the clamp to
 // [0,1] for C, :363-371 for T)
-template <bool COMP>
+template <bool COMP, bool POW2>
 __device__ __forceinline__ double filter_new(const double* __restrict__ T0, const double* __restrict__ dts,
                                              const Grid& g, const FD& F, int i, int k, const FScale& sc,
                                              double cfac, double courant)
 {
     const size_t c = g.at(i, k);
-    double Trr = fdiv_r(dot3(F.D2, T0, c, 1), sc);
-    double Tss = fdiv_s(dot3(F.D2, T0, c, g.ld), sc);
+    double Trr = fdiv_r<POW2>(dot3(F.D2, T0, c, 1), sc);
+    double Tss = fdiv_s<POW2>(dot3(F.D2, T0, c, g.ld), sc);
     double temp = Trr + Tss;
     double d = dts[c] * courant;
     if constexpr (COMP) {
@@ -167,7 +175,7 @@ __device__ __forceinline__ double filter_new(const double* __restrict__ T0, cons
 
 // the value of cell (i,k), i in -SPAN..nx+SPAN, k in -SPAN..nz+SPAN, after
 // one filter step and the boundary routine
-template <bool COMP>
+template <bool COMP, bool POW2>
 __device__ __forceinline__ double filter_cell(const double* __restrict__ T0, const double* __restrict__ dts,
                                               const Grid& g, const FD& F, int i, int k, const FScale& sc,
                                               double cfac, double courant, int Tbot)
@@ -175,7 +183,7 @@ __device__ __forceinline__ double filter_cell(const double* __restrict__ T0, con
     const int nx = g.nx, nz = g.nz;
     // column mirror of the side BCs: ghost columns read interior column 2nx-i / -i
     const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
-    auto W = [&](int ii, int kk) { return filter_new<COMP>(T0, dts, g, F, ii, kk, sc, cfac, courant); };
+    auto W = [&](int ii, int kk) { return filter_new<COMP, POW2>(T0, dts, g, F, ii, kk, sc, cfac, courant); };
     if constexpr (COMP) {
         // enforceBCs_comp (composition.f90:792-803) in its interleaved
         // order: column +-1, row +-1, column +-2, row +-2; the first row
@@ -205,7 +213,7 @@ __device__ __forceinline__ double filter_cell(const double* __restrict__ T0, con
 constexpr int FI_BX = 64, FI_BY = 4, FI_TILE = FI_BY * FILT_ROWS;
 constexpr int FI_HALO = (FI_BX + 1) * 2 + FI_TILE;  // cells the block evaluates beyond its own
 
-template <bool COMP>
+template <bool COMP, bool POW2>
 __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ T0, double* __restrict__ T,
                                                      const double* __restrict__ dts, Grid g, FD F, FScale sc,
                                                      double cfac, double courant, int Tbot,
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ 
     // x from i = -ORG_PAD, so that each row segment starts a cache line
     const int i0 = blockIdx.x * FI_BX - ORG_PAD, k0 = blockIdx.y * FI_TILE - SPAN;
     const int i = i0 + threadIdx.x;
-    auto V = [&](int ii, int kk) { return filter_cell<COMP>(T0, dts, g, F, ii, kk, sc, cfac, courant, Tbot); };
+    auto V = [&](int ii, int kk) { return filter_cell<COMP, POW2>(T0, dts, g, F, ii, kk, sc, cfac, courant, Tbot); };
     for (int r = 0; r < FILT_ROWS; ++r) {
         const int tk = threadIdx.y + FI_BY * r, k = k0 + tk;
         if (i >= -SPAN && i <= nx + SPAN && k <= nz + SPAN) {
@@ -301,6 +309,7 @@ constexpr int FM_SETS = 3 * FM_K;
 struct FmFac { double v[2 * FM_K]; };    // factors of the previous launch's iterations, then this one's
 struct FmBufs { double* v[3]; };
 
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, const double* __restrict__ dts, Grid g,
                                                       FD F, FScale sc, double courant, FmFac fac, int m,
                                                       double delta, unsigned long long* __restrict__ slots,
@@ -311,19 +320,20 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
     __shared__ double sdts[FM_RY][FM_RX];
     __shared__ int s_j0;
     const int tid = threadIdx.y * FM_TX + threadIdx.x;
-    // the stop test on the iterates the previous launch produced (or the input)
-    if (tid == 0) {
+    // the stop test on the iterates the previous launch produced (or the
+    // input): the first wave reads one slot per lane of each set and takes
+    // the maxima by shuffles
+    if (tid < 64) {
         int j0 = -1;
-        for (int j = (m - FM_K + 1 > 0 ? m - FM_K + 1 : 0); j <= m && j0 < 0; ++j) {
-            const unsigned long long* set = slots + (size_t)(j % FM_SETS) * FILT_SET;
-            unsigned long long v = 0ull;
-            for (int s = 0; s < FILT_SLOTS; ++s) {
-                const unsigned long long w = set[s * FILT_STRIDE];
+        for (int j = (m - FM_K + 1 > 0 ? m - FM_K + 1 : 0); j <= m; ++j) {
+            unsigned long long v = slots[(size_t)(j % FM_SETS) * FILT_SET + tid * FILT_STRIDE];
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long w = __shfl_xor(v, o, 64);
                 v = (w > v) ? w : v;
             }
-            if (__longlong_as_double((long long)v) < delta) j0 = j;
+            if (j0 < 0 && __longlong_as_double((long long)v) < delta) j0 = j;
         }
-        s_j0 = j0;
+        if (tid == 0) s_j0 = j0;
     }
     __syncthreads();
     const int j0 = s_j0;
@@ -377,11 +387,11 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
             double acc = F.D2[1] * P[vk][vi - 1];
             acc += F.D2[2] * P[vk][vi];
             acc += F.D2[3] * P[vk][vi + 1];
-            const double Trr = fdiv_r(acc, sc);
+            const double Trr = fdiv_r<POW2>(acc, sc);
             acc = F.D2[1] * P[vk - 1][vi];
             acc += F.D2[2] * P[vk][vi];
             acc += F.D2[3] * P[vk + 1][vi];
-            const double Tss = fdiv_s(acc, sc);
+            const double Tss = fdiv_s<POW2>(acc, sc);
             const double temp = Trr + Tss;
             const double d = sdts[vk][vi] * courant;
             double v = P[vk][vi] + cfac * d * temp;
@@ -974,8 +984,13 @@ void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const 
     dim3 blk(FM_TX, 4);
     dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FM_TX - 1) / FM_TX),
             (unsigned)((g.nz + 1 + 2 * SPAN + FM_TY - 1) / FM_TY));
-    hipLaunchKernelGGL(k_filter_multi, gr, blk, 0, st, b, b_in, dts, g, F, filter_scale(dr2, ds2), courant, f, m, delta,
-                       slots, ctl);
+    const FScale sc = filter_scale(dr2, ds2);
+    if (sc.pow2)
+        hipLaunchKernelGGL((k_filter_multi<true>), gr, blk, 0, st, b, b_in, dts, g, F, sc, courant, f, m, delta, slots,
+                           ctl);
+    else
+        hipLaunchKernelGGL((k_filter_multi<false>), gr, blk, 0, st, b, b_in, dts, g, F, sc, courant, f, m, delta, slots,
+                           ctl);
 }
 
 void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, const double* dts, const Grid& g,
@@ -986,12 +1001,18 @@ void launch_filter_iter(hipStream_t st, bool comp, const double* T0, double* T, 
     dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FI_BX - 1) / FI_BX),
             (unsigned)((g.nz + 1 + 2 * SPAN + FI_TILE - 1) / FI_TILE));
     const FScale sc = filter_scale(dr2, ds2);
-    if (comp)
-        hipLaunchKernelGGL((k_filter_iter<true>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots, m,
-                           delta, ctl);
+    if (comp && sc.pow2)
+        hipLaunchKernelGGL((k_filter_iter<true, true>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots,
+                           m, delta, ctl);
+    else if (comp)
+        hipLaunchKernelGGL((k_filter_iter<true, false>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots,
+                           m, delta, ctl);
+    else if (sc.pow2)
+        hipLaunchKernelGGL((k_filter_iter<false, true>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots,
+                           m, delta, ctl);
     else
-        hipLaunchKernelGGL((k_filter_iter<false>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot, slots, m,
-                           delta, ctl);
+        hipLaunchKernelGGL((k_filter_iter<false, false>), gr, b, 0, st, T0, T, dts, g, F, sc, cfac, courant, Tbot,
+                           slots, m, delta, ctl);
 }
 
 void launch_zero_i32(hipStream_t st, int32_t* p, size_t n)
