@@ -352,8 +352,9 @@ def main():
                     help="initial-guess candidates of the headline's solves")
     ap.add_argument("--gcr", type=int, default=16,
                     help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
-    ap.add_argument("--coarse-refresh", type=int, default=1,
-                    help="refactorise the coarsest operator every n-th solve of the defect-correction forms")
+    ap.add_argument("--coarse-refresh", type=int, default=4,
+                    help="refactorise the coarsest operator every n-th solve of the defect-correction forms "
+                         "(the outer cycles still converge to the tolerance on the current operator)")
     ap.add_argument("--filter-blocking", type=int, default=1,
                     help="several C-filter iterations per launch (1) or one launch per iteration (0)")
     ap.add_argument("--legs", default="fp64,dc64,learned",

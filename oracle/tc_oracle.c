@@ -1586,7 +1586,9 @@ void or_compute_strain_rate_invariant(or_state* s)
 /* T and C filters                                                           */
 /* ------------------------------------------------------------------------ */
 
-/* smoothness (smoother.f90:75-95): largest jump to the 5 forward neighbours. */
+/* smoothness (smoother.f90:75-95): largest jump to the 5 forward neighbours.
+   max(a1,...,a5) is gfortran's MAX, which returns the non-NaN argument (fmax);
+   a NaN result fails "a > diff".  So NaN jumps are ignored. */
 static double smoothness(const or_state* s, const double* T0)
 {
     double diff = 0.0;
@@ -1598,11 +1600,7 @@ static double smoothness(const or_state* s, const double* T0)
             double a3 = fabs(T0[IX(s, i + 1, k)] - c);
             double a4 = fabs(T0[IX(s, i, k + 1)] - c);
             double a5 = fabs(T0[IX(s, i + 1, k + 1)] - c);
-            double a = a1;
-            if (a2 > a) a = a2;
-            if (a3 > a) a = a3;
-            if (a4 > a) a = a4;
-            if (a5 > a) a = a5;
+            double a = fmax(fmax(fmax(fmax(a1, a2), a3), a4), a5);
             if (a > diff) diff = a;
         }
     return diff;

@@ -1,5 +1,6 @@
-"""The blocked C filter loop (several iterations per launch, LDS-staged,
-tc_set_filter_blocking) against the one-launch-per-iteration loop: the
+"""The blocked C filter loop (several iterations per launch, each wavefront
+holding its tile in registers, tc_set_filter_blocking) against the
+one-launch-per-iteration loop: the
 same iterates, so Cvis, n_comp and the iteration counts are bit-identical,
 at start-up and through a time step; and the start-up Cvis equals the
 oracle's (smoother_time, smoother.f90:286-340)."""
@@ -28,10 +29,14 @@ def _run(params, inputs, blocking, steps):
 
 @pytest.mark.parametrize("preset,ov,steps", [("cfg2", dict(nx=256, nz=128, tpc=4), 2),
                                              ("cfg1", dict(nx=192, nz=64, tpc=6), 2),
-                                             # the last column tile holds only ghost columns (shifted LDS
-                                             # region); start-up only: this coarse grid's first steps push
+                                             # start-up only: these coarse grids' first steps push
                                              # tracers out of the domain
                                              ("cfg1", dict(nx=96, nz=32, tpc=8), 0),
+                                             # the last column tile (from i=106) and the last row tile
+                                             # (from k=62) hold only ghost cells, so their held regions
+                                             # are shifted inward; dr^2 = (3/104)^2 is not a power of
+                                             # two (the general instance)
+                                             ("cfg1", dict(nx=104, nz=60, tpc=8), 0),
                                              ("cfg0", dict(nx=64, nz=32, aspect=2.0, tpc=6, ntype=2, ivis=2,
                                                            RaC=(0.5e5, 0.2e5), visC=(1.0 / 30.0, 0.1),
                                                            delta_visC=(2.0, 2.0)), 2)])

@@ -96,18 +96,19 @@ constexpr int FILT_SLOTS = 64, FILT_STRIDE = 16;  // 16 x 8 B = one 128-byte lin
 constexpr int FILT_SET = FILT_SLOTS * FILT_STRIDE;  // u64 words per slot set
 constexpr int FILT_ROWS = 4;                      // rows per thread of both kernels
 
-// the largest jump at one node from its own value t0 and the five forward
-// neighbours, merged into the running maximum a (a NaN jump is kept)
+// the largest jump at one node from its own value t0 to the five forward
+// neighbours, merged into the running maximum a.  fmax ignores a NaN jump,
+// as the reference's max() (gfortran's MAX returns the non-NaN argument)
+// and its "if (a.gt.diff)" do; a starts at 0 and never becomes NaN.  The
+// result is one of the jumps (or 0) whatever the order of the merges.
 __device__ __forceinline__ double smooth_merge(double a, double t0, double n1, double n2, double n3, double n4,
                                                double n5)
 {
-    double b = fabs(n1 - t0);
-    const double a2 = fabs(n2 - t0), a3 = fabs(n3 - t0), a4 = fabs(n4 - t0), a5 = fabs(n5 - t0);
-    if (a2 > b) b = a2;
-    if (a3 > b) b = a3;
-    if (a4 > b) b = a4;
-    if (a5 > b) b = a5;
-    return (b > a || b != b) ? b : a;
+    a = fmax(a, fabs(n1 - t0));
+    a = fmax(a, fabs(n2 - t0));
+    a = fmax(a, fabs(n3 - t0));
+    a = fmax(a, fabs(n4 - t0));
+    return fmax(a, fabs(n5 - t0));
 }
 
 __device__ __forceinline__ void slot_max(unsigned long long* __restrict__ set, double a)
@@ -280,15 +281,25 @@ __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Several iterations of smoother_time's C filter per launch (temporal
-// blocking).  A block owns a FM_TX x FM_TY tile of the ghosted plane and
-// stages the tile with a halo of FM_H = K+1 cells (the iterate and dts) in
-// LDS.  Iteration j evaluates, for every cell of the halo region shrunk by j,
-// the value after one filter step and enforceBCs_comp (filter_cell's rules:
-// the step at the cell a ghost mirrors, the row-1 corner read from the
-// previous iterate) from the previous iterate in LDS, so the results are
-// bit-identical to K single launches.  After each iteration the block's own
-// nodes give that iterate's smoothness maximum (one slot set per iterate).
+// Several iterations of smoother_time's C filter per launch, with the
+// iterates held in registers.  Each wavefront owns a tile of FM_TXO x FM_TY
+// cells of the ghosted plane and holds it with a halo of FM_H = K+1 cells:
+// 64 columns, one per lane, by FM_RY rows in the lane's registers.  An
+// iteration updates the held rows from the previous iterate -- the step at
+// every interior node with the left/right values from the neighbouring
+// lanes (cross-lane shuffles) and the upper/lower ones from the lane's own
+// rows -- then applies enforceBCs_comp: ghost columns copy the lane they
+// mirror, ghost rows the row they mirror, and the row-1 corner cells
+// (-2 or nx+2, -1 or nz+1) the previous iterate's value at row 1 (nz-1), as
+// the reference's interleaved mirrors do.  The iterates are therefore
+// bit-identical to one launch per iteration (filter_cell).  The valid part
+// of the held region shrinks by one cell per iteration; the halo keeps the
+// tile valid through K iterations and the last iterate's measure.  A ghost
+// takes the step at the cell it mirrors, so a tile starting right of the
+// last interior column (or above the last interior row) holds its region
+// shifted inward by the shortfall.  dts_array is one value on the uniform
+// grids the library accepts.  After each iteration the tile's own nodes give
+// that iterate's smoothness maximum (one slot set per iterate).
 //
 // The stop test needs the whole grid's maximum, which a launch only sees
 // for the iterates of the launch before it.  Launch l (iterate m = input)
@@ -300,26 +311,50 @@ __global__ __launch_bounds__(256) void k_filter_iter(const double* __restrict__ 
 // Slot sets: 3K, set j mod 3K for iterate j; a launch reads K, writes K and
 // clears the K its successor writes.
 // ---------------------------------------------------------------------------
+// a double from lane l-1 (wave_shr:1) or lane l+1 (wave_shl:1) of the
+// wavefront, as DPP moves of its two halves; lane 0 (63) gets 0
+__device__ __forceinline__ double dpp_from_left(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x138, 0xF, 0xF, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double dpp_from_right(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x130, 0xF, 0xF, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x130, 0xF, 0xF, true));
+}
+
 constexpr int FM_K = 4;                  // iterations per launch
 constexpr int FM_H = FM_K + 1;           // halo: one cell per iteration, one for the last measure
-constexpr int FM_TX = 64, FM_TY = 16;    // tile (256 threads, 64 x 4)
-constexpr int FM_RX = FM_TX + 2 * FM_H, FM_RY = FM_TY + 2 * FM_H;
+constexpr int FM_TXO = 64 - 2 * FM_H;    // own columns of a wavefront's tile
+constexpr int FM_TY = 16;                // own rows
+constexpr int FM_RY = FM_TY + 2 * FM_H;  // held rows
+constexpr int FM_WAVES = 4;              // tiles (stacked in z) per workgroup
 constexpr int FM_SETS = 3 * FM_K;
 
 struct FmFac { double v[2 * FM_K]; };    // factors of the previous launch's iterations, then this one's
 struct FmBufs { double* v[3]; };
 
-template <bool POW2>
+// FAST: dr^2 and ds^2 are powers of two and D2 is (1, -2, 1) at offsets
+// -1, 0, 1 (the second-order table); the products by 1 and -2 are then exact
+// and are left out, and the quotients are products by the reciprocals: the
+// same bits as the general form.
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, const double* __restrict__ dts, Grid g,
                                                       FD F, FScale sc, double courant, FmFac fac, int m,
                                                       double delta, unsigned long long* __restrict__ slots,
                                                       int32_t* __restrict__ ctl)
 {
-    if (ctl[0]) return;
-    __shared__ double sbuf[2][FM_RY][FM_RX];
-    __shared__ double sdts[FM_RY][FM_RX];
+    // ctl[0] = m+1 marks the stop taken by the launch with input iterate m:
+    // the later launches return, the blocks of that launch still do its work
+    {
+        const int s = ctl[0];
+        if (s != 0 && s != m + 1) return;
+    }
     __shared__ int s_j0;
-    const int tid = threadIdx.y * FM_TX + threadIdx.x;
+    const int lane = threadIdx.x, tid = threadIdx.y * 64 + lane;
     // the stop test on the iterates the previous launch produced (or the
     // input): the first wave reads one slot per lane of each set and takes
     // the maxima by shuffles
@@ -340,7 +375,7 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
     const bool first = blockIdx.x == 0 && blockIdx.y == 0;
     const int b_prev = (b_in + 2) % 3, b_out = (b_in + 1) % 3;
     if (j0 == m) {
-        if (first && tid == 0) { ctl[0] = 1; ctl[1] = m; ctl[2] = b_in; }
+        if (first && tid == 0) { ctl[0] = m + 1; ctl[1] = m; ctl[2] = b_in; }
         return;
     }
     const bool replay = j0 >= 0;
@@ -352,90 +387,138 @@ __global__ __launch_bounds__(256) void k_filter_multi(FmBufs bufs, int b_in, con
         for (int j = m + FM_K + 1; j <= m + 2 * FM_K; ++j)
             slots[(size_t)(j % FM_SETS) * FILT_SET + tid * FILT_STRIDE] = 0ull;
     const int nx = g.nx, nz = g.nz;
-    // tile origin: rows from k = -SPAN, columns from i = -ORG_PAD (row segments start a cache line)
-    const int i0 = blockIdx.x * FM_TX - ORG_PAD, k0 = blockIdx.y * FM_TY - SPAN;
-    // region origin.  A ghost cell nx+d takes the step at nx-d, whose
-    // dependencies over the launch reach nx-d-(K-1)-1 = nx-2-K for d=2; a
-    // tile starting right of nx-1 (ghost columns only, or one interior
-    // column) has its region shifted left by the shortfall (its own cells
-    // end at nx+2, well inside the region's right part).  Rows alike.
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);  // the wave's tile (uniform)
+    const int i0 = blockIdx.x * FM_TXO - SPAN, k0 = (blockIdx.y * FM_WAVES + wy) * FM_TY - SPAN;
     const int sx = i0 - nx + 1 > 0 ? i0 - nx + 1 : 0, sk = k0 - nz + 1 > 0 ? k0 - nz + 1 : 0;
-    const int ri0 = i0 - FM_H - sx, rk0 = k0 - FM_H - sk;
-    auto inplane = [&](int i, int k) { return i >= -SPAN && i <= nx + SPAN && k >= -SPAN && k <= nz + SPAN; };
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    for (int vk = ty; vk < FM_RY; vk += 4)
-        for (int vi = tx; vi < FM_RX; vi += FM_TX) {
-            const int i = ri0 + vi, k = rk0 + vk;
-            double v = 0.0, d = 0.0;
-            if (inplane(i, k)) {
-                const size_t c = g.at(i, k);
-                v = src[c];
-                if (i >= 0 && i <= nx && k >= 0 && k <= nz) d = dts[c];
-            }
-            sbuf[0][vk][vi] = v;
-            sdts[vk][vi] = d;
-        }
-    __syncthreads();
-    int cur = 0;
+    const int ri0 = i0 - FM_H - sx, rk0 = k0 - FM_H - sk;  // held region origin
+    const int i = ri0 + lane;                               // this lane's column
+    const bool col_in = i >= -SPAN && i <= nx + SPAN, col_int = i >= 0 && i <= nx;
+    const bool corner_col = i == -SPAN || i == nx + SPAN;
+    const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
+    int ml = lane + (im - i);                               // lane holding the mirrored column
+    ml = ml < 0 ? 0 : (ml > 63 ? 63 : ml);
+    const double dtc = dts[g.at(0, 0)] * courant;
+    double c[FM_RY];
+#pragma unroll
+    for (int r = 0; r < FM_RY; ++r) {
+        const int k = rk0 + r;
+        c[r] = (col_in && k >= -SPAN && k <= nz + SPAN) ? src[g.at(i, k)] : 0.0;
+    }
+    const bool ghost_col = col_in && !col_int;
+    const bool gcols = ri0 <= -1 || ri0 + 63 >= nx + 1;  // the held columns include ghost columns (uniform)
+    const bool own_col = i >= i0 && i < i0 + FM_TXO && i >= 0 && i <= nx - 1;  // columns measured
     for (int it = 1; it <= n; ++it) {
-        const double cfac = fac.v[fo + it - 1];
-        const double (*P)[FM_RX] = sbuf[cur];
-        double (*Q)[FM_RX] = sbuf[cur ^ 1];
-        // the step at interior node (ii, kk) from the previous iterate P
-        auto W = [&](int ii, int kk) {
-            const int vi = ii - ri0, vk = kk - rk0;
-            double acc = F.D2[1] * P[vk][vi - 1];
-            acc += F.D2[2] * P[vk][vi];
-            acc += F.D2[3] * P[vk][vi + 1];
-            const double Trr = fdiv_r<POW2>(acc, sc);
-            acc = F.D2[1] * P[vk - 1][vi];
-            acc += F.D2[2] * P[vk][vi];
-            acc += F.D2[3] * P[vk + 1][vi];
-            const double Tss = fdiv_s<POW2>(acc, sc);
+        const double cfd = fac.v[fo + it - 1] * dtc;
+        // the row predicates below are recomputed from this copy in every
+        // iteration rather than held across the loop
+        int rk = rk0;
+        asm volatile("" : "+s"(rk));
+        const bool top = rk <= -1, bot = rk + FM_RY - 1 >= nz + 1;  // the held rows include ghost rows
+        // the previous iterate's rows 1 and nz-1 (the row-1 corners read them)
+        double old1 = 0.0, oldn1 = 0.0;
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < FM_RY; ++r)
+                if (rk + r == 1) old1 = c[r];
+        }
+        if (bot) {
+#pragma unroll
+            for (int r = 0; r < FM_RY; ++r)
+                if (rk + r == nz - 1) oldn1 = c[r];
+        }
+        // the step (filter_new's operation order) on every held row but the
+        // first and last; ghost cells are overwritten below, cells outside
+        // the plane are never read by the valid part of the region
+        double om = c[0];
+#pragma unroll
+        for (int r = 1; r < FM_RY - 1; ++r) {
+            const double oc = c[r];
+            const double lf = dpp_from_left(oc), rt = dpp_from_right(oc);
+            double Trr, Tss;
+            if constexpr (FAST) {
+                const double m2 = -2.0 * oc;
+                Trr = ((lf + m2) + rt) * sc.idr2;
+                Tss = ((om + m2) + c[r + 1]) * sc.ids2;
+            } else {
+                double acc = F.D2[1] * lf;
+                acc += F.D2[2] * oc;
+                acc += F.D2[3] * rt;
+                Trr = acc / sc.dr2;
+                acc = F.D2[1] * om;
+                acc += F.D2[2] * oc;
+                acc += F.D2[3] * c[r + 1];
+                Tss = acc / sc.ds2;
+            }
             const double temp = Trr + Tss;
-            const double d = sdts[vk][vi] * courant;
-            double v = P[vk][vi] + cfac * d * temp;
+            double v = oc + cfd * temp;
             if (v < 0.0) v = 0.0;
             if (v > 1.0) v = 1.0;
-            return v;
-        };
-        for (int vk = it + ty; vk < FM_RY - it; vk += 4)
-            for (int vi = it + tx; vi < FM_RX - it; vi += FM_TX) {
-                const int i = ri0 + vi, k = rk0 + vk;
-                if (!inplane(i, k)) continue;
-                const int im = (i < 0) ? -i : (i > nx ? 2 * nx - i : i);
-                double v;
-                if (k >= 0 && k <= nz) {
-                    v = W(im, k);
-                } else {
-                    const int ks = (k < 0) ? -k : 2 * nz - k;
-                    const bool row1 = (k == -1 || k == nz + 1);
-                    v = (row1 && (i == -2 || i == nx + 2)) ? P[ks - rk0][vi] : W(im, ks);
-                }
-                Q[vk][vi] = v;
+            c[r] = v;
+            om = oc;
+        }
+        // ghost columns: the mirrored lane's new value (ghost rows are
+        // overwritten next)
+        if (gcols) {
+#pragma unroll
+            for (int r = 1; r < FM_RY - 1; ++r) {
+                const double mv = __shfl(c[r], ml, 64);
+                c[r] = ghost_col ? mv : c[r];
             }
-        __syncthreads();
-        cur ^= 1;
+        }
+        // ghost rows: the mirrored row (columns already mirrored), the row-1
+        // corners from the previous iterate
+        if (top) {
+#pragma unroll
+            for (int r = 0; r + 4 < FM_RY; ++r) {
+                if (rk + r == -1) c[r] = corner_col ? old1 : c[r + 2];
+                if (rk + r == -2) c[r] = c[r + 4];
+            }
+        }
+        if (bot) {
+#pragma unroll
+            for (int r = 4; r < FM_RY; ++r) {
+                if (rk + r == nz + 1) c[r] = corner_col ? oldn1 : c[r - 2];
+                if (rk + r == nz + 2) c[r] = c[r - 4];
+            }
+        }
         if (!replay) {
-            // the new iterate's smoothness over the block's own nodes
+            // the new iterate's smoothness over the tile's own nodes of
+            // rows 1..nz-1 (held rows ra..rb): the jumps of smooth_merge,
+            // each vertical pair once (node r's jump up is node r-1's jump
+            // down, |x-y| = |y-x|); the maximum is that of smooth_merge
+            const int ka = k0 > 1 ? k0 : 1, kb = k0 + FM_TY - 1 < nz - 1 ? k0 + FM_TY - 1 : nz - 1;
+            const int ra = ka - rk, rb = kb - rk;  // ra >= FM_H, rb <= FM_RY - 1 - FM_H
             double a = 0.0;
-            const int i = i0 + tx, vi = i - ri0;
-            for (int r = 0; r < FM_TY / 4; ++r) {
-                const int k = k0 + ty + 4 * r, vk = k - rk0;
-                if (i >= 0 && i <= nx - 1 && k >= 1 && k <= nz - 1)
-                    a = smooth_merge(a, Q[vk][vi], Q[vk - 1][vi], Q[vk - 1][vi + 1], Q[vk][vi + 1], Q[vk + 1][vi],
-                                     Q[vk + 1][vi + 1]);
+            double rm = 0.0, r0 = 0.0;  // the right lane's c[r-1], c[r]
+#pragma unroll
+            for (int r = 1; r < FM_RY - 1; ++r) {
+                // rows outside ra-2..rb are skipped by the whole wave
+                if (r + 2 >= ra && r <= rb) {
+                    const double rp = dpp_from_right(c[r + 1]);
+                    if (r >= ra) {
+                        if (r == ra) a = fmax(a, fabs(c[r - 1] - c[r]));
+                        a = fmax(a, fabs(rm - c[r]));
+                        a = fmax(a, fabs(r0 - c[r]));
+                        a = fmax(a, fabs(c[r + 1] - c[r]));
+                        a = fmax(a, fabs(rp - c[r]));
+                    }
+                    rm = r0;
+                    r0 = rp;
+                }
             }
+            a = own_col ? a : 0.0;
             a = block_max(a);
             if (tid == 0) slot_max(slots + (size_t)((m + it) % FM_SETS) * FILT_SET, a);
         }
     }
-    // the block's own cells of the last iterate
-    for (int r = 0; r < FM_TY / 4; ++r) {
-        const int i = i0 + tx, k = k0 + ty + 4 * r;
-        if (inplane(i, k)) dst[g.at(i, k)] = sbuf[cur][k - rk0][i - ri0];
+    // the tile's own cells of the last iterate
+    const bool own_w = col_in && i >= i0 && i < i0 + FM_TXO;
+#pragma unroll
+    for (int r = 0; r < FM_RY; ++r) {
+        const int k = rk0 + r;
+        if (own_w && k >= k0 && k < k0 + FM_TY && k >= -SPAN && k <= nz + SPAN) dst[g.at(i, k)] = c[r];
     }
-    if (replay && first && tid == 0) { ctl[0] = 1; ctl[1] = j0; ctl[2] = b_out; }
+    if (replay && first && tid == 0) { ctl[0] = m + 1; ctl[1] = j0; ctl[2] = b_out; }
 }
 
 // enforceBCs (energy.f90:455-475), one workgroup.
@@ -981,11 +1064,13 @@ void launch_filter_multi(hipStream_t st, double* const bufs[3], int b_in, const 
     FmBufs b{{bufs[0], bufs[1], bufs[2]}};
     FmFac f{};
     for (int j = 0; j < 2 * FM_K; ++j) f.v[j] = fac[j];
-    dim3 blk(FM_TX, 4);
-    dim3 gr((unsigned)((g.nx + 1 + SPAN + ORG_PAD + FM_TX - 1) / FM_TX),
-            (unsigned)((g.nz + 1 + 2 * SPAN + FM_TY - 1) / FM_TY));
+    // tiles of FM_TXO x FM_TY cells over the plane's -SPAN..nx+SPAN x
+    // -SPAN..nz+SPAN, FM_WAVES of them stacked in z per workgroup
+    const int tz = (g.nz + 1 + 2 * SPAN + FM_TY - 1) / FM_TY;
+    dim3 blk(64, FM_WAVES);
+    dim3 gr((unsigned)((g.nx + 1 + 2 * SPAN + FM_TXO - 1) / FM_TXO), (unsigned)((tz + FM_WAVES - 1) / FM_WAVES));
     const FScale sc = filter_scale(dr2, ds2);
-    if (sc.pow2)
+    if (sc.pow2 && F.D2[1] == 1.0 && F.D2[2] == -2.0 && F.D2[3] == 1.0)
         hipLaunchKernelGGL((k_filter_multi<true>), gr, blk, 0, st, b, b_in, dts, g, F, sc, courant, f, m, delta, slots,
                            ctl);
     else
