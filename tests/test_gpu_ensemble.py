@@ -37,3 +37,20 @@ def test_concurrent_equals_alone(mode, guess):
     for a, b in zip(*fields):
         for k in a:
             assert np.array_equal(a[k], b[k]), k
+
+
+def test_drawn_cases_at_full_size():
+    """Six cases of the batched-ensemble distribution (full grids from
+    {256x256, 512x256, 768x256, 1024x512}, drawn physics, 10-40 tracers per
+    node), mixed mode with GCR(16) and the learned guess's candidates, two
+    steps: the concurrent arm is bit-identical to running the cases one
+    after another, and the last solve of every case met the reference's
+    stop test (mean |r| <= tolerance * res_ref; a solve that cannot meet it
+    returns an error)."""
+    cases = ensemble.draw_cases(6, seed=0)
+    print(f"[ensemble full size {[(c['nx'], c['nz']) for c in cases]}]", flush=True)
+    res = ensemble.run(cases, 2, mode="mixed", guess="extrap", gcr=16, log=lambda m: print(m, flush=True))
+    assert res["identical_to_sequential"] == len(cases)
+    for arm in ("sequential", "concurrent"):
+        assert res[arm]["max_last_residual_over_tolerance"] <= 1.0
+        assert min(res[arm]["cycles_per_step"]) > 0

@@ -6,8 +6,9 @@ position inside the domain that is always inside the arrays.  The oracle (and
 the HIP kernel) clamp i and k so that an out-of-domain position cannot read
 outside the ghost ring, and the oracle counts every clamped call.
 
-* The round-1 faulting case (768x256 physics at 96x32, RK4 stages driven
-  back to back from one solve's velocities, as the GPU parity test does):
+* The case during which a test run once ended with a segmentation fault in
+  several host threads (768x256 physics at 96x32, RK4 stages driven back to
+  back from one solve's velocities, as the GPU parity test does):
   every stage position stays in the domain and the clamp is never reached,
   under an AddressSanitizer build of the oracle too.
 * A full time step of the same physics from another seed, where the third

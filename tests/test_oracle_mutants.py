@@ -3,9 +3,9 @@
 Each mutant is a copy of oracle/tc_oracle.c with one deliberate mistake of
 the kind a restatement could make (a swapped constant, a wrong index, a
 dropped factor, a flipped sign).  It is compiled on the CPU into a temporary
-library, and tests/test_oracle_pins.py is run against it (ORACLE_LIB): every
-mutant must make at least one pin fail, while the unmodified oracle passes
-them all (test_oracle_pins itself).
+library, and tests/test_oracle_pins.py and tests/test_oracle_bcs.py are run
+against it (ORACLE_LIB): every mutant must make at least one pin fail, while
+the unmodified oracle passes them all (those two files themselves).
 """
 import concurrent.futures as cf
 import os
@@ -55,6 +55,10 @@ MUTANTS = [
      "const double lb = -5.0, la = -1.0;", "const double lb = -4.0, la = -1.0;"),
     ("smoother_time without courantC", "smoother.f90:323",
      "double dts = s->dts[cc] * s->p.courantC;", "double dts = s->dts[cc];"),
+    ("enforceBCs top ghosts copied, not reflected about 0", "energy.f90:470",
+     "T0[IX(s, i, nz + k)] = -T0[IX(s, i, nz - k)];", "T0[IX(s, i, nz + k)] = T0[IX(s, i, nz - k)];"),
+    ("enforceBCs_comp bottom ghosts from the row one further in", "composition.f90:800",
+     "C0[IX(s, ii, -i)] = C0[IX(s, ii, i)];", "C0[IX(s, ii, -i)] = C0[IX(s, ii, i + 1)];"),
 ]
 
 
@@ -76,7 +80,8 @@ def _build(tmp, idx, old, new):
 def _run_pins(lib):
     env = dict(os.environ, ORACLE_LIB=lib, OMP_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_pins.py")],
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_bcs.py")],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
     return r.returncode, r.stdout[-400:]
 

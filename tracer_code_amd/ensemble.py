@@ -14,6 +14,7 @@ the throughput with running the same cases one after another.
 """
 import argparse
 import ctypes
+import hashlib
 import json
 import math
 import os
@@ -44,12 +45,14 @@ def draw_cases(n, seed=0):
     return cases
 
 
-def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None, gcr=0):
+def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None, gcr=0, log=None):
     """(solvers, networks) initialised from each case's seeded inputs; `tune`
     (per case: dict of sweeps, tiles, level2, coarse) applies autotuned
     settings; gcr: GCR directions over the defect-correction cycles."""
     solvers, nets = [], []
-    for c in cases:
+    for j, c in enumerate(cases):
+        if log and j % 16 == 0:
+            log(f"[ensemble] initialising cases {j}..{min(j + 16, len(cases)) - 1} of {len(cases)}")
         params = config.make_params(**c)
         assert (params.nx, params.nz) == (c["nx"], c["nz"]), "grid changed by optimize_grid"
         T, r, s, ty = ic.workload(c, seed=c["seed"])
@@ -78,13 +81,15 @@ def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune
     return solvers, nets
 
 
-def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70), gcr=0):
+def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70), gcr=0, log=None):
     """Per case: the sweep counts and the tiles the autotuners pick (on a
     scratch copy of the case, advanced by the tuning steps).  With GCR the
     levels from 3 down and level 2 get counts of their own, and the common
     count is tuned again after them."""
     out = []
-    for c in cases:
+    for j, c in enumerate(cases):
+        if log and j % 8 == 0:
+            log(f"[ensemble] autotuning cases {j}..{min(j + 8, len(cases)) - 1} of {len(cases)}")
         (g,), _ = make_solvers([c], mode, gcr=gcr)
         sweeps, _ = g.autotune_dc_sweeps(candidates, 1)
         level2 = coarse = 0
@@ -114,27 +119,36 @@ def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune
     """Throughput of the cases run concurrently and one after another, from
     the same initial states (each after one untimed warm-up step).  graphs:
     replay the coarse part of each defect-correction cycle as a HIP graph."""
-    out = {}
+    out, hashes = {}, []
     for concurrent in (False, True):
-        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune, gcr=gcr)
+        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune, gcr=gcr, log=log)
         for g in solvers:
             g.set_graphs(graphs)
         ensemble_step(solvers, 1, threads)  # warm-up, concurrently in both arms
         for g in solvers:
             g.sync()
+        c0 = [g.stats().mg_cycles for g in solvers]
         t0 = time.perf_counter()
         if concurrent:
             ensemble_step(solvers, steps, threads)
         else:
-            for g in solvers:
+            for j, g in enumerate(solvers):
                 g.time_step(steps)
+                if j % 16 == 15:
+                    log(f"[ensemble] sequential: {j + 1} of {len(solvers)} cases done")
         for g in solvers:
             g.sync()
         el = time.perf_counter() - t0
         cells = sum(c["nx"] * c["nz"] for c in cases)
-        out["concurrent" if concurrent else "sequential"] = dict(
-            seconds=el, case_steps_per_s=len(cases) * steps / el, cell_steps_per_s=cells * steps / el)
-        log(f"[ensemble] {'concurrent' if concurrent else 'sequential'}: {el:.2f}s for {len(cases)} cases x {steps} steps")
+        arm = "concurrent" if concurrent else "sequential"
+        # every solve stops on the reference's test (a solve that cannot
+        # reach it returns an error); the last one's residual is recorded
+        resid = [g.stats().residual_last / (g.params.tolerance * g.stats().res_ref_last) for g in solvers]
+        out[arm] = dict(seconds=el, case_steps_per_s=len(cases) * steps / el, cell_steps_per_s=cells * steps / el,
+                        cycles_per_step=[(g.stats().mg_cycles - c) / steps for g, c in zip(solvers, c0)],
+                        max_last_residual_over_tolerance=max(resid))
+        hashes.append([state_hash(g) for g in solvers])
+        log(f"[ensemble] {arm}: {el:.2f}s for {len(cases)} cases x {steps} steps")
         for g in solvers:
             if guess == "learned":
                 g.learn_attach(None)
@@ -142,7 +156,19 @@ def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune
         for net in nets:
             net.close()
     out["speedup"] = out["sequential"]["seconds"] / out["concurrent"]["seconds"]
+    out["identical_to_sequential"] = sum(a == b for a, b in zip(*hashes))
     return out
+
+
+def state_hash(g):
+    """SHA-256 of a case's streamfunction, temperature, composition and
+    tracer positions."""
+    h = hashlib.sha256()
+    for k in ("SF", "T", "Cnew"):
+        h.update(np.ascontiguousarray(g.get(k)).tobytes())
+    for a in g.get_tracers("x"):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
 
 
 def main(argv=None):
@@ -162,9 +188,9 @@ def main(argv=None):
     if a.guess == "learned":
         w, _ = learn.load_weights(os.path.join(os.path.dirname(__file__), "weights", "unet_cfg1_L4.npz"))
     gcr = a.gcr if a.mode != "fp64" else 0
-    tune = autotune_cases(cases, a.mode, gcr=gcr) if (a.autotune and a.mode != "fp64") else None
-    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune,
-              log=lambda m: print(m, file=sys.stderr, flush=True), graphs=a.graphs, gcr=gcr)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    tune = autotune_cases(cases, a.mode, gcr=gcr, log=log) if (a.autotune and a.mode != "fp64") else None
+    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune, log=log, graphs=a.graphs, gcr=gcr)
     res["graphs"] = a.graphs
     res["gcr"] = gcr
     if tune:
