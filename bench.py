@@ -83,7 +83,7 @@ def copy_calibration(mib):
     return 0 if ok else 1
 
 
-def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1):
+def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1, sort_interval=0):
     """A solver on the seeded inputs; returns (solver, network or None, guess)."""
     from tracer_code_amd import Solver
     T, r, s, ty = inputs
@@ -92,6 +92,7 @@ def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1)
     solver.initialize(T, r, s, ty)
     net = None
     solver.set_filter_blocking(FILTER_BLOCKING[0])
+    solver.set_tracer_sort(sort_interval)
     if mode == "fp64":
         return solver, None, "plain"
     if guess == "learned":
@@ -355,6 +356,9 @@ def main():
     ap.add_argument("--coarse-refresh", type=int, default=4,
                     help="refactorise the coarsest operator every n-th solve of the defect-correction forms "
                          "(the outer cycles still converge to the tolerance on the current operator)")
+    ap.add_argument("--sort-interval", type=int, default=20,
+                    help="sort the device tracers by node at the start of every n-th step (0: never); "
+                         "tc_get_tracers keeps the caller's order and results are bit-identical")
     ap.add_argument("--filter-blocking", type=int, default=1,
                     help="several C-filter iterations per launch (1) or one launch per iteration (0)")
     ap.add_argument("--legs", default="fp64,dc64,learned",
@@ -389,7 +393,8 @@ def main():
 
     # ---- headline form ----------------------------------------------------
     gcr = args.gcr if args.mode != "fp64" else 0
-    solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config, args.coarse_refresh)
+    solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config, args.coarse_refresh,
+                                     args.sort_interval)
     counts, rec = ({"common": params.Nmulti, "coarse": 0, "level2": 0, "tiles": None}, {})
     if args.mode != "fp64":
         counts, rec = tune_counts(solver, params, args.mode, args)
@@ -423,7 +428,8 @@ def main():
     fp64_state = None
     for name in want:
         if name == "learned":
-            s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config, args.coarse_refresh)
+            s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config, args.coarse_refresh,
+                                     args.sort_interval)
             apply_counts(s2, params, counts)
             res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned", until_time=t_head)
             close_solver(s2, n2)
@@ -432,7 +438,8 @@ def main():
                                              counts["coarse"] or counts["common"]]})
             legs["mixed_learned"] = res
         elif name == "dc64":
-            s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config, args.coarse_refresh)
+            s2, n2, g2 = make_solver(params, inputs, "dc64", args.guess, gcr, args.config, args.coarse_refresh,
+                                     args.sort_interval)
             c2, _ = tune_counts(s2, params, "dc64", args)
             res = run_timed(s2, params, 1, args.leg_steps, "dc64", until_time=t_head)
             close_solver(s2, n2)
@@ -449,7 +456,7 @@ def main():
             failed = []
             for tol in tols:
                 p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol, max_cycles=FP64_MAX_CYCLES))
-                s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config)
+                s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config, sort_interval=args.sort_interval)
                 try:
                     res = run_timed(s2, p64, 1, args.leg_steps, "fp64", until_time=t_head)
                 except TcError as e:
@@ -501,6 +508,7 @@ def main():
             "guess": guess,
             "gcr_directions": gcr,
             "coarse_refresh_solves": args.coarse_refresh,
+            "tracer_sort_interval": args.sort_interval,
             "sweeps_per_visit": counts["common"],
             "sweeps_per_visit_level_2": counts["level2"] or counts["common"],
             "sweeps_per_visit_levels_3_up": counts["coarse"] or counts["common"],

@@ -405,6 +405,9 @@ int tc_sort_tracers(tc_ctx* ctx);
 /* Sort automatically at the start of every `interval`-th tc_time_step step
  * (0: never, the default). */
 int tc_set_tracer_sort(tc_ctx* ctx, int32_t interval);
+/* caller_index[j] (ntr entries) = the caller's index of the tracer held in
+ * device slot j (the identity before any sort). */
+int tc_tracer_slots(tc_ctx* ctx, int64_t* caller_index);
 
 /* ---- measurement hooks (no counterpart in the reference) --------------- */
 /* Kernel ids for tc_kernel_timing. */

@@ -2445,9 +2445,7 @@ int sort_tracers(tc_ctx* ctx)
     const uint32_t bad = (uint32_t)(g.nx + 1) * (uint32_t)(g.nz + 1);
     int bits = 1;
     while (bits < 32 && ((uint64_t)1 << bits) <= bad) ++bits;
-    size_t need = 0;
-    TC_TRY(sort_pairs(ctx->stream, nullptr, &need, ctx->sort_k[0], ctx->sort_k[1], ctx->sort_v[0], ctx->sort_v[1], n,
-                      bits));
+    const size_t need = radix_scratch_words(n) * sizeof(uint32_t);
     if (need > ctx->sort_tmp_bytes) {
         if (ctx->sort_tmp) TC_HIP(hipFree(ctx->sort_tmp));
         TC_HIP(hipMalloc(&ctx->sort_tmp, need));
@@ -2455,10 +2453,15 @@ int sort_tracers(tc_ctx* ctx)
     }
     launch_tracer_keys(ctx->stream, ctx->rtr, ctx->str, n, g.nx, ctx->lv[1].dr, ctx->lv[1].ds, P.aspect, bad,
                        ctx->sort_k[0], ctx->sort_v[0]);
-    size_t bytes = ctx->sort_tmp_bytes;
-    TC_TRY(sort_pairs(ctx->stream, ctx->sort_tmp, &bytes, ctx->sort_k[0], ctx->sort_k[1], ctx->sort_v[0],
-                      ctx->sort_v[1], n, bits));
-    const uint32_t* perm = ctx->sort_v[1];
+    int cur = 0;
+    const int e = radix_sort_pairs<uint32_t>(ctx->stream, ctx->sort_k[0], ctx->sort_k[1], ctx->sort_v[0], ctx->sort_v[1],
+                                             true, n, bits, static_cast<uint32_t*>(ctx->sort_tmp),
+                                             ctx->sort_tmp_bytes / sizeof(uint32_t), &cur);
+    if (e) {
+        ctx->err = "tracer sort: radix sort launch failed";
+        return e;
+    }
+    const uint32_t* perm = ctx->sort_v[cur];
     double** arrays[] = {&ctx->rtr, &ctx->str, &ctx->rtr0, &ctx->str0, &ctx->rtr1, &ctx->str1, &ctx->rtr2, &ctx->str2};
     for (double** a : arrays) {
         launch_gather_d(ctx->stream, *a, perm, ctx->spare_d, n);
@@ -2478,6 +2481,20 @@ int tc_sort_tracers(tc_ctx* ctx)
     if (!ctx) return TC_ERR_ARG;
     TC_TRY(sort_tracers(ctx));
     TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tc_tracer_slots(tc_ctx* ctx, int64_t* caller_index)
+{
+    if (!ctx || (!caller_index && ctx->ntr > 0)) return TC_ERR_ARG;
+    if (ctx->ntr == 0) return 0;
+    if (ctx->tr_id) {
+        TC_HIP(hipMemcpyAsync(caller_index, ctx->tr_id, (size_t)ctx->ntr * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        for (int64_t j = 0; j < ctx->ntr; ++j) caller_index[j] = j;
+    }
     return 0;
 }
 

@@ -14,7 +14,6 @@
 //     donors of their type; the visit-order rank of a node among those of its
 //     type comes from wavefront ballots and a scan of per-block counts.
 // The node index is c = k*(nx+1) + i; the tracer index is the caller's.
-#include <hipcub/hipcub.hpp>
 
 #include "tc_kernels.h"
 
@@ -587,15 +586,21 @@ void launch_eq_pass(hipStream_t st, const int32_t* rows, const int32_t* cols, in
 }
 
 // LSD radix sort of (64-bit key, slot) pairs on bits [0, bits) with
-// ping-pong buffers; the result is in k[*cur], v[*cur]
+// ping-pong buffers (radix_sort_pairs); the result is in k[*cur], v[*cur].
+// With temp == nullptr only sets *temp_bytes.
 int eq_sort(hipStream_t st, void* temp, size_t* temp_bytes, uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1,
             int64_t n, int bits, int* cur)
 {
-    hipcub::DoubleBuffer<uint64_t> dk(k0, k1);
-    hipcub::DoubleBuffer<uint32_t> dv(v0, v1);
-    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, dk, dv, (int)n, 0, bits, st);
-    if (cur) *cur = dk.selector;
-    return e == hipSuccess ? 0 : TC_ERR_HIP;
+    const size_t words = radix_scratch_words(n);
+    if (!temp) {
+        *temp_bytes = words * sizeof(uint32_t);
+        return 0;
+    }
+    if (*temp_bytes < words * sizeof(uint32_t)) return TC_ERR_ARG;
+    int c = 0;
+    const int e = radix_sort_pairs<uint64_t>(st, k0, k1, v0, v1, false, n, bits, static_cast<uint32_t*>(temp), words, &c);
+    if (cur) *cur = c;
+    return e;
 }
 
 }  // namespace tc

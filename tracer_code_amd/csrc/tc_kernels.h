@@ -204,8 +204,14 @@ struct CopyList {
 };
 void launch_copy_list(hipStream_t st, const CopyList& L, void* dst);
 void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n);
-int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
-               const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
+// Stable LSD radix sort of (key, value) pairs on the key's bits [0, bits)
+// (tc_radix.hip).  k0/v0 hold the input (identity_values: v0 is not read,
+// the values are the indices 0..n-1); the result is in k0/v0 (*cur = 0) or
+// k1/v1 (*cur = 1).  scratch: radix_scratch_words(n) words.
+size_t radix_scratch_words(int64_t n);
+template <typename K>
+int radix_sort_pairs(hipStream_t st, K* k0, K* k1, uint32_t* v0, uint32_t* v1, bool identity_values, int64_t n,
+                     int bits, uint32_t* scratch, size_t scratch_words, int* cur);
 
 // equalize_tracers (tc_equalize.hip)
 uint64_t eq_stream_key(uint64_t seed, uint64_t call);

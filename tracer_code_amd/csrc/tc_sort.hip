@@ -7,8 +7,6 @@
 // each slot back to the caller's tracer index for the boundary copies.
 #include <algorithm>
 
-#include <hipcub/hipcub.hpp>
-
 #include "tc_kernels.h"
 
 namespace tc {
@@ -114,16 +112,6 @@ void launch_copy_list(hipStream_t st, const CopyList& L, void* dst)
     for (int j = 0; j < L.n; ++j) most = std::max(most, L.bytes[j] / 16);
     const unsigned gx = (unsigned)std::min<size_t>(std::max<size_t>((most + 255) / 256, 1), 1024);
     hipLaunchKernelGGL(k_copy_list, dim3(gx, (unsigned)L.n), dim3(256), 0, st, L, static_cast<char*>(dst));
-}
-
-// LSD radix sort of (key, slot) pairs on the key's low `bits` bits; with
-// temp == nullptr only sets *temp_bytes
-int sort_pairs(hipStream_t st, void* temp, size_t* temp_bytes, const uint32_t* kin, uint32_t* kout,
-               const uint32_t* vin, uint32_t* vout, int64_t n, int bits)
-{
-    const hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, (int)n, 0, bits,
-                                                           st);
-    return e == hipSuccess ? 0 : TC_ERR_HIP;
 }
 
 }  // namespace tc

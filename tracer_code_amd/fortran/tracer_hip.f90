@@ -338,6 +338,11 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: interval
     end function
+    integer(c_int) function tc_tracer_slots(ctx, caller_index) bind(C, name="tc_tracer_slots")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: caller_index(*)
+    end function
     integer(c_int) function tc_set_dc_sweeps(ctx, sweeps) bind(C, name="tc_set_dc_sweeps")
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: ctx

@@ -276,6 +276,12 @@ class Solver:
         """Sort the tracers at the start of every interval-th time step (0: never)."""
         self._check(self.lib.tc_set_tracer_sort(self.h, int(interval)), "set_tracer_sort")
 
+    def tracer_slots(self):
+        """The caller's index of the tracer in each device slot."""
+        out = np.empty(self.ntr, dtype=np.int64)
+        self._check(self.lib.tc_tracer_slots(self.h, out.ctypes.data), "tracer_slots")
+        return out
+
     def set_dc_sweeps(self, sweeps):
         """Sweeps per level visit of the defect-correction cycle (0: Nmulti)."""
         self._check(self.lib.tc_set_dc_sweeps(self.h, int(sweeps)), "set_dc_sweeps")
