@@ -215,6 +215,16 @@ int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves);
  * default), or one launch per iteration (enable=0).  Both give the same
  * iterates, stop test, iteration counts and bits. */
 int tc_set_filter_blocking(tc_ctx* ctx, int32_t enable);
+/* Level-1 smoother of the FP32 defect-correction cycle (mode TC_MODE_MIXED;
+ * no counterpart in the reference, whose smoother is the Chebyshev-
+ * accelerated Jacobi-Richardson sweep of iterate_stream,
+ * iterative_solver.f90:860-912): with sweeps > 0 each level-1 visit runs
+ * `sweeps` multicolour Gauss-Seidel sweeps (9 colours, tiles staged in LDS
+ * and coupled between launches) instead of its Jacobi-Richardson sweeps,
+ * `per_launch` (1..8) of them per kernel launch, relaxation factor omega
+ * (0 < omega < 2).  sweeps = 0 (the default) keeps the Jacobi-Richardson
+ * sweeps.  The outer residual, stop test and tolerance are unchanged. */
+int tc_set_gs_smoother(tc_ctx* ctx, int32_t sweeps, int32_t per_launch, double omega);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

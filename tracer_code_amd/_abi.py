@@ -131,6 +131,7 @@ SIGNATURES = {
     "tc_ensemble_step": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _VP]),
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_tracer_slots": (ctypes.c_int, [_VP, _VP]),
+    "tc_set_gs_smoother": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     "tc_autotune_dc_level_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _VP, ctypes.c_int32,
                                                    ctypes.c_int32, _VP, _VP]),

@@ -328,6 +328,17 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
     const tc_params& P = ctx->p;
     const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
     const bool use_ratio = (dir == 0) && (level == ctx->ratio_level);
+    if (f32 && level == 1 && ctx->gs_sweeps > 0 && ctx->mode == TC_MODE_MIXED) {
+        // multicolour Gauss-Seidel in place of the visit's Jacobi-Richardson sweeps
+        for (int n = 0; n < ctx->gs_sweeps;) {
+            const int nk = std::min(ctx->gs_per_launch, ctx->gs_sweeps - n);
+            launch_gs32(ctx->stream, L, rhs32, L.err32[L.cur32], L.err32[1 - L.cur32], nk, ctx->gs_omega, zsg);
+            L.cur32 = 1 - L.cur32;
+            n += nk;
+        }
+        ctx->stats.sweeps_exec[0] += ctx->gs_sweeps;
+        return 0;
+    }
     const bool tb = d.tb && level >= 2 && ctx->mode != TC_MODE_FP64;
     // sweeps per launch: 4, or 8 on FP32 levels of at most 16 tiles
     const int K = (tb && f32 && L.tb_tile != 32) ? tb_sweeps_per_launch(L.g) : 4;
@@ -989,6 +1000,13 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
         if (d.tb)
             for (int L = 2; L < G; ++L)
                 if (tb_uses_w13(ctx->lv[L])) launch_tb_weights32(ctx->stream, ctx->lv[L], d.F, d.h[L]);
+        if (ctx->gs_sweeps > 0) {
+            launch_tb_weights32(ctx->stream, L1, d.F, d.h[1]);
+            if (!L1.w13) {
+                ctx->err = "out of device memory for the level-1 Gauss-Seidel weights";
+                return TC_ERR_NOMEM;
+            }
+        }
     }
     L1.cur = 0;
     L1.cur32 = 0;
@@ -2161,6 +2179,19 @@ int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves)
     }
     ctx->coarse_refresh = solves;
     ctx->coarse_since = 0;
+    return 0;
+}
+
+int tc_set_gs_smoother(tc_ctx* ctx, int32_t sweeps, int32_t per_launch, double omega)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (sweeps < 0 || per_launch < 1 || per_launch > 8 || !(omega > 0.0 && omega < 2.0)) {
+        ctx->err = "tc_set_gs_smoother: sweeps >= 0, per_launch in 1..8, 0 < omega < 2";
+        return TC_ERR_ARG;
+    }
+    ctx->gs_sweeps = sweeps;
+    ctx->gs_per_launch = per_launch;
+    ctx->gs_omega = omega;
     return 0;
 }
 

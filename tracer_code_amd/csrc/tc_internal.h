@@ -185,6 +185,10 @@ struct tc_ctx {
     // the C filter loop of smoother_time with several iterations per launch
     // (tc_set_filter_blocking); off: one launch per iteration
     bool filter_blocking = true;
+    // multicolour Gauss-Seidel on level 1 of the FP32 correction cycle
+    // (tc_set_gs_smoother): sweeps per visit (0: Jacobi-Richardson), per launch
+    int gs_sweeps = 0, gs_per_launch = 2;
+    double gs_omega = 1.0;
     // tracers (SoA)
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;

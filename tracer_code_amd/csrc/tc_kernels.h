@@ -204,6 +204,11 @@ struct CopyList {
 };
 void launch_copy_list(hipStream_t st, const CopyList& L, void* dst);
 void launch_scatter_i32(hipStream_t st, const int32_t* in, const int64_t* id, int32_t* out, int64_t n);
+// multicolour Gauss-Seidel sweeps of a level's FP32 correction (tc_gs.hip):
+// nk sweeps from Ein into Eout, with the level's 13 dt-scaled tap planes w13
+void launch_gs32(hipStream_t st, const Level& L, const float* rhs, const float* Ein, float* Eout, int nk, double omega,
+                 double zsg);
+
 // Stable LSD radix sort of (key, value) pairs on the key's bits [0, bits)
 // (tc_radix.hip).  k0/v0 hold the input (identity_values: v0 is not read,
 // the values are the indices 0..n-1); the result is in k0/v0 (*cur = 0) or

@@ -107,27 +107,6 @@ __device__ __forceinline__ void tb_weights(const Coefs<T>& C, const Tables<T>& T
     }
 }
 
-// Mirror writes of enforceBCs_stream for interior (i,k) into the staged
-// region with global origin (gi0, gk0); only cells inside it.
-template <int R, typename T>
-__device__ __forceinline__ void tb_ghosts(T* __restrict__ B, const Grid& g, int i, int k, int gi0, int gk0, T v, T zsg)
-{
-    const int nx = g.nx, nz = g.nz;
-    auto put = [&](int ii, int kk, T x) {
-        const int li = ii - gi0, lk = kk - gk0;
-        if (li >= 0 && li < R && lk >= 0 && lk < R) B[lk * R + li] = x;
-    };
-    const bool lo_i = i <= SPAN, hi_i = i >= nx - SPAN, lo_k = k <= SPAN, hi_k = k >= nz - SPAN;
-    if (lo_i) put(-i, k, -v);
-    if (hi_i) put(2 * nx - i, k, -v);
-    if (lo_k) put(i, -k, zsg * v);
-    if (hi_k) put(i, 2 * nz - k, zsg * v);
-    if (lo_i && lo_k) put(-i, -k, zsg * (-v));
-    if (lo_i && hi_k) put(-i, 2 * nz - k, zsg * (-v));
-    if (hi_i && lo_k) put(2 * nx - i, -k, zsg * (-v));
-    if (hi_i && hi_k) put(2 * nx - i, 2 * nz - k, zsg * (-v));
-}
-
 #pragma clang fp contract(fast)
 // The 13 dt-scaled tap weights of every interior cell of a level, once per
 // solve (the operator's coefficients are fixed during a solve): plane q of

@@ -338,6 +338,12 @@ module tracer_hip
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: interval
     end function
+    integer(c_int) function tc_set_gs_smoother(ctx, sweeps, per_launch, omega) bind(C, name="tc_set_gs_smoother")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: sweeps, per_launch
+      real(c_double), value :: omega
+    end function
     integer(c_int) function tc_tracer_slots(ctx, caller_index) bind(C, name="tc_tracer_slots")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: ctx

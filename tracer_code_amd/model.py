@@ -276,6 +276,12 @@ class Solver:
         """Sort the tracers at the start of every interval-th time step (0: never)."""
         self._check(self.lib.tc_set_tracer_sort(self.h, int(interval)), "set_tracer_sort")
 
+    def set_gs_smoother(self, sweeps, per_launch=2, omega=1.0):
+        """Multicolour Gauss-Seidel sweeps per level-1 visit of the mixed
+        mode's correction cycle (0: the Jacobi-Richardson sweeps)."""
+        self._check(self.lib.tc_set_gs_smoother(self.h, int(sweeps), int(per_launch), float(omega)),
+                    "set_gs_smoother")
+
     def tracer_slots(self):
         """The caller's index of the tracer in each device slot."""
         out = np.empty(self.ntr, dtype=np.int64)
