@@ -93,6 +93,7 @@ int dalloc(tc_ctx* ctx, double** p, size_t n)
     TC_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), ctx->stream));
     return 0;
 }
+int falloc(tc_ctx* ctx, float** p, size_t n);
 int dalloc_i(tc_ctx* ctx, int32_t** p, size_t n)
 {
     TC_HIP(hipMalloc((void**)p, n * sizeof(int32_t)));
@@ -425,12 +426,14 @@ int coarse_setup(tc_ctx* ctx, tc_dev& d, bool lagged)
         ctx->coarse_since = 0;
         if (ctx->next_pending) {
             std::swap(ctx->Ainv, ctx->Ainv_next);
+            std::swap(ctx->Ainv32, ctx->Ainv32_next);
             std::swap(ctx->RowB, ctx->RowB_next);
             std::swap(ctx->d_info, ctx->d_info_next);
             std::swap(ctx->ev_inv_ready, ctx->ev_inv_next);
         }
         if (!ctx->Ainv_next) {
             TC_TRY(dalloc(ctx, &ctx->Ainv_next, (size_t)ctx->ngc * ctx->ngc));
+            TC_TRY(falloc(ctx, &ctx->Ainv32_next, (size_t)ctx->ngc * ctx->ngc));
             TC_TRY(dalloc(ctx, &ctx->RowB_next, ctx->ngc));
             TC_HIP(hipMalloc((void**)&ctx->d_info_next, sizeof(int32_t)));
             TC_HIP(hipMemsetAsync(ctx->d_info_next, 0, sizeof(int32_t), ctx->stream));
@@ -449,6 +452,7 @@ int coarse_setup(tc_ctx* ctx, tc_dev& d, bool lagged)
                            ctx->ngc);
     launch_dgbtf2(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, info);
     launch_band_inverse(ctx->side, ctx->Mc, ctx->ldab, ctx->ngc, ctx->kl, ctx->ku, ctx->ipiv, Ainv, ctx->Ur);
+    launch_to_f32(ctx->side, Ainv, to_next ? ctx->Ainv32_next : ctx->Ainv32, (size_t)ctx->ngc * ctx->ngc);
     TC_HIP(hipEventRecord(to_next ? ctx->ev_inv_next : ctx->ev_inv_ready, ctx->side));
     ctx->next_pending = to_next;
     d.coarse_ready = true;
@@ -1095,6 +1099,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     const bool gcr = ctx->gcr_m > 0 && !ctx->gcr_suspended &&
                      !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
     if (gcr) TC_TRY(gcr_alloc(ctx));
+    bool coarse_b_ready = false;  // the coarse right-hand side was formed by the descent
     // one step p of the cycle schedule: the descent into the level, its
     // visit, and (level 1) the stop test or the ascent out of it
     auto step = [&](int p, bool& done, bool& brk) -> int {
@@ -1107,13 +1112,23 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             if (f32) {
                 if (level - 1 == 1) {
                     launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
+                } else if (level == G) {
+                    // the coarse solve's right-hand side is formed here, from
+                    // the equilibration of the inverse in use (ready after the wait)
+                    TC_TRY(wait_coarse(ctx, d));
+                    launch_residual_restrict32(ctx->stream, Lf, Lf.err32[Lf.cur32], Lf.rhs32, d.F, d.h[level - 1],
+                                               L.rhs32, L.g, ctx->RowB, ctx->coarse_b);
+                    ctx->stats.residuals_exec[level - 2]++;
+                    coarse_b_ready = true;
                 } else {
                     launch_residual_restrict32(ctx->stream, Lf, Lf.err32[Lf.cur32], Lf.rhs32, d.F, d.h[level - 1],
                                                L.rhs32, L.g);
                     ctx->stats.residuals_exec[level - 2]++;
                 }
-                // a fill kernel, not a memset: this can be recorded in a graph
-                if (!zero_start)
+                // a fill kernel, not a memset: this can be recorded in a graph.  The
+                // coarse solve writes every interior cell and ghost of its level
+                // (the walls stay 0), so it needs none.
+                if (!zero_start && level < G)
                     launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(L.err32[L.cur32]), L.g.elems);
             } else {
                 launch_restrict(ctx->stream, Lf.res, Lf.g, L.rhs, L.g);
@@ -1124,9 +1139,11 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
         } else {
             TC_TRY(wait_coarse(ctx, d));
-            if (f32)
-                launch_coarse_apply32(ctx->stream, ctx->Ainv, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg,
-                                      ctx->coarse_b);
+            if (f32) {
+                launch_coarse_apply32(ctx->stream, ctx->Ainv32, ctx->RowB, L, L.rhs32, L.err32[L.cur32], ctx->ngc, zsg,
+                                      ctx->coarse_b, coarse_b_ready);
+                coarse_b_ready = false;
+            }
             else launch_coarse_apply(ctx->stream, ctx->Ainv, ctx->RowB, L, L.err[L.cur], ctx->ngc, zsg);
             ctx->stats.coarse_solves++;
         }
@@ -1917,6 +1934,7 @@ void free_ctx(tc_ctx* ctx)
                    ctx->nn_scales, ctx->rec_scales, ctx->nn_part, ctx->guess_psi, ctx->guess_lo, ctx->nn_T,
                    ctx->nn_C, ctx->nn_p1, ctx->nn_p2, ctx->nn_ext, ctx->nn_psi, ctx->nn_inc, ctx->nn_ra, ctx->nn_rb};
     for (double* p : b) F(p);
+    F(ctx->Ainv32); F(ctx->Ainv32_next);
     for (int t = 0; t < TC_MAX_TYPES; ++t) { F(ctx->Cnew[t]); F(ctx->Cvis[t]); F(ctx->Cbuoy[t]); F(ctx->C0[t]); }
     for (int t = 0; t < 4; ++t) { F(ctx->sh1[t]); F(ctx->sh2[t]); F(ctx->Tc1[t]); }
     for (int t = 0; t <= TC_MAX_TYPES; ++t) F(ctx->count[t]);
@@ -2057,6 +2075,7 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_TRY(dalloc(ctx, &ctx->Mc, (size_t)ctx->ldab * ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->RowB, ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Ainv, (size_t)ctx->ngc * ctx->ngc));
+        TC_TRY(falloc(ctx, &ctx->Ainv32, (size_t)ctx->ngc * ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Bc, ctx->ngc));
         TC_TRY(dalloc(ctx, &ctx->Ur, (size_t)ctx->ngc * (size_t)(ctx->kl + ctx->ku + 1)));
         TC_TRY(dalloc_i(ctx, &ctx->ipiv, ctx->ngc));

@@ -299,12 +299,15 @@ __global__ __launch_bounds__(256) void k_restrict32(const float* __restrict__ in
 // 604-609, reads (ii-1,kk+1) twice and not (ii+1,kk-1)), each rounded to FP32
 // exactly as k_residual32 stores it, and combines them in k_restrict32's
 // order -- the same values as the two kernels, without the residual plane.
+// With bout (the coarsest level), also the equilibrated right-hand side of
+// the coarse solve at the interior unknowns, as k_coarse_rhs32 forms it.
 __global__ __launch_bounds__(256) void k_residual_restrict32(const float* __restrict__ S, const float* __restrict__ vis,
                                                              const float* __restrict__ vx, const float* __restrict__ vz,
                                                              const float* __restrict__ a1,
                                                              const float* __restrict__ vxz,
                                                              const float* __restrict__ rhs, Grid gi, FD32 F, H32 h,
-                                                             float* __restrict__ out, Grid go)
+                                                             float* __restrict__ out, Grid go,
+                                                             const double* __restrict__ RowB, double* __restrict__ bout)
 {
     const int2 tile = xcd_tile();
     const int i = tile.x * blockDim.x + threadIdx.x + 1;
@@ -319,7 +322,12 @@ __global__ __launch_bounds__(256) void k_residual_restrict32(const float* __rest
     const float rm0 = r(ii - 1, kk), r0m = r(ii, kk - 1), rp0 = r(ii + 1, kk), r0p = r(ii, kk + 1), r00 = r(ii, kk);
     float corners = (rmm + rmp + rpp + rmp) * (1.0f / 16.0f);
     float sides = (rm0 + r0m + rp0 + r0p) * (1.0f / 8.0f);
-    out[go.at(i, k)] = corners + sides + r00 * 0.25f;
+    const float v = corners + sides + r00 * 0.25f;
+    out[go.at(i, k)] = v;
+    if (bout) {
+        const int q = i * (go.nz + 1) + k;
+        bout[q] = (double)v * RowB[q];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_prolong32(float* __restrict__ Ef, Grid gf, const float* __restrict__ Ec,
@@ -347,16 +355,18 @@ __global__ __launch_bounds__(256) void k_coarse_rhs32(const double* __restrict__
     b[q] = (iq > 0 && iq < g.nx && kq > 0 && kq < g.nz) ? (double)rhs[g.at(iq, kq)] * RowB[q] : 0.0;
 }
 
-// Coarse solve from an FP32 right-hand side with the FP64 inverse, result
-// rounded to FP32: one workgroup per unknown, its row of the inverse times b.
-__global__ __launch_bounds__(256) void k_coarse_apply32(const double* __restrict__ Ainv, const double* __restrict__ b,
+// Coarse solve of the FP32 correction cycle: the explicit inverse rounded to
+// FP32 (half the bytes of the GEMV; the cycle's corrections are FP32), FP64
+// right-hand side and accumulation, result rounded to FP32: one workgroup
+// per unknown, its row of the inverse times b.
+__global__ __launch_bounds__(256) void k_coarse_apply32(const float* __restrict__ Ainv, const double* __restrict__ b,
                                                         float* __restrict__ E, Grid g, int n, float zsg)
 {
     const int N = g.nz + 1;
     const int ip = blockIdx.x + 1, kp = blockIdx.y + 1;
-    const double* row = Ainv + (size_t)(N * ip + kp) * (size_t)n;
+    const float* row = Ainv + (size_t)(N * ip + kp) * (size_t)n;
     double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += row[q] * b[q];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += (double)row[q] * b[q];
     acc = block_sum(acc);
     if (threadIdx.x == 0) {
         const float v = (float)acc;
@@ -678,11 +688,11 @@ void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* o
 }
 
 void launch_residual_restrict32(hipStream_t st, const Level& Lf, const float* S, const float* rhs, const FD& F,
-                                const Spacing& h, float* out, const Grid& go)
+                                const Spacing& h, float* out, const Grid& go, const double* RowB, double* bout)
 {
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_residual_restrict32, grid2(go.nx - 1, go.nz - 1, b), b, 0, st, S, Lf.vis32, Lf.visx32,
-                       Lf.visz32, Lf.a132, Lf.visxz32, rhs, Lf.g, fd32_from(F), h32_from(h), out, go);
+                       Lf.visz32, Lf.a132, Lf.visxz32, rhs, Lf.g, fd32_from(F), h32_from(h), out, go, RowB, bout);
 }
 
 void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg)
@@ -691,12 +701,26 @@ void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec
     hipLaunchKernelGGL(k_prolong32, grid2(gf.nx - 1, gf.nz - 1, b), b, 0, st, Ef, gf, Ec, gc, (float)zsg);
 }
 
-void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
-                           float* E, int ngc, double zsg, double* b)
+void launch_coarse_apply32(hipStream_t st, const float* Ainv, const double* RowB, const Level& L, const float* rhs,
+                           float* E, int ngc, double zsg, double* b, bool b_ready)
 {
-    hipLaunchKernelGGL(k_coarse_rhs32, dim3((unsigned)((ngc + 255) / 256)), dim3(256), 0, st, RowB, rhs, L.g, ngc, b);
+    if (!b_ready)
+        hipLaunchKernelGGL(k_coarse_rhs32, dim3((unsigned)((ngc + 255) / 256)), dim3(256), 0, st, RowB, rhs, L.g, ngc,
+                           b);
     hipLaunchKernelGGL(k_coarse_apply32, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, b, E, L.g, ngc,
                        (float)zsg);
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n)
+{
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
+        out[j] = (float)in[j];
+}
+
+void launch_to_f32(hipStream_t st, const double* in, float* out, size_t n)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_to_f32, dim3(blocks), dim3(256), 0, st, in, out, n);
 }
 
 void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis)

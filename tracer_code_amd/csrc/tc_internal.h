@@ -176,6 +176,8 @@ struct tc_ctx {
     // defect-correction modes: the factorisation of this solve's coarse
     // operator is computed into these while the solve uses the previous one
     double *RowB_next = nullptr, *Ainv_next = nullptr;
+    // the explicit inverses rounded to FP32, for the FP32 correction cycle
+    float *Ainv32 = nullptr, *Ainv32_next = nullptr;
     double* coarse_b = nullptr;   // equilibrated right-hand side of the FP32-cycle coarse solve
     int32_t* d_info_next = nullptr;
     bool next_pending = false;

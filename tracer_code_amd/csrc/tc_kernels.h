@@ -134,12 +134,16 @@ void launch_sweep32(hipStream_t st, bool use_ratio, const Level& L, const float*
 void launch_residual32(hipStream_t st, const Level& L, const float* S, float* R, const float* rhs, const FD& F,
                        const Spacing& h);
 void launch_restrict32(hipStream_t st, const float* in, const Grid& gi, float* out, const Grid& go);
+// with RowB and bout (descent into the coarsest level): also the coarse
+// solve's equilibrated right-hand side, as launch_coarse_apply32 forms it
 void launch_residual_restrict32(hipStream_t st, const Level& Lf, const float* S, const float* rhs, const FD& F,
-                                const Spacing& h, float* out, const Grid& go);
+                                const Spacing& h, float* out, const Grid& go, const double* RowB = nullptr,
+                                double* bout = nullptr);
 void launch_prolong32(hipStream_t st, float* Ef, const Grid& gf, const float* Ec, const Grid& gc, double zsg);
-void launch_coarse_apply32(hipStream_t st, const double* Ainv, const double* RowB, const Level& L, const float* rhs,
-                           float* E, int ngc, double zsg,
-                           double* b);
+// b_ready: b already holds the right-hand side (launch_residual_restrict32)
+void launch_coarse_apply32(hipStream_t st, const float* Ainv, const double* RowB, const Level& L, const float* rhs,
+                           float* E, int ngc, double zsg, double* b, bool b_ready);
+void launch_to_f32(hipStream_t st, const double* in, float* out, size_t n);
 void launch_coeffs_to_f32(hipStream_t st, Level& L, bool with_lb, int ivis);
 
 // --- tc_tb.hip ------------------------------------------------------------------
