@@ -78,6 +78,14 @@ struct tc_dev {
     unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (rotating slot sets)
     double* fbuf = nullptr;    // third iterate buffer of the blocked C filter loop
     int32_t* h_ctl = nullptr;  // pinned mirror
+    // device loop control of smoother_vis (k_vis_decide): state, factor and
+    // ratio, their pinned mirrors, and the pinned initial state
+    int32_t* vctl = nullptr;
+    double* vdctl = nullptr;
+    int32_t* h_vctl = nullptr;
+    double* h_vdctl = nullptr;
+    int32_t* h_vinit = nullptr;
+    int vis_hint[TC_MAX_LEVELS + 1][2] = {};  // iterations the last call of (level, interpolate) took
     std::vector<double> hbuf;  // host staging plane
 };
 
@@ -212,8 +220,9 @@ int grad_eig_min(tc_ctx* ctx, tc_dev& d, int level, double* rmin)
     return 0;
 }
 
-// smoother_vis (smoother.f90:165-263)
-int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
+// smoother_vis (smoother.f90:165-263), host-controlled: one read-back of
+// the ratio minimum per iteration (passes longer than VIS_CMAX)
+int smoother_vis_host(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
 {
     Level& L = ctx->lv[level];
     const tc_params& p = ctx->p;
@@ -260,6 +269,96 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
             ctx->err = "smoother_vis: eigenvalue-ratio test not met after 100000 passes";
             return TC_ERR_HIP;
         }
+    }
+    return 0;
+}
+
+// smoother_vis (smoother.f90:165-263) with the loop control on the device
+// (k_vis_decide): the host queues iterations -- the gradients and
+// eigenvalue ratios, the exit test, the copy, the Chebyshev sweep and the
+// boundary values -- in batches sized by the iterations the previous call
+// on this level took, and reads the loop state once per batch; queued
+// iterations after the exit do nothing.  The Chebyshev factors are the host
+// table of the same formula, so the sweeps see the same values.
+int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
+{
+    Level& L = ctx->lv[level];
+    const tc_params& p = ctx->p;
+    int option, Niter;
+    double lb, la;
+    if (level == 1 && interpolate == 0) { option = 1; Niter = 6; lb = -5.0; la = -2.0; }
+    else { option = 0; Niter = p.vis_iter; lb = -2.0; la = -1.0; }
+    if (Niter > VIS_CMAX || p.vis_iter > VIS_CMAX) return smoother_vis_host(ctx, d, level, interpolate);
+    if (interpolate == 0 || level == 1) { ctx->Nvis_store = 0; ctx->nn_store = 1; }
+    int kmin, kmax;
+    if (p.RaT == 0.0) { kmin = 0; kmax = L.g.nz; } else { kmin = 1; kmax = L.g.nz - 1; }
+    const Grid* gf = level > 1 ? &ctx->lv[level - 1].g : &L.g;
+    const double* Vf = level > 1 ? ctx->lv[level - 1].vis : nullptr;
+    auto cfac = [](int N, int nn, double lbv, double lav) {
+        const int ntemp = N - nn + 1;
+        return 2.0 / (-lbv - lav + (lbv - lav) * std::cos((double)(2 * ntemp - 1) * PII / (double)(2 * N)));
+    };
+    VisDecide P{};
+    P.option = option;
+    P.interpolate = interpolate;
+    P.ivis = p.ivis;
+    P.eig_ratio_min = p.eig_ratio_min;
+    P.eig_ratio = p.eig_ratio;
+    const int N1 = (option == 1) ? 6 : Niter;  // the passes after the first (option 1) or every pass
+    for (int nn = 1; nn <= Niter; ++nn) P.c0[nn - 1] = cfac(Niter, nn, lb, la);
+    for (int nn = 1; nn <= N1; ++nn) P.c1[nn - 1] = cfac(N1, nn, -2.0, -1.0);
+    // the state at entry; a first pass that starts beyond its end is only
+    // the pass bookkeeping (the reference's inner loop does not run)
+    int nn0 = ctx->nn_store, n0 = 0, Nvs = ctx->Nvis_store, tab = (option == 1) ? 0 : 1, Ncur = Niter;
+    while (nn0 > Ncur) {
+        ++Nvs;
+        nn0 = 1;
+        if (option == 1) { Ncur = 6; tab = 1; }
+        ++n0;
+    }
+    int32_t* h = d.h_vinit;
+    for (int j = 0; j < VC_COUNT; ++j) h[j] = 0;
+    h[VC_NN] = nn0; h[VC_N] = n0; h[VC_NITER] = Ncur; h[VC_NVIS] = Nvs; h[VC_NNSTORE] = (nn0 != ctx->nn_store) ? 1 : ctx->nn_store;
+    h[VC_TAB] = tab;
+    TC_HIP(hipMemcpyAsync(d.vctl, h, VC_COUNT * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    ctx->ratio_level = (p.ivis == 0) ? -1 : level;
+    const EigScalars e = eig_scalars(p, L);
+    int batch = std::max(1, d.vis_hint[level][interpolate ? 1 : 0]);
+    for (int queued = 0;;) {
+        for (int b = 0; b < batch; ++b) {
+            const int nb = launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1,
+                                               d.partialA, d.vctl + VC_DONE);
+            launch_vis_decide(ctx->stream, d.partialA, nb, P, d.vctl, d.vdctl);
+            launch_copy_gated(ctx->stream, L.vtmp, L.vis, L.g.elems, d.vctl + VC_NOSWEEP);
+            launch_vis_sweep(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax, 0.0,
+                             d.vdctl + VD_C, d.vctl + VC_NOSWEEP);
+            launch_visc_bcs(ctx->stream, L.vis, L.g, Vf, *gf, level, 0, p.RaT, p.visT * p.visT, d.vctl + VC_NOSWEEP);
+        }
+        queued += batch;
+        TC_HIP(hipGetLastError());
+        TC_HIP(hipMemcpyAsync(d.h_vctl, d.vctl, VC_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipMemcpyAsync(d.h_vdctl, d.vdctl, VD_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        if (d.h_vctl[VC_ERR]) {
+            ctx->err = "smoother_vis: eigenvalue-ratio test not met after 100000 passes";
+            return TC_ERR_HIP;
+        }
+        if (d.h_vctl[VC_DONE]) break;
+        // the reference loops until the test passes; a NaN ratio never passes it
+        if (queued > 100000 * (VIS_CMAX + 1)) {
+            ctx->err = "smoother_vis: eigenvalue-ratio test not met";
+            return TC_ERR_HIP;
+        }
+        batch = std::max(2, batch / 2);
+    }
+    const int iters = d.h_vctl[VC_ITERS];
+    d.vis_hint[level][interpolate ? 1 : 0] = iters;
+    ctx->stats.vis_iters += iters;
+    ctx->Nvis_store = d.h_vctl[VC_NVIS];
+    ctx->nn_store = d.h_vctl[VC_NNSTORE];
+    if (option == 1) {
+        ctx->stats.eig_fine = d.h_vdctl[VD_EIG];
+        ctx->stats.n_eta = d.h_vctl[VC_NETA];
     }
     return 0;
 }
@@ -1962,8 +2061,12 @@ void free_ctx(tc_ctx* ctx)
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
         F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->fbuf); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
+        F(d->vctl); F(d->vdctl);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
+        if (d->h_vctl) (void)hipHostFree(d->h_vctl);
+        if (d->h_vdctl) (void)hipHostFree(d->h_vdctl);
+        if (d->h_vinit) (void)hipHostFree(d->h_vinit);
         delete d;
     }
     for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
@@ -2101,6 +2204,11 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_HIP(hipMalloc((void**)&d->fslots, filter_slot_words() * sizeof(unsigned long long)));
         TC_TRY(dalloc(ctx, &d->fbuf, g.elems));
         TC_HIP(hipHostMalloc((void**)&d->h_ctl, 4 * sizeof(int32_t), hipHostMallocDefault));
+        TC_TRY(dalloc_i(ctx, &d->vctl, VC_COUNT));
+        TC_TRY(dalloc(ctx, &d->vdctl, VD_COUNT));
+        TC_HIP(hipHostMalloc((void**)&d->h_vctl, VC_COUNT * sizeof(int32_t), hipHostMallocDefault));
+        TC_HIP(hipHostMalloc((void**)&d->h_vdctl, VD_COUNT * sizeof(double), hipHostMallocDefault));
+        TC_HIP(hipHostMalloc((void**)&d->h_vinit, VC_COUNT * sizeof(int32_t), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_scal, 16 * sizeof(double), hipHostMallocDefault));
     }
     // grid-dependent tables: dt_vis per level, dts_array

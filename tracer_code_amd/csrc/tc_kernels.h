@@ -81,12 +81,26 @@ void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, d
 void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
                  const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC);
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
-                        const EigScalars& e, int want_ratio, double* partial);
+                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate = nullptr);
 void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream);
+// cptr / gate (device-controlled smoother_vis): the factor read from the
+// device, and the launch skipped while gate[0] is set
 void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
-                      const FD& F, const Spacing& h, int kmin, int kmax, double c);
+                      const FD& F, const Spacing& h, int kmin, int kmax, double c, const double* cptr = nullptr,
+                      const int32_t* gate = nullptr);
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
-                     double RaT, double visT2);
+                     double RaT, double visT2, const int32_t* gate = nullptr);
+void launch_copy_gated(hipStream_t st, double* dst, const double* src, size_t n, const int32_t* gate);
+// smoother_vis's loop state on the device (k_vis_decide)
+enum { VC_DONE = 0, VC_ERR, VC_NOSWEEP, VC_NN, VC_N, VC_NITER, VC_NVIS, VC_NNSTORE, VC_TAB, VC_NETA, VC_ITERS, VC_COUNT };
+enum { VD_C = 0, VD_EIG, VD_COUNT };
+constexpr int VIS_CMAX = 32;  // longest pass (Niter) the device control takes
+struct VisDecide {
+    int option, interpolate, ivis;
+    double eig_ratio_min, eig_ratio;
+    double c0[VIS_CMAX], c1[VIS_CMAX];  // factors of the first pass (option 1) and of the later / only passes
+};
+void launch_vis_decide(hipStream_t st, const double* partial, int nb, const VisDecide& P, int32_t* ctl, double* dctl);
 
 // --- tc_coarse.hip -------------------------------------------------------------
 void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Level& L, const FD& F, const Spacing& h,

@@ -226,8 +226,10 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
                                                       double* __restrict__ vz, double* __restrict__ vxx,
                                                       double* __restrict__ vzz, double* __restrict__ vxz,
                                                       double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
-                                                      EigScalars e, int want_ratio, double* __restrict__ partial)
+                                                      EigScalars e, int want_ratio, double* __restrict__ partial,
+                                                      const int32_t* __restrict__ gate)
 {
+    if (gate && gate[0]) return;  // the device-controlled loop has ended
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     double m = INFINITY;
@@ -303,8 +305,11 @@ __global__ __launch_bounds__(256) void k_local_dt(const double* __restrict__ V, 
 template <bool POW2>
 __global__ __launch_bounds__(256) void k_vis_sweep(const double* __restrict__ T0, double* __restrict__ V,
                                                    const double* __restrict__ dtv, Grid g, FD F, Spacing h, int kmin,
-                                                   int kmax, double c)
+                                                   int kmax, double c_arg, const double* __restrict__ cptr,
+                                                   const int32_t* __restrict__ gate)
 {
+    if (gate && gate[0]) return;
+    const double c = cptr ? cptr[0] : c_arg;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y * blockDim.y + threadIdx.y + kmin;
     if (i > g.nx || k > kmax) return;
@@ -338,8 +343,10 @@ __device__ __forceinline__ double visc_val1(const double* __restrict__ V, const 
 }
 
 __global__ __launch_bounds__(256) void k_visc_bcs(double* __restrict__ V, Grid g, const double* __restrict__ Vf,
-                                                  Grid gf, int level, int initialize, double RaT, double visT2)
+                                                  Grid gf, int level, int initialize, double RaT, double visT2,
+                                                  const int32_t* __restrict__ gate)
 {
+    if (gate && gate[0]) return;
     const int nx = g.nx, nz = g.nz;
     const int nA = (level > 1) ? 2 * (nx + 1) + (initialize ? 2 * (nz - 1) : 0) : 0;  // phase 1 cells
     const int nB = 2 * SPAN * (nz + 1);                                                // side ghosts
@@ -465,16 +472,16 @@ void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& 
 }
 
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
-                        const EigScalars& e, int want_ratio, double* partial)
+                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     if (pow2)
         hipLaunchKernelGGL((k_vis_grad_eig<true>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate);
     else
         hipLaunchKernelGGL((k_vis_grad_eig<false>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate);
     return (int)(gr.x * gr.y);
 }
 
@@ -486,23 +493,103 @@ void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double coura
 }
 
 void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
-                      const FD& F, const Spacing& h, int kmin, int kmax, double c)
+                      const FD& F, const Spacing& h, int kmin, int kmax, double c, const double* cptr,
+                      const int32_t* gate)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(g.nx + 1, kmax - kmin + 1, b);
     if (pow2)
-        hipLaunchKernelGGL((k_vis_sweep<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c);
+        hipLaunchKernelGGL((k_vis_sweep<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c, cptr, gate);
     else
-        hipLaunchKernelGGL((k_vis_sweep<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c);
+        hipLaunchKernelGGL((k_vis_sweep<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c, cptr, gate);
 }
 
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
-                     double RaT, double visT2)
+                     double RaT, double visT2, const int32_t* gate)
 {
     const int n = ((level > 1) ? 2 * (g.nx + 1) + (init ? 2 * (g.nz - 1) : 0) : 0) + 2 * SPAN * (g.nz + 1) +
                   2 * SPAN * (g.nx + 2 * SPAN + 1);
     hipLaunchKernelGGL(k_visc_bcs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, V, g, Vf, gf, level, init, RaT,
-                       visT2);
+                       visT2, gate);
+}
+
+// A plane copy that a device-controlled loop can switch off.
+__global__ __launch_bounds__(256) void k_copy_gated(double* __restrict__ dst, const double* __restrict__ src, size_t n,
+                                                    const int32_t* __restrict__ gate)
+{
+    if (gate[0]) return;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
+        dst[j] = src[j];
+}
+
+void launch_copy_gated(hipStream_t st, double* dst, const double* src, size_t n, const int32_t* gate)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_copy_gated, dim3(blocks), dim3(256), 0, st, dst, src, n, gate);
+}
+
+// The loop control of smoother_vis (smoother.f90:165-263) on the device, one
+// iteration per launch: the minimum of the per-block eigenvalue ratios (as
+// k_reduce_final), the exit test of the pass, and either the stop (ctl[0],
+// with no sweep: ctl[2]) or the Chebyshev factor of this iteration's sweep
+// (dctl[0]) and the advance to the next iteration, with the pass
+// bookkeeping at the end of a pass.  Once stopped, later launches change
+// nothing, so the host may queue more iterations than the loop needs.
+__global__ __launch_bounds__(256) void k_vis_decide(const double* __restrict__ partial, int nb, VisDecide P,
+                                                    int32_t* __restrict__ ctl, double* __restrict__ dctl)
+{
+    double a = INFINITY;
+    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+        const double v = partial[t];
+        a = v < a ? v : a;
+    }
+    a = block_min(a);
+    if (threadIdx.x != 0) return;
+    if (ctl[VC_DONE]) {
+        ctl[VC_NOSWEEP] = 1;
+        return;
+    }
+    const double rmin = (P.ivis == 0) ? 1e99 : a;
+    ctl[VC_ITERS] += 1;
+    const int nn = ctl[VC_NN], n = ctl[VC_N], Nvs = ctl[VC_NVIS];
+    bool trig;
+    if (P.option == 1) trig = (Nvs >= 1 && rmin > P.eig_ratio_min) || P.ivis == 0 || n > 20;
+    else if (P.interpolate == 0) trig = rmin > P.eig_ratio || P.ivis == 0;
+    else trig = Nvs >= 1 || P.ivis == 0;
+    if (trig) {
+        ctl[VC_DONE] = 1;
+        ctl[VC_NOSWEEP] = 1;
+        if (P.option == 1) {
+            dctl[VD_EIG] = rmin;
+            ctl[VC_NETA] = n;
+        } else if (P.interpolate == 0) {
+            ctl[VC_NNSTORE] = nn;
+        }
+        return;
+    }
+    dctl[VD_C] = (ctl[VC_TAB] == 0) ? P.c0[nn - 1] : P.c1[nn - 1];
+    ctl[VC_NOSWEEP] = 0;
+    int next = nn + 1;
+    if (next > ctl[VC_NITER]) {  // end of the pass
+        ctl[VC_NVIS] = Nvs + 1;
+        ctl[VC_NNSTORE] = 1;
+        if (P.option == 1) {
+            ctl[VC_NITER] = 6;
+            ctl[VC_TAB] = 1;
+        }
+        ctl[VC_N] = n + 1;
+        next = 1;
+        if (n + 1 > 100000) {
+            ctl[VC_ERR] = 1;
+            ctl[VC_DONE] = 1;
+        }
+    }
+    ctl[VC_NN] = next;
+}
+
+void launch_vis_decide(hipStream_t st, const double* partial, int nb, const VisDecide& P, int32_t* ctl, double* dctl)
+{
+    hipLaunchKernelGGL(k_vis_decide, dim3(1), dim3(256), 0, st, partial, nb, P, ctl, dctl);
 }
 
 }  // namespace tc
