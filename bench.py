@@ -477,15 +477,21 @@ def main():
             # the timed stretch (its rounding floor grows with |psi|); a leg
             # that stalls is redone with the next one from the start
             tols = [t for t in FP64_TOLERANCES if t >= params.tolerance] if params.nx >= 2048 else [params.tolerance]
+            # a preset that keeps the reference's unzeroed coarse corrections
+            # (fixes=0) is retried with them zeroed where that cycle diverges
+            attempts = [(tol, params.fixes) for tol in tols]
+            if params.fixes == 0:
+                attempts += [(tol, 1) for tol in tols]
             failed = []
-            for tol in tols:
-                p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol, max_cycles=FP64_MAX_CYCLES))
+            for tol, fixes in attempts:
+                p64, _ = config.preset_params(args.config, **dict(ov, tolerance=tol, max_cycles=FP64_MAX_CYCLES,
+                                                                   fixes=fixes))
                 s2, n2, _ = make_solver(p64, inputs, "fp64", "plain", 0, args.config, sort_interval=args.sort_interval)
                 try:
                     res = run_timed(s2, p64, 1, args.leg_steps, "fp64", until_time=t_head)
                 except TcError as e:
-                    log(f"fp64 at {tol:g}: {e}")
-                    failed.append(tol)
+                    log(f"fp64 at {tol:g}, fixes={fixes}: {e}")
+                    failed.append([tol, fixes])
                     close_solver(s2, n2)
                     continue
                 T, SF = s2.get("T"), s2.get("SF")
