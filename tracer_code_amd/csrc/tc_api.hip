@@ -441,7 +441,7 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
     }
     const bool tb = d.tb && level >= 2 && ctx->mode != TC_MODE_FP64;
     // sweeps per launch: 4, or 8 on FP32 levels of at most 16 tiles
-    const int K = (tb && f32 && L.tb_tile != 32) ? tb_sweeps_per_launch(L.g) : 4;
+    const int K = (tb && f32 && L.tb_tile != 32 && L.tb_tile != 48) ? tb_sweeps_per_launch(L.g) : 4;
     // level-1 timing (tc_set_profiling): one event pair around the visit's
     // back-to-back launches, or around each launch when the first sweep is
     // followed by the wall reset of the reference form
@@ -2361,7 +2361,8 @@ int tc_set_dc_sweeps(tc_ctx* ctx, int32_t sweeps)
 
 int tc_set_tb_tile(tc_ctx* ctx, int32_t level, int32_t tile)
 {
-    if (!ctx || level < 2 || level >= ctx->grids || (tile != 0 && tile != 16 && tile != 32)) return TC_ERR_ARG;
+    if (!ctx || level < 2 || level >= ctx->grids || (tile != 0 && tile != 16 && tile != 32 && tile != 48))
+        return TC_ERR_ARG;
     ctx->lv[level].tb_tile = tile;
     return 0;
 }
@@ -2379,12 +2380,20 @@ int tc_autotune_tb_tiles(tc_ctx* ctx, int32_t steps_each, int32_t* tiles)
     for (int L = 2; L < ctx->grids; ++L) {
         Level& lv = ctx->lv[L];
         const int keep = lv.tb_tile;
-        double ms16 = 0.0, ms32 = 0.0;
-        lv.tb_tile = 16;
-        TC_TRY(step_ms(&ms16));
-        lv.tb_tile = 32;
-        TC_TRY(step_ms(&ms32));
-        lv.tb_tile = (ms32 < 0.98 * ms16) ? 32 : (ms16 < 0.98 * ms32 ? 16 : keep);
+        // 16, 32 and 48 (where it still gives 128 workgroups); a candidate
+        // replaces the kept tile only when it is 2% faster
+        const int cand[3] = {16, 32, 48};
+        double best_ms = 0.0;
+        int best = keep;
+        for (int c = 0; c < 3; ++c) {
+            const dim3 t = dim3((lv.g.nx - 1 + cand[c] - 1) / cand[c], (lv.g.nz - 1 + cand[c] - 1) / cand[c]);
+            if (cand[c] == 48 && t.x * t.y < 128) continue;
+            double ms = 0.0;
+            lv.tb_tile = cand[c];
+            TC_TRY(step_ms(&ms));
+            if (c == 0 || ms < 0.98 * best_ms) { best_ms = ms; best = cand[c]; }
+        }
+        lv.tb_tile = best;
     }
     if (tiles)
         for (int L = 0; L < TC_MAX_LEVELS; ++L)

@@ -300,6 +300,15 @@ static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, cons
                       int tile = 0, const Prolong<T>& pr = Prolong<T>{})
 {
     if constexpr (sizeof(T) == 4) {
+        // 48x48 tiles (1.78x staged cells per tile cell against 2.25x for 32x32), precomputed weights only
+        if (tile == 48 && K == 4 && W) {
+            const dim3 gr = tb_grid(g, 48), b(TB_THREADS);
+            if (use_ratio)
+                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+            else
+                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+            return;
+        }
         if ((tile ? tile : tb_tile<T>(g)) == 32 && K == 4) {
             tb_launch_t<T, 32>(st, use_ratio, K, g, S0, E, C, tb, sf, nk, zsg, W, pr);
             return;
