@@ -564,9 +564,10 @@ __global__ void k_zero_i32(int32_t* __restrict__ p, size_t n)
 
 // tracers_to_corners counting loop (composition.f90:704-715): nearest node
 // by Fortran NINT (half away from zero) of r/dr and s/ds, domain check.
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ r, const double* __restrict__ s,
                                              const int32_t* __restrict__ ty, int64_t n, CntArr cnt, Grid g, double dr,
-                                             double ds, double aspect, int32_t* __restrict__ flag)
+                                             double ds, double idr, double ids, double aspect, int32_t* __restrict__ flag)
 {
     const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // key of the (type, node) counter this tracer adds to; none = ~0
@@ -580,8 +581,8 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ r, const
         if (!(rv > 0.0 && rv < aspect && sv > 0.0 && sv < 1.0)) {
             atomicOr(flag, 1);
         } else {
-            const int i = (int)round(rv / dr);
-            const int k = (int)round(sv / ds);
+            const int i = (int)round(sdiv<POW2>(rv, dr, idr));
+            const int k = (int)round(sdiv<POW2>(sv, ds, ids));
             node = g.at(i, k);
             t = ty[id];
             key = (uint32_t)node * (uint32_t)(TC_MAX_TYPES + 1) + (uint32_t)t;
@@ -689,13 +690,17 @@ __global__ __launch_bounds__(256) void k_derivs(const double* __restrict__ u, co
     Dw[c] = b / dr;
 }
 
-// interpolate (composition.f90:838-898), order=2.
+// interpolate (composition.f90:838-898), order=2.  POW2: dr and ds are
+// powers of two, so the quotients are the products by the reciprocals
+// (idr, ids), bit for bit.
+template <bool POW2>
 __device__ __forceinline__ void interp_uw(double r, double s, const double* __restrict__ u, const double* __restrict__ w,
                                           const double* __restrict__ Du, const double* __restrict__ Dw, const Grid& g,
-                                          const D3Arr& D, double dr, double ds, double& ou, double& ow)
+                                          const D3Arr& D, double dr, double ds, double idr, double ids, double& ou,
+                                          double& ow)
 {
-    int i = (int)(r / dr);
-    int k = (int)(s / ds);
+    int i = (int)sdiv<POW2>(r, dr, idr);
+    int k = (int)sdiv<POW2>(s, ds, ids);
     // in-domain positions give 0<=i<=nx-1, 0<=k<=nz-1; the clamp only keeps
     // the gathers inside the ghost ring for invalid input
     i = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
@@ -716,26 +721,28 @@ __device__ __forceinline__ void interp_uw(double r, double s, const double* __re
     if (hs <= 0.5 * ds) {
         double au = D.v[0] * fu[0]; au += D.v[1] * fu[1]; au += D.v[2] * fu[2];
         double aw = D.v[0] * fw[0]; aw += D.v[1] * fw[1]; aw += D.v[2] * fw[2];
-        ou = fu[1] + (au / ds) * hs;
-        ow = fw[1] + (aw / ds) * hs;
+        ou = fu[1] + sdiv<POW2>(au, ds, ids) * hs;
+        ow = fw[1] + sdiv<POW2>(aw, ds, ids) * hs;
     } else {
         hs = ds - hs;
         double au = D.v[0] * fu[1]; au += D.v[1] * fu[2]; au += D.v[2] * fu[3];
         double aw = D.v[0] * fw[1]; aw += D.v[1] * fw[2]; aw += D.v[2] * fw[3];
-        ou = fu[2] + (au / ds) * (-hs);
-        ow = fw[2] + (aw / ds) * (-hs);
+        ou = fu[2] + sdiv<POW2>(au, ds, ids) * (-hs);
+        ow = fw[2] + sdiv<POW2>(aw, ds, ids) * (-hs);
     }
 }
 
 // RK stage `stage` of the tracer loops (energy.f90:182-225, 270-312,
 // 355-378, 420-440), tracer=0, uniform grids, global dt.
+template <bool POW2>
 __global__ __launch_bounds__(256) void k_tracer_stage(int stage, int torder, double dt, int64_t n, double* __restrict__ r,
                                                       double* __restrict__ s, double* __restrict__ r0,
                                                       double* __restrict__ s0, double* __restrict__ r1,
                                                       double* __restrict__ s1, double* __restrict__ r2,
                                                       double* __restrict__ s2, const double* __restrict__ u,
                                                       const double* __restrict__ w, const double* __restrict__ Du,
-                                                      const double* __restrict__ Dw, Grid g, D3Arr D, double dr, double ds)
+                                                      const double* __restrict__ Dw, Grid g, D3Arr D, double dr, double ds,
+                                                      double idr, double ids)
 {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
@@ -743,13 +750,13 @@ __global__ __launch_bounds__(256) void k_tracer_stage(int stage, int torder, dou
     if (stage == 1) {
         const double x0 = r[id], z0 = s[id];  // rtr0 = rtr (energy.f90:147-148)
         r0[id] = x0; s0[id] = z0;
-        interp_uw(x0, z0, u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        interp_uw<POW2>(x0, z0, u, w, Du, Dw, g, D, dr, ds, idr, ids, tu, tw);
         const double dtstage = (torder == 2) ? dt : 0.5 * dt;
         r[id] = tu; s[id] = tw;
         r1[id] = x0 + dtstage * tu;
         s1[id] = z0 + dtstage * tw;
     } else if (stage == 2) {
-        interp_uw(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        interp_uw<POW2>(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, idr, ids, tu, tw);
         const double dtstage = 0.5 * dt;
         if (torder == 2) {
             double a = r[id] + tu, b = s[id] + tw;
@@ -762,13 +769,13 @@ __global__ __launch_bounds__(256) void k_tracer_stage(int stage, int torder, dou
             s2[id] = s0[id] + dtstage * tw;
         }
     } else if (stage == 3) {
-        interp_uw(r2[id], s2[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        interp_uw<POW2>(r2[id], s2[id], u, w, Du, Dw, g, D, dr, ds, idr, ids, tu, tw);
         r[id] = r[id] + 2.0 * tu;
         r1[id] = r0[id] + dt * tu;
         s[id] = s[id] + 2.0 * tw;
         s1[id] = s0[id] + dt * tw;
     } else {
-        interp_uw(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, tu, tw);
+        interp_uw<POW2>(r1[id], s1[id], u, w, Du, Dw, g, D, dr, ds, idr, ids, tu, tw);
         const double dto6 = dt / 6.0;
         r[id] = r0[id] + dto6 * (r[id] + tu);
         s[id] = s0[id] + dto6 * (s[id] + tw);
@@ -1112,8 +1119,14 @@ void launch_bin(hipStream_t st, const double* r, const double* s, const int32_t*
                 const Grid& g, double dr, double ds, double aspect, int32_t* flag)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, s, ty, n, cnt, g, dr, ds, aspect,
-                       flag);
+    int e;
+    const bool pow2 = std::frexp(dr, &e) == 0.5 && std::frexp(ds, &e) == 0.5;
+    if (pow2)
+        hipLaunchKernelGGL((k_bin<true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, s, ty, n, cnt, g, dr, ds,
+                           1.0 / dr, 1.0 / ds, aspect, flag);
+    else
+        hipLaunchKernelGGL((k_bin<false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, r, s, ty, n, cnt, g, dr,
+                           ds, 1.0 / dr, 1.0 / ds, aspect, flag);
 }
 
 void launch_ratio(hipStream_t st, const CntArr& cnt, int32_t* nempty, const PtrArr& Cnew, const Grid& g, int ntype,
@@ -1145,8 +1158,14 @@ void launch_tracer_stage(hipStream_t st, int stage, int torder, double dt, int64
                          const double* Du, const double* Dw, const Grid& g, const D3Arr& Dint, double dr, double ds)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_tracer_stage, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, torder, dt, n, r, s,
-                       r0, s0, r1, s1, r2, s2, u, w, Du, Dw, g, Dint, dr, ds);
+    int e;
+    const bool pow2 = std::frexp(dr, &e) == 0.5 && std::frexp(ds, &e) == 0.5;
+    if (pow2)
+        hipLaunchKernelGGL((k_tracer_stage<true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, torder, dt,
+                           n, r, s, r0, s0, r1, s1, r2, s2, u, w, Du, Dw, g, Dint, dr, ds, 1.0 / dr, 1.0 / ds);
+    else
+        hipLaunchKernelGGL((k_tracer_stage<false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, torder,
+                           dt, n, r, s, r0, s0, r1, s1, r2, s2, u, w, Du, Dw, g, Dint, dr, ds, 1.0 / dr, 1.0 / ds);
 }
 
 void launch_energy_stage(hipStream_t st, int stage, double dt, double* T, const double* T0, double* T1, double* T2,
