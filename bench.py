@@ -64,6 +64,7 @@ SWEEP_KERNEL = {"fp64": ("k_sweep", "sweep64"), "dc64": ("k_sweep", "sweep64"), 
 FP64_TOLERANCES = (1e-7, 3e-7, 1e-6)
 FP64_MAX_CYCLES = 200
 FILTER_BLOCKING = [True]  # several C-filter iterations per launch (--filter-blocking)
+TUNE_STEPS = 2  # real time steps timed per autotuner candidate (one step's time varies by ~2%)
 
 
 def log(msg):
@@ -149,20 +150,21 @@ def tune_counts(solver, params, mode, args):
     common, coarse, level2 = params.Nmulti, 0, 0
     if args.dc_sweeps is None:
         t0 = time.perf_counter()
-        common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+        common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, TUNE_STEPS)
         log(f"{mode}: autotuned sweeps per visit: {common} ({time.perf_counter() - t0:.1f}s; ms/step {rec['tune']})")
         if params.grids > 3:
             t0 = time.perf_counter()
-            coarse, rec["coarse_tune"] = solver.autotune_dc_level_sweeps(3, params.grids - 1, COARSE_SWEEP_CANDIDATES, 1)
-            level2, rec["level2_tune"] = solver.autotune_dc_level_sweeps(2, 2, LEVEL2_SWEEP_CANDIDATES, 1)
+            coarse, rec["coarse_tune"] = solver.autotune_dc_level_sweeps(3, params.grids - 1, COARSE_SWEEP_CANDIDATES,
+                                                                           TUNE_STEPS)
+            level2, rec["level2_tune"] = solver.autotune_dc_level_sweeps(2, 2, LEVEL2_SWEEP_CANDIDATES, TUNE_STEPS)
             log(f"{mode}: levels >= 3: {coarse or common}, level 2: {level2 or common} "
                 f"({time.perf_counter() - t0:.1f}s; ms/step {rec['coarse_tune']}, {rec['level2_tune']})")
             t0 = time.perf_counter()
-            common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, 1)
+            common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, TUNE_STEPS)
             log(f"{mode}: levels 1-2 tuned again: {common} ({time.perf_counter() - t0:.1f}s; ms/step {rec['tune']})")
         if mode == "mixed":
             t0 = time.perf_counter()
-            rec["tiles"] = solver.autotune_tb_tiles(1)
+            rec["tiles"] = solver.autotune_tb_tiles(TUNE_STEPS)
             log(f"{mode}: autotuned tiles by level: {rec['tiles']} ({time.perf_counter() - t0:.1f}s)")
     else:
         common = args.dc_sweeps
