@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_POINT = {"fp64": 88.0, "dc64": 88.0, "mixed": 44.0}      # algorithmic
 KERNEL_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}     # what the kernels move
 TRACER_BYTES_PER_STEP = 304.0 + 5 * 20.0  # RK4 stages + five binnings, per tracer
-DC_SWEEP_CANDIDATES = (28, 32, 36, 40, 48, 56, 70)
+DC_SWEEP_CANDIDATES = (16, 20, 24, 28, 32, 40, 48, 70)
 # sweeps per visit on levels 3 and below, then on level 2 (0: the common count), tuned after it
 COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
 LEVEL2_SWEEP_CANDIDATES = (8, 12, 16, 24, 0)

@@ -20,10 +20,10 @@ pytestmark = pytest.mark.gpu
 STEPS = 3
 
 
-# the measured form of the 2048x1024 benchmark: GCR over the cycles with 16
-# directions, 40 sweeps per visit on level 1, 16 on level 2, 8 below, and
-# the learned guess (network of that workload) judged beside the stage
-# increment and the plain start
+# the measured forms of the 2048x1024 benchmark: GCR over the cycles with 16
+# directions, 16 sweeps per visit on level 2, 8 below, the learned guess
+# (network of that workload) judged beside the stage increment and the plain
+# start, and the level-1 counts its autotuner picks (20 to 40)
 BENCH_FORM = dict(gcr=16, level_sweeps={2: 16, "coarse": 8}, guess="learned")
 
 
@@ -85,7 +85,7 @@ def test_accelerated_forms_against_fp64():
     # the defect-correction modes with the reference's sweep count, and the
     # mixed mode with a tuned count and the extrapolated starts judged
     forms = [("dc64", 0, "plain", {}), ("mixed", 0, "plain", {}), ("mixed", 32, "extrap", {}),
-             ("mixed", 40, BENCH_FORM["guess"], BENCH_FORM)]
+             ("mixed", 40, BENCH_FORM["guess"], BENCH_FORM), ("mixed", 20, BENCH_FORM["guess"], BENCH_FORM)]
     for mode, sweeps, guess, extra in forms:
         kw = {k: v for k, v in extra.items() if k != "guess"}
         f = run(mode, params, T, r, s, ty, sweeps, guess, **kw)
