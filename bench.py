@@ -65,6 +65,13 @@ FP64_TOLERANCES = (1e-7, 3e-7, 1e-6)
 FP64_MAX_CYCLES = 200
 FILTER_BLOCKING = [True]  # several C-filter iterations per launch (--filter-blocking)
 TUNE_STEPS = 2  # real time steps timed per autotuner candidate (one step's time varies by ~2%)
+# sweeps per visit (level 1, level 2, levels 3 and below) of the mixed form
+# where they were measured: six steps after two, from the same state, per
+# setting, at 2048x1024 (level 1 16/20/24/28: 97.4/94.2/95.8/98.5 ms per
+# step with 16 and 8 below; level 2 at 12 or levels 3-5 at 6 slower).  The
+# autotuner's one- or two-step timings scatter by a few percent across runs,
+# so the headline uses these unless --autotune is given.
+MEASURED_COUNTS = {"cfg2": (20, 16, 8)}
 
 
 def log(msg):
@@ -143,12 +150,19 @@ def close_solver(solver, net):
 
 
 def tune_counts(solver, params, mode, args):
-    """Sweep counts of the defect-correction cycle: autotuned with real time
-    steps (common count, levels 3 and below, level 2, the common count again)
-    or fixed by the command line.  Returns the counts and the tuning record."""
-    rec = {"tune": None, "coarse_tune": None, "level2_tune": None, "tiles": None}
+    """Sweep counts of the defect-correction cycle: fixed by the command line,
+    the preset's measured counts (MEASURED_COUNTS, mixed form), or autotuned
+    with real time steps (common count, levels 3 and below, level 2, the
+    common count again, then the tiles).  Returns the counts and the tuning
+    record."""
+    rec = {"tune": None, "coarse_tune": None, "level2_tune": None, "tiles": None, "source": "command line"}
     common, coarse, level2 = params.Nmulti, 0, 0
-    if args.dc_sweeps is None:
+    fixed = MEASURED_COUNTS.get(args.config) if mode == "mixed" else None
+    if args.dc_sweeps is None and fixed and not args.autotune:
+        common, level2, coarse = fixed
+        rec["source"] = "measured default"
+    elif args.dc_sweeps is None:
+        rec["source"] = "autotuned"
         t0 = time.perf_counter()
         common, rec["tune"] = solver.autotune_dc_sweeps(DC_SWEEP_CANDIDATES, TUNE_STEPS)
         log(f"{mode}: autotuned sweeps per visit: {common} ({time.perf_counter() - t0:.1f}s; ms/step {rec['tune']})")
@@ -369,7 +383,10 @@ def main():
     ap.add_argument("--delta-visc", type=float, default=None,
                     help="delta_visC of the C filter threshold (default: the preset's)")
     ap.add_argument("--dc-sweeps", type=int, default=None,
-                    help="sweeps per visit of the defect-correction cycle (default: autotuned before the warm-up)")
+                    help="sweeps per visit of the defect-correction cycle (default: the measured counts of the "
+                         "preset, else autotuned before the warm-up)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the sweep counts and tiles before the warm-up instead of the measured defaults")
     ap.add_argument("--coarse-sweeps", type=int, default=None,
                     help="sweeps per visit on levels 3 and below with --dc-sweeps (default: that count)")
     ap.add_argument("--level2-sweeps", type=int, default=None,
@@ -545,6 +562,7 @@ def main():
             "sweeps_per_visit_level_2": counts["level2"] or counts["common"],
             "sweeps_per_visit_levels_3_up": counts["coarse"] or counts["common"],
             "reference_sweeps_per_visit": params.Nmulti,
+            "counts_source": rec.get("source"),
             "autotune_ms_per_step": rec.get("tune"),
             "coarse_autotune_ms_per_step": rec.get("coarse_tune"),
             "level2_autotune_ms_per_step": rec.get("level2_tune"),
