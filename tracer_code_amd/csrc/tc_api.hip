@@ -73,6 +73,8 @@ struct tc_dev {
     bool trace = false;
     bool coarse_lu = false;
     bool tb = true;            // temporally blocked sweeps on levels >= 2
+    bool fuse_rr = true;       // residual+restriction in the last blocked launch of a visit (TC_FUSED_RR=0: apart)
+    int cus = 256;             // compute units of the device
     bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
     int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
     unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (rotating slot sets)
@@ -421,9 +423,12 @@ void cheb_factors(int n, int Niter, int dir, double& temp, double& cconst)
 // FP32 one.  In the defect-correction modes the levels >= 2 run several
 // sweeps per launch (k_sweep_tb, the 13-tap form of the operator) unless
 // TC_TB=0; the reference-form path and level 1 run one sweep per launch.
+// rr (FP32 blocked sweeps): the visit's last launch also forms the next
+// coarser level's right-hand side; *rr_done says whether it did.
 int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, const Level& view, const float* rhs32,
-               bool* first_sweep, bool zero_start = false)
+               bool* first_sweep, bool zero_start = false, const RRTarget* rr = nullptr, bool* rr_done = nullptr)
 {
+    if (rr_done) *rr_done = false;
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
     const double zsg = (P.Vbc == 0) ? -1.0 : 1.0;
@@ -459,8 +464,12 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
                 // the first launch after an ascent adds the prolonged coarse correction
                 const float* in = (zero_start && n == 1) ? nullptr : L.err32[L.cur32];
                 const bool pro = n == 1 && ctx->pend_src && ctx->pend_level == level;
-                launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, in, L.err32[1 - L.cur32], d.F, d.h[level], temp,
-                                  cc, nk, zsg, pro ? ctx->pend_src : nullptr, pro ? &ctx->lv[ctx->pend_from].g : nullptr);
+                const bool last = n + nk - 1 == Niter;
+                const bool fused =
+                    launch_sweep_tb32(ctx->stream, use_ratio, K, L, rhs32, in, L.err32[1 - L.cur32], d.F, d.h[level],
+                                      temp, cc, nk, zsg, pro ? ctx->pend_src : nullptr,
+                                      pro ? &ctx->lv[ctx->pend_from].g : nullptr, last ? rr : nullptr);
+                if (fused && rr_done) *rr_done = true;
                 if (pro) ctx->pend_src = nullptr;
                 L.cur32 = 1 - L.cur32;
             } else {
@@ -930,7 +939,8 @@ bool tb_zero_start(const tc_ctx* ctx, const tc_dev& d, int level, bool f32)
     return f32 && d.tb && level >= 2 && level < ctx->grids && ctx->mode != TC_MODE_FP64;
 }
 
-int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32, bool zero_start = false)
+int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32, bool zero_start = false,
+               const RRTarget* rr = nullptr, bool* rr_done = nullptr)
 {
     Level& L = ctx->lv[level];
     const tc_params& P = ctx->p;
@@ -941,7 +951,7 @@ int dc_iterate(tc_ctx* ctx, tc_dev& d, int level, int dir, bool f32, bool zero_s
     // a per-level count applies to the tuned cycle, not to the Nmulti retry
     int n = ctx->dc_sweeps_active;
     if (ctx->dc_level_sweeps[level] > 0 && ctx->dc_sweeps > 0 && n == ctx->dc_sweeps) n = ctx->dc_level_sweeps[level];
-    TC_TRY(run_sweeps(ctx, d, level, n, dir, f32, view, rhs32, nullptr, zero_start));
+    TC_TRY(run_sweeps(ctx, d, level, n, dir, f32, view, rhs32, nullptr, zero_start, rr, rr_done));
     // FP64: compute_residual now; FP32: fused into the restriction of the
     // next descent (k_residual_restrict32), the only reader of the residual
     if (level > 1 && !f32) {
@@ -1199,6 +1209,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
                      !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
     if (gcr) TC_TRY(gcr_alloc(ctx));
     bool coarse_b_ready = false;  // the coarse right-hand side was formed by the descent
+    int rr_level = 0;             // the level whose right-hand side the last visit formed
     // one step p of the cycle schedule: the descent into the level, its
     // visit, and (level 1) the stop test or the ascent out of it
     auto step = [&](int p, bool& done, bool& brk) -> int {
@@ -1211,6 +1222,10 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             if (f32) {
                 if (level - 1 == 1) {
                     launch_restrict32(ctx->stream, Lf.res32, Lf.g, L.rhs32, L.g);
+                } else if (rr_level == level) {
+                    // formed by the last launch of the visit above
+                    rr_level = 0;
+                    if (level == G) coarse_b_ready = true;
                 } else if (level == G) {
                     // the coarse solve's right-hand side is formed here, from
                     // the equilibration of the inverse in use (ready after the wait)
@@ -1235,7 +1250,28 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             }
         }
         if (level < G) {
-            TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start));
+            // a descent follows: the visit's last blocked launch forms the
+            // next level's right-hand side (for the coarsest level with the
+            // equilibration of the inverse in use, ready after the wait)
+            RRTarget rrt;
+            const bool want_rr = f32 && d.fuse_rr && level >= 2 && multi[p + 1] == level + 1;
+            if (want_rr) {
+                Level& Lc = ctx->lv[level + 1];
+                rrt.out = Lc.rhs32;
+                rrt.go = Lc.g;
+                rrt.cus = d.cus;
+                if (level + 1 == G) {
+                    TC_TRY(wait_coarse(ctx, d));
+                    rrt.RowB = ctx->RowB;
+                    rrt.bout = ctx->coarse_b;
+                }
+            }
+            bool fused = false;
+            TC_TRY(dc_iterate(ctx, d, level, dir, f32, zero_start, want_rr ? &rrt : nullptr, &fused));
+            if (fused) {
+                rr_level = level + 1;
+                ctx->stats.residuals_exec[level - 1]++;
+            }
         } else {
             TC_TRY(wait_coarse(ctx, d));
             if (f32) {
@@ -2092,6 +2128,14 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         d->coarse_lu = c && c[0] == '1';
         const char* t = getenv("TC_TB");
         d->tb = !(t && t[0] == '0');
+        const char* r = getenv("TC_FUSED_RR");
+        d->fuse_rr = !(r && r[0] == '0');
+    }
+    {
+        int dev = 0, cus = 0;
+        TC_HIP(hipGetDevice(&dev));
+        TC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cus > 0) d->cus = cus;
     }
     TC_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     TC_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));

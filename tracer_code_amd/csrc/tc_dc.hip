@@ -320,9 +320,7 @@ __global__ __launch_bounds__(256) void k_residual_restrict32(const float* __rest
     };
     const float rmm = r(ii - 1, kk - 1), rmp = r(ii - 1, kk + 1), rpp = r(ii + 1, kk + 1);
     const float rm0 = r(ii - 1, kk), r0m = r(ii, kk - 1), rp0 = r(ii + 1, kk), r0p = r(ii, kk + 1), r00 = r(ii, kk);
-    float corners = (rmm + rmp + rpp + rmp) * (1.0f / 16.0f);
-    float sides = (rm0 + r0m + rp0 + r0p) * (1.0f / 8.0f);
-    const float v = corners + sides + r00 * 0.25f;
+    const float v = restrict_fw32(rmm, rmp, rpp, rm0, r0m, rp0, r0p, r00);
     out[go.at(i, k)] = v;
     if (bout) {
         const int q = i * (go.nz + 1) + k;

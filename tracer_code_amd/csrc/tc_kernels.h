@@ -169,11 +169,24 @@ void launch_tb_weights32(hipStream_t st, Level& L, const FD& F, const Spacing& h
 void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, const double* rhs, const double* S0,
                        double* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
                        double zsg);
+// The next coarser level's right-hand side, formed at the end of a visit's
+// last blocked launch (what launch_residual_restrict32 would form from the
+// visit's result); RowB/bout as there.
+struct RRTarget {
+    float* out = nullptr;
+    Grid go;
+    const double* RowB = nullptr;
+    double* bout = nullptr;
+    int cus = 256;  // compute units: the fused launch serves levels of at most one tile per CU
+};
 // Pc (optional): a coarse correction on grid *gc to prolong into S0 while
-// staging (the prolongation folded into the visit's first launch)
-void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
+// staging (the prolongation folded into the visit's first launch).
+// rr (optional): also form rr's right-hand side; returns whether it did
+// (16x16 tiles on levels of at most rr->cus tiles only).
+bool launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
                        float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
-                       double zsg, const float* Pc = nullptr, const Grid* gc = nullptr);
+                       double zsg, const float* Pc = nullptr, const Grid* gc = nullptr,
+                       const RRTarget* rr = nullptr);
 
 // --- tc_fields.hip -------------------------------------------------------------
 void launch_velocities(hipStream_t st, const double* SF, double* u, double* w, const Grid& g, const FD& F, double dr,

@@ -76,6 +76,25 @@ __device__ __forceinline__ A stream_space32(const FD32& F, const H32& h, const f
 {
     return stream_space32_g<A>(F, h, PlaneAt<float>{S, c, ld}, vis, vx, vz, a1, vxz, rhs);
 }
+
+// S(i+di, k+dk) of a tile staged in LDS with row pitch ld
+struct LdsAt {
+    const float* f;
+    int c;
+    int ld;
+    __device__ __forceinline__ float operator()(int di, int dk) const { return f[c + dk * ld + di]; }
+};
+
+// restrict (iterative_solver.f90:596-626) at one coarse point from the fine
+// residuals around its fine centre: the reference's weights, whose corner sum
+// counts (ii-1,kk+1) twice and (ii+1,kk-1) not at all
+__device__ __forceinline__ float restrict_fw32(float rmm, float rmp, float rpp, float rm0, float r0m, float rp0,
+                                               float r0p, float r00)
+{
+    const float corners = (rmm + rmp + rpp + rmp) * (1.0f / 16.0f);
+    const float sides = (rm0 + r0m + rp0 + r0p) * (1.0f / 8.0f);
+    return corners + sides + r00 * 0.25f;
+}
 #pragma clang fp contract(off)
 
 }  // namespace tc

@@ -16,18 +16,21 @@
 // rounding order, so these kernels serve the defect-correction modes; the
 // reference-form FP64 path keeps its one-sweep kernels.
 #include "tc_kernels.h"
+#include "tc_stencil32.h"
 
 namespace tc {
 
 // Tile TT x TT; K sweeps per launch need a halo of 2K cells, so the
 // staged region is (TT + 4K)^2: 32^2 for TT=16, K=4 (4x the tile), 48^2 for
 // TT=32, K=4 (2.25x) -- the larger tile stages less halo per updated cell
-// and serves levels with enough tiles to fill the chip.
+// and serves levels with enough tiles to fill the chip.  A launch that also
+// forms the next coarser level's right-hand side stages X = 3 more cells
+// (below).
 constexpr int TB_KMAX = 8;
 constexpr int TB_THREADS = 1024;
-template <int TT, int K>
+template <int TT, int K, int X = 0>
 struct TBGeom {
-    static constexpr int H = 2 * K;
+    static constexpr int H = 2 * K + X;
     static constexpr int R = TT + 2 * H;
     static constexpr int CELLS = R * R;
     static constexpr int NPT = (CELLS + TB_THREADS - 1) / TB_THREADS;  // staged cells per thread
@@ -59,6 +62,20 @@ template <typename T>
 struct Coefs {
     const T *vis, *vx, *vz, *vxx, *vzz, *a1, *vxz, *rhs, *dts, *ratio, *lb;
 };
+
+// The residual+restriction into the next coarser level, done by the last
+// launch of a visit before a descent (RR): out (the coarse right-hand side),
+// its grid, and for the coarsest level RowB/bout (the equilibrated
+// right-hand side of the coarse solve); F, h the FP32 stencil tables.
+struct RRArgs {
+    float* out;
+    Grid go;
+    const double* RowB;
+    double* bout;
+    FD32 F;
+    H32 h;
+};
+struct NoRR {};
 
 // dt_stream-scaled weights of the 13 taps and the scaled RHS of one cell.
 template <typename T>
@@ -127,12 +144,22 @@ __global__ __launch_bounds__(256) void k_tb_weights32(Coefs<float> C, Tables<flo
     for (int q = 0; q < 13; ++q) W[q * n + c] = v[q];
 }
 
-template <typename T, bool RATIO, int K, int TT, bool PRE = false>
+// RR: after the nk <= K sweeps, the fine residual rhs - L(S) of the cells
+// [TT*tx+1, TT*tx+TT+1]^2 (the tile and one more row and column: the
+// restriction footprint of the tile's coarse points, coarse i in
+// [TT/2*tx+1, TT/2*tx+TT/2] with fine centre 2i) from the staged values, and
+// their restriction.  The residual reads S 2 cells out, so the staged values
+// must be exact from 2 cells below the tile to 3 above: the halo is 2K+3,
+// and after nk sweeps every cell at least 2nk from the region's edge is.
+// The arithmetic is k_residual_restrict32's, so the values are the same.
+template <typename T, bool RATIO, int K, int TT, bool PRE = false, bool RR = false>
 __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ Ein, T* __restrict__ Eout, Coefs<T> C,
                                                         Tables<T> Tb, Grid g, SweepFactors<T> sf, int nk, T zsg,
-                                                        const T* __restrict__ W, Prolong<T> pr)
+                                                        const T* __restrict__ W, Prolong<T> pr,
+                                                        std::conditional_t<RR, RRArgs, NoRR> rr)
 {
-    using G = TBGeom<TT, K>;
+    static_assert(!RR || (sizeof(T) == 4 && PRE), "the fused restriction serves the FP32 cycle");
+    using G = TBGeom<TT, K, RR ? 3 : 0>;
     constexpr int R = G::R, H = G::H, NPT = G::NPT;
     __shared__ T buf[2][G::CELLS];
     const int2 tile = xcd_tile();
@@ -186,6 +213,18 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
             }
         }
     }
+    // RR: this thread's fine residual cell and its coefficients, loaded now so
+    // that their latency overlaps the sweeps
+    constexpr int NF = TT + 1;
+    const int rl = threadIdx.x, ri = 1 + TT * tile.x + rl % NF, rk = 1 + TT * tile.y + rl / NF;
+    const bool rv = RR && rl < NF * NF && ri <= g.nx - 1 && rk <= g.nz - 1;
+    float rc[6] = {};
+    if constexpr (RR) {
+        if (rv) {
+            const size_t c = g.at(ri, rk);
+            rc[0] = C.vis[c]; rc[1] = C.vx[c]; rc[2] = C.vz[c]; rc[3] = C.a1[c]; rc[4] = C.vxz[c]; rc[5] = C.rhs[c];
+        }
+    }
     __syncthreads();
     int cur = 0;
     for (int s = 0; s < nk; ++s) {
@@ -221,6 +260,29 @@ __global__ __launch_bounds__(TB_THREADS) void k_sweep_tb(const T* __restrict__ E
             const T r = buf[cur][l];
             Eout[g.at(i, k)] = r;
             stream_ghosts(Eout, g, i, k, r, zsg);
+        }
+    }
+    if constexpr (RR) {
+        constexpr int NC = TT / 2;
+        __shared__ float rres[NF * NF];
+        const int fi0 = 1 + TT * tile.x, fk0 = 1 + TT * tile.y;
+        static_assert(NF * NF <= TB_THREADS, "one fine residual per thread");
+        if (rl < NF * NF)
+            rres[rl] = rv ? (float)stream_space32_g<double>(rr.F, rr.h, LdsAt{buf[cur], (rk - gk0) * R + (ri - gi0), R},
+                                                            rc[0], rc[1], rc[2], rc[3], rc[4], rc[5])
+                          : 0.0f;
+        __syncthreads();
+        for (int l = threadIdx.x; l < NC * NC; l += TB_THREADS) {
+            const int ic = NC * tile.x + 1 + l % NC, kc = NC * tile.y + 1 + l / NC;
+            if (ic > rr.go.nx - 1 || kc > rr.go.nz - 1) continue;
+            const int c0 = (2 * kc - fk0) * NF + (2 * ic - fi0);  // the fine centre in rres
+            const float v = restrict_fw32(rres[c0 - NF - 1], rres[c0 + NF - 1], rres[c0 + NF + 1], rres[c0 - 1],
+                                          rres[c0 - NF], rres[c0 + 1], rres[c0 + NF], rres[c0]);
+            rr.out[rr.go.at(ic, kc)] = v;
+            if (rr.bout) {
+                const int q = ic * (rr.go.nz + 1) + kc;
+                rr.bout[q] = (double)v * rr.RowB[q];
+            }
         }
     }
 }
@@ -266,32 +328,50 @@ static void tb_launch_t(hipStream_t st, bool use_ratio, int K, const Grid& g, co
                         const Tables<T>& tb, const SweepFactors<T>& sf, int nk, T zsg, const T* W, const Prolong<T>& pr)
 {
     const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
+    const NoRR no{};
     if constexpr (sizeof(T) == 4) {
         if (W && K == 4) {
             if (use_ratio)
-                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr, no);
             else
-                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr,
+                                   no);
             return;
         }
         if constexpr (TT == 16) {
             if (W && K == 8) {
                 if (use_ratio)
-                    hipLaunchKernelGGL((k_sweep_tb<T, true, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+                    hipLaunchKernelGGL((k_sweep_tb<T, true, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr,
+                                       no);
                 else
                     hipLaunchKernelGGL((k_sweep_tb<T, false, 8, 16, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg,
-                                       W, pr);
+                                       W, pr, no);
                 return;
             }
         }
     }
     if (K == 8) {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr, no);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 8, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr, no);
     } else {
-        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
-        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+        if (use_ratio) hipLaunchKernelGGL((k_sweep_tb<T, true, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr, no);
+        else hipLaunchKernelGGL((k_sweep_tb<T, false, 4, TT>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr, no);
     }
+}
+
+// the FP32 launch that also forms the next coarser level's right-hand side
+template <int TT, int K>
+static void tb_launch_rr(hipStream_t st, bool use_ratio, const Grid& g, const float* S0, float* E,
+                         const Coefs<float>& C, const Tables<float>& tb, const SweepFactors<float>& sf, int nk,
+                         float zsg, const float* W, const Prolong<float>& pr, const RRArgs& rr)
+{
+    const dim3 gr = tb_grid(g, TT), b(TB_THREADS);
+    if (use_ratio)
+        hipLaunchKernelGGL((k_sweep_tb<float, true, K, TT, true, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W,
+                           pr, rr);
+    else
+        hipLaunchKernelGGL((k_sweep_tb<float, false, K, TT, true, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W,
+                           pr, rr);
 }
 
 template <typename T>
@@ -303,10 +383,13 @@ static void tb_launch(hipStream_t st, bool use_ratio, int K, const Grid& g, cons
         // 48x48 tiles (1.78x staged cells per tile cell against 2.25x for 32x32), precomputed weights only
         if (tile == 48 && K == 4 && W) {
             const dim3 gr = tb_grid(g, 48), b(TB_THREADS);
+            const NoRR no{};
             if (use_ratio)
-                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+                hipLaunchKernelGGL((k_sweep_tb<T, true, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr,
+                                   no);
             else
-                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr);
+                hipLaunchKernelGGL((k_sweep_tb<T, false, 4, 48, true>), gr, b, 0, st, S0, E, C, tb, g, sf, nk, zsg, W, pr,
+                                   no);
             return;
         }
         if ((tile ? tile : tb_tile<T>(g)) == 32 && K == 4) {
@@ -347,9 +430,9 @@ void launch_sweep_tb64(hipStream_t st, bool use_ratio, int K, const Level& L, co
     tb_launch<double>(st, use_ratio, K, L.g, S0, E, C, make_tables<double>(F, h), sf, nk, zsg);
 }
 
-void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
+bool launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, const float* rhs, const float* S0,
                        float* E, const FD& F, const Spacing& h, const double* temp, const double* cconst, int nk,
-                       double zsg, const float* Pc, const Grid* gc)
+                       double zsg, const float* Pc, const Grid* gc, const RRTarget* rr)
 {
     Coefs<float> C{L.vis32, L.visx32, L.visz32, nullptr, nullptr, L.a132, L.visxz32, rhs, L.dts32, nullptr, L.lb32};
     SweepFactors<float> sf{};
@@ -359,8 +442,27 @@ void launch_sweep_tb32(hipStream_t st, bool use_ratio, int K, const Level& L, co
         pr.Ec = Pc;
         pr.gc = *gc;
     }
-    tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, make_tables<float>(F, h), sf, nk, (float)zsg,
-                     tb_uses_w13(L) ? L.w13 : nullptr, L.tb_tile, pr);
+    const Tables<float> tbl = make_tables<float>(F, h);
+    const float* W = tb_uses_w13(L) ? L.w13 : nullptr;
+    if (rr && rr->out && W) {
+        // 16x16 tiles only: a 32x32 tile's 33^2 residuals take two per thread,
+        // and its FP64 residual arithmetic then spills.  The wider staging
+        // (1444 cells for K=4) needs ~85 VGPRs, so one workgroup fits a CU where
+        // the plain launch fits two: on a level of more tiles than CUs the
+        // fused launch runs in two rounds and costs more than the launch it
+        // saves (16.0 against 8.3 + 6.3 us at 512 tiles; 8.6 against 5.7 + 6.3
+        // at 128 tiles).
+        const int TT = L.tb_tile ? L.tb_tile : tb_tile<float>(L.g);
+        const dim3 t16 = tb_grid(L.g, 16);
+        const RRArgs a{rr->out, rr->go, rr->RowB, rr->bout, fd32_from(F), h32_from(h)};
+        if (TT == 16 && (K == 4 || K == 8) && (int)(t16.x * t16.y) <= rr->cus) {
+            if (K == 4) tb_launch_rr<16, 4>(st, use_ratio, L.g, S0, E, C, tbl, sf, nk, (float)zsg, W, pr, a);
+            else tb_launch_rr<16, 8>(st, use_ratio, L.g, S0, E, C, tbl, sf, nk, (float)zsg, W, pr, a);
+            return true;
+        }
+    }
+    tb_launch<float>(st, use_ratio, K, L.g, S0, E, C, tbl, sf, nk, (float)zsg, W, L.tb_tile, pr);
+    return false;
 }
 
 }  // namespace tc
