@@ -74,6 +74,10 @@ struct tc_dev {
     bool coarse_lu = false;
     bool tb = true;            // temporally blocked sweeps on levels >= 2
     bool fuse_rr = true;       // residual+restriction in the last blocked launch of a visit (TC_FUSED_RR=0: apart)
+    // defect-correction cycle: level-1 residual updated by increments (k_absorb_inc) with an exact
+    // double-double residual before the stop test accepts (TC_INC_RESID=0: exact after every correction)
+    bool inc_res = true;
+    bool res_exact = true;     // L1.res is the double-double residual of psi (not an incremental one)
     int cus = 256;             // compute units of the device
     bool inv_pending = false;  // the main stream has not yet waited for the coarse inverse
     int32_t* ctl = nullptr;    // device loop control of the filter loops: stop flag, stop index
@@ -967,12 +971,26 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
 {
     Level& L1 = ctx->lv[1];
     const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
+    if (d.inc_res) {
+        const int nb = launch_absorb_inc(ctx->stream, ctx->psi_hi, ctx->psi_lo, f32 ? L1.err32[L1.cur32] : nullptr,
+                                         f32 ? nullptr : L1.err[L1.cur], nullptr, L1, L1.res,
+                                         f32 ? L1.res32 : nullptr, d.F, d.h[1], ctx->p.ivis, d.pow2[1], zsg,
+                                         d.partialA);
+        // e(1) back to zero for the next visit (a fill kernel: capturable)
+        if (f32) launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(L1.err32[L1.cur32]), L1.g.elems);
+        else launch_zero_i32(ctx->stream, reinterpret_cast<int32_t*>(L1.err[L1.cur]), 2 * L1.g.elems);
+        launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
+        ctx->stats.residuals_exec[0]++;
+        d.res_exact = false;
+        return 0;
+    }
     launch_dd_absorb(ctx->stream, ctx->psi_hi, ctx->psi_lo, f32 ? L1.err32[L1.cur32] : nullptr,
                      f32 ? nullptr : L1.err[L1.cur], L1.g, zsg);
     int nb = launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr,
                                 d.F, d.h[1], ctx->p.ivis, d.partialA);
     launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
     ctx->stats.residuals_exec[0]++;
+    d.res_exact = true;
     return 0;
 }
 
@@ -983,10 +1001,11 @@ int dc_absorb_residual(tc_ctx* ctx, tc_dev& d, bool f32)
 // psi = psi_s + alpha z minimises |r|_2 over them; when the set is full it
 // starts again empty.  The residual of the new psi is r_s - alpha q: the
 // directions and their images are FP64 differences of double-double psi and
-// of exact residuals, so it is exact to FP64 rounding of q (an FP32 copy of
+// of level-1 defects (exact, or kept by FP64 increments -L(e)), so it is
+// RHS - L(psi) to FP64 rounding of q and of the increments (an FP32 copy of
 // z would carry its rounding times the fourth-order operator's spread of
-// eigenvalues).  The next level-1 visit smooths against it; its absorb
-// computes the exact residual, so the stop test and the tolerance are the
+// eigenvalues).  The next level-1 visit smooths against it; a stop test that
+// passes is confirmed on the exact residual, so the tolerance is the
 // cycles'.  With no kept direction and alpha = 1 this is the plain cycle.
 static_assert(GCR_MAX == 16, "tc_ctx keeps 16 GCR slots");
 
@@ -1080,6 +1099,7 @@ int gcr_step(tc_ctx* ctx, bool f32)
     const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;
     launch_gcr_update(ctx->stream, nb, Z, Q, cf, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs,
                       L1.res, f32 ? L1.res32 : nullptr, g, zsg);
+    if (DEV(ctx).inc_res) DEV(ctx).res_exact = false;
     TC_HIP(hipMemcpyAsync(ctx->gcr_hflag, ctx->gcr_flag, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     TC_HIP(hipEventRecord(ctx->gcr_ev, ctx->stream));
     ctx->gcr_new = sn;
@@ -1136,6 +1156,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     auto defect = [&](const double* hi, double* mag) -> int {
         const int nb = launch_residual_dd(ctx->stream, hi, ctx->psi_lo, L1, L1.rhs, L1.res, f32 ? L1.res32 : nullptr,
                                           d.F, d.h[1], P.ivis, d.partialA);
+        d.res_exact = true;
         if (mag) {
             launch_reduce_final(ctx->stream, d.partialA, nb, 0, d.d_scal, S_RESID1);
             TC_TRY(fetch_scalars(ctx, d, S_COUNT));
@@ -1208,6 +1229,19 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
     const bool gcr = ctx->gcr_m > 0 && !ctx->gcr_suspended &&
                      !(ctx->dc_sweeps > 0 && ctx->dc_sweeps_active != ctx->dc_sweeps);
     if (gcr) TC_TRY(gcr_alloc(ctx));
+    // the stop test on an incremental residual: a pass is confirmed on the
+    // double-double residual of psi (which then replaces it)
+    auto stop_test_dc = [&](int qq, bool& done) -> int {
+        int st = stop_test(ctx, d, qq, done);
+        if (st == 0 && done && !d.res_exact) {
+            double mag = 0.0;
+            TC_TRY(defect(ctx->psi_hi, &mag));
+            d.res_exact = true;
+            ctx->stats.residuals_exec[0]++;
+            st = stop_test(ctx, d, qq, done);
+        }
+        return st;
+    };
     bool coarse_b_ready = false;  // the coarse right-hand side was formed by the descent
     int rr_level = 0;             // the level whose right-hand side the last visit formed
     // one step p of the cycle schedule: the descent into the level, its
@@ -1284,7 +1318,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
         }
         if (level == 1) {
             TC_TRY(dc_absorb_residual(ctx, d, f32));
-            status = stop_test(ctx, d, q, done);
+            status = stop_test_dc(q, done);
             if (status || done) {
                 brk = true;
                 return 0;
@@ -1295,10 +1329,17 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
             if (level == 2) {
                 // fold the coarse-grid correction into psi; level 1 then
                 // post-smooths a zero correction against the new defect
-                launch_prolong_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf.g, f32 ? L.err32[L.cur32] : nullptr,
-                                  f32 ? nullptr : L.err[L.cur], L.g, zsg);
-                launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res, f32 ? Lf.res32 : nullptr,
-                                   d.F, d.h[1], P.ivis, nullptr);
+                if (d.inc_res) {
+                    launch_absorb_inc(ctx->stream, ctx->psi_hi, ctx->psi_lo, f32 ? L.err32[L.cur32] : nullptr,
+                                      f32 ? nullptr : L.err[L.cur], &L.g, Lf, Lf.res, f32 ? Lf.res32 : nullptr, d.F,
+                                      d.h[1], P.ivis, d.pow2[1], zsg, nullptr);
+                    d.res_exact = false;
+                } else {
+                    launch_prolong_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf.g, f32 ? L.err32[L.cur32] : nullptr,
+                                      f32 ? nullptr : L.err[L.cur], L.g, zsg);
+                    launch_residual_dd(ctx->stream, ctx->psi_hi, ctx->psi_lo, Lf, Lf.rhs, Lf.res,
+                                       f32 ? Lf.res32 : nullptr, d.F, d.h[1], P.ivis, nullptr);
+                }
                 ctx->stats.residuals_exec[0]++;
             } else if (f32 && tb_zero_start(ctx, d, level - 1, f32) && multi[p + 1] == level - 1) {
                 // folded into the first blocked launch of level-1's next visit
@@ -1393,7 +1434,7 @@ int dc_solve(tc_ctx* ctx, int32_t* q_out, int max_cycles)
         bool done = false, brk = false;
         if (first_visit_done && q == 1) {
             // level 1's first visit ran while judging the guess
-            status = stop_test(ctx, d, q, done);
+            status = stop_test_dc(q, done);
             brk = status || done;
         } else {
             TC_TRY(step(1, done, brk));
@@ -2130,6 +2171,8 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         d->tb = !(t && t[0] == '0');
         const char* r = getenv("TC_FUSED_RR");
         d->fuse_rr = !(r && r[0] == '0');
+        const char* ir = getenv("TC_INC_RESID");
+        d->inc_res = !(ir && ir[0] == '0');
     }
     {
         int dev = 0, cus = 0;

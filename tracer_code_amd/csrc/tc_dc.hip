@@ -239,6 +239,86 @@ __global__ __launch_bounds__(256) void k_prolong_dd(double* __restrict__ hi, dou
     stream_ghosts(lo, gf, i, k, s.lo, zsg);
 }
 
+// The bilinear prolongation of a coarse correction at fine interior (i,k)
+// (prolong, iterative_solver.f90:175-233), in FP64 as k_prolong_dd adds it.
+template <typename T>
+__device__ __forceinline__ double prolong_val(const T* __restrict__ Ec, const Grid& gc, int i, int k)
+{
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) return (double)Ec[gc.at(i / 2, k / 2)];
+    if (ie) return ((double)Ec[gc.at(i / 2, (k - 1) / 2)] + (double)Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
+    if (ke) return ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
+    return ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
+            (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+}
+
+// The defect kept up to date by increments: psi(hi,lo) += e (as k_dd_absorb
+// or k_prolong_dd) and r -= L(e), so r stays RHS - L(psi) without the
+// double-double residual of the whole psi.  L(e) is the reference-form FP64
+// operator (stream_space, as k_residual, with the FP64 coefficients): its
+// rounding is that of the FP64 residual of psi scaled by |e|/|psi|, far
+// below the tolerance for a correction; the solve confirms convergence on
+// the exact residual (k_residual_dd) before it stops.  e is staged in LDS
+// with the residual's 2-cell halo.  PRO = false: e is the level-1 correction
+// (with the ghosts the sweeps wrote); PRO = true: the bilinear prolongation of
+// the level-2 correction Ec, formed at the staged interior cells, 0 on the
+// walls and the enforceBCs_stream mirrors at the ghosts.  The kernel does not
+// clear e: other workgroups read it as their halo.  With partial, the block
+// sums of |r| for the stop test.
+template <typename T, bool PRO, bool POW2>
+__global__ __launch_bounds__(256) void k_absorb_inc(double* __restrict__ hi, double* __restrict__ lo,
+                                                    const T* __restrict__ e, Grid gc, const double* __restrict__ vis,
+                                                    const double* __restrict__ vx, const double* __restrict__ vz,
+                                                    const double* __restrict__ vxx, const double* __restrict__ vzz,
+                                                    const double* __restrict__ vxz, double* __restrict__ r64,
+                                                    float* __restrict__ r32, Grid g, FD F, Spacing h, int ivis,
+                                                    double zsg, double* __restrict__ partial)
+{
+    __shared__ double se[RD_CELLS];
+    const int2 tile = xcd_tile();
+    {
+        const int i0 = tile.x * RD_BX + 1 - RD_H, k0 = tile.y * RD_BY + 1 - RD_H;
+        for (int l = threadIdx.y * RD_BX + threadIdx.x; l < RD_CELLS; l += RD_BX * RD_BY) {
+            const int ii = i0 + l % RD_W, kk = k0 + l / RD_W;
+            double v = 0.0;
+            if constexpr (PRO) {
+                int mi = ii, mk = kk;
+                double sg = 1.0;
+                if (mi < 0) { mi = -mi; sg = -sg; } else if (mi > g.nx) { mi = 2 * g.nx - mi; sg = -sg; }
+                if (mk < 0) { mk = -mk; sg = sg * zsg; } else if (mk > g.nz) { mk = 2 * g.nz - mk; sg = sg * zsg; }
+                if (mi >= 1 && mi <= g.nx - 1 && mk >= 1 && mk <= g.nz - 1) v = sg * prolong_val(e, gc, mi, mk);
+            } else {
+                if (ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN) v = (double)e[g.at(ii, kk)];
+            }
+            se[l] = v;
+        }
+    }
+    __syncthreads();
+    const int i = tile.x * RD_BX + threadIdx.x + 1;
+    const int k = tile.y * RD_BY + threadIdx.y + 1;
+    double a = 0.0;
+    if (i <= g.nx - 1 && k <= g.nz - 1) {
+        const size_t c = g.at(i, k);
+        const size_t cl = (size_t)(threadIdx.y + RD_H) * RD_W + threadIdx.x + RD_H;
+        const dd s = dd_add({hi[c], lo[c]}, {se[cl], 0.0});
+        hi[c] = s.hi;
+        lo[c] = s.lo;
+        stream_ghosts(hi, g, i, k, s.hi, zsg);
+        stream_ghosts(lo, g, i, k, s.lo, zsg);
+        // 0 - L(e)
+        const double le = stream_space_g<POW2>(F, h, ivis, PlaneAt<double>{se, cl, RD_W}, vis[c], vx[c], vz[c], vxx[c],
+                                               vzz[c], vxz[c], 0.0);
+        const double rv = r64[c] + le;
+        r64[c] = rv;
+        if (r32) r32[c] = (float)rv;
+        a = fabs(rv);
+    }
+    if (partial) {
+        a = block_sum(a);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[tile.y * gridDim.x + tile.x] = a;
+    }
+}
+
 // ---- FP32 inner cycle (stencil in tc_stencil32.h) ---------------------------
 #pragma clang fp contract(fast)
 template <bool RATIO>
@@ -644,6 +724,36 @@ void launch_dd_absorb(hipStream_t st, double* hi, double* lo, float* e32, double
     dim3 b(64, 4);
     if (e32) hipLaunchKernelGGL((k_dd_absorb<float>), grid2(g.nx - 1, g.nz - 1, b), b, 0, st, hi, lo, e32, g, zsg);
     else hipLaunchKernelGGL((k_dd_absorb<double>), grid2(g.nx - 1, g.nz - 1, b), b, 0, st, hi, lo, e64, g, zsg);
+}
+
+template <typename T, bool PRO>
+static void absorb_inc_t(hipStream_t st, bool pow2, dim3 gr, dim3 b, double* hi, double* lo, const T* e, const Grid& gc,
+                         const Level& L, double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double zsg,
+                         double* partial)
+{
+    if (pow2)
+        hipLaunchKernelGGL((k_absorb_inc<T, PRO, true>), gr, b, 0, st, hi, lo, e, gc, L.vis, L.visx, L.visz, L.visxx,
+                           L.viszz, L.visxz, r64, r32, L.g, F, h, ivis, zsg, partial);
+    else
+        hipLaunchKernelGGL((k_absorb_inc<T, PRO, false>), gr, b, 0, st, hi, lo, e, gc, L.vis, L.visx, L.visz, L.visxx,
+                           L.viszz, L.visxz, r64, r32, L.g, F, h, ivis, zsg, partial);
+}
+
+int launch_absorb_inc(hipStream_t st, double* hi, double* lo, const float* e32, const double* e64, const Grid* gc,
+                      const Level& L, double* r64, float* r32, const FD& F, const Spacing& h, int ivis, bool pow2,
+                      double zsg, double* partial)
+{
+    dim3 b(RD_BX, RD_BY);
+    dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
+    const Grid gcv = gc ? *gc : Grid{};
+    if (e32) {
+        if (gc) absorb_inc_t<float, true>(st, pow2, gr, b, hi, lo, e32, gcv, L, r64, r32, F, h, ivis, zsg, partial);
+        else absorb_inc_t<float, false>(st, pow2, gr, b, hi, lo, e32, gcv, L, r64, r32, F, h, ivis, zsg, partial);
+    } else {
+        if (gc) absorb_inc_t<double, true>(st, pow2, gr, b, hi, lo, e64, gcv, L, r64, r32, F, h, ivis, zsg, partial);
+        else absorb_inc_t<double, false>(st, pow2, gr, b, hi, lo, e64, gcv, L, r64, r32, F, h, ivis, zsg, partial);
+    }
+    return (int)(gr.x * gr.y);
 }
 
 void launch_prolong_dd(hipStream_t st, double* hi, double* lo, const Grid& gf, const float* e32, const double* e64,

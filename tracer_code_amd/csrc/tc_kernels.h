@@ -141,6 +141,12 @@ void launch_extrap(hipStream_t st, const double* a, const double* b, double c, d
 int launch_residual_dd(hipStream_t st, const double* hi, const double* lo, const Level& L, const double* rhs,
                        double* r64, float* r32, const FD& F, const Spacing& h, int ivis, double* partial);
 void launch_dd_absorb(hipStream_t st, double* hi, double* lo, float* e32, double* e64, const Grid& g, double zsg);
+// psi(hi,lo) += e and r64 -= L(e) (r32 its FP32 rounding), e the level-1
+// correction (gc == nullptr) or the prolongation of the level-2 correction on
+// grid *gc; with partial, the block sums of |r| (returns the block count)
+int launch_absorb_inc(hipStream_t st, double* hi, double* lo, const float* e32, const double* e64, const Grid* gc,
+                      const Level& L, double* r64, float* r32, const FD& F, const Spacing& h, int ivis, bool pow2,
+                      double zsg, double* partial);
 void launch_prolong_dd(hipStream_t st, double* hi, double* lo, const Grid& gf, const float* e32, const double* e64,
                        const Grid& gc, double zsg);
 void launch_sweep32(hipStream_t st, bool use_ratio, const Level& L, const float* S0, float* E, const float* rhs,
