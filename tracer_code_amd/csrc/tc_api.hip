@@ -1092,8 +1092,7 @@ int gcr_step(tc_ctx* ctx, bool f32)
     idx.v[nb] = sn;
     Z.v[nb] = ctx->gcr_Z[sn];
     Q.v[nb] = ctx->gcr_Q[sn];
-    launch_gcr_dots(ctx->stream, nb, ctx->psi_hi, ctx->psi_lo, ctx->gcr_hs, ctx->gcr_ls, ctx->gcr_rs, L1.res,
-                    Z.v[nb], Q.v[nb], Qk, g, ctx->gcr_part, ctx->gcr_out);
+    launch_gcr_dots(ctx->stream, nb, ctx->gcr_rs, L1.res, Qk, g, ctx->gcr_part, ctx->gcr_out);
     GcrCoef* cf = reinterpret_cast<GcrCoef*>(ctx->gcr_coef);
     launch_gcr_coef(ctx->stream, nb, ctx->gcr_out, idx, ctx->gcr_qq, cf, ctx->gcr_flag);
     const double zsg = (ctx->p.Vbc == 0) ? -1.0 : 1.0;

@@ -509,25 +509,24 @@ __global__ __launch_bounds__(256) void k_add_diff(const double* __restrict__ a, 
 // Interior points only: the update writes psi's ghosts by the mirrors of the
 // boundary routine.
 //
-// k_gcr_dots forms z and q into the new slot and the per-block partial sums
-// over the interior of (q, Q_j) and (r_s, Q_j) for the NB kept directions,
-// then (q, q) and (r_s, q); block b takes rows 1+b, 1+b+gridDim.x, ...
+// k_gcr_dots: the per-block partial sums over the interior of (q, Q_j) and
+// (r_s, Q_j) for the NB kept directions, then (q, q) and (r_s, q), with
+// q = r_s - r'; block b takes rows 1+b, 1+b+gridDim.x, ...  It only reads
+// (z and q are stored by k_gcr_update), so the loads of successive points
+// are free to overlap.  Each value is summed over the block as block_sum
+// does (wave shuffles, then the waves in order), with one barrier for all.
 template <int NB>
-__global__ __launch_bounds__(256) void k_gcr_dots(const double* __restrict__ hi, const double* __restrict__ lo,
-                                                  const double* __restrict__ hs, const double* __restrict__ ls,
-                                                  const double* __restrict__ rs, const double* __restrict__ r,
-                                                  double* __restrict__ zn, double* __restrict__ qn, GcrSet Q, Grid g,
-                                                  double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_gcr_dots(const double* __restrict__ rs, const double* __restrict__ r,
+                                                  GcrSet Q, Grid g, double* __restrict__ part)
 {
-    double acc[2 * NB + 2];
+    constexpr int ND = 2 * NB + 2;
+    double acc[ND];
 #pragma unroll
-    for (int j = 0; j < 2 * NB + 2; ++j) acc[j] = 0.0;
+    for (int j = 0; j < ND; ++j) acc[j] = 0.0;
     for (int k = 1 + blockIdx.x; k <= g.nz - 1; k += gridDim.x)
         for (int i = 1 + threadIdx.x; i <= g.nx - 1; i += blockDim.x) {
             const size_t c = g.at(i, k);
             const double rv = rs[c], qv = rv - r[c];
-            zn[c] = (hi[c] - hs[c]) + (lo[c] - ls[c]);
-            qn[c] = qv;
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 const double b = Q.v[j][c];
@@ -537,10 +536,20 @@ __global__ __launch_bounds__(256) void k_gcr_dots(const double* __restrict__ hi,
             acc[2 * NB] += qv * qv;
             acc[2 * NB + 1] += rv * qv;
         }
+    __shared__ double sh[4][ND];
+    static_assert(ND <= 256, "one thread per value");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int j = 0; j < 2 * NB + 2; ++j) {
-        const double t = block_sum(acc[j]);
-        if (threadIdx.x == 0) part[(size_t)blockIdx.x * GCR_NDOT + j] = t;
+    for (int j = 0; j < ND; ++j) {
+        double a = acc[j];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0) sh[w][j] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < ND) {
+        double t = sh[0][threadIdx.x];
+        for (int q = 1; q < 4; ++q) t += sh[q][threadIdx.x];
+        part[(size_t)blockIdx.x * GCR_NDOT + threadIdx.x] = t;
     }
 }
 
@@ -591,10 +600,11 @@ __global__ void k_gcr_coef(const double* __restrict__ out, int nb, GcrIdx idx, d
     *flag = f;
 }
 
-// the new direction (position NB of Z and Q) orthogonalised against the kept
-// ones and stored back; psi = psi_s + alpha z in double-double, written to
-// psi (with its ghosts) and to psi_s; its residual r_s - alpha q written to
-// r (FP64 and, for the FP32 cycle, FP32) and to r_s
+// the new direction z = (hi - hs) + (lo - ls) and its image q = r_s - r,
+// orthogonalised against the kept ones and stored at position NB of Z and Q;
+// psi = psi_s + alpha z in double-double, written to psi (with its ghosts)
+// and to psi_s; its residual r_s - alpha q written to r (FP64 and, for the
+// FP32 cycle, FP32) and to r_s
 template <int NB>
 __global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, const GcrCoef* __restrict__ cfp,
                                                     double* __restrict__ hi,
@@ -608,7 +618,7 @@ __global__ __launch_bounds__(256) void k_gcr_update(GcrSetW Z, GcrSetW Q, const 
     if (i > g.nx - 1 || k > g.nz - 1) return;
     const size_t c = g.at(i, k);
     const GcrCoef& cf = *cfp;
-    double z = Z.v[NB][c], q = Q.v[NB][c];
+    double z = (hi[c] - hs[c]) + (lo[c] - ls[c]), q = rs[c] - r[c];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         z -= cf.beta[j] * Z.v[j][c];
@@ -634,14 +644,12 @@ static inline dim3 grid2(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / 
 
 int gcr_blocks(const Grid& g) { return std::min(g.nz - 1, 1024); }
 
-void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo, const double* hs, const double* ls,
-                     const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
+void launch_gcr_dots(hipStream_t st, int nb, const double* rs, const double* r, const GcrSet& Q, const Grid& g,
                      double* part, double* out)
 {
     const int nblk = gcr_blocks(g);
     const dim3 gr(nblk), b(256);
-#define TC_GCR_DOTS(N)                                                                                             \
-    hipLaunchKernelGGL((k_gcr_dots<N>), gr, b, 0, st, hi, lo, hs, ls, rs, r, zn, qn, Q, g, part)
+#define TC_GCR_DOTS(N) hipLaunchKernelGGL((k_gcr_dots<N>), gr, b, 0, st, rs, r, Q, g, part)
     switch (nb) {
     case 0: TC_GCR_DOTS(0); break;
     case 1: TC_GCR_DOTS(1); break;

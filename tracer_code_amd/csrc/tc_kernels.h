@@ -124,11 +124,9 @@ struct GcrSetW { double* v[GCR_MAX]; };
 struct GcrCoef { double beta[GCR_MAX]; double alpha; };
 struct GcrIdx { int v[GCR_MAX]; };  // slot of each kept direction, then of the new one
 int gcr_blocks(const Grid& g);
-// the new direction z = (hi,lo) - (hs,ls) and q = rs - r into zn, qn, and
-// out[2j] = (q, Q_j), out[2j+1] = (rs, Q_j) for j < nb; out[2nb] = (q, q),
-// out[2nb+1] = (rs, q)
-void launch_gcr_dots(hipStream_t st, int nb, const double* hi, const double* lo, const double* hs, const double* ls,
-                     const double* rs, const double* r, double* zn, double* qn, const GcrSet& Q, const Grid& g,
+// with q = rs - r: out[2j] = (q, Q_j), out[2j+1] = (rs, Q_j) for j < nb;
+// out[2nb] = (q, q), out[2nb+1] = (rs, q)
+void launch_gcr_dots(hipStream_t st, int nb, const double* rs, const double* r, const GcrSet& Q, const Grid& g,
                      double* part, double* out);
 // the coefficients of the step (device-resident cf; flag: 0, or 1 = the new
 // direction used alone, 2 = the cycle's result kept; qq: squared norm per slot)
