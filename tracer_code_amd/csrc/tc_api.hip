@@ -332,13 +332,13 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
     int batch = std::max(1, d.vis_hint[level][interpolate ? 1 : 0]);
     for (int queued = 0;;) {
         for (int b = 0; b < batch; ++b) {
+            // gradients, ratio and the copy that the sweep reads in one launch
             const int nb = launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1,
-                                               d.partialA, d.vctl + VC_DONE);
+                                               d.partialA, d.vctl + VC_DONE, L.vtmp);
             launch_vis_decide(ctx->stream, d.partialA, nb, P, d.vctl, d.vdctl);
-            launch_copy_gated(ctx->stream, L.vtmp, L.vis, L.g.elems, d.vctl + VC_NOSWEEP);
-            launch_vis_sweep(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax, 0.0,
-                             d.vdctl + VD_C, d.vctl + VC_NOSWEEP);
-            launch_visc_bcs(ctx->stream, L.vis, L.g, Vf, *gf, level, 0, p.RaT, p.visT * p.visT, d.vctl + VC_NOSWEEP);
+            // the sweep and viscosityBCs in one launch
+            launch_vis_sweep_bcs(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax,
+                                 d.vdctl + VD_C, d.vctl + VC_NOSWEEP, Vf, *gf, level, p.RaT, p.visT * p.visT);
         }
         queued += batch;
         TC_HIP(hipGetLastError());

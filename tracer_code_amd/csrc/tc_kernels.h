@@ -80,17 +80,23 @@ int launch_abs_partial(hipStream_t st, const double* f, const Grid& g, double* p
 void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, double* out, int slot);
 void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
                  const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC);
+// copy_dst: also copy the plane L.vis there (the source of smoother_vis's sweep)
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
-                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate = nullptr);
+                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate = nullptr,
+                        double* copy_dst = nullptr);
 void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream);
 // cptr / gate (device-controlled smoother_vis): the factor read from the
 // device, and the launch skipped while gate[0] is set
 void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
                       const FD& F, const Spacing& h, int kmin, int kmax, double c, const double* cptr = nullptr,
                       const int32_t* gate = nullptr);
+// one smoother_vis iteration's sweep (T0 -> V, factor *cptr) and the
+// viscosityBCs after it, in one launch skipped while gate[0] is set
+void launch_vis_sweep_bcs(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
+                          const FD& F, const Spacing& h, int kmin, int kmax, const double* cptr, const int32_t* gate,
+                          const double* Vf, const Grid& gf, int level, double RaT, double visT2);
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
                      double RaT, double visT2, const int32_t* gate = nullptr);
-void launch_copy_gated(hipStream_t st, double* dst, const double* src, size_t n, const int32_t* gate);
 // smoother_vis's loop state on the device (k_vis_decide)
 enum { VC_DONE = 0, VC_ERR, VC_NOSWEEP, VC_NN, VC_N, VC_NITER, VC_NVIS, VC_NNSTORE, VC_TAB, VC_NETA, VC_ITERS, VC_COUNT };
 enum { VD_C = 0, VD_EIG, VD_COUNT };

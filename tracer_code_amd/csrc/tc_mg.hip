@@ -218,18 +218,75 @@ __global__ __launch_bounds__(256) void k_visf(double* __restrict__ vis, const do
     vis[c] = v;
 }
 
+// The loop control of smoother_vis (smoother.f90:165-263) for one iteration
+// of a loop that has not stopped, from the minimum a of the eigenvalue
+// ratios: the exit test of the pass, and either the stop (ctl[VC_DONE],
+// with no sweep: ctl[VC_NOSWEEP]) or the Chebyshev factor of this
+// iteration's sweep (dctl[VD_C]) and the advance to the next iteration, with
+// the pass bookkeeping at the end of a pass.
+__device__ void vis_decide(double a, const VisDecide& P, int32_t* __restrict__ ctl, double* __restrict__ dctl)
+{
+    const double rmin = (P.ivis == 0) ? 1e99 : a;
+    ctl[VC_ITERS] += 1;
+    const int nn = ctl[VC_NN], n = ctl[VC_N], Nvs = ctl[VC_NVIS];
+    bool trig;
+    if (P.option == 1) trig = (Nvs >= 1 && rmin > P.eig_ratio_min) || P.ivis == 0 || n > 20;
+    else if (P.interpolate == 0) trig = rmin > P.eig_ratio || P.ivis == 0;
+    else trig = Nvs >= 1 || P.ivis == 0;
+    if (trig) {
+        ctl[VC_DONE] = 1;
+        ctl[VC_NOSWEEP] = 1;
+        if (P.option == 1) {
+            dctl[VD_EIG] = rmin;
+            ctl[VC_NETA] = n;
+        } else if (P.interpolate == 0) {
+            ctl[VC_NNSTORE] = nn;
+        }
+        return;
+    }
+    dctl[VD_C] = (ctl[VC_TAB] == 0) ? P.c0[nn - 1] : P.c1[nn - 1];
+    ctl[VC_NOSWEEP] = 0;
+    int next = nn + 1;
+    if (next > ctl[VC_NITER]) {  // end of the pass
+        ctl[VC_NVIS] = Nvs + 1;
+        ctl[VC_NNSTORE] = 1;
+        if (P.option == 1) {
+            ctl[VC_NITER] = 6;
+            ctl[VC_TAB] = 1;
+        }
+        ctl[VC_N] = n + 1;
+        next = 1;
+        if (n + 1 > 100000) {
+            ctl[VC_ERR] = 1;
+            ctl[VC_DONE] = 1;
+        }
+    }
+    ctl[VC_NN] = next;
+}
+
 // viscosity_gradients + test_eigenvalues (iterative_solver.f90:290-329,
 // 666-698, eigI 750-803, eigR 806-857) on uniform grids, with per-block
-// minima of the ratio for smoother_vis (smoother.f90:209).
+// minima of the ratio for smoother_vis (smoother.f90:209).  gate: the
+// device-controlled smoother_vis loop has stopped; copy_dst: the plane V is
+// also copied there (the source of the iteration's sweep, which k_vis_decide
+// may still switch off; the copy is then unused).
 template <bool POW2>
 __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__ V, double* __restrict__ vx,
                                                       double* __restrict__ vz, double* __restrict__ vxx,
                                                       double* __restrict__ vzz, double* __restrict__ vxz,
                                                       double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
                                                       EigScalars e, int want_ratio, double* __restrict__ partial,
-                                                      const int32_t* __restrict__ gate)
+                                                      const int32_t* __restrict__ gate,
+                                                      double* __restrict__ copy_dst)
 {
     if (gate && gate[0]) return;  // the device-controlled loop has ended
+    if (copy_dst) {
+        const size_t nt = (size_t)gridDim.x * gridDim.y * blockDim.x * blockDim.y;
+        for (size_t j = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x * blockDim.y) +
+                        threadIdx.y * blockDim.x + threadIdx.x;
+             j < g.elems; j += nt)
+            copy_dst[j] = V[j];
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     double m = INFINITY;
@@ -383,6 +440,60 @@ __global__ __launch_bounds__(256) void k_visc_bcs(double* __restrict__ V, Grid g
     }
 }
 
+// One iteration's sweep of smoother_vis (k_vis_sweep) with the
+// viscosityBCs that follow it (k_visc_bcs with initialize = 0) in one launch.
+// Thread (i,k), i = 0..nx, k = 0..nz: its value after the sweep (rows
+// kmin..kmax; the other rows keep theirs), after phase 1 of the boundary
+// routine (rows 0 and nz of a coarser level from the finer level), and then
+// the cells the routine fills from it: the side ghosts of columns 1..SPAN
+// and nx-SPAN..nx-1 (phase 2) and, from rows 1..SPAN and nz-SPAN..nz-1, the
+// bottom and top ghost rows in its own column and in its mirror ghost
+// column (phase 3).  Every cell k_visc_bcs writes is written here by the one
+// thread that owns its source, with the same arithmetic.
+template <bool POW2>
+__global__ __launch_bounds__(256) void k_vis_sweep_bcs(const double* __restrict__ T0, double* __restrict__ V,
+                                                       const double* __restrict__ dtv, Grid g, FD F, Spacing h,
+                                                       int kmin, int kmax, const double* __restrict__ cptr,
+                                                       const int32_t* __restrict__ gate,
+                                                       const double* __restrict__ Vf, Grid gf, int level, double RaT,
+                                                       double visT2)
+{
+    if (gate[0]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = g.nx, nz = g.nz;
+    if (i > nx || k > nz) return;
+    const size_t cc = g.at(i, k);
+    double v;
+    if (k >= kmin && k <= kmax) {
+        const double c = cptr[0];
+        double Trr = sdiv<POW2>(dot3(F.D2, T0, cc, 1), h.dr2, h.idr2);
+        double Tss = sdiv<POW2>(dot3(F.D2, T0, cc, g.ld), h.ds2, h.ids2);
+        double temp = Trr + Tss;
+        double dts = dtv[cc] * 0.99;
+        v = T0[cc] + c * dts * temp;
+    } else {
+        v = T0[cc];
+    }
+    if (level > 1 && (k == 0 || k == nz)) v = Vf[gf.at(2 * i, k == 0 ? 0 : gf.nz)];
+    V[cc] = v;
+    const bool lo_i = i >= 1 && i <= SPAN, hi_i = i >= nx - SPAN && i <= nx - 1;
+    if (lo_i) V[g.at(-i, k)] = v;
+    if (hi_i) V[g.at(2 * nx - i, k)] = v;
+    if (k >= 1 && k <= SPAN) {
+        const double b = (RaT != 0.0) ? 1.0 / v / visT2 : v;
+        V[g.at(i, -k)] = b;
+        if (lo_i) V[g.at(-i, -k)] = b;
+        if (hi_i) V[g.at(2 * nx - i, -k)] = b;
+    }
+    if (k >= nz - SPAN && k <= nz - 1) {
+        const double t = (RaT != 0.0) ? 1.0 / v : v;
+        V[g.at(i, 2 * nz - k)] = t;
+        if (lo_i) V[g.at(-i, 2 * nz - k)] = t;
+        if (hi_i) V[g.at(2 * nx - i, 2 * nz - k)] = t;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
@@ -472,16 +583,17 @@ void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& 
 }
 
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
-                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate)
+                        const EigScalars& e, int want_ratio, double* partial, const int32_t* gate,
+                        double* copy_dst)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     if (pow2)
         hipLaunchKernelGGL((k_vis_grad_eig<true>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst);
     else
         hipLaunchKernelGGL((k_vis_grad_eig<false>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst);
     return (int)(gr.x * gr.y);
 }
 
@@ -504,6 +616,20 @@ void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, co
         hipLaunchKernelGGL((k_vis_sweep<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, c, cptr, gate);
 }
 
+void launch_vis_sweep_bcs(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
+                          const FD& F, const Spacing& h, int kmin, int kmax, const double* cptr, const int32_t* gate,
+                          const double* Vf, const Grid& gf, int level, double RaT, double visT2)
+{
+    dim3 b(64, 4);
+    dim3 gr = grid2(g.nx + 1, g.nz + 1, b);
+    if (pow2)
+        hipLaunchKernelGGL((k_vis_sweep_bcs<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, cptr, gate, Vf, gf,
+                           level, RaT, visT2);
+    else
+        hipLaunchKernelGGL((k_vis_sweep_bcs<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, cptr, gate, Vf, gf,
+                           level, RaT, visT2);
+}
+
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
                      double RaT, double visT2, const int32_t* gate)
 {
@@ -513,28 +639,13 @@ void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf,
                        visT2, gate);
 }
 
-// A plane copy that a device-controlled loop can switch off.
-__global__ __launch_bounds__(256) void k_copy_gated(double* __restrict__ dst, const double* __restrict__ src, size_t n,
-                                                    const int32_t* __restrict__ gate)
-{
-    if (gate[0]) return;
-    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x)
-        dst[j] = src[j];
-}
 
-void launch_copy_gated(hipStream_t st, double* dst, const double* src, size_t n, const int32_t* gate)
-{
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_copy_gated, dim3(blocks), dim3(256), 0, st, dst, src, n, gate);
-}
 
-// The loop control of smoother_vis (smoother.f90:165-263) on the device, one
-// iteration per launch: the minimum of the per-block eigenvalue ratios (as
-// k_reduce_final), the exit test of the pass, and either the stop (ctl[0],
-// with no sweep: ctl[2]) or the Chebyshev factor of this iteration's sweep
-// (dctl[0]) and the advance to the next iteration, with the pass
-// bookkeeping at the end of a pass.  Once stopped, later launches change
-// nothing, so the host may queue more iterations than the loop needs.
+// The loop control of smoother_vis on the device, one iteration per launch:
+// the minimum of the per-block eigenvalue ratios (as k_reduce_final), then
+// vis_decide, or, once the loop has stopped, only no sweep.  Later launches
+// change nothing after the stop, so the host may queue more iterations than
+// the loop needs.
 __global__ __launch_bounds__(256) void k_vis_decide(const double* __restrict__ partial, int nb, VisDecide P,
                                                     int32_t* __restrict__ ctl, double* __restrict__ dctl)
 {
@@ -549,42 +660,7 @@ __global__ __launch_bounds__(256) void k_vis_decide(const double* __restrict__ p
         ctl[VC_NOSWEEP] = 1;
         return;
     }
-    const double rmin = (P.ivis == 0) ? 1e99 : a;
-    ctl[VC_ITERS] += 1;
-    const int nn = ctl[VC_NN], n = ctl[VC_N], Nvs = ctl[VC_NVIS];
-    bool trig;
-    if (P.option == 1) trig = (Nvs >= 1 && rmin > P.eig_ratio_min) || P.ivis == 0 || n > 20;
-    else if (P.interpolate == 0) trig = rmin > P.eig_ratio || P.ivis == 0;
-    else trig = Nvs >= 1 || P.ivis == 0;
-    if (trig) {
-        ctl[VC_DONE] = 1;
-        ctl[VC_NOSWEEP] = 1;
-        if (P.option == 1) {
-            dctl[VD_EIG] = rmin;
-            ctl[VC_NETA] = n;
-        } else if (P.interpolate == 0) {
-            ctl[VC_NNSTORE] = nn;
-        }
-        return;
-    }
-    dctl[VD_C] = (ctl[VC_TAB] == 0) ? P.c0[nn - 1] : P.c1[nn - 1];
-    ctl[VC_NOSWEEP] = 0;
-    int next = nn + 1;
-    if (next > ctl[VC_NITER]) {  // end of the pass
-        ctl[VC_NVIS] = Nvs + 1;
-        ctl[VC_NNSTORE] = 1;
-        if (P.option == 1) {
-            ctl[VC_NITER] = 6;
-            ctl[VC_TAB] = 1;
-        }
-        ctl[VC_N] = n + 1;
-        next = 1;
-        if (n + 1 > 100000) {
-            ctl[VC_ERR] = 1;
-            ctl[VC_DONE] = 1;
-        }
-    }
-    ctl[VC_NN] = next;
+    vis_decide(a, P, ctl, dctl);
 }
 
 void launch_vis_decide(hipStream_t st, const double* partial, int nb, const VisDecide& P, int32_t* ctl, double* dctl)
