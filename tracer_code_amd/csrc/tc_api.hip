@@ -332,9 +332,9 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
     int batch = std::max(1, d.vis_hint[level][interpolate ? 1 : 0]);
     for (int queued = 0;;) {
         for (int b = 0; b < batch; ++b) {
-            // gradients, ratio and the copy that the sweep reads in one launch
+            // the ratio's block minima and the copy that the sweep reads in one launch
             const int nb = launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1,
-                                               d.partialA, d.vctl + VC_DONE, L.vtmp);
+                                               d.partialA, d.vctl + VC_DONE, L.vtmp, false);
             launch_vis_decide(ctx->stream, d.partialA, nb, P, d.vctl, d.vdctl);
             // the sweep and viscosityBCs in one launch
             launch_vis_sweep_bcs(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax,
@@ -357,6 +357,9 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
         }
         batch = std::max(2, batch / 2);
     }
+    // the gradient and ratio planes of the final viscosity (what the loop's
+    // last iteration computed before its stop)
+    launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1, nullptr);
     const int iters = d.h_vctl[VC_ITERS];
     d.vis_hint[level][interpolate ? 1 : 0] = iters;
     ctx->stats.vis_iters += iters;

@@ -80,10 +80,11 @@ int launch_abs_partial(hipStream_t st, const double* f, const Grid& g, double* p
 void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, double* out, int slot);
 void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
                  const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC);
-// copy_dst: also copy the plane L.vis there (the source of smoother_vis's sweep)
+// copy_dst: also copy the plane L.vis there (the source of smoother_vis's
+// sweep); store = false: only the block minima of the ratio (no planes)
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
                         const EigScalars& e, int want_ratio, double* partial, const int32_t* gate = nullptr,
-                        double* copy_dst = nullptr);
+                        double* copy_dst = nullptr, bool store = true);
 void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream);
 // cptr / gate (device-controlled smoother_vis): the factor read from the
 // device, and the launch skipped while gate[0] is set

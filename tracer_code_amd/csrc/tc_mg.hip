@@ -269,7 +269,9 @@ __device__ void vis_decide(double a, const VisDecide& P, int32_t* __restrict__ c
 // minima of the ratio for smoother_vis (smoother.f90:209).  gate: the
 // device-controlled smoother_vis loop has stopped; copy_dst: the plane V is
 // also copied there (the source of the iteration's sweep, which k_vis_decide
-// may still switch off; the copy is then unused).
+// may still switch off; the copy is then unused); store = 0: the gradient and
+// ratio planes are not written (inside the loop only the block minima are
+// read, and the planes are written from the final viscosity after it).
 template <bool POW2>
 __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__ V, double* __restrict__ vx,
                                                       double* __restrict__ vz, double* __restrict__ vxx,
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
                                                       double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
                                                       EigScalars e, int want_ratio, double* __restrict__ partial,
                                                       const int32_t* __restrict__ gate,
-                                                      double* __restrict__ copy_dst)
+                                                      double* __restrict__ copy_dst, int store)
 {
     if (gate && gate[0]) return;  // the device-controlled loop has ended
     if (copy_dst) {
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
             gzz = sdiv<POW2>(dot3(F.D2, V, c, g.ld), h.ds2, h.ids2);
             gxz = sdiv<POW2>(sum9(F.Drs, V, c, g.ld), h.drds, h.idrds);
         }
-        vx[c] = gx; vz[c] = gz; vxx[c] = gxx; vzz[c] = gzz; vxz[c] = gxz;
+        if (store) { vx[c] = gx; vz[c] = gz; vxx[c] = gxx; vzz[c] = gzz; vxz[c] = gxz; }
         if (want_ratio && ivis != 0) {
             const double vis = V[c];
             // eigI with xr=zs=1, xrr=..=0: only the odd-derivative terms survive
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
             double r6 = 2.0 * gz * (0.0 + 0.0);
             double eR = -(r1 + r2 + r3 + r4 + r5 + r6);
             double r = fabs(eR / eI);
-            ratio[c] = r;
+            if (store) ratio[c] = r;
             m = r;
         }
     }
@@ -584,16 +586,16 @@ void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& 
 
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
                         const EigScalars& e, int want_ratio, double* partial, const int32_t* gate,
-                        double* copy_dst)
+                        double* copy_dst, bool store)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     if (pow2)
         hipLaunchKernelGGL((k_vis_grad_eig<true>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0);
     else
         hipLaunchKernelGGL((k_vis_grad_eig<false>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0);
     return (int)(gr.x * gr.y);
 }
 
