@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 SWEEP_BYTES_PER_POINT = {"fp64": 88.0, "dc64": 88.0, "mixed": 44.0}      # algorithmic
 KERNEL_BYTES_PER_POINT = {"fp64": 80.0, "dc64": 80.0, "mixed": 36.0}     # what the kernels move
 TRACER_BYTES_PER_STEP = 304.0 + 5 * 20.0  # RK4 stages + five binnings, per tracer
-DC_SWEEP_CANDIDATES = (16, 20, 24, 28, 32, 40, 48, 70)
+DC_SWEEP_CANDIDATES = (10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 70)
 # sweeps per visit on levels 3 and below, then on level 2 (0: the common count), tuned after it
 COARSE_SWEEP_CANDIDATES = (4, 8, 12, 16, 24, 0)
 LEVEL2_SWEEP_CANDIDATES = (8, 12, 16, 24, 0)
@@ -66,12 +66,19 @@ FP64_MAX_CYCLES = 200
 FILTER_BLOCKING = [True]  # several C-filter iterations per launch (--filter-blocking)
 TUNE_STEPS = 2  # real time steps timed per autotuner candidate (one step's time varies by ~2%)
 # sweeps per visit (level 1, level 2, levels 3 and below) of the mixed form
-# where they were measured: six steps after two, from the same state, per
-# setting, at 2048x1024 (level 1 16/20/24/28: 97.4/94.2/95.8/98.5 ms per
-# step with 16 and 8 below; level 2 at 12 or levels 3-5 at 6 slower).  The
-# autotuner's one- or two-step timings scatter by a few percent across runs,
-# so the headline uses these unless --autotune is given.
-MEASURED_COUNTS = {"cfg2": (20, 16, 8)}
+# where they were measured, with the level-1 Chebyshev interval below: eight
+# steps after two, from the same state, per setting, at 2048x1024 (level 1
+# 8/10/12/14/16 with la = -0.3: 89.7/80.5/80.3/81.3/82.2 ms per step, with
+# la = -0.25 at 10: 79.0, 78.1, 78.7; the reference interval at 20: 85.0 to
+# 85.4).  The autotuner's one- or two-step timings scatter by a few percent
+# across runs, so the headline uses these unless --autotune is given.
+MEASURED_COUNTS = {"cfg2": (10, 16, 8)}
+# Chebyshev interval (lb, la) of the mixed form's level-1 visits where it was
+# measured (tc_set_dc_cheb_interval; the reference's is (-2, -1)).  At
+# 2048x1024 la = -0.3 / -0.25 / -0.2 at 10-12 sweeps take 50.1 / 50.0 /
+# 52.9 cycles per step where the reference interval needs 20 sweeps for 50.2;
+# a wider interval on level 2 diverged and on levels 3-5 cost cycles.
+LEVEL1_CHEB = {"cfg2": (-2.0, -0.25)}
 
 
 def log(msg):
@@ -91,8 +98,10 @@ def copy_calibration(mib):
     return 0 if ok else 1
 
 
-def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1, sort_interval=0):
-    """A solver on the seeded inputs; returns (solver, network or None, guess)."""
+def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1, sort_interval=0, cheb=None):
+    """A solver on the seeded inputs; returns (solver, network or None, guess).
+    cheb: the mixed form's level-1 Chebyshev interval (lb, la), None for the
+    reference's."""
     from tracer_code_amd import Solver
     T, r, s, ty = inputs
     solver = Solver(params, len(r))
@@ -116,6 +125,8 @@ def make_solver(params, inputs, mode, guess, gcr, config_name, coarse_refresh=1,
     solver.guess_enable(guess)
     solver.set_gcr(gcr)
     solver.set_coarse_refresh(coarse_refresh)
+    if mode == "mixed" and cheb is not None:
+        solver.set_dc_cheb_interval(*cheb)
     return solver, net, guess
 
 
@@ -395,6 +406,9 @@ def main():
                     help="initial-guess candidates of the headline's solves")
     ap.add_argument("--gcr", type=int, default=16,
                     help="GCR directions kept over the defect-correction cycles (0: plain cycles)")
+    ap.add_argument("--level1-cheb-la", type=float, default=None,
+                    help="la of the mixed form's level-1 Chebyshev interval (-2, la) (default: the preset's "
+                         "measured one where there is one, else the reference's -1)")
     ap.add_argument("--coarse-refresh", type=int, default=4,
                     help="refactorise the coarsest operator every n-th solve of the defect-correction forms "
                          "(the outer cycles still converge to the tolerance on the current operator)")
@@ -435,8 +449,9 @@ def main():
 
     # ---- headline form ----------------------------------------------------
     gcr = args.gcr if args.mode != "fp64" else 0
+    cheb = (-2.0, args.level1_cheb_la) if args.level1_cheb_la is not None else LEVEL1_CHEB.get(args.config)
     solver, net, guess = make_solver(params, inputs, args.mode, args.guess, gcr, args.config, args.coarse_refresh,
-                                     args.sort_interval)
+                                     args.sort_interval, cheb)
     counts, rec = ({"common": params.Nmulti, "coarse": 0, "level2": 0, "tiles": None}, {})
     if args.mode != "fp64":
         counts, rec = tune_counts(solver, params, args.mode, args)
@@ -472,7 +487,7 @@ def main():
     for name in want:
         if name == "learned":
             s2, n2, g2 = make_solver(params, inputs, "mixed", "learned", gcr, args.config, args.coarse_refresh,
-                                     args.sort_interval)
+                                     args.sort_interval, cheb)
             apply_counts(s2, params, counts)
             res = run_timed(s2, params, 1, args.leg_steps, "mixed+learned", until_time=t_head)
             close_solver(s2, n2)
@@ -557,6 +572,7 @@ def main():
             "guess": guess,
             "gcr_directions": gcr,
             "coarse_refresh_solves": args.coarse_refresh,
+            "level1_cheb_interval": list(cheb) if (cheb and args.mode == "mixed") else [-2.0, -1.0],
             "tracer_sort_interval": args.sort_interval,
             "sweeps_per_visit": counts["common"],
             "sweeps_per_visit_level_2": counts["level2"] or counts["common"],

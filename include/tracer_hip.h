@@ -225,6 +225,16 @@ int tc_set_filter_blocking(tc_ctx* ctx, int32_t enable);
  * (0 < omega < 2).  sweeps = 0 (the default) keeps the Jacobi-Richardson
  * sweeps.  The outer residual, stop test and tolerance are unchanged. */
 int tc_set_gs_smoother(tc_ctx* ctx, int32_t sweeps, int32_t per_launch, double omega);
+/* Chebyshev interval of the level-1 visits of the FP32 defect-correction
+ * cycle (mode TC_MODE_MIXED).  The reference's sweep factors
+ * (iterate_stream, iterative_solver.f90:873-899) take the step of sweep n of
+ * N as c = 2 / (-lb - la + (lb - la) cos((2(N-n+1)-1) pi / 2N)) with
+ * lb = -2, la = -1 (the default here too).  A wider interval (la closer to
+ * 0, lb <= la < 0 required) also damps the middle of the spectrum, so fewer
+ * sweeps per visit do the same work; the coarser levels, the reference
+ * form and the dc64 mode keep the reference's factors.  The outer residual,
+ * stop test and tolerance are unchanged. */
+int tc_set_dc_cheb_interval(tc_ctx* ctx, double lb, double la);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps
  * the largest candidate whose wall time per step is within 3% of the

@@ -23,11 +23,14 @@ STEPS = 3
 # the measured forms of the 2048x1024 benchmark: GCR over the cycles with 16
 # directions, 16 sweeps per visit on level 2, 8 below, the learned guess
 # (network of that workload) judged beside the stage increment and the plain
-# start, and the level-1 counts its autotuner picks (20 to 40)
+# start, and the level-1 counts its autotuner picks (20 to 40); the current
+# headline form: 10 level-1 sweeps on the Chebyshev interval (-2, -0.25),
+# the stage increment judged beside the plain start
 BENCH_FORM = dict(gcr=16, level_sweeps={2: 16, "coarse": 8}, guess="learned")
+HEADLINE_FORM = dict(gcr=16, level_sweeps={2: 16, "coarse": 8}, guess="extrap", cheb=(-2.0, -0.25))
 
 
-def run(mode, params, T, r, s, ty, dc_sweeps=0, guess="plain", gcr=0, level_sweeps=None):
+def run(mode, params, T, r, s, ty, dc_sweeps=0, guess="plain", gcr=0, level_sweeps=None, cheb=None):
     g = Solver(params, len(r))
     g.initialize(T, r, s, ty)
     g.set_mode(mode)
@@ -46,6 +49,8 @@ def run(mode, params, T, r, s, ty, dc_sweeps=0, guess="plain", gcr=0, level_swee
             g.learn_attach(net)
         g.guess_enable(guess)
         g.set_gcr(gcr)
+        if cheb is not None:
+            g.set_dc_cheb_interval(*cheb)
         for L, n in (level_sweeps or {}).items():
             for lv in (range(3, params.grids) if L == "coarse" else (L,)):
                 g.set_dc_level_sweeps(lv, n)
@@ -85,7 +90,8 @@ def test_accelerated_forms_against_fp64():
     # the defect-correction modes with the reference's sweep count, and the
     # mixed mode with a tuned count and the extrapolated starts judged
     forms = [("dc64", 0, "plain", {}), ("mixed", 0, "plain", {}), ("mixed", 32, "extrap", {}),
-             ("mixed", 40, BENCH_FORM["guess"], BENCH_FORM), ("mixed", 20, BENCH_FORM["guess"], BENCH_FORM)]
+             ("mixed", 40, BENCH_FORM["guess"], BENCH_FORM), ("mixed", 20, BENCH_FORM["guess"], BENCH_FORM),
+             ("mixed", 10, HEADLINE_FORM["guess"], HEADLINE_FORM)]
     for mode, sweeps, guess, extra in forms:
         kw = {k: v for k, v in extra.items() if k != "guess"}
         f = run(mode, params, T, r, s, ty, sweeps, guess, **kw)

@@ -132,6 +132,7 @@ SIGNATURES = {
     "tc_set_tracer_sort": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "tc_tracer_slots": (ctypes.c_int, [_VP, _VP]),
     "tc_set_gs_smoother": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+    "tc_set_dc_cheb_interval": (ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_double]),
     "tc_autotune_dc_sweeps": (ctypes.c_int, [_VP, _VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP]),
     "tc_autotune_dc_level_sweeps": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _VP, ctypes.c_int32,
                                                    ctypes.c_int32, _VP, _VP]),

@@ -489,6 +489,9 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
         }
         double temp, cconst;
         cheb_factors(n, Niter, dir, temp, cconst);
+        // the FP32 cycle's level-1 interval (tc_set_dc_cheb_interval)
+        if (f32 && level == 1 && dir != 1)
+            cconst = 2.0 / (-ctx->cheb1_lb - ctx->cheb1_la + (ctx->cheb1_lb - ctx->cheb1_la) * temp);
         if (timed && !timed_visit) TC_TRY(prof_begin(ctx));
         if (f32) {
             launch_sweep32(ctx->stream, use_ratio, L, L.err32[L.cur32], L.err32[1 - L.cur32], rhs32, d.F, d.h[level],
@@ -2395,6 +2398,18 @@ int tc_set_coarse_refresh(tc_ctx* ctx, int32_t solves)
     }
     ctx->coarse_refresh = solves;
     ctx->coarse_since = 0;
+    return 0;
+}
+
+int tc_set_dc_cheb_interval(tc_ctx* ctx, double lb, double la)
+{
+    if (!ctx) return TC_ERR_ARG;
+    if (!(lb < 0.0 && la < 0.0 && lb <= la && std::isfinite(lb))) {
+        ctx->err = "tc_set_dc_cheb_interval: need lb <= la < 0";
+        return TC_ERR_ARG;
+    }
+    ctx->cheb1_lb = lb;
+    ctx->cheb1_la = la;
     return 0;
 }
 

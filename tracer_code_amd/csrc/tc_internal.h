@@ -191,6 +191,9 @@ struct tc_ctx {
     // (tc_set_gs_smoother): sweeps per visit (0: Jacobi-Richardson), per launch
     int gs_sweeps = 0, gs_per_launch = 2;
     double gs_omega = 1.0;
+    // Chebyshev interval of the level-1 visits of the FP32 correction cycle
+    // (tc_set_dc_cheb_interval; the reference's -2, -1 by default)
+    double cheb1_lb = -2.0, cheb1_la = -1.0;
     // tracers (SoA)
     double *rtr = nullptr, *str = nullptr, *rtr0 = nullptr, *str0 = nullptr;
     double *rtr1 = nullptr, *str1 = nullptr, *rtr2 = nullptr, *str2 = nullptr;

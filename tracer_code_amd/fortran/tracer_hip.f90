@@ -344,6 +344,11 @@ module tracer_hip
       integer(c_int32_t), value :: sweeps, per_launch
       real(c_double), value :: omega
     end function
+    integer(c_int) function tc_set_dc_cheb_interval(ctx, lb, la) bind(C, name="tc_set_dc_cheb_interval")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), value :: lb, la
+    end function
     integer(c_int) function tc_tracer_slots(ctx, caller_index) bind(C, name="tc_tracer_slots")
       import :: c_int, c_int64_t, c_ptr
       type(c_ptr), value :: ctx

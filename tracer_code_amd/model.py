@@ -282,6 +282,12 @@ class Solver:
         self._check(self.lib.tc_set_gs_smoother(self.h, int(sweeps), int(per_launch), float(omega)),
                     "set_gs_smoother")
 
+    def set_dc_cheb_interval(self, lb=-2.0, la=-1.0):
+        """Chebyshev interval of the mixed mode's level-1 visits (the
+        reference's -2, -1 by default; la closer to 0 damps more of the
+        spectrum per visit)."""
+        self._check(self.lib.tc_set_dc_cheb_interval(self.h, float(lb), float(la)), "set_dc_cheb_interval")
+
     def tracer_slots(self):
         """The caller's index of the tracer in each device slot."""
         out = np.empty(self.ntr, dtype=np.int64)
