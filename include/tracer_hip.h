@@ -231,9 +231,11 @@ int tc_set_gs_smoother(tc_ctx* ctx, int32_t sweeps, int32_t per_launch, double o
  * N as c = 2 / (-lb - la + (lb - la) cos((2(N-n+1)-1) pi / 2N)) with
  * lb = -2, la = -1 (the default here too).  A wider interval (la closer to
  * 0, lb <= la < 0 required) also damps the middle of the spectrum, so fewer
- * sweeps per visit do the same work; the coarser levels, the reference
- * form and the dc64 mode keep the reference's factors.  The outer residual,
- * stop test and tolerance are unchanged. */
+ * sweeps per visit do the same work.  It applies to the solves at a tuned
+ * count (tc_set_dc_sweeps, other than Nmulti); the Nmulti retry of a solve
+ * that diverged or stalled, the coarser levels, the reference form and the
+ * dc64 mode keep the reference's factors.  The outer residual, stop test
+ * and tolerance are unchanged. */
 int tc_set_dc_cheb_interval(tc_ctx* ctx, double lb, double la);
 /* Autotuner of the defect-correction sweep count: runs steps_each time
  * steps (real steps of the run) with each of the n candidates, and keeps

@@ -489,8 +489,13 @@ int run_sweeps(tc_ctx* ctx, tc_dev& d, int level, int Niter, int dir, bool f32, 
         }
         double temp, cconst;
         cheb_factors(n, Niter, dir, temp, cconst);
-        // the FP32 cycle's level-1 interval (tc_set_dc_cheb_interval)
-        if (f32 && level == 1 && dir != 1)
+        // the FP32 cycle's level-1 interval (tc_set_dc_cheb_interval), at a
+        // tuned count only: the Nmulti retry of a solve that diverged or
+        // stalled runs the reference's factors (a wide interval's steps reach
+        // 4x the local time step, and Nmulti of them amplify the FP32
+        // rounding of the high modes until the solve blows up)
+        if (f32 && level == 1 && dir != 1 && ctx->dc_sweeps > 0 && ctx->dc_sweeps != P.Nmulti &&
+            ctx->dc_sweeps_active == ctx->dc_sweeps)
             cconst = 2.0 / (-ctx->cheb1_lb - ctx->cheb1_la + (ctx->cheb1_lb - ctx->cheb1_la) * temp);
         if (timed && !timed_visit) TC_TRY(prof_begin(ctx));
         if (f32) {

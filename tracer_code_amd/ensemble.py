@@ -45,10 +45,11 @@ def draw_cases(n, seed=0):
     return cases
 
 
-def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None, gcr=0, log=None):
+def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune=None, gcr=0, log=None, cheb=None):
     """(solvers, networks) initialised from each case's seeded inputs; `tune`
     (per case: dict of sweeps, tiles, level2, coarse) applies autotuned
-    settings; gcr: GCR directions over the defect-correction cycles."""
+    settings; gcr: GCR directions over the defect-correction cycles; cheb:
+    the mixed mode's level-1 Chebyshev interval (lb, la)."""
     solvers, nets = [], []
     for j, c in enumerate(cases):
         if log and j % 16 == 0:
@@ -67,6 +68,8 @@ def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune
                 nets.append(net)
             g.guess_enable(guess)
             g.set_gcr(gcr)
+            if cheb is not None and mode == "mixed":
+                g.set_dc_cheb_interval(*cheb)
             if tune:
                 t = tune[len(solvers)]
                 g.set_dc_sweeps(t["sweeps"])
@@ -81,16 +84,17 @@ def make_solvers(cases, mode="mixed", guess="plain", weights=None, level=4, tune
     return solvers, nets
 
 
-def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70), gcr=0, log=None):
+def autotune_cases(cases, mode="mixed", candidates=(28, 32, 40, 48, 70), gcr=0, log=None, cheb=None):
     """Per case: the sweep counts and the tiles the autotuners pick (on a
     scratch copy of the case, advanced by the tuning steps).  With GCR the
     levels from 3 down and level 2 get counts of their own, and the common
-    count is tuned again after them."""
+    count is tuned again after them.  cheb: the level-1 interval the cases
+    run with (the tuning uses it too)."""
     out = []
     for j, c in enumerate(cases):
         if log and j % 8 == 0:
             log(f"[ensemble] autotuning cases {j}..{min(j + 8, len(cases)) - 1} of {len(cases)}")
-        (g,), _ = make_solvers([c], mode, gcr=gcr)
+        (g,), _ = make_solvers([c], mode, gcr=gcr, cheb=cheb)
         sweeps, _ = g.autotune_dc_sweeps(candidates, 1)
         level2 = coarse = 0
         if gcr and g.params.grids > 3:
@@ -115,13 +119,14 @@ def ensemble_step(solvers, steps, threads=0):
 
 
 def run(cases, steps, mode="mixed", guess="plain", weights=None, threads=0, tune=None, log=print, graphs=False,
-        gcr=0):
+        gcr=0, cheb=None):
     """Throughput of the cases run concurrently and one after another, from
     the same initial states (each after one untimed warm-up step).  graphs:
-    replay the coarse part of each defect-correction cycle as a HIP graph."""
+    replay the coarse part of each defect-correction cycle as a HIP graph;
+    cheb: the mixed mode's level-1 Chebyshev interval."""
     out, hashes = {}, []
     for concurrent in (False, True):
-        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune, gcr=gcr, log=log)
+        solvers, nets = make_solvers(cases, mode, guess, weights, tune=tune, gcr=gcr, log=log, cheb=cheb)
         for g in solvers:
             g.set_graphs(graphs)
         ensemble_step(solvers, 1, threads)  # warm-up, concurrently in both arms
@@ -182,15 +187,22 @@ def main(argv=None):
     ap.add_argument("--autotune", action="store_true", help="tune sweeps and tiles per case first (untimed)")
     ap.add_argument("--graphs", action="store_true", help="HIP-graph replay of each cycle's coarse part")
     ap.add_argument("--gcr", type=int, default=0, help="GCR directions over the defect-correction cycles")
+    ap.add_argument("--cheb-la", type=float, default=None,
+                    help="la of the mixed mode's level-1 Chebyshev interval (-2, la); the autotuner then also "
+                         "tries 10 to 20 sweeps per visit (default: the reference's interval)")
     a = ap.parse_args(argv)
+    cheb = (-2.0, a.cheb_la) if a.cheb_la is not None else None
     cases = draw_cases(a.cases, a.seed)
     w = None
     if a.guess == "learned":
         w, _ = learn.load_weights(os.path.join(os.path.dirname(__file__), "weights", "unet_cfg1_L4.npz"))
     gcr = a.gcr if a.mode != "fp64" else 0
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    tune = autotune_cases(cases, a.mode, gcr=gcr, log=log) if (a.autotune and a.mode != "fp64") else None
-    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune, log=log, graphs=a.graphs, gcr=gcr)
+    cand = (10, 12, 16, 20, 28, 32, 40, 48, 70) if cheb else (28, 32, 40, 48, 70)
+    tune = (autotune_cases(cases, a.mode, candidates=cand, gcr=gcr, log=log, cheb=cheb)
+            if (a.autotune and a.mode != "fp64") else None)
+    res = run(cases, a.steps, a.mode, a.guess, w, a.threads, tune, log=log, graphs=a.graphs, gcr=gcr, cheb=cheb)
+    res["cheb"] = list(cheb) if cheb else None
     res["graphs"] = a.graphs
     res["gcr"] = gcr
     if tune:
