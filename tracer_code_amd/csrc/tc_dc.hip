@@ -211,6 +211,19 @@ __global__ __launch_bounds__(256) void k_dd_absorb(double* __restrict__ hi, doub
     stream_ghosts(e, g, i, k, T(0), T(zsg));
 }
 
+// The bilinear prolongation of a coarse correction at fine interior (i,k)
+// (prolong, iterative_solver.f90:175-233), in FP64 as k_prolong_dd adds it.
+template <typename T>
+__device__ __forceinline__ double prolong_val(const T* __restrict__ Ec, const Grid& gc, int i, int k)
+{
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    if (ie && ke) return (double)Ec[gc.at(i / 2, k / 2)];
+    if (ie) return ((double)Ec[gc.at(i / 2, (k - 1) / 2)] + (double)Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
+    if (ke) return ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
+    return ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
+            (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+}
+
 // psi(hi,lo) += bilinear prolongation of the level-2 correction
 // (prolong, iterative_solver.f90:175-233) with the ghost mirrors.  Folding
 // the coarse-grid correction into psi before the level-1 post-smoothing keeps
@@ -224,32 +237,12 @@ __global__ __launch_bounds__(256) void k_prolong_dd(double* __restrict__ hi, dou
     const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > gf.nx - 1 || k > gf.nz - 1) return;
     const size_t c = gf.at(i, k);
-    double add;
-    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
-    if (ie && ke) add = (double)Ec[gc.at(i / 2, k / 2)];
-    else if (ie) add = ((double)Ec[gc.at(i / 2, (k - 1) / 2)] + (double)Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
-    else if (ke) add = ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
-    else
-        add = ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
-               (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+    const double add = prolong_val(Ec, gc, i, k);
     dd s = dd_add({hi[c], lo[c]}, {add, 0.0});
     hi[c] = s.hi;
     lo[c] = s.lo;
     stream_ghosts(hi, gf, i, k, s.hi, zsg);
     stream_ghosts(lo, gf, i, k, s.lo, zsg);
-}
-
-// The bilinear prolongation of a coarse correction at fine interior (i,k)
-// (prolong, iterative_solver.f90:175-233), in FP64 as k_prolong_dd adds it.
-template <typename T>
-__device__ __forceinline__ double prolong_val(const T* __restrict__ Ec, const Grid& gc, int i, int k)
-{
-    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
-    if (ie && ke) return (double)Ec[gc.at(i / 2, k / 2)];
-    if (ie) return ((double)Ec[gc.at(i / 2, (k - 1) / 2)] + (double)Ec[gc.at(i / 2, (k + 1) / 2)]) / 2.0;
-    if (ke) return ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
-    return ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
-            (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
 }
 
 // The defect kept up to date by increments: psi(hi,lo) += e (as k_dd_absorb
