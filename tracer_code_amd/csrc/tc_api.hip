@@ -136,19 +136,90 @@ EigScalars eig_scalars(const tc_params& p, const Level& L)
 
 size_t plane_bytes(const Grid& g) { return g.elems * sizeof(double); }
 
-// host <-> device copies of a ghosted Fortran-layout plane (ld = nx+5)
+// Host <-> device copies of rows of a pitched device array through two pinned
+// chunk buffers: the caller's arrays are pageable (a Fortran driver's module
+// arrays), and a pitched copy from pageable memory runs at ~2 GB/s.  Chunk c
+// goes by DMA between the device and buffer c%2 while the host copies the
+// previous chunk between the other buffer and the caller's array.  Both
+// return with the host array read or written (the boundary's contract).
+constexpr size_t XFER_CHUNK = (size_t)2 << 20;  // bytes per pinned buffer
+static int xfer_init(tc_ctx* ctx)
+{
+    for (int b = 0; b < 2; ++b) {
+        if (!ctx->xfer_buf[b]) TC_HIP(hipHostMalloc(&ctx->xfer_buf[b], XFER_CHUNK, hipHostMallocDefault));
+        if (!ctx->xfer_ev[b]) TC_HIP(hipEventCreateWithFlags(&ctx->xfer_ev[b], hipEventDisableTiming));
+    }
+    return 0;
+}
+static int copy_rows_d2h(tc_ctx* ctx, void* host, const void* dev, size_t dpitch, size_t width, size_t rows)
+{
+    TC_TRY(xfer_init(ctx));
+    const size_t per = std::max<size_t>(1, XFER_CHUNK / width);
+    if (per * width > XFER_CHUNK) {  // one row wider than a buffer: a plain copy
+        TC_HIP(hipMemcpy2DAsync(host, width, dev, dpitch, width, rows, hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    const size_t nch = (rows + per - 1) / per;
+    auto issue = [&](size_t c) -> int {
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        TC_HIP(hipMemcpy2DAsync(ctx->xfer_buf[c % 2], width, static_cast<const char*>(dev) + r0 * dpitch, dpitch, width,
+                                nr, hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipEventRecord(ctx->xfer_ev[c % 2], ctx->stream));
+        return 0;
+    };
+    for (size_t c = 0; c < std::min<size_t>(2, nch); ++c) TC_TRY(issue(c));
+    for (size_t c = 0; c < nch; ++c) {
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        TC_HIP(hipEventSynchronize(ctx->xfer_ev[c % 2]));
+        std::memcpy(static_cast<char*>(host) + r0 * width, ctx->xfer_buf[c % 2], nr * width);
+        if (c + 2 < nch) TC_TRY(issue(c + 2));
+    }
+    return 0;
+}
+static int copy_rows_h2d(tc_ctx* ctx, void* dev, size_t dpitch, const void* host, size_t width, size_t rows)
+{
+    TC_TRY(xfer_init(ctx));
+    const size_t per = std::max<size_t>(1, XFER_CHUNK / width);
+    if (per * width > XFER_CHUNK) {
+        TC_HIP(hipMemcpy2DAsync(dev, dpitch, host, width, width, rows, hipMemcpyHostToDevice, ctx->stream));
+        TC_HIP(hipStreamSynchronize(ctx->stream));
+        return 0;
+    }
+    const size_t nch = (rows + per - 1) / per;
+    for (size_t c = 0; c < nch; ++c) {
+        const size_t r0 = c * per, nr = std::min(per, rows - r0);
+        if (c >= 2) TC_HIP(hipEventSynchronize(ctx->xfer_ev[c % 2]));  // its buffer's last DMA is done
+        std::memcpy(ctx->xfer_buf[c % 2], static_cast<const char*>(host) + r0 * width, nr * width);
+        TC_HIP(hipMemcpy2DAsync(static_cast<char*>(dev) + r0 * dpitch, dpitch, ctx->xfer_buf[c % 2], width, width, nr,
+                                hipMemcpyHostToDevice, ctx->stream));
+        TC_HIP(hipEventRecord(ctx->xfer_ev[c % 2], ctx->stream));
+    }
+    TC_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// host <-> device copies of a ghosted Fortran-layout plane (ld = nx+5),
+// complete on return
 int put_plane(tc_ctx* ctx, double* dev, const Grid& g, const double* host)
 {
     const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
-    TC_HIP(hipMemcpy2DAsync(dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double), host, w * sizeof(double),
-                            w * sizeof(double), g.nz + 2 * SPAN + 1, hipMemcpyHostToDevice, ctx->stream));
-    return 0;
+    return copy_rows_h2d(ctx, dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double), host, w * sizeof(double),
+                         (size_t)(g.nz + 2 * SPAN + 1));
 }
 int get_plane(tc_ctx* ctx, const double* dev, const Grid& g, double* host)
 {
     const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
-    TC_HIP(hipMemcpy2DAsync(host, w * sizeof(double), dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double),
-                            w * sizeof(double), g.nz + 2 * SPAN + 1, hipMemcpyDeviceToHost, ctx->stream));
+    return copy_rows_d2h(ctx, host, dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double), w * sizeof(double),
+                         (size_t)(g.nz + 2 * SPAN + 1));
+}
+// the same as put_plane without the pinned buffers (one-time set-up copies)
+int put_plane_direct(tc_ctx* ctx, double* dev, const Grid& g, const double* host)
+{
+    const size_t w = (size_t)(g.nx + 2 * SPAN + 1);
+    TC_HIP(hipMemcpy2DAsync(dev + g.at(-SPAN, -SPAN), g.ld * sizeof(double), host, w * sizeof(double),
+                            w * sizeof(double), g.nz + 2 * SPAN + 1, hipMemcpyHostToDevice, ctx->stream));
+    TC_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 int put_iplane(tc_ctx* ctx, int32_t* dev, const Grid& g, const int32_t* host)
@@ -2145,6 +2216,10 @@ void free_ctx(tc_ctx* ctx)
     if (ctx->ev_snap_h) (void)hipEventDestroy(ctx->ev_snap_h);
     if (ctx->snap_stream) (void)hipStreamDestroy(ctx->snap_stream);
     if (ctx->h_flag) (void)hipHostFree(ctx->h_flag);
+    for (int b = 0; b < 2; ++b) {
+        if (ctx->xfer_buf[b]) (void)hipHostFree(ctx->xfer_buf[b]);
+        if (ctx->xfer_ev[b]) (void)hipEventDestroy(ctx->xfer_ev[b]);
+    }
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
         F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->fbuf); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
@@ -2313,11 +2388,11 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         std::vector<double> plane;
         for (int j = 1; j <= ctx->grids; ++j) {
             host_dt_vis(*p, ctx->lv[j], plane);
-            TC_TRY(put_plane(ctx, ctx->lv[j].dtvis, ctx->lv[j].g, plane.data()));
+            TC_TRY(put_plane_direct(ctx, ctx->lv[j].dtvis, ctx->lv[j].g, plane.data()));
             TC_HIP(hipStreamSynchronize(ctx->stream));
         }
         host_dts(*p, ctx->lv[1], plane);
-        TC_TRY(put_plane(ctx, ctx->dts, ctx->g, plane.data()));
+        TC_TRY(put_plane_direct(ctx, ctx->dts, ctx->g, plane.data()));
         TC_HIP(hipStreamSynchronize(ctx->stream));
     }
     TC_HIP(hipDeviceSynchronize());

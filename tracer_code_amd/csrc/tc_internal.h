@@ -268,6 +268,10 @@ struct tc_ctx {
     hipStream_t snap_stream = nullptr;
     hipEvent_t ev_snap_d = nullptr, ev_snap_h = nullptr;
     bool snap_staged = false;
+    // host <-> device plane copies through two pinned chunk buffers
+    // (tc_get_field / tc_put_field): allocated on first use
+    void* xfer_buf[2] = {nullptr, nullptr};
+    hipEvent_t xfer_ev[2] = {nullptr, nullptr};
     // kernel timing (tc_set_profiling)
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
