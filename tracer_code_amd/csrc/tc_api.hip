@@ -86,8 +86,9 @@ struct tc_dev {
     int32_t* h_ctl = nullptr;  // pinned mirror
     // device loop control of smoother_vis (k_vis_decide): state, factor and
     // ratio, their pinned mirrors, and the pinned initial state
-    int32_t* vctl = nullptr;
-    double* vdctl = nullptr;
+    int32_t* vctl = nullptr;   // two loop-state buffers of smoother_vis (VC_COUNT each)
+    double* vdctl = nullptr;   // two (VD_COUNT each)
+    unsigned long long* vslots = nullptr;  // two sets of VIS_SLOTS minimum-ratio slots
     int32_t* h_vctl = nullptr;
     double* h_vdctl = nullptr;
     int32_t* h_vinit = nullptr;
@@ -397,24 +398,35 @@ int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
     for (int j = 0; j < VC_COUNT; ++j) h[j] = 0;
     h[VC_NN] = nn0; h[VC_N] = n0; h[VC_NITER] = Ncur; h[VC_NVIS] = Nvs; h[VC_NNSTORE] = (nn0 != ctx->nn_store) ? 1 : ctx->nn_store;
     h[VC_TAB] = tab;
+    // the loop state of iteration j is buffer j % 2 (k_vis_sweep_bcs writes the
+    // next one); the slot sets of the first two iterations start at the
+    // largest bit pattern (atomicMin)
     TC_HIP(hipMemcpyAsync(d.vctl, h, VC_COUNT * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    TC_HIP(hipMemsetAsync(d.vdctl, 0, VD_COUNT * sizeof(double), ctx->stream));
+    TC_HIP(hipMemsetAsync(d.vslots, 0xff, 2 * VIS_SLOTS * VIS_SLOT_STRIDE * sizeof(unsigned long long), ctx->stream));
     ctx->ratio_level = (p.ivis == 0) ? -1 : level;
     const EigScalars e = eig_scalars(p, L);
     int batch = std::max(1, d.vis_hint[level][interpolate ? 1 : 0]);
+    const size_t slot_set = (size_t)VIS_SLOTS * VIS_SLOT_STRIDE;
     for (int queued = 0;;) {
         for (int b = 0; b < batch; ++b) {
-            // the ratio's block minima and the copy that the sweep reads in one launch
-            const int nb = launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1,
-                                               d.partialA, d.vctl + VC_DONE, L.vtmp, false);
-            launch_vis_decide(ctx->stream, d.partialA, nb, P, d.vctl, d.vdctl);
-            // the sweep and viscosityBCs in one launch
+            const int j = (queued + b) & 1, jn = 1 - j;
+            VisLoopArgs lp{d.vctl + j * VC_COUNT, d.vctl + jn * VC_COUNT, d.vdctl + j * VD_COUNT,
+                           d.vdctl + jn * VD_COUNT, d.vslots + j * slot_set, d.vslots + jn * slot_set, P};
+            // the ratio's block minima into the slots and the copy that the sweep reads
+            launch_vis_grad_eig(ctx->stream, d.pow2[level], L, d.F, d.h[level], p.ivis, e, 1, nullptr,
+                                lp.ctl_in + VC_DONE, L.vtmp, false, d.vslots + j * slot_set);
+            // the decision, the sweep and viscosityBCs in one launch
             launch_vis_sweep_bcs(ctx->stream, d.pow2[level], L.vtmp, L.vis, L.dtvis, L.g, d.F, d.h[level], kmin, kmax,
-                                 d.vdctl + VD_C, d.vctl + VC_NOSWEEP, Vf, *gf, level, p.RaT, p.visT * p.visT);
+                                 lp, Vf, *gf, level, p.RaT, p.visT * p.visT);
         }
         queued += batch;
+        const int jf = queued & 1;  // the state after the last queued iteration
         TC_HIP(hipGetLastError());
-        TC_HIP(hipMemcpyAsync(d.h_vctl, d.vctl, VC_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-        TC_HIP(hipMemcpyAsync(d.h_vdctl, d.vdctl, VD_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        TC_HIP(hipMemcpyAsync(d.h_vctl, d.vctl + jf * VC_COUNT, VC_COUNT * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        TC_HIP(hipMemcpyAsync(d.h_vdctl, d.vdctl + jf * VD_COUNT, VD_COUNT * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
         TC_HIP(hipStreamSynchronize(ctx->stream));
         if (d.h_vctl[VC_ERR]) {
             ctx->err = "smoother_vis: eigenvalue-ratio test not met after 100000 passes";
@@ -2223,7 +2235,7 @@ void free_ctx(tc_ctx* ctx)
     tc_dev* d = static_cast<tc_dev*>(ctx->impl);
     if (d) {
         F(d->d_scal); F(d->partialA); F(d->partialB); F(d->ctl); F(d->fslots); F(d->fbuf); F(d->diag); F(d->diag_part); F(d->stats_part); F(d->stats_out);
-        F(d->vctl); F(d->vdctl);
+        F(d->vctl); F(d->vdctl); F(d->vslots);
         if (d->h_scal) (void)hipHostFree(d->h_scal);
         if (d->h_ctl) (void)hipHostFree(d->h_ctl);
         if (d->h_vctl) (void)hipHostFree(d->h_vctl);
@@ -2376,8 +2388,9 @@ int init_ctx(tc_ctx* ctx, const tc_params* p, int64_t ntr)
         TC_HIP(hipMalloc((void**)&d->fslots, filter_slot_words() * sizeof(unsigned long long)));
         TC_TRY(dalloc(ctx, &d->fbuf, g.elems));
         TC_HIP(hipHostMalloc((void**)&d->h_ctl, 4 * sizeof(int32_t), hipHostMallocDefault));
-        TC_TRY(dalloc_i(ctx, &d->vctl, VC_COUNT));
-        TC_TRY(dalloc(ctx, &d->vdctl, VD_COUNT));
+        TC_TRY(dalloc_i(ctx, &d->vctl, 2 * VC_COUNT));
+        TC_TRY(dalloc(ctx, &d->vdctl, 2 * VD_COUNT));
+        TC_HIP(hipMalloc((void**)&d->vslots, 2 * VIS_SLOTS * VIS_SLOT_STRIDE * sizeof(unsigned long long)));
         TC_HIP(hipHostMalloc((void**)&d->h_vctl, VC_COUNT * sizeof(int32_t), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_vdctl, VD_COUNT * sizeof(double), hipHostMallocDefault));
         TC_HIP(hipHostMalloc((void**)&d->h_vinit, VC_COUNT * sizeof(int32_t), hipHostMallocDefault));

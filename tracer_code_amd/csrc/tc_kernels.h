@@ -81,20 +81,22 @@ void launch_reduce_final(hipStream_t st, const double* partial, int n, int op, d
 void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& Cvis, const double* zg,
                  const Grid& g, int ivis, int comp, int ntype, double RaT, double lvisT, double lvisP, const RaCArr& lvisC);
 // copy_dst: also copy the plane L.vis there (the source of smoother_vis's
-// sweep); store = false: only the block minima of the ratio (no planes)
+// sweep); store = false: only the block minima of the ratio (no planes);
+// slots: the minima also into the loop's slots (VisLoopArgs)
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
                         const EigScalars& e, int want_ratio, double* partial, const int32_t* gate = nullptr,
-                        double* copy_dst = nullptr, bool store = true);
+                        double* copy_dst = nullptr, bool store = true, unsigned long long* slots = nullptr);
 void launch_local_dt(hipStream_t st, Level& L, const EigScalars& e, double courant_stream);
 // cptr / gate (device-controlled smoother_vis): the factor read from the
 // device, and the launch skipped while gate[0] is set
 void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
                       const FD& F, const Spacing& h, int kmin, int kmax, double c, const double* cptr = nullptr,
                       const int32_t* gate = nullptr);
-// one smoother_vis iteration's sweep (T0 -> V, factor *cptr) and the
-// viscosityBCs after it, in one launch skipped while gate[0] is set
+struct VisLoopArgs;
+// one smoother_vis iteration: its decision, then (unless it stops the loop)
+// the sweep (T0 -> V) and the viscosityBCs after it, in one launch
 void launch_vis_sweep_bcs(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
-                          const FD& F, const Spacing& h, int kmin, int kmax, const double* cptr, const int32_t* gate,
+                          const FD& F, const Spacing& h, int kmin, int kmax, const VisLoopArgs& lp,
                           const double* Vf, const Grid& gf, int level, double RaT, double visT2);
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
                      double RaT, double visT2, const int32_t* gate = nullptr);
@@ -107,7 +109,20 @@ struct VisDecide {
     double eig_ratio_min, eig_ratio;
     double c0[VIS_CMAX], c1[VIS_CMAX];  // factors of the first pass (option 1) and of the later / only passes
 };
-void launch_vis_decide(hipStream_t st, const double* partial, int nb, const VisDecide& P, int32_t* ctl, double* dctl);
+// The device-side loop of smoother_vis: k_vis_grad_eig puts each block's
+// minimum ratio into VIS_SLOTS slots (atomicMin, VIS_SLOT_STRIDE words apart)
+// and k_vis_sweep_bcs makes the iteration's decision from them.  The loop
+// state alternates between two buffers with the iteration's parity.
+constexpr int VIS_SLOTS = 64, VIS_SLOT_STRIDE = 16;
+struct VisLoopArgs {
+    const int32_t* ctl_in;
+    int32_t* ctl_out;
+    const double* dctl_in;
+    double* dctl_out;
+    const unsigned long long* slots_in;
+    unsigned long long* slots_reset;
+    VisDecide P;
+};
 
 // --- tc_coarse.hip -------------------------------------------------------------
 void launch_coarse_assemble(hipStream_t st, double* Mc, double* RowB, const Level& L, const FD& F, const Spacing& h,

@@ -271,7 +271,9 @@ __device__ void vis_decide(double a, const VisDecide& P, int32_t* __restrict__ c
 // also copied there (the source of the iteration's sweep, which k_vis_decide
 // may still switch off; the copy is then unused); store = 0: the gradient and
 // ratio planes are not written (inside the loop only the block minima are
-// read, and the planes are written from the final viscosity after it).
+// read, and the planes are written from the final viscosity after it);
+// slots: each block's minimum also goes into slot blk % VIS_SLOTS by atomicMin
+// (the loop's decision reads the minimum of the slots in the next launch).
 template <bool POW2>
 __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__ V, double* __restrict__ vx,
                                                       double* __restrict__ vz, double* __restrict__ vxx,
@@ -279,7 +281,8 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
                                                       double* __restrict__ ratio, Grid g, FD F, Spacing h, int ivis,
                                                       EigScalars e, int want_ratio, double* __restrict__ partial,
                                                       const int32_t* __restrict__ gate,
-                                                      double* __restrict__ copy_dst, int store)
+                                                      double* __restrict__ copy_dst, int store,
+                                                      unsigned long long* __restrict__ slots)
 {
     if (gate && gate[0]) return;  // the device-controlled loop has ended
     if (copy_dst) {
@@ -325,9 +328,15 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
             m = r;
         }
     }
-    if (partial) {
+    if (partial || slots) {
         m = block_min(m);
-        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = m;
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            const unsigned blk = blockIdx.y * gridDim.x + blockIdx.x;
+            if (partial) partial[blk] = m;
+            // the ratio is >= 0 (or NaN, skipped as by the comparison minimum):
+            // its bit pattern orders as the value
+            if (slots) atomicMin(slots + (size_t)(blk % VIS_SLOTS) * VIS_SLOT_STRIDE, (unsigned long long)__double_as_longlong(m));
+        }
     }
 }
 
@@ -452,15 +461,55 @@ __global__ __launch_bounds__(256) void k_visc_bcs(double* __restrict__ V, Grid g
 // bottom and top ghost rows in its own column and in its mirror ghost
 // column (phase 3).  Every cell k_visc_bcs writes is written here by the one
 // thread that owns its source, with the same arithmetic.
+//
+// The iteration's decision is made here too (VisLoop): every block takes the
+// minimum of the slots k_vis_grad_eig filled and runs vis_decide on a copy of
+// the loop state of this iteration (ctl_in, dctl_in); all blocks reach the
+// same decision.  Block 0 writes the new state to the other buffer (ctl_out,
+// dctl_out: the buffers alternate with the iteration's parity, so no block
+// reads a state another block writes) and resets the slot set of the next
+// iteration.  Once the loop has stopped, the state is only carried over.
 template <bool POW2>
 __global__ __launch_bounds__(256) void k_vis_sweep_bcs(const double* __restrict__ T0, double* __restrict__ V,
                                                        const double* __restrict__ dtv, Grid g, FD F, Spacing h,
-                                                       int kmin, int kmax, const double* __restrict__ cptr,
-                                                       const int32_t* __restrict__ gate,
+                                                       int kmin, int kmax, VisLoopArgs lp,
                                                        const double* __restrict__ Vf, Grid gf, int level, double RaT,
                                                        double visT2)
 {
-    if (gate[0]) return;
+    __shared__ int s_sweep;
+    __shared__ double s_c;
+    static_assert(VIS_SLOTS == 64, "one slot per lane of the first wave (blockDim.x = 64)");
+    if (threadIdx.y == 0) {
+        // the minimum of the slots, a slot each lane; an unset slot (NaN bit
+        // pattern) and a NaN ratio count as +inf, as the comparison minimum
+        // of the block minima skips them
+        double a = __longlong_as_double((long long)lp.slots_in[(size_t)threadIdx.x * VIS_SLOT_STRIDE]);
+        if (!(a < INFINITY)) a = INFINITY;
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_xor(a, o, 64);
+            a = w < a ? w : a;
+        }
+        if (threadIdx.x == 0) {
+            int32_t st[VC_COUNT];
+            double dst[VD_COUNT];
+            for (int q = 0; q < VC_COUNT; ++q) st[q] = lp.ctl_in[q];
+            for (int q = 0; q < VD_COUNT; ++q) dst[q] = lp.dctl_in[q];
+            if (st[VC_DONE]) {
+                st[VC_NOSWEEP] = 1;
+            } else {
+                vis_decide(a, lp.P, st, dst);
+            }
+            s_sweep = !st[VC_NOSWEEP];
+            s_c = dst[VD_C];
+            if (blockIdx.x == 0 && blockIdx.y == 0) {
+                for (int q = 0; q < VC_COUNT; ++q) lp.ctl_out[q] = st[q];
+                for (int q = 0; q < VD_COUNT; ++q) lp.dctl_out[q] = dst[q];
+                for (int q = 0; q < VIS_SLOTS; ++q) lp.slots_reset[(size_t)q * VIS_SLOT_STRIDE] = ~0ull;
+            }
+        }
+    }
+    __syncthreads();
+    if (!s_sweep) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y * blockDim.y + threadIdx.y;
     const int nx = g.nx, nz = g.nz;
@@ -468,7 +517,7 @@ __global__ __launch_bounds__(256) void k_vis_sweep_bcs(const double* __restrict_
     const size_t cc = g.at(i, k);
     double v;
     if (k >= kmin && k <= kmax) {
-        const double c = cptr[0];
+        const double c = s_c;
         double Trr = sdiv<POW2>(dot3(F.D2, T0, cc, 1), h.dr2, h.idr2);
         double Tss = sdiv<POW2>(dot3(F.D2, T0, cc, g.ld), h.ds2, h.ids2);
         double temp = Trr + Tss;
@@ -586,16 +635,16 @@ void launch_visf(hipStream_t st, double* vis, const double* Tvis, const PtrArr& 
 
 int launch_vis_grad_eig(hipStream_t st, bool pow2, Level& L, const FD& F, const Spacing& h, int ivis,
                         const EigScalars& e, int want_ratio, double* partial, const int32_t* gate,
-                        double* copy_dst, bool store)
+                        double* copy_dst, bool store, unsigned long long* slots)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(L.g.nx - 1, L.g.nz - 1, b);
     if (pow2)
         hipLaunchKernelGGL((k_vis_grad_eig<true>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0, slots);
     else
         hipLaunchKernelGGL((k_vis_grad_eig<false>), gr, b, 0, st, L.vis, L.visx, L.visz, L.visxx, L.viszz, L.visxz,
-                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0);
+                           L.ratio, L.g, F, h, ivis, e, want_ratio, partial, gate, copy_dst, store ? 1 : 0, slots);
     return (int)(gr.x * gr.y);
 }
 
@@ -619,17 +668,17 @@ void launch_vis_sweep(hipStream_t st, bool pow2, const double* T0, double* V, co
 }
 
 void launch_vis_sweep_bcs(hipStream_t st, bool pow2, const double* T0, double* V, const double* dtv, const Grid& g,
-                          const FD& F, const Spacing& h, int kmin, int kmax, const double* cptr, const int32_t* gate,
+                          const FD& F, const Spacing& h, int kmin, int kmax, const VisLoopArgs& lp,
                           const double* Vf, const Grid& gf, int level, double RaT, double visT2)
 {
     dim3 b(64, 4);
     dim3 gr = grid2(g.nx + 1, g.nz + 1, b);
     if (pow2)
-        hipLaunchKernelGGL((k_vis_sweep_bcs<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, cptr, gate, Vf, gf,
-                           level, RaT, visT2);
+        hipLaunchKernelGGL((k_vis_sweep_bcs<true>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, lp, Vf, gf, level,
+                           RaT, visT2);
     else
-        hipLaunchKernelGGL((k_vis_sweep_bcs<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, cptr, gate, Vf, gf,
-                           level, RaT, visT2);
+        hipLaunchKernelGGL((k_vis_sweep_bcs<false>), gr, b, 0, st, T0, V, dtv, g, F, h, kmin, kmax, lp, Vf, gf, level,
+                           RaT, visT2);
 }
 
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
@@ -643,31 +692,5 @@ void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf,
 
 
 
-// The loop control of smoother_vis on the device, one iteration per launch:
-// the minimum of the per-block eigenvalue ratios (as k_reduce_final), then
-// vis_decide, or, once the loop has stopped, only no sweep.  Later launches
-// change nothing after the stop, so the host may queue more iterations than
-// the loop needs.
-__global__ __launch_bounds__(256) void k_vis_decide(const double* __restrict__ partial, int nb, VisDecide P,
-                                                    int32_t* __restrict__ ctl, double* __restrict__ dctl)
-{
-    double a = INFINITY;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-        const double v = partial[t];
-        a = v < a ? v : a;
-    }
-    a = block_min(a);
-    if (threadIdx.x != 0) return;
-    if (ctl[VC_DONE]) {
-        ctl[VC_NOSWEEP] = 1;
-        return;
-    }
-    vis_decide(a, P, ctl, dctl);
-}
-
-void launch_vis_decide(hipStream_t st, const double* partial, int nb, const VisDecide& P, int32_t* ctl, double* dctl)
-{
-    hipLaunchKernelGGL(k_vis_decide, dim3(1), dim3(256), 0, st, partial, nb, P, ctl, dctl);
-}
 
 }  // namespace tc
