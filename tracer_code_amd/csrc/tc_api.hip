@@ -84,8 +84,8 @@ struct tc_dev {
     unsigned long long* fslots = nullptr;  // the filter iterations' smoothness maxima (rotating slot sets)
     double* fbuf = nullptr;    // third iterate buffer of the blocked C filter loop
     int32_t* h_ctl = nullptr;  // pinned mirror
-    // device loop control of smoother_vis (k_vis_decide): state, factor and
-    // ratio, their pinned mirrors, and the pinned initial state
+    // device loop control of smoother_vis (vis_decide in k_vis_sweep_bcs):
+    // state, factor and ratio, their pinned mirrors, and the pinned initial state
     int32_t* vctl = nullptr;   // two loop-state buffers of smoother_vis (VC_COUNT each)
     double* vdctl = nullptr;   // two (VD_COUNT each)
     unsigned long long* vslots = nullptr;  // two sets of VIS_SLOTS minimum-ratio slots
@@ -351,13 +351,13 @@ int smoother_vis_host(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
     return 0;
 }
 
-// smoother_vis (smoother.f90:165-263) with the loop control on the device
-// (k_vis_decide): the host queues iterations -- the gradients and
-// eigenvalue ratios, the exit test, the copy, the Chebyshev sweep and the
-// boundary values -- in batches sized by the iterations the previous call
-// on this level took, and reads the loop state once per batch; queued
-// iterations after the exit do nothing.  The Chebyshev factors are the host
-// table of the same formula, so the sweeps see the same values.
+// smoother_vis (smoother.f90:165-263) with the loop control on the device:
+// the host queues iterations -- two launches each: the eigenvalue ratios'
+// minimum and the copy; the exit test, the Chebyshev sweep and the boundary
+// values -- in batches sized by the iterations the previous call on this
+// level took, and reads the loop state once per batch; queued iterations
+// after the exit do nothing.  The Chebyshev factors are the host table of
+// the same formula, so the sweeps see the same values.
 int smoother_vis(tc_ctx* ctx, tc_dev& d, int level, int interpolate)
 {
     Level& L = ctx->lv[level];

@@ -100,7 +100,7 @@ void launch_vis_sweep_bcs(hipStream_t st, bool pow2, const double* T0, double* V
                           const double* Vf, const Grid& gf, int level, double RaT, double visT2);
 void launch_visc_bcs(hipStream_t st, double* V, const Grid& g, const double* Vf, const Grid& gf, int level, int init,
                      double RaT, double visT2, const int32_t* gate = nullptr);
-// smoother_vis's loop state on the device (k_vis_decide)
+// smoother_vis's loop state on the device (vis_decide)
 enum { VC_DONE = 0, VC_ERR, VC_NOSWEEP, VC_NN, VC_N, VC_NITER, VC_NVIS, VC_NNSTORE, VC_TAB, VC_NETA, VC_ITERS, VC_COUNT };
 enum { VD_C = 0, VD_EIG, VD_COUNT };
 constexpr int VIS_CMAX = 32;  // longest pass (Niter) the device control takes

@@ -268,8 +268,8 @@ __device__ void vis_decide(double a, const VisDecide& P, int32_t* __restrict__ c
 // 666-698, eigI 750-803, eigR 806-857) on uniform grids, with per-block
 // minima of the ratio for smoother_vis (smoother.f90:209).  gate: the
 // device-controlled smoother_vis loop has stopped; copy_dst: the plane V is
-// also copied there (the source of the iteration's sweep, which k_vis_decide
-// may still switch off; the copy is then unused); store = 0: the gradient and
+// also copied there (the source of the iteration's sweep, which the loop's
+// decision may still switch off; the copy is then unused); store = 0: the gradient and
 // ratio planes are not written (inside the loop only the block minima are
 // read, and the planes are written from the final viscosity after it);
 // slots: each block's minimum also goes into slot blk % VIS_SLOTS by atomicMin
