@@ -124,6 +124,7 @@ __device__ __forceinline__ dd dd_mul(dd a, dd b)
 // double-double reciprocals (InvSpacing).
 constexpr int RD_BX = 64, RD_BY = 4, RD_H = 2;              // block shape (launch_residual_dd), halo
 constexpr int RD_W = RD_BX + 2 * RD_H, RD_CELLS = RD_W * (RD_BY + 2 * RD_H);
+constexpr int RD_NST = (RD_CELLS + RD_BX * RD_BY - 1) / (RD_BX * RD_BY);  // staged cells per thread
 
 __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ hi, const double* __restrict__ lo,
                                                      const double* __restrict__ vis, const double* __restrict__ vx,
@@ -137,30 +138,41 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
     // the 62 stencil taps of each point read LDS instead of the L1/TA path
     __shared__ double sh[RD_CELLS], sl[RD_CELLS];
     const int2 tile = xcd_tile();
+    const int i = tile.x * RD_BX + threadIdx.x + 1;
+    const int k = tile.y * RD_BY + threadIdx.y + 1;
+    const bool in = i <= g.nx - 1 && k <= g.nz - 1;
+    // the point's coefficients and right-hand side, loaded with the staged
+    // cells (every thread, branch-free, as in k_absorb_inc)
+    const size_t c = g.at(in ? i : 1, in ? k : 1);
+    const double v = vis[c], cvx = vx[c], cvz = vz[c], cvxx = vxx[c], cvzz = vzz[c], cvxz = vxz[c], crhs = rhs[c];
     {
         const int i0 = tile.x * RD_BX + 1 - RD_H, k0 = tile.y * RD_BY + 1 - RD_H;
-        for (int l = threadIdx.y * RD_BX + threadIdx.x; l < RD_CELLS; l += RD_BX * RD_BY) {
+        double h[RD_NST], o[RD_NST];
+#pragma unroll
+        for (int u = 0; u < RD_NST; ++u) {
+            const int l = threadIdx.y * RD_BX + threadIdx.x + u * RD_BX * RD_BY;
             const int ii = i0 + l % RD_W, kk = k0 + l / RD_W;
-            double h = 0.0, o = 0.0;
-            if (ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN) {
-                const size_t q = g.at(ii, kk);
-                h = hi[q];
-                o = lo[q];
+            const bool ok = l < RD_CELLS && ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN;
+            const size_t q = g.at(ok ? ii : 0, ok ? kk : 0);
+            const double hv = hi[q], ov = lo[q];
+            h[u] = ok ? hv : 0.0;
+            o[u] = ok ? ov : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RD_NST; ++u) {
+            const int l = threadIdx.y * RD_BX + threadIdx.x + u * RD_BX * RD_BY;
+            if (l < RD_CELLS) {
+                sh[l] = h[u];
+                sl[l] = o[u];
             }
-            sh[l] = h;
-            sl[l] = o;
         }
     }
     __syncthreads();
-    const int i = tile.x * blockDim.x + threadIdx.x + 1;
-    const int k = tile.y * blockDim.y + threadIdx.y + 1;
     double a = 0.0;
-    if (i <= g.nx - 1 && k <= g.nz - 1) {
-        const size_t c = g.at(i, k);
+    if (in) {
         const size_t cl = (size_t)(threadIdx.y + RD_H) * RD_W + threadIdx.x + RD_H;
         const ptrdiff_t ld = RD_W;
         auto sc = [&](dd x, int q) { return dd_mul(x, dd{ih.hi[q], ih.lo[q]}); };
-        const double v = vis[c];
         dd Srrrr = sc(csum_dot(F.D4, -2, 2, sh, sl, cl, 1), IH_R4);
         dd Sssss = sc(csum_dot(F.D4, -2, 2, sh, sl, cl, ld), IH_S4);
         dd Srrss = sc(csum_9(F.Drrss, sh, sl, cl, ld), IH_R2S2);
@@ -174,12 +186,12 @@ __global__ __launch_bounds__(256) void k_residual_dd(const double* __restrict__ 
             dd Srs = sc(csum_9(F.Drs, sh, sl, cl, ld), IH_RS);
             dd Srss = sc(csum_9(F.Drss, sh, sl, cl, ld), IH_RS2);
             dd Srrs = sc(csum_9(F.Drrs, sh, sl, cl, ld), IH_R2S);
-            sum = dd_add(sum, dd_mul_d(dd_add(Srr, dd_neg(Sss)), vxx[c] - vzz[c]));
-            sum = dd_add(sum, dd_mul_d(Srs, 4.0 * vxz[c]));
-            sum = dd_add(sum, dd_mul_d(dd_add(Srrr, Srss), 2.0 * vx[c]));
-            sum = dd_add(sum, dd_mul_d(dd_add(Ssss, Srrs), 2.0 * vz[c]));
+            sum = dd_add(sum, dd_mul_d(dd_add(Srr, dd_neg(Sss)), cvxx - cvzz));
+            sum = dd_add(sum, dd_mul_d(Srs, 4.0 * cvxz));
+            sum = dd_add(sum, dd_mul_d(dd_add(Srrr, Srss), 2.0 * cvx));
+            sum = dd_add(sum, dd_mul_d(dd_add(Ssss, Srrs), 2.0 * cvz));
         }
-        dd r = dd_add({rhs[c], 0.0}, dd_neg(sum));
+        dd r = dd_add({crhs, 0.0}, dd_neg(sum));
         const double rv = r.hi + r.lo;
         r64[c] = rv;
         if (r32) r32[c] = (float)rv;
@@ -222,6 +234,24 @@ __device__ __forceinline__ double prolong_val(const T* __restrict__ Ec, const Gr
     if (ke) return ((double)Ec[gc.at((i - 1) / 2, k / 2)] + (double)Ec[gc.at((i + 1) / 2, k / 2)]) / 2.0;
     return ((double)Ec[gc.at((i - 1) / 2, (k - 1) / 2)] + (double)Ec[gc.at((i + 1) / 2, (k - 1) / 2)] +
             (double)Ec[gc.at((i + 1) / 2, (k + 1) / 2)] + (double)Ec[gc.at((i - 1) / 2, (k + 1) / 2)]) / 4.0;
+}
+
+// prolong_val with its loads outside the branches: the four corners' values
+// are loaded at every point (one address four times at an even-even point)
+// and combined by the same case formulas, so the result has the same bits
+// and a caller's loads can all be in flight together.  (i, k) interior.
+template <typename T>
+__device__ __forceinline__ double prolong_val_nb(const T* __restrict__ Ec, const Grid& gc, int i, int k)
+{
+    const bool ie = (i % 2) == 0, ke = (k % 2) == 0;
+    const int ia = ie ? i / 2 : (i - 1) / 2, ib = ie ? i / 2 : (i + 1) / 2;
+    const int ka = ke ? k / 2 : (k - 1) / 2, kb = ke ? k / 2 : (k + 1) / 2;
+    const double aa = (double)Ec[gc.at(ia, ka)], ba = (double)Ec[gc.at(ib, ka)];
+    const double ab = (double)Ec[gc.at(ia, kb)], bb = (double)Ec[gc.at(ib, kb)];
+    // every case formed and one selected (branches would take the loads
+    // into them)
+    const double ve = (aa + ab) / 2.0, vk = (aa + ba) / 2.0, vo = (aa + ba + bb + ab) / 4.0;
+    return (ie && ke) ? aa : (ie ? ve : (ke ? vk : vo));
 }
 
 // psi(hi,lo) += bilinear prolongation of the level-2 correction
@@ -269,39 +299,61 @@ __global__ __launch_bounds__(256) void k_absorb_inc(double* __restrict__ hi, dou
 {
     __shared__ double se[RD_CELLS];
     const int2 tile = xcd_tile();
+    const int i = tile.x * RD_BX + threadIdx.x + 1;
+    const int k = tile.y * RD_BY + threadIdx.y + 1;
+    const bool in = i <= g.nx - 1 && k <= g.nz - 1;
+    // the point's own values, loaded before the staging so that their
+    // latency overlaps the halo's.  Every thread loads (a thread beyond the
+    // interior reads point (1,1) and discards it): behind a branch the
+    // compiler folds the operator's coefficient products into it and waits
+    // for the loads there.
+    const size_t c = g.at(in ? i : 1, in ? k : 1);
+    const double ph = hi[c], pl = lo[c], pr = r64[c];
+    const double cv[6] = {vis[c], vx[c], vz[c], vxx[c], vzz[c], vxz[c]};
     {
+        // the staged cells of this thread, all loaded before the first is
+        // stored (a rolled loop waits for each load in turn)
         const int i0 = tile.x * RD_BX + 1 - RD_H, k0 = tile.y * RD_BY + 1 - RD_H;
-        for (int l = threadIdx.y * RD_BX + threadIdx.x; l < RD_CELLS; l += RD_BX * RD_BY) {
+        double v[RD_NST];
+#pragma unroll
+        for (int u = 0; u < RD_NST; ++u) {
+            const int l = threadIdx.y * RD_BX + threadIdx.x + u * RD_BX * RD_BY;
             const int ii = i0 + l % RD_W, kk = k0 + l / RD_W;
-            double v = 0.0;
+            // branch-free loads: a cell that takes no value reads a valid
+            // address and is set to zero after
             if constexpr (PRO) {
                 int mi = ii, mk = kk;
                 double sg = 1.0;
                 if (mi < 0) { mi = -mi; sg = -sg; } else if (mi > g.nx) { mi = 2 * g.nx - mi; sg = -sg; }
                 if (mk < 0) { mk = -mk; sg = sg * zsg; } else if (mk > g.nz) { mk = 2 * g.nz - mk; sg = sg * zsg; }
-                if (mi >= 1 && mi <= g.nx - 1 && mk >= 1 && mk <= g.nz - 1) v = sg * prolong_val(e, gc, mi, mk);
+                const bool ok = l < RD_CELLS && mi >= 1 && mi <= g.nx - 1 && mk >= 1 && mk <= g.nz - 1;
+                const double pv = prolong_val_nb(e, gc, ok ? mi : 1, ok ? mk : 1);
+                v[u] = ok ? sg * pv : 0.0;
             } else {
-                if (ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN) v = (double)e[g.at(ii, kk)];
+                const bool ok = l < RD_CELLS && ii >= -SPAN && ii <= g.nx + SPAN && kk >= -SPAN && kk <= g.nz + SPAN;
+                const double ev = (double)e[g.at(ok ? ii : 0, ok ? kk : 0)];
+                v[u] = ok ? ev : 0.0;
             }
-            se[l] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < RD_NST; ++u) {
+            const int l = threadIdx.y * RD_BX + threadIdx.x + u * RD_BX * RD_BY;
+            if (l < RD_CELLS) se[l] = v[u];
         }
     }
     __syncthreads();
-    const int i = tile.x * RD_BX + threadIdx.x + 1;
-    const int k = tile.y * RD_BY + threadIdx.y + 1;
     double a = 0.0;
-    if (i <= g.nx - 1 && k <= g.nz - 1) {
-        const size_t c = g.at(i, k);
+    if (in) {
         const size_t cl = (size_t)(threadIdx.y + RD_H) * RD_W + threadIdx.x + RD_H;
-        const dd s = dd_add({hi[c], lo[c]}, {se[cl], 0.0});
+        const dd s = dd_add({ph, pl}, {se[cl], 0.0});
         hi[c] = s.hi;
         lo[c] = s.lo;
         stream_ghosts(hi, g, i, k, s.hi, zsg);
         stream_ghosts(lo, g, i, k, s.lo, zsg);
         // 0 - L(e)
-        const double le = stream_space_g<POW2>(F, h, ivis, PlaneAt<double>{se, cl, RD_W}, vis[c], vx[c], vz[c], vxx[c],
-                                               vzz[c], vxz[c], 0.0);
-        const double rv = r64[c] + le;
+        const double le =
+            stream_space_g<POW2>(F, h, ivis, PlaneAt<double>{se, cl, RD_W}, cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], 0.0);
+        const double rv = pr + le;
         r64[c] = rv;
         if (r32) r32[c] = (float)rv;
         a = fabs(rv);
@@ -519,12 +571,16 @@ __global__ __launch_bounds__(256) void k_gcr_dots(const double* __restrict__ rs,
     for (int k = 1 + blockIdx.x; k <= g.nz - 1; k += gridDim.x)
         for (int i = 1 + threadIdx.x; i <= g.nx - 1; i += blockDim.x) {
             const size_t c = g.at(i, k);
+            // every load of the point first: issued back to back, they
+            // overlap (loaded one per use, each waited for the last)
+            double b[NB > 0 ? NB : 1];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) b[j] = Q.v[j][c];
             const double rv = rs[c], qv = rv - r[c];
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
-                const double b = Q.v[j][c];
-                acc[2 * j] += qv * b;
-                acc[2 * j + 1] += rv * b;
+                acc[2 * j] += qv * b[j];
+                acc[2 * j + 1] += rv * b[j];
             }
             acc[2 * NB] += qv * qv;
             acc[2 * NB + 1] += rv * qv;
