@@ -284,6 +284,13 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
                                                       double* __restrict__ copy_dst, int store,
                                                       unsigned long long* __restrict__ slots)
 {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const bool in = i <= g.nx - 1 && k <= g.nz - 1;
+    // the point's neighbourhood, loaded first so that it arrives with the
+    // gate and the copy's values (every thread: one beyond the interior
+    // reads the neighbourhood of point (1,1) and discards it)
+    const Reg9 S = load9(V, g.at(in ? i : 1, in ? k : 1), g.ld);
     if (gate && gate[0]) return;  // the device-controlled loop has ended
     if (copy_dst) {
         const size_t nt = (size_t)gridDim.x * gridDim.y * blockDim.x * blockDim.y;
@@ -292,22 +299,20 @@ __global__ __launch_bounds__(256) void k_vis_grad_eig(const double* __restrict__
              j < g.elems; j += nt)
             copy_dst[j] = V[j];
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y + 1;
     double m = INFINITY;
-    if (i <= g.nx - 1 && k <= g.nz - 1) {
+    if (in) {
         const size_t c = g.at(i, k);
         double gx = 0.0, gz = 0.0, gxx = 0.0, gzz = 0.0, gxz = 0.0;
         if (ivis != 0) {
-            gx = sdiv<POW2>(dot3(F.D1, V, c, 1), h.dr, h.idr);
-            gxx = sdiv<POW2>(dot3(F.D2, V, c, 1), h.dr2, h.idr2);
-            gz = sdiv<POW2>(dot3(F.D1, V, c, g.ld), h.ds, h.ids);
-            gzz = sdiv<POW2>(dot3(F.D2, V, c, g.ld), h.ds2, h.ids2);
-            gxz = sdiv<POW2>(sum9(F.Drs, V, c, g.ld), h.drds, h.idrds);
+            gx = sdiv<POW2>(dot3g<0>(F.D1, S), h.dr, h.idr);
+            gxx = sdiv<POW2>(dot3g<0>(F.D2, S), h.dr2, h.idr2);
+            gz = sdiv<POW2>(dot3g<1>(F.D1, S), h.ds, h.ids);
+            gzz = sdiv<POW2>(dot3g<1>(F.D2, S), h.ds2, h.ids2);
+            gxz = sdiv<POW2>(sum9g(F.Drs, S), h.drds, h.idrds);
         }
         if (store) { vx[c] = gx; vz[c] = gz; vxx[c] = gxx; vzz[c] = gzz; vxz[c] = gxz; }
         if (want_ratio && ivis != 0) {
-            const double vis = V[c];
+            const double vis = S(0, 0);
             // eigI with xr=zs=1, xrr=..=0: only the odd-derivative terms survive
             double t1 = (gxx + gzz) * (0.0 + 0.0);
             double t2 = vis * (0.0 + 0.0);
@@ -479,6 +484,20 @@ __global__ __launch_bounds__(256) void k_vis_sweep_bcs(const double* __restrict_
     __shared__ int s_sweep;
     __shared__ double s_c;
     static_assert(VIS_SLOTS == 64, "one slot per lane of the first wave (blockDim.x = 64)");
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.y + threadIdx.y;
+    const int nx = g.nx, nz = g.nz;
+    const bool act = i <= nx && k <= nz;
+    // the sweep's values, loaded before the decision so that they arrive with
+    // the slots (every thread: one beyond the grid reads around point (0,0)
+    // and discards it)
+    const size_t cc = g.at(act ? i : 0, act ? k : 0);
+    const double t0 = T0[cc], tm = T0[cc - 1], tp = T0[cc + 1], tdm = T0[cc - g.ld], tdp = T0[cc + g.ld];
+    const double dv = dtv[cc];
+    // rows 0 and nz of a coarser level take the finer level's value
+    const bool fromf = act && level > 1 && (k == 0 || k == nz);
+    const double* __restrict__ fsrc = Vf ? Vf : T0;
+    const double vf = fsrc[fromf ? gf.at(2 * i, k == 0 ? 0 : gf.nz) : cc];
     if (threadIdx.y == 0) {
         // the minimum of the slots, a slot each lane; an unset slot (NaN bit
         // pattern) and a NaN ratio count as +inf, as the comparison minimum
@@ -510,23 +529,26 @@ __global__ __launch_bounds__(256) void k_vis_sweep_bcs(const double* __restrict_
     }
     __syncthreads();
     if (!s_sweep) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y * blockDim.y + threadIdx.y;
-    const int nx = g.nx, nz = g.nz;
-    if (i > nx || k > nz) return;
-    const size_t cc = g.at(i, k);
+    if (!act) return;
     double v;
     if (k >= kmin && k <= kmax) {
         const double c = s_c;
-        double Trr = sdiv<POW2>(dot3(F.D2, T0, cc, 1), h.dr2, h.idr2);
-        double Tss = sdiv<POW2>(dot3(F.D2, T0, cc, g.ld), h.ds2, h.ids2);
+        // dot3 along each axis, on the loaded values
+        double ar = F.D2[1] * tm;
+        ar += F.D2[2] * t0;
+        ar += F.D2[3] * tp;
+        double as = F.D2[1] * tdm;
+        as += F.D2[2] * t0;
+        as += F.D2[3] * tdp;
+        double Trr = sdiv<POW2>(ar, h.dr2, h.idr2);
+        double Tss = sdiv<POW2>(as, h.ds2, h.ids2);
         double temp = Trr + Tss;
-        double dts = dtv[cc] * 0.99;
-        v = T0[cc] + c * dts * temp;
+        double dts = dv * 0.99;
+        v = t0 + c * dts * temp;
     } else {
-        v = T0[cc];
+        v = t0;
     }
-    if (level > 1 && (k == 0 || k == nz)) v = Vf[gf.at(2 * i, k == 0 ? 0 : gf.nz)];
+    if (fromf) v = vf;
     V[cc] = v;
     const bool lo_i = i >= 1 && i <= SPAN, hi_i = i >= nx - SPAN && i <= nx - 1;
     if (lo_i) V[g.at(-i, k)] = v;

@@ -70,6 +70,22 @@ struct PlaneAt {
     __device__ __forceinline__ T operator()(int di, int dk) const { return f[c + (ptrdiff_t)dk * ld + di]; }
 };
 
+// a 3x3 neighbourhood of element c held in registers, for the accessor
+// forms: its nine loads are issued together, ahead of other work
+struct Reg9 {
+    double v[9];
+    __device__ __forceinline__ double operator()(int di, int dk) const { return v[(dk + 1) * 3 + di + 1]; }
+};
+__device__ __forceinline__ Reg9 load9(const double* __restrict__ f, size_t c, ptrdiff_t ld)
+{
+    Reg9 r;
+#pragma unroll
+    for (int dk = -1; dk <= 1; ++dk)
+#pragma unroll
+        for (int di = -1; di <= 1; ++di) r.v[(dk + 1) * 3 + di + 1] = f[c + (ptrdiff_t)dk * ld + di];
+    return r;
+}
+
 __device__ __forceinline__ double dot3(const double* D, const double* __restrict__ f, size_t c, ptrdiff_t st)
 {
     double acc = D[1] * f[c - st];
