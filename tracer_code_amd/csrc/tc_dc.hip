@@ -437,15 +437,33 @@ __global__ __launch_bounds__(256) void k_residual_restrict32(const float* __rest
     const int2 tile = xcd_tile();
     const int i = tile.x * blockDim.x + threadIdx.x + 1;
     const int k = tile.y * blockDim.y + threadIdx.y + 1;
-    if (i > go.nx - 1 || k > go.nz - 1) return;
-    const int ii = 2 * i, kk = 2 * k;
-    auto r = [&](int a, int b) {
-        const size_t c = gi.at(a, b);
-        return (float)stream_space32<double>(F, h, S, c, gi.ld, vis[c], vx[c], vz[c], a1[c], vxz[c], rhs[c]);
-    };
-    const float rmm = r(ii - 1, kk - 1), rmp = r(ii - 1, kk + 1), rpp = r(ii + 1, kk + 1);
-    const float rm0 = r(ii - 1, kk), r0m = r(ii, kk - 1), rp0 = r(ii + 1, kk), r0p = r(ii, kk + 1), r00 = r(ii, kk);
-    const float v = restrict_fw32(rmm, rmp, rpp, rm0, r0m, rp0, r0p, r00);
+    const bool in = i <= go.nx - 1 && k <= go.nz - 1;
+    // every value the eight residuals read, loaded before the first of them
+    // (computed one after another from memory, each waited for its own
+    // loads): the 7x7 fine neighbourhood and the residual points'
+    // coefficients.  A thread beyond the interior loads around coarse point
+    // (1,1) and stores nothing.
+    const int ii = 2 * (in ? i : 1), kk = 2 * (in ? k : 1);
+    float P[49];
+#pragma unroll
+    for (int dk = -3; dk <= 3; ++dk)
+#pragma unroll
+        for (int di = -3; di <= 3; ++di) P[(dk + 3) * 7 + di + 3] = S[gi.at(ii + di, kk + dk)];
+    // the residual points, in the order the restriction takes them
+    constexpr int RA[8] = {-1, -1, 1, -1, 0, 1, 0, 0}, RB[8] = {-1, 1, 1, 0, -1, 0, 1, 0};
+    float cf[8][6];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const size_t c = gi.at(ii + RA[t], kk + RB[t]);
+        cf[t][0] = vis[c]; cf[t][1] = vx[c]; cf[t][2] = vz[c]; cf[t][3] = a1[c]; cf[t][4] = vxz[c]; cf[t][5] = rhs[c];
+    }
+    if (!in) return;
+    float rv[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+        rv[t] = (float)stream_space32_g<double>(F, h, Patch7At{P, RA[t], RB[t]}, cf[t][0], cf[t][1], cf[t][2], cf[t][3],
+                                                cf[t][4], cf[t][5]);
+    const float v = restrict_fw32(rv[0], rv[1], rv[2], rv[3], rv[4], rv[5], rv[6], rv[7]);
     out[go.at(i, k)] = v;
     if (bout) {
         const int q = i * (go.nz + 1) + k;
@@ -481,7 +499,11 @@ __global__ __launch_bounds__(256) void k_coarse_rhs32(const double* __restrict__
 // Coarse solve of the FP32 correction cycle: the explicit inverse rounded to
 // FP32 (half the bytes of the GEMV; the cycle's corrections are FP32), FP64
 // right-hand side and accumulation, result rounded to FP32: one workgroup
-// per unknown, its row of the inverse times b.
+// per unknown, its row of the inverse times b.  J > 0: n <= 256 J, and a
+// thread loads its J chunks' values before the first product (a rolled loop
+// waits for each chunk's loads in turn); the products are summed in the
+// loop's order, so the result has the same bits.  J = 0: the loop.
+template <int J>
 __global__ __launch_bounds__(256) void k_coarse_apply32(const float* __restrict__ Ainv, const double* __restrict__ b,
                                                         float* __restrict__ E, Grid g, int n, float zsg)
 {
@@ -489,7 +511,21 @@ __global__ __launch_bounds__(256) void k_coarse_apply32(const float* __restrict_
     const int ip = blockIdx.x + 1, kp = blockIdx.y + 1;
     const float* row = Ainv + (size_t)(N * ip + kp) * (size_t)n;
     double acc = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) acc += (double)row[q] * b[q];
+    if constexpr (J > 0) {
+        float a[J];
+        double x[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int q = threadIdx.x + 256 * j;
+            a[j] = row[q < n ? q : 0];
+            x[j] = b[q < n ? q : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+            if (threadIdx.x + 256 * j < n) acc += (double)a[j] * x[j];
+    } else {
+        for (int q = threadIdx.x; q < n; q += blockDim.x) acc += (double)row[q] * b[q];
+    }
     acc = block_sum(acc);
     if (threadIdx.x == 0) {
         const float v = (float)acc;
@@ -872,8 +908,14 @@ void launch_coarse_apply32(hipStream_t st, const float* Ainv, const double* RowB
     if (!b_ready)
         hipLaunchKernelGGL(k_coarse_rhs32, dim3((unsigned)((ngc + 255) / 256)), dim3(256), 0, st, RowB, rhs, L.g, ngc,
                            b);
-    hipLaunchKernelGGL(k_coarse_apply32, dim3(L.g.nx - 1, L.g.nz - 1), dim3(256), 0, st, Ainv, b, E, L.g, ngc,
-                       (float)zsg);
+    const dim3 gr(L.g.nx - 1, L.g.nz - 1), bl(256);
+    const int chunks = (ngc + 255) / 256;
+    if (chunks <= 2) hipLaunchKernelGGL(k_coarse_apply32<2>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
+    else if (chunks <= 4) hipLaunchKernelGGL(k_coarse_apply32<4>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
+    else if (chunks <= 8) hipLaunchKernelGGL(k_coarse_apply32<8>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
+    else if (chunks <= 12) hipLaunchKernelGGL(k_coarse_apply32<12>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
+    else if (chunks <= 16) hipLaunchKernelGGL(k_coarse_apply32<16>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
+    else hipLaunchKernelGGL(k_coarse_apply32<0>, gr, bl, 0, st, Ainv, b, E, L.g, ngc, (float)zsg);
 }
 
 __global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n)

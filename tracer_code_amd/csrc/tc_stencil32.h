@@ -77,6 +77,14 @@ __device__ __forceinline__ A stream_space32(const FD32& F, const H32& h, const f
     return stream_space32_g<A>(F, h, PlaneAt<float>{S, c, ld}, vis, vx, vz, a1, vxz, rhs);
 }
 
+// S(i+di, k+dk) of a 7x7 neighbourhood held in registers (row-major, centre
+// at index 24) for the point (a, b) cells from its centre
+struct Patch7At {
+    const float* p;
+    int a, b;
+    __device__ __forceinline__ float operator()(int di, int dk) const { return p[(b + dk + 3) * 7 + a + di + 3]; }
+};
+
 // S(i+di, k+dk) of a tile staged in LDS with row pitch ld
 struct LdsAt {
     const float* f;
